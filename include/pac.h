@@ -1,0 +1,163 @@
+/*
+ * pac.h -- C ABI of the MI355X weighted alpha complex engine (libpac.so).
+ *
+ * Drop-in boundary.  The reference (forest113/ParallelAC-xmp) has no
+ * library: its only interface is the process CLI
+ *     parallelac <crd-file> <out-file> <sol-rad> <alpha>
+ * (AlphaComplexParallel.cu:1053-1057; chunked variant
+ * AlphaComplexLargeData.cu:1054-1058).  Each entry point below replaces one
+ * step of that program; the CLI `parallelac` built from this library is a
+ * thin wrapper over them and writes the reference's file format.
+ *
+ * Conventions: every function returns PAC_OK (0) or a negative status and
+ * never prints; pac_last_error() gives a thread-local message.  Results are
+ * library-owned and opaque; free them with pac_result_free().  Distinct
+ * results / streams may be used from distinct threads; one result is not
+ * re-entrant.  Simplex vertex ids are original 0-based input positions.
+ */
+#ifndef PAC_H
+#define PAC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PAC_OK = 0,
+  PAC_ERR_IO = -1,          /* file missing / unreadable / short read */
+  PAC_ERR_INVALID = -2,     /* bad argument, non-finite input, empty grid */
+  PAC_ERR_DEGENERATE = -3,  /* reserved: degenerate input reported */
+  PAC_ERR_NOMEM = -4,       /* host or device allocation failed */
+  PAC_ERR_DEVICE = -5,      /* HIP runtime error */
+  PAC_ERR_OVERFLOW = -6,    /* a count would exceed the index width */
+  PAC_ERR_RANGE = -7        /* input exponent range exceeds exact capacity */
+};
+
+enum { PAC_CELL_EXACT = 0, PAC_CELL_REFERENCE_RAW = 1 };
+enum { PAC_PREC_PLAIN_FP64 = 0, PAC_PREC_STAGED = 1 };
+enum { PAC_ORDER_REFERENCE = 0, PAC_ORDER_CANONICAL = 1 };
+
+typedef struct pac_options {
+  int32_t cell_policy;   /* PAC_CELL_*: exact = grid-independent stencil  */
+  int32_t precision;     /* PAC_PREC_*: FP64->int, or FP32->FP64->DD->int */
+  int32_t skip_tet_ac2;  /* debug / fixture mode: keep all alpha tets     */
+  int32_t order;         /* PAC_ORDER_*: row order of the output lists    */
+  int32_t device_inputs; /* 1: xyzr is a device pointer (e.g. torch)      */
+  int32_t device;        /* HIP device ordinal                            */
+  void* stream;          /* hipStream_t to run on, NULL = library stream  */
+  int32_t block;         /* threads per block override (0 = tuned)        */
+  int32_t reserved[7];
+} pac_options;
+
+typedef struct pac_result pac_result;
+
+/* Per-run statistics.  stage_evals[p][s]: predicate family p (0 edge filter,
+ * 1 triangle filter, 2 tet filter, 3 tet emptiness, 4 triangle emptiness,
+ * 5 edge emptiness) evaluated at stage s (0 FP32, 1 FP64, 2 double-double,
+ * 3 wide integer); sos_calls counts exact ties broken by SoS. */
+typedef struct pac_stats {
+  int64_t n;
+  int64_t edge_candidates, edge_alpha, edges_final;
+  int64_t tri_candidates, tri_alpha, tris_final;
+  int64_t tet_candidates, tet_alpha, tets_final;
+  int64_t stage_evals[6][4];
+  int64_t sos_calls;
+  int64_t ac2_pairs[3]; /* (simplex, ball) pairs tested: tet, tri, edge */
+  int32_t k_enum, k_ac2, scale_bits, grid_cells;
+  /* phase times in ms with the reference's labels
+   * (AlphaComplexParallel.cu:1818-1830): Data transfer, Grid computation,
+   * Potential edges, Potential triangles, Potential tetrahedra,
+   * AC2 test for tetrahedra, AC2 test for triangles, AC2 test for edges */
+  double phase_ms[8];
+  double total_ms;
+} pac_stats;
+
+void pac_options_init(pac_options* opt);
+
+/* Whole pipeline on one GPU.  xyzr: n x (x, y, z, r_raw) FP32 in input
+ * order; balls carry fl32(r_raw + solvent) (AlphaComplexParallel.cu:1086).
+ * Replaces AlphaComplexParallel.cu:1126-1806 (everything between parsing
+ * and writing). */
+int pac_compute(const float* xyzr, int64_t n, float solvent, float alpha,
+                const pac_options* opt, pac_result** out);
+
+/* Same with FP64 inputs (continuous coordinates; no reference counterpart,
+ * the reference is FP32-only: predicates.h:142). */
+int pac_compute_f64(const double* xyzr, int64_t n, double solvent,
+                    double alpha, const pac_options* opt, pac_result** out);
+
+/* counts[0..3] = vertices, edges, triangles, tetrahedra
+ * (AlphaComplexParallel.cu:1809-1816). */
+int pac_result_counts(const pac_result* r, int64_t counts[4]);
+/* Copy simplex lists (row-major, 2/3/4 int32 per row) to host memory
+ * (AlphaComplexParallel.cu:1794-1806 + the id mapping at :1842-1856). */
+int pac_result_copy_edges(const pac_result* r, int32_t* dst);
+int pac_result_copy_triangles(const pac_result* r, int32_t* dst);
+int pac_result_copy_tets(const pac_result* r, int32_t* dst);
+/* Device pointer of the list for dim 1/2/3 (valid until pac_result_free). */
+const int32_t* pac_result_device_list(const pac_result* r, int dim);
+/* FP64 alpha values (squared orthoradius rho) of the output simplices of
+ * dim 1/2/3, in list order.  The reference computes them for triangles and
+ * tets (float4 .w, AlphaComplexParallel.cu:370, :603) and discards them. */
+int pac_result_copy_alpha(const pac_result* r, int dim, double* rho);
+int pac_result_stats(const pac_result* r, pac_stats* st);
+void pac_result_free(pac_result* r);
+
+/* Batched ensemble: n_mol molecules concatenated, offsets[n_mol+1]
+ * (int64) into the ball array.  All molecules are processed by one graph
+ * launch on the device; per-molecule lists are returned with offsets.  No
+ * reference counterpart (the reference handles one molecule per process). */
+typedef struct pac_batch_result pac_batch_result;
+int pac_compute_batch(const float* xyzr, const int64_t* offsets, int64_t n_mol,
+                      float solvent, float alpha, const pac_options* opt,
+                      pac_batch_result** out);
+/* per-molecule counts: out[m*4 + {0..3}] */
+int pac_batch_counts(const pac_batch_result* r, int64_t* out);
+/* lists of molecule m (ids local to the molecule) */
+int pac_batch_copy(const pac_batch_result* r, int64_t m, int dim, int32_t* dst);
+int pac_batch_stats(const pac_batch_result* r, pac_stats* st);
+void pac_batch_free(pac_batch_result* r);
+
+/* Adjoint of the per-simplex alpha values with the combinatorics held
+ * fixed: given dL/drho for the output triangles and tets (list order),
+ * accumulate dL/d(x,y,z) (n x 3) and dL/d(r) (n) in input order.  No
+ * reference counterpart. */
+int pac_alpha_adjoint(const pac_result* r, const double* dl_drho_tri,
+                      const double* dl_drho_tet, double* dl_dxyz,
+                      double* dl_dr);
+
+/* CRD input (AlphaComplexParallel.cu:1069-1088): returns a malloc'ed
+ * n x 4 FP32 array (x, y, z, r_raw); free with pac_free(). */
+int pac_read_crd(const char* path, float** xyzr, int64_t* n);
+/* Output file in the reference's format (AlphaComplexParallel.cu:1834-1856). */
+int pac_write_output(const char* path, const float* xyzr_raw, int64_t n,
+                     float solvent, const pac_result* r);
+void pac_free(void* p);
+
+/* Synthetic protein-like ball sets (deterministic; SplitMix64 seeds).
+ * globule: Poisson-disk centres (min separation min_sep) in a sphere,
+ * radii drawn from {1.70:.63, 1.55:.17, 1.52:.19, 1.80:.01}, coordinates
+ * rounded to 10^-decimals (decimals < 0: no rounding). */
+int pac_synth_globule(int64_t n, double sphere_radius, double min_sep,
+                      uint64_t seed, int decimals, float* xyzr_out);
+int pac_synth_globule_f64(int64_t n, double sphere_radius, double min_sep,
+                          uint64_t seed, double jitter_sigma,
+                          double* xyzr_out);
+/* ensemble sizes: n_mol sizes log-uniform in [nmin, nmax]; offsets_out has
+ * n_mol+1 entries; call with xyzr_out = NULL first to size the buffer. */
+int pac_synth_batch(int64_t n_mol, int64_t nmin, int64_t nmax, uint64_t seed,
+                    int64_t* offsets_out, float* xyzr_out);
+/* stress: lattice_frac of the balls form a central cubic lattice of
+ * spacing `spacing` with raw radius lattice_r, the rest Poisson-disk. */
+int pac_synth_stress(int64_t n, double sphere_radius, uint64_t seed,
+                     double lattice_frac, double spacing, double lattice_r,
+                     float* xyzr_out);
+
+const char* pac_last_error(void);
+const char* pac_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,248 @@
+/*
+ * k_prims.hip -- device-wide primitives written for wave64 / gfx950:
+ *   - exclusive scan (block tiles of 256 threads x 8 items, wave64 shuffle
+ *     scans, recursive over tile sums), replacing the reference's Thrust
+ *     inclusive_scan (AlphaComplexParallel.cu:1207, :1287, :1389) and the
+ *     lower/upper_bound CSR construction (:1179-1193, :1260-1263);
+ *   - stable LSD radix sort of (key, value) pairs with 8-bit digits, stable
+ *     in-tile ranking through 64-bit ballots, replacing
+ *     thrust::sort_by_key of the cell keys (AlphaComplexParallel.cu:1157).
+ */
+#include "pac_internal.h"
+
+namespace pac {
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+/* block-wide exclusive scan of one value per thread; returns block total */
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v,
+                                                        uint64_t* total) {
+  __shared__ uint64_t wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t inc = wave_incl_scan_u64(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t base = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q) {
+    if (q < w) base += wsum[q];
+    tot += wsum[q];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+template <class TIn, bool kFlag>
+__device__ __forceinline__ uint64_t load_val(const TIn* in, int64_t i,
+                                             int64_t n) {
+  if (i >= n) return 0;
+  if (kFlag) return in[i] != 0 ? 1 : 0;
+  return (uint64_t)in[i];
+}
+
+template <class TIn, bool kFlag>
+__global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, int64_t n,
+                                                      uint64_t* sums) {
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+  uint64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) s += load_val<TIn, kFlag>(in, base + q, n);
+  uint64_t tot;
+  block_excl_scan_u64(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class TIn, bool kFlag>
+__global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
+                                                      const uint64_t* tile_off,
+                                                      uint64_t* out) {
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    v[q] = load_val<TIn, kFlag>(in, base + q, n);
+    s += v[q];
+  }
+  uint64_t tot;
+  uint64_t ex = block_excl_scan_u64(s, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < n) out[base + q] = ex;
+    ex += v[q];
+  }
+  /* the last thread of the last tile writes the grand total at out[n] */
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
+}
+
+size_t scan_tmp_bytes(int64_t n) {
+  size_t b = 0;
+  int64_t m = n;
+  while (m > kTile) {
+    int64_t t = (m + kTile - 1) / kTile;
+    b += (size_t)(t + 1) * 2 * sizeof(uint64_t);
+    m = t;
+  }
+  return b + 256;
+}
+
+__global__ void k_store_u64(uint64_t* out, uint64_t v) { *out = v; }
+
+/* recursive scan over uint64 tile sums */
+static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
+                         uint64_t* tmp, hipStream_t s);
+
+template <class TIn, bool kFlag>
+static void scan_impl(const TIn* in, uint64_t* out, int64_t n, uint64_t* tmp,
+                      hipStream_t s) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, s, out, (uint64_t)0);
+    return;
+  }
+  int64_t tiles = (n + kTile - 1) / kTile;
+  if (tiles == 1) {
+    hipLaunchKernelGGL((k_tile_scan<TIn, kFlag>), dim3(1), dim3(kBlock), 0, s,
+                       in, n, (const uint64_t*)nullptr, out);
+    return;
+  }
+  uint64_t* sums = tmp;
+  uint64_t* offs = tmp + (tiles + 1);
+  hipLaunchKernelGGL((k_tile_sums<TIn, kFlag>), dim3((unsigned)tiles),
+                     dim3(kBlock), 0, s, in, n, sums);
+  scan_u64_rec(sums, offs, tiles, tmp + 2 * (tiles + 1), s);
+  hipLaunchKernelGGL((k_tile_scan<TIn, kFlag>), dim3((unsigned)tiles),
+                     dim3(kBlock), 0, s, in, n, (const uint64_t*)offs, out);
+}
+
+static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
+                         uint64_t* tmp, hipStream_t s) {
+  scan_impl<uint64_t, false>(in, out, n, tmp, s);
+}
+
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n, void* tmp,
+                     size_t, hipStream_t s) {
+  scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s);
+}
+void scan_u8flag_to_u64(const uint8_t* in, uint64_t* out, int64_t n, void* tmp,
+                        size_t, hipStream_t s) {
+  scan_impl<uint8_t, true>(in, out, n, (uint64_t*)tmp, s);
+}
+
+/* ------------------------------------------------------------------ */
+/* radix sort                                                          */
+/* ------------------------------------------------------------------ */
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys,
+                                                       int64_t n, int shift,
+                                                       int64_t ntiles,
+                                                       uint32_t* hist) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    int64_t i = base + (int64_t)q * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+/* Stable scatter: items are ranked round by round (256 consecutive items
+ * per round); inside a wave, lanes with equal digits are found with eight
+ * 64-bit ballots; across the four waves and earlier rounds, LDS counters
+ * give the offset. */
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const uint32_t* keys, const uint32_t* vals, int64_t n, int shift,
+    int64_t ntiles, const uint64_t* hoff, uint32_t* keys_out,
+    uint32_t* vals_out) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcnt[kBlock / 64][256];
+  __shared__ uint64_t base_of[256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  running[threadIdx.x] = 0;
+  base_of[threadIdx.x] = hoff[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  for (int q = 0; q < kBlock / 64; ++q) wcnt[q][threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kTile;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kScanItems; ++r) {
+    int64_t i = base + (int64_t)r * kBlock + threadIdx.x;
+    bool valid = i < n;
+    uint32_t key = valid ? keys[i] : 0u;
+    uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    bool leader = valid && (peers & lt_mask) == 0;
+    if (leader) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = running[d] + rank;
+      for (int q = 0; q < w; ++q) off += wcnt[q][d];
+      uint64_t dst = base_of[d] + off;
+      keys_out[dst] = key;
+      vals_out[dst] = vals[i];
+    }
+    __syncthreads();
+    {
+      uint32_t add = 0;
+      for (int q = 0; q < kBlock / 64; ++q) {
+        add += wcnt[q][threadIdx.x];
+        wcnt[q][threadIdx.x] = 0;
+      }
+      running[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+size_t radix_tmp_bytes(int64_t n) {
+  int64_t ntiles = (n + kTile - 1) / kTile;
+  size_t h = (size_t)256 * ntiles;
+  return h * sizeof(uint32_t) + (h + 1) * sizeof(uint64_t) + scan_tmp_bytes(h) + 512;
+}
+
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                      uint32_t* vals_alt, int64_t n, int bits, void* tmp,
+                      size_t, hipStream_t s, bool* result_in_alt) {
+  *result_in_alt = false;
+  if (n <= 1 || bits <= 0) return;
+  int64_t ntiles = (n + kTile - 1) / kTile;
+  size_t h = (size_t)256 * ntiles;
+  uint32_t* hist = (uint32_t*)tmp;
+  uint64_t* hoff = (uint64_t*)(((uintptr_t)(hist + h) + 15) & ~(uintptr_t)15);
+  uint64_t* stmp = hoff + h + 1;
+  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  bool alt = false;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0,
+                       s, ki, n, shift, ntiles, hist);
+    scan_impl<uint32_t, false>(hist, hoff, (int64_t)h, stmp, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock),
+                       0, s, ki, vi, n, shift, ntiles, hoff, ko, vo);
+    uint32_t* t = ki;
+    ki = ko;
+    ko = t;
+    t = vi;
+    vi = vo;
+    vo = t;
+    alt = !alt;
+  }
+  *result_in_alt = alt;
+}
+
+}  // namespace pac

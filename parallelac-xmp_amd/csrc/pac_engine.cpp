@@ -1,0 +1,802 @@
+/*
+ * pac_engine.cpp -- host orchestration of the MI355X pipeline and the C ABI
+ * declared in include/pac.h.
+ *
+ * Replaces the reference's main() phase sequence (AlphaComplexParallel.cu:
+ * 1126-1806): one HIP stream per call, device-resident intermediates taken
+ * from a caching pool (no hipMalloc in steady state), counts read back
+ * between stages only where a buffer must be sized, hipEvents around the
+ * reference's eight phases.  Errors are returned as status codes, never
+ * printed (the reference prints and continues: :55-61).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pac.h"
+#include "pac_internal.h"
+
+using namespace pac;
+
+/* ------------------------------------------------------------------ */
+/* errors                                                              */
+/* ------------------------------------------------------------------ */
+static thread_local std::string g_err;
+struct PacError : std::runtime_error {
+  int code;
+  PacError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+static void fail(int code, const std::string& m) { throw PacError(code, m); }
+void pac_set_error(const char* m) { g_err = m ? m : ""; }
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess)                                                \
+      fail(e_ == hipErrorOutOfMemory ? PAC_ERR_NOMEM : PAC_ERR_DEVICE,   \
+           std::string(#x) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+#define KCHK() HIPCHK(hipGetLastError())
+
+template <class F>
+static int guarded(F f) {
+  try {
+    f();
+    return PAC_OK;
+  } catch (const PacError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return PAC_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PAC_ERR_INVALID;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* caching device pool                                                 */
+/* ------------------------------------------------------------------ */
+struct Pool {
+  std::mutex m;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> cls_;
+  static size_t klass(size_t b) {
+    size_t c = 4096;
+    while (c < b) c <<= 1;
+    return c;
+  }
+  void* get(size_t bytes) {
+    size_t c = klass(bytes ? bytes : 1);
+    {
+      std::lock_guard<std::mutex> g(m);
+      auto it = free_.find(c);
+      if (it != free_.end()) {
+        void* p = it->second;
+        free_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, c);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(PAC_ERR_NOMEM, "device allocation of " + std::to_string(c) +
+                                " bytes failed");
+      }
+    }
+    std::lock_guard<std::mutex> g(m);
+    cls_[p] = c;
+    return p;
+  }
+  void put(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(m);
+    free_.insert({cls_[p], p});
+  }
+  void trim() {
+    std::lock_guard<std::mutex> g(m);
+    for (auto& kv : free_) {
+      (void)hipFree(kv.second);
+      cls_.erase(kv.second);
+    }
+    free_.clear();
+  }
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  std::mutex run_mu;
+  void* pinned = nullptr; /* small pinned staging for scalar read-backs */
+};
+static std::mutex g_eng_mu;
+static std::map<int, Engine*> g_engines;
+static Engine* engine_for(int dev) {
+  std::lock_guard<std::mutex> g(g_eng_mu);
+  auto it = g_engines.find(dev);
+  if (it != g_engines.end()) return it->second;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    fail(PAC_ERR_DEVICE, "no HIP device available");
+  }
+  if (dev < 0 || dev >= ndev) fail(PAC_ERR_INVALID, "bad device ordinal");
+  HIPCHK(hipSetDevice(dev));
+  Engine* e = new Engine();
+  e->device = dev;
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc(&e->pinned, 4096, hipHostMallocDefault));
+  g_engines[dev] = e;
+  return e;
+}
+
+template <class T>
+struct DBuf {
+  Pool* pool = nullptr;
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() {}
+  DBuf(Pool* pl, size_t count) : pool(pl), n(count) {
+    p = (T*)pl->get(std::max<size_t>(count, 1) * sizeof(T));
+  }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    release();
+    pool = o.pool;
+    p = o.p;
+    n = o.n;
+    o.p = nullptr;
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p && pool) pool->put(p);
+    p = nullptr;
+  }
+  T* get() const { return p; }
+};
+
+/* ------------------------------------------------------------------ */
+/* result                                                              */
+/* ------------------------------------------------------------------ */
+struct pac_result {
+  Engine* eng = nullptr;
+  int64_t counts[4] = {0, 0, 0, 0};
+  DBuf<int32_t> lists[3];     /* edges, tris, tets (original ids) */
+  DBuf<float4> balls;         /* original order, r = fl(r_raw + s) */
+  double alpha = 0;
+  pac_stats st;
+};
+
+/* ------------------------------------------------------------------ */
+/* helpers                                                             */
+/* ------------------------------------------------------------------ */
+static int neg_lsb_exp_host(double v) {
+  if (v == 0) return 0;
+  int e;
+  double m = std::frexp(v, &e);
+  long long mi = (long long)std::ldexp(std::fabs(m), 53);
+  int lsb = e - 53 + __builtin_ctzll((unsigned long long)mi);
+  return lsb < 0 ? -lsb : 0;
+}
+
+/* v * 2^shift as a 512-bit two's-complement integer (v*2^shift integral) */
+static WI wi_from_scaled_host(double v, int shift) {
+  WI r;
+  std::memset(&r, 0, sizeof(r));
+  if (v == 0) return r;
+  int e;
+  double m = std::frexp(v, &e);
+  long long mi = (long long)std::ldexp(m, 53);
+  bool neg = mi < 0;
+  unsigned long long mag = (unsigned long long)(neg ? -mi : mi);
+  int ex = e - 53 + shift;
+  while (ex < 0 && (mag & 1ull) == 0) {
+    mag >>= 1;
+    ++ex;
+  }
+  if (ex < 0) fail(PAC_ERR_RANGE, "alpha not representable at the scale");
+  /* place mag << ex into limbs */
+  for (int b = 0; b < 64; ++b)
+    if (mag >> b & 1ull) {
+      int pos = b + ex;
+      if (pos >= 32 * WL - 1) fail(PAC_ERR_RANGE, "alpha too large");
+      r.l[pos / 32] |= 1u << (pos % 32);
+    }
+  if (neg) {
+    /* two's complement negate */
+    uint64_t carry = 1;
+    for (int i = 0; i < WL; ++i) {
+      uint64_t t = (uint64_t)(uint32_t)~r.l[i] + carry;
+      r.l[i] = (uint32_t)t;
+      carry = t >> 32;
+    }
+  }
+  return r;
+}
+
+struct Timer {
+  hipEvent_t ev[10];
+  int used = 0;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  }
+  ~Timer() {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  void mark(int i) { HIPCHK(hipEventRecord(ev[i], s)); }
+  double ms(int a, int b) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return t;
+  }
+};
+
+template <class T>
+static T readback(Engine* E, hipStream_t s, const T* dptr) {
+  HIPCHK(hipMemcpyAsync(E->pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  T v;
+  std::memcpy(&v, E->pinned, sizeof(T));
+  return v;
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_zero_u32(unsigned* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+static void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     s, p, n, v);
+}
+static void zero_u32(unsigned* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+/* ------------------------------------------------------------------ */
+/* the pipeline                                                        */
+/* ------------------------------------------------------------------ */
+static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
+                         float alpha, const pac_options& opt, pac_result* R,
+                         hipStream_t s, Timer& T) {
+  Pool* P = &E->pool;
+  PipeLaunch L{s, kBlock};
+  pac_stats& st = R->st;
+  st.n = n;
+  if (n == 0) return;
+  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
+
+  /* ---- bounds, grid (AlphaComplexParallel.cu:1089-1124) ---- */
+  DBuf<Bounds> bpart(P, 1024 * 2 + 4), bout(P, 1);
+  launch_bounds(raw, n, solvent, bpart.get(), bout.get(), L);
+  KCHK();
+  Bounds B = readback(E, s, bout.get());
+  if (B.nonfinite) fail(PAC_ERR_INVALID, "non-finite coordinate or radius");
+  volatile float m2 = B.max_raw_r * B.max_raw_r;
+  volatile float m2a = m2 + alpha;
+  volatile float pmr = sqrtf(m2a);
+  float step = 2 * pmr;
+  if (!(step > 0) || !std::isfinite(step))
+    fail(PAC_ERR_INVALID, "grid step 2*sqrt(maxR^2+alpha) is not positive");
+  volatile float qx = (B.maxx - B.minx) / step, qy = (B.maxy - B.miny) / step,
+                 qz = (B.maxz - B.minz) / step;
+  int64_t ex = (int64_t)(int)qx + 1, ey = (int64_t)(int)qy + 1,
+          ez = (int64_t)(int)qz + 1;
+  int64_t G = ex * ey * ez;
+  if (G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grid has too many cells");
+  GridP g;
+  g.minx = B.minx;
+  g.miny = B.miny;
+  g.minz = B.minz;
+  g.step = step;
+  g.ex = (int)ex;
+  g.ey = (int)ey;
+  g.ez = (int)ez;
+  double maxw = (double)B.max_r * (double)B.max_r;
+  double reach2 = maxw + (double)alpha;
+  bool no_edges = reach2 < 0;
+  double reach = reach2 > 0 ? std::sqrt(reach2) * (1 + 1e-12) + 1e-12 : 0.0;
+  double maxext = std::max({(double)B.maxx - B.minx, (double)B.maxy - B.miny,
+                            (double)B.maxz - B.minz});
+  double margin = (maxext + (double)step) * 3.0 * 0x1p-24 + 1e-9;
+  if (opt.cell_policy == PAC_CELL_REFERENCE_RAW) {
+    g.k_enum = 1;
+    g.k_ac2 = 1;
+    reach = (double)step * (1 - 1e-6);
+    margin = 0;
+  } else {
+    g.k_enum = std::max(1, (int)std::ceil((2 * reach + 2 * margin) / step));
+    g.k_ac2 = std::max(1, (int)std::ceil((reach + margin) / step));
+  }
+  st.k_enum = g.k_enum;
+  st.k_ac2 = g.k_ac2;
+  st.grid_cells = (int32_t)G;
+
+  /* exact embedding scale */
+  int S = std::max(B.shift, (neg_lsb_exp_host(alpha) + 1) / 2);
+  double mag = std::max({(double)B.max_abs, (double)B.max_r,
+                         std::sqrt(std::fabs((double)alpha)), 1.0});
+  double bits = std::log2(mag) + S + 2;
+  if (8 * bits + 16 > 32.0 * WL - 2)
+    fail(PAC_ERR_RANGE, "input exponent range exceeds the 512-bit exact stage");
+  st.scale_bits = S;
+  Stage SG;
+  SG.n = n;
+  SG.g = g;
+  SG.alpha = alpha;
+  SG.ex.S = S;
+  SG.ex.alpha = wi_from_scaled_host((double)alpha, 2 * S);
+  SG.staged = opt.precision == PAC_PREC_STAGED;
+  SG.reach = reach;
+  SG.margin = margin;
+  DBuf<unsigned long long> ctr(P, CTR_COUNT);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ctr.get(), CTR_COUNT);
+  SG.ctr = ctr.get();
+  DBuf<unsigned> qn(P, 4);
+
+  /* ---- binning: key, stable radix sort, gather, cell starts ---- */
+  DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
+  launch_cell_key(raw, n, g, key.get(), idx.get(), L);
+  KCHK();
+  int kbits = 0;
+  while (kbits < 32 && ((int64_t)1 << kbits) < G) ++kbits;
+  size_t rtb = radix_tmp_bytes(n);
+  DBuf<uint8_t> rtmp(P, rtb);
+  bool alt = false;
+  radix_sort_pairs(key.get(), idx.get(), key2.get(), idx2.get(), n, kbits,
+                   rtmp.get(), rtb, s, &alt);
+  KCHK();
+  const uint32_t* skey = alt ? key2.get() : key.get();
+  const uint32_t* sidx = alt ? idx2.get() : idx.get();
+  DBuf<float4> ball(P, n);
+  DBuf<uint32_t> perm(P, n), cellk(P, n), cstart(P, G + 1);
+  launch_gather(raw, sidx, skey, n, solvent, ball.get(), perm.get(),
+                cellk.get(), L);
+  launch_cell_starts(cellk.get(), n, G, cstart.get(), L);
+  KCHK();
+  rtmp.release();
+  key.release();
+  key2.release();
+  idx.release();
+  idx2.release();
+  SG.ball = ball.get();
+  SG.cellk = cellk.get();
+  SG.cstart = cstart.get();
+  T.mark(2);
+
+  auto scan_tmp = [&](int64_t m) { return DBuf<uint8_t>(P, scan_tmp_bytes(m) + 64); };
+
+  /* ---- edges ---- */
+  int64_t Ecand = 0, Ealpha = 0;
+  DBuf<uint64_t> erow(P, n + 1);
+  DBuf<uint32_t> ecol, eleft;
+  if (!no_edges) {
+    DBuf<uint32_t> cnt(P, n);
+    DBuf<uint64_t> off(P, n + 1);
+    launch_edge_count(SG, cnt.get(), L);
+    {
+      auto tmp = scan_tmp(n);
+      scan_u32_to_u64(cnt.get(), off.get(), n, tmp.get(), 0, s);
+    }
+    KCHK();
+    Ecand = (int64_t)readback(E, s, off.get() + n);
+    if (Ecand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "edge candidates exceed 2^32");
+    DBuf<uint32_t> cand(P, Ecand), keep(P, n);
+    DBuf<uint8_t> flag(P, Ecand);
+    DBuf<QEntry> q(P, std::max<int64_t>(Ecand, 1));
+    zero_u32(qn.get(), 1, s);
+    launch_edge_fill(SG, off.get(), cand.get(), flag.get(), keep.get(), q.get(),
+                     qn.get(), (uint32_t)std::max<int64_t>(Ecand, 1), L);
+    KCHK();
+    uint32_t nq = readback(E, s, qn.get());
+    launch_edge_exact(SG, q.get(), nq, cand.get(), flag.get(), keep.get(), L);
+    {
+      auto tmp = scan_tmp(n);
+      scan_u32_to_u64(keep.get(), erow.get(), n, tmp.get(), 0, s);
+    }
+    KCHK();
+    Ealpha = (int64_t)readback(E, s, erow.get() + n);
+    ecol = DBuf<uint32_t>(P, Ealpha);
+    eleft = DBuf<uint32_t>(P, Ealpha);
+    launch_edge_compact(n, off.get(), cand.get(), flag.get(), erow.get(),
+                        ecol.get(), eleft.get(), L);
+    KCHK();
+  } else {
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 1 + 255) / 256)),
+                       dim3(256), 0, s, (unsigned long long*)erow.get(),
+                       (int)(n + 1));
+    ecol = DBuf<uint32_t>(P, 1);
+    eleft = DBuf<uint32_t>(P, 1);
+  }
+  st.edge_candidates = Ecand;
+  st.edge_alpha = Ealpha;
+  T.mark(3);
+
+  /* ---- triangles ---- */
+  int64_t Tcand = 0, Talpha = 0;
+  DBuf<uint64_t> trow(P, Ealpha + 1);
+  DBuf<uint32_t> tk, tedge;
+  {
+    DBuf<uint32_t> cnt(P, Ealpha);
+    DBuf<uint64_t> off(P, Ealpha + 1);
+    launch_tri_count(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
+                     cnt.get(), L);
+    {
+      auto tmp = scan_tmp(Ealpha);
+      scan_u32_to_u64(cnt.get(), off.get(), Ealpha, tmp.get(), 0, s);
+    }
+    KCHK();
+    Tcand = (int64_t)readback(E, s, off.get() + Ealpha);
+    if (Tcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "triangle candidates exceed 2^32");
+    DBuf<uint32_t> cand(P, Tcand), keep(P, Ealpha);
+    DBuf<uint8_t> flag(P, Tcand);
+    DBuf<QEntry> q(P, std::max<int64_t>(Tcand, 1));
+    zero_u32(qn.get(), 1, s);
+    if (Ealpha)
+      launch_tri_fill(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
+                      off.get(), cand.get(), flag.get(), keep.get(), q.get(),
+                      qn.get(), (uint32_t)std::max<int64_t>(Tcand, 1), L);
+    KCHK();
+    uint32_t nq = readback(E, s, qn.get());
+    launch_tri_exact(SG, q.get(), nq, ecol.get(), eleft.get(), cand.get(),
+                     flag.get(), keep.get(), L);
+    {
+      auto tmp = scan_tmp(Ealpha);
+      scan_u32_to_u64(keep.get(), trow.get(), Ealpha, tmp.get(), 0, s);
+    }
+    KCHK();
+    Talpha = (int64_t)readback(E, s, trow.get() + Ealpha);
+    tk = DBuf<uint32_t>(P, Talpha);
+    tedge = DBuf<uint32_t>(P, Talpha);
+    if (Ealpha)
+      launch_seg_compact(Ealpha, off.get(), cand.get(), flag.get(), trow.get(),
+                         tk.get(), tedge.get(), L);
+    KCHK();
+  }
+  st.tri_candidates = Tcand;
+  st.tri_alpha = Talpha;
+  T.mark(4);
+
+  /* ---- tets ---- */
+  int64_t Kcand = 0, Kalpha = 0;
+  DBuf<uint4> tets;
+  {
+    DBuf<uint32_t> cnt(P, Talpha);
+    DBuf<uint64_t> off(P, Talpha + 1);
+    if (Talpha)
+      launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
+                       tk.get(), tedge.get(), cnt.get(), L);
+    {
+      auto tmp = scan_tmp(Talpha);
+      scan_u32_to_u64(cnt.get(), off.get(), Talpha, tmp.get(), 0, s);
+    }
+    KCHK();
+    Kcand = (int64_t)readback(E, s, off.get() + Talpha);
+    if (Kcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "tet candidates exceed 2^32");
+    DBuf<uint32_t> cand(P, Kcand), keep(P, Talpha);
+    DBuf<uint8_t> flag(P, Kcand);
+    DBuf<QEntry> q(P, std::max<int64_t>(Kcand, 1));
+    zero_u32(qn.get(), 1, s);
+    if (Talpha)
+      launch_tet_fill(SG, Talpha, erow.get(), ecol.get(), eleft.get(),
+                      trow.get(), tk.get(), tedge.get(), off.get(), cand.get(),
+                      flag.get(), keep.get(), q.get(), qn.get(),
+                      (uint32_t)std::max<int64_t>(Kcand, 1), L);
+    KCHK();
+    uint32_t nq = readback(E, s, qn.get());
+    launch_tet_exact(SG, q.get(), nq, ecol.get(), eleft.get(), tk.get(),
+                     tedge.get(), cand.get(), flag.get(), keep.get(), L);
+    DBuf<uint64_t> orow(P, Talpha + 1);
+    {
+      auto tmp = scan_tmp(Talpha);
+      scan_u32_to_u64(keep.get(), orow.get(), Talpha, tmp.get(), 0, s);
+    }
+    KCHK();
+    Kalpha = (int64_t)readback(E, s, orow.get() + Talpha);
+    tets = DBuf<uint4>(P, Kalpha);
+    if (Talpha)
+      launch_tet_compact(Talpha, off.get(), cand.get(), flag.get(), orow.get(),
+                         ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                         tets.get(), L);
+    KCHK();
+  }
+  st.tet_candidates = Kcand;
+  st.tet_alpha = Kalpha;
+  T.mark(5);
+
+  /* ---- emptiness with retry on queue overflow ---- */
+  auto run_queued = [&](int64_t items, auto&& scan_kernel, auto&& exact_kernel) {
+    uint32_t cap = (uint32_t)std::min<int64_t>(
+        std::max<int64_t>(1 << 20, 2 * items), (int64_t)UINT32_MAX - 1);
+    for (;;) {
+      DBuf<QEntry> q(P, cap);
+      zero_u32(qn.get(), 1, s);
+      scan_kernel(q.get(), cap);
+      KCHK();
+      uint32_t nq = readback(E, s, qn.get());
+      if (nq <= cap) {
+        exact_kernel(q.get(), nq);
+        KCHK();
+        return;
+      }
+      cap = (uint32_t)std::min<int64_t>((int64_t)nq + nq / 4 + 1024,
+                                        (int64_t)UINT32_MAX - 1);
+    }
+  };
+
+  DBuf<uint8_t> kflag(P, Kalpha);
+  if (opt.skip_tet_ac2) {
+    fill_u8(kflag.get(), Kalpha, 1, s);
+  } else if (Kalpha) {
+    run_queued(
+        Kalpha,
+        [&](QEntry* q, uint32_t cap) {
+          launch_tet_ac2(SG, Kalpha, tets.get(), perm.get(), kflag.get(), q,
+                         qn.get(), cap, L);
+        },
+        [&](QEntry* q, uint32_t nq) {
+          launch_tet_ac2_exact(SG, q, nq, tets.get(), perm.get(), kflag.get(), L);
+        });
+  }
+  T.mark(6);
+
+  /* ---- triangle closure + dangling emptiness ---- */
+  DBuf<uint8_t> tri_in(P, Talpha), tflag(P, Talpha);
+  fill_u8(tri_in.get(), Talpha, 0, s);
+  launch_mark_tet_faces(Kalpha, tets.get(), kflag.get(), erow.get(), ecol.get(),
+                        trow.get(), tk.get(), tri_in.get(), L);
+  if (Talpha)
+    run_queued(
+        Talpha,
+        [&](QEntry* q, uint32_t cap) {
+          launch_tri_ac2(SG, Talpha, ecol.get(), eleft.get(), tk.get(),
+                         tedge.get(), tri_in.get(), tflag.get(), q, qn.get(),
+                         cap, L);
+        },
+        [&](QEntry* q, uint32_t nq) {
+          launch_tri_ac2_exact(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
+                               tedge.get(), tflag.get(), L);
+        });
+  T.mark(7);
+
+  /* ---- edge closure + dangling emptiness ---- */
+  DBuf<uint8_t> edge_in(P, Ealpha), eflag(P, Ealpha);
+  fill_u8(edge_in.get(), Ealpha, 0, s);
+  launch_mark_tri_edges(Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                        tflag.get(), erow.get(), edge_in.get(), L);
+  if (Ealpha)
+    run_queued(
+        Ealpha,
+        [&](QEntry* q, uint32_t cap) {
+          launch_edge_ac2(SG, Ealpha, ecol.get(), eleft.get(), edge_in.get(),
+                          eflag.get(), q, qn.get(), cap, L);
+        },
+        [&](QEntry* q, uint32_t nq) {
+          launch_edge_ac2_exact(SG, q, nq, ecol.get(), eleft.get(), eflag.get(),
+                                L);
+        });
+
+  /* ---- emit final lists (original ids, reference order) ---- */
+  DBuf<uint64_t> pos_k(P, Kalpha + 1), pos_t(P, Talpha + 1),
+      pos_e(P, Ealpha + 1);
+  {
+    auto tmp = scan_tmp(std::max({Kalpha, Talpha, Ealpha, (int64_t)1}));
+    scan_u8flag_to_u64(kflag.get(), pos_k.get(), Kalpha, tmp.get(), 0, s);
+    scan_u8flag_to_u64(tflag.get(), pos_t.get(), Talpha, tmp.get(), 0, s);
+    scan_u8flag_to_u64(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
+  }
+  KCHK();
+  int64_t Kf = (int64_t)readback(E, s, pos_k.get() + Kalpha);
+  int64_t Tf = (int64_t)readback(E, s, pos_t.get() + Talpha);
+  int64_t Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
+  R->lists[0] = DBuf<int32_t>(P, 2 * Ef);
+  R->lists[1] = DBuf<int32_t>(P, 3 * Tf);
+  R->lists[2] = DBuf<int32_t>(P, 4 * Kf);
+  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), eflag.get(), pos_e.get(),
+                    perm.get(), R->lists[0].get(), L);
+  launch_emit_tris(Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                   tflag.get(), pos_t.get(), perm.get(), R->lists[1].get(), L);
+  launch_emit_tets(Kalpha, tets.get(), kflag.get(), pos_k.get(), perm.get(),
+                   R->lists[2].get(), L);
+  KCHK();
+  R->counts[1] = Ef;
+  R->counts[2] = Tf;
+  R->counts[3] = Kf;
+  st.edges_final = Ef;
+  st.tris_final = Tf;
+  st.tets_final = Kf;
+  T.mark(8);
+
+  unsigned long long h[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(h, ctr.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int f = 0; f < 6; ++f)
+    for (int q = 0; q < 4; ++q)
+      st.stage_evals[f][q] = (int64_t)h[CTR_STAGE + 4 * f + q];
+  st.sos_calls = (int64_t)h[CTR_SOS];
+  for (int q = 0; q < 3; ++q) st.ac2_pairs[q] = (int64_t)h[CTR_PAIRS + q];
+}
+
+/* copy raw input with solvent-added radii, original order (kept for rho /
+ * adjoint queries) */
+__global__ void k_balls_orig(const float4* raw, int64_t n, float solvent,
+                             float4* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 a = raw[i];
+  out[i] = make_float4(a.x, a.y, a.z, __fadd_rn(a.w, solvent));
+}
+
+static void compute_f32(const float* xyzr, int64_t n, float solvent,
+                        float alpha, const pac_options* o, pac_result** out) {
+  if (!out) fail(PAC_ERR_INVALID, "out is NULL");
+  if (n < 0 || (n > 0 && !xyzr)) fail(PAC_ERR_INVALID, "bad input array");
+  if (!std::isfinite(solvent) || !std::isfinite(alpha))
+    fail(PAC_ERR_INVALID, "non-finite solvent or alpha");
+  pac_options opt;
+  pac_options_init(&opt);
+  if (o) opt = *o;
+  Engine* E = engine_for(opt.device);
+  std::lock_guard<std::mutex> lk(E->run_mu);
+  HIPCHK(hipSetDevice(E->device));
+  hipStream_t s = opt.stream ? (hipStream_t)opt.stream : E->stream;
+  std::unique_ptr<pac_result> R(new pac_result());
+  R->eng = E;
+  R->alpha = alpha;
+  std::memset(&R->st, 0, sizeof(R->st));
+  R->counts[0] = n;
+  Timer T(s);
+  T.mark(0);
+  auto t_host0 = std::chrono::steady_clock::now();
+  DBuf<float4> raw_own;
+  const float4* raw = (const float4*)xyzr;
+  if (n > 0 && !opt.device_inputs) {
+    raw_own = DBuf<float4>(&E->pool, n);
+    HIPCHK(hipMemcpyAsync(raw_own.get(), xyzr, (size_t)n * 16,
+                          hipMemcpyHostToDevice, s));
+    raw = raw_own.get();
+  }
+  T.mark(1);
+  if (n > 0) {
+    R->balls = DBuf<float4>(&E->pool, n);
+    hipLaunchKernelGGL(k_balls_orig, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, s, raw, n, solvent, R->balls.get());
+    KCHK();
+    run_pipeline(E, raw, n, solvent, alpha, opt, R.get(), s, T);
+  } else {
+    for (int i = 2; i <= 8; ++i) T.mark(i);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  auto t_host1 = std::chrono::steady_clock::now();
+  pac_stats& st = R->st;
+  st.phase_ms[0] = T.ms(0, 1);
+  st.phase_ms[1] = T.ms(1, 2);
+  st.phase_ms[2] = T.ms(2, 3);
+  st.phase_ms[3] = T.ms(3, 4);
+  st.phase_ms[4] = T.ms(4, 5);
+  st.phase_ms[5] = T.ms(5, 6);
+  st.phase_ms[6] = T.ms(6, 7);
+  st.phase_ms[7] = T.ms(7, 8);
+  st.total_ms =
+      std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
+  *out = R.release();
+}
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                               */
+/* ------------------------------------------------------------------ */
+extern "C" {
+
+void pac_options_init(pac_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->cell_policy = PAC_CELL_EXACT;
+  o->precision = PAC_PREC_STAGED;
+  o->order = PAC_ORDER_REFERENCE;
+}
+
+int pac_compute(const float* xyzr, int64_t n, float solvent, float alpha,
+                const pac_options* opt, pac_result** out) {
+  return guarded([&] { compute_f32(xyzr, n, solvent, alpha, opt, out); });
+}
+
+int pac_compute_f64(const double*, int64_t, double, double, const pac_options*,
+                    pac_result**) {
+  g_err = "pac_compute_f64: not built in this version";
+  return PAC_ERR_INVALID;
+}
+
+int pac_result_counts(const pac_result* r, int64_t counts[4]) {
+  if (!r || !counts) return PAC_ERR_INVALID;
+  for (int i = 0; i < 4; ++i) counts[i] = r->counts[i];
+  return PAC_OK;
+}
+
+static int copy_list(const pac_result* r, int dim, int32_t* dst) {
+  if (!r || !dst || dim < 1 || dim > 3) return PAC_ERR_INVALID;
+  return guarded([&] {
+    size_t bytes = (size_t)r->counts[dim] * (dim + 1) * sizeof(int32_t);
+    if (!bytes) return;
+    HIPCHK(hipSetDevice(r->eng->device));
+    HIPCHK(hipMemcpy(dst, r->lists[dim - 1].get(), bytes,
+                     hipMemcpyDeviceToHost));
+  });
+}
+int pac_result_copy_edges(const pac_result* r, int32_t* dst) {
+  return copy_list(r, 1, dst);
+}
+int pac_result_copy_triangles(const pac_result* r, int32_t* dst) {
+  return copy_list(r, 2, dst);
+}
+int pac_result_copy_tets(const pac_result* r, int32_t* dst) {
+  return copy_list(r, 3, dst);
+}
+const int32_t* pac_result_device_list(const pac_result* r, int dim) {
+  if (!r || dim < 1 || dim > 3) return nullptr;
+  return r->lists[dim - 1].get();
+}
+int pac_result_copy_alpha(const pac_result*, int, double*) {
+  g_err = "pac_result_copy_alpha: not built in this version";
+  return PAC_ERR_INVALID;
+}
+int pac_result_stats(const pac_result* r, pac_stats* st) {
+  if (!r || !st) return PAC_ERR_INVALID;
+  *st = r->st;
+  return PAC_OK;
+}
+void pac_result_free(pac_result* r) { delete r; }
+
+int pac_compute_batch(const float*, const int64_t*, int64_t, float, float,
+                      const pac_options*, pac_batch_result**) {
+  g_err = "pac_compute_batch: not built in this version";
+  return PAC_ERR_INVALID;
+}
+int pac_batch_counts(const pac_batch_result*, int64_t*) { return PAC_ERR_INVALID; }
+int pac_batch_copy(const pac_batch_result*, int64_t, int, int32_t*) {
+  return PAC_ERR_INVALID;
+}
+int pac_batch_stats(const pac_batch_result*, pac_stats*) { return PAC_ERR_INVALID; }
+void pac_batch_free(pac_batch_result*) {}
+int pac_alpha_adjoint(const pac_result*, const double*, const double*, double*,
+                      double*) {
+  g_err = "pac_alpha_adjoint: not built in this version";
+  return PAC_ERR_INVALID;
+}
+
+const char* pac_last_error(void) { return g_err.c_str(); }
+const char* pac_version(void) { return "parallelac-mi355x 0.1"; }
+
+/* test hook: the SoS term table compiled into the device code */
+int pac_internal_sos_table(int32_t* out3, int cap) {
+  return sos_table_host(out3, cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+/*
+ * pac_internal.h -- shared between the host engine (pac_engine.cpp) and the
+ * kernel translation units.  Not part of the C ABI.
+ */
+#ifndef PAC_INTERNAL_H
+#define PAC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pac_arith.h"
+
+namespace pac {
+
+constexpr int kBlock = 256;      /* threads per block, 4 wave64s */
+constexpr int kScanItems = 8;    /* items per thread in scans / radix */
+constexpr int kTile = kBlock * kScanItems;
+
+/* Reference grid (AlphaComplexParallel.cu:1120-1124) plus the stencil
+ * half-widths used for enumeration and emptiness scans. */
+struct GridP {
+  float minx, miny, minz, step;
+  int ex, ey, ez;
+  int k_enum, k_ac2;
+};
+
+/* raw-input bounds reduced on the device */
+struct Bounds {
+  float minx, miny, minz, maxx, maxy, maxz;
+  float max_raw_r, max_r;  /* raw radius, and fl(r_raw + solvent) */
+  int shift;               /* max over inputs of -lsb exponent (>= 0) */
+  int nonfinite;           /* count of non-finite inputs */
+  float max_abs;           /* max |coordinate| */
+  float pad;
+};
+
+struct QEntry {
+  uint32_t item;  /* candidate / simplex index */
+  uint32_t aux;   /* segment (ball / edge / triangle) or ball index */
+};
+
+/* counters layout (unsigned long long) */
+enum {
+  CTR_STAGE = 0,        /* [6][4] stage evaluations */
+  CTR_SOS = 24,
+  CTR_PAIRS = 25,       /* [3] tet, tri, edge pairs */
+  CTR_COUNT = 28
+};
+enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
+       FAM_TRI_AC2 = 4, FAM_EDGE_AC2 = 5 };
+
+/* ---- primitives (k_prims.hip) ---- */
+/* exclusive scan; out has n+1 entries, out[n] = total */
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n,
+                     void* tmp, size_t tmp_bytes, hipStream_t s);
+void scan_u8flag_to_u64(const uint8_t* in, uint64_t* out, int64_t n,
+                        void* tmp, size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(int64_t n);
+/* stable LSD radix sort of (key, val) by the low `bits` bits of key */
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                      uint32_t* vals_alt, int64_t n, int bits, void* tmp,
+                      size_t tmp_bytes, hipStream_t s, bool* result_in_alt);
+size_t radix_tmp_bytes(int64_t n);
+
+/* ---- pipeline (k_pipeline.hip) ---- */
+struct PipeLaunch {
+  hipStream_t s;
+  int block;
+};
+void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
+                   Bounds* out, PipeLaunch L);
+void launch_cell_key(const float4* raw, int64_t n, GridP g, uint32_t* key,
+                     uint32_t* idx, PipeLaunch L);
+void launch_gather(const float4* raw, const uint32_t* sidx,
+                   const uint32_t* skey, int64_t n, float solvent,
+                   float4* ball, uint32_t* perm, uint32_t* cellk,
+                   PipeLaunch L);
+void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
+                        uint32_t* cstart, PipeLaunch L);
+
+struct Stage {
+  const float4* ball;
+  int64_t n;
+  GridP g;
+  const uint32_t* cellk;
+  const uint32_t* cstart;
+  float alpha;
+  ExactScale ex;
+  unsigned long long* ctr;
+  int staged;
+  double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
+  double margin;  /* FP32 cell-key rounding, in length units */
+};
+
+/* edges */
+void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
+void launch_edge_fill(const Stage& S, const uint64_t* off, uint32_t* cand,
+                      uint8_t* flag, uint32_t* keep, QEntry* q,
+                      unsigned int* qn, uint32_t qcap, PipeLaunch L);
+void launch_edge_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                       const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                       PipeLaunch L);
+void launch_edge_compact(int64_t n, const uint64_t* off, const uint32_t* cand,
+                         const uint8_t* flag, const uint64_t* erow,
+                         uint32_t* ecol, uint32_t* eleft, PipeLaunch L);
+/* triangles */
+void launch_tri_count(const Stage& S, int64_t ne, const uint64_t* erow,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      uint32_t* cnt, PipeLaunch L);
+void launch_tri_fill(const Stage& S, int64_t ne, const uint64_t* erow,
+                     const uint32_t* ecol, const uint32_t* eleft,
+                     const uint64_t* off, uint32_t* cand, uint8_t* flag,
+                     uint32_t* keep, QEntry* q, unsigned int* qn,
+                     uint32_t qcap, PipeLaunch L);
+void launch_tri_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                      PipeLaunch L);
+void launch_seg_compact(int64_t nseg, const uint64_t* off,
+                        const uint32_t* cand, const uint8_t* flag,
+                        const uint64_t* orow, uint32_t* ocol, uint32_t* oseg,
+                        PipeLaunch L);
+/* tets */
+void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+                      const uint32_t* eleft, const uint64_t* trow,
+                      const uint32_t* tk, const uint32_t* tedge,
+                      uint32_t* cnt, PipeLaunch L);
+void launch_tet_fill(const Stage& S, int64_t nt, const uint64_t* erow,
+                     const uint32_t* ecol, const uint32_t* eleft,
+                     const uint64_t* trow, const uint32_t* tk,
+                     const uint32_t* tedge, const uint64_t* off,
+                     uint32_t* cand, uint8_t* flag, uint32_t* keep, QEntry* q,
+                     unsigned int* qn, uint32_t qcap, PipeLaunch L);
+void launch_tet_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      const uint32_t* tk, const uint32_t* tedge,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                      PipeLaunch L);
+void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
+                        const uint8_t* flag, const uint64_t* orow,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        uint4* tets, PipeLaunch L);
+/* emptiness (AC2) */
+void launch_tet_ac2(const Stage& S, int64_t nk, const uint4* tets,
+                    const uint32_t* perm, uint8_t* flag, QEntry* q,
+                    unsigned int* qn, uint32_t qcap, PipeLaunch L);
+void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                          const uint4* tets, const uint32_t* perm,
+                          uint8_t* flag, PipeLaunch L);
+void launch_mark_tet_faces(int64_t nk, const uint4* tets, const uint8_t* kflag,
+                           const uint64_t* erow, const uint32_t* ecol,
+                           const uint64_t* trow, const uint32_t* tk,
+                           uint8_t* tri_in, PipeLaunch L);
+void launch_tri_ac2(const Stage& S, int64_t nt, const uint32_t* ecol,
+                    const uint32_t* eleft, const uint32_t* tk,
+                    const uint32_t* tedge, const uint8_t* tri_in,
+                    uint8_t* flag, QEntry* q, unsigned int* qn, uint32_t qcap,
+                    PipeLaunch L);
+void launch_tri_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                          const uint32_t* ecol, const uint32_t* eleft,
+                          const uint32_t* tk, const uint32_t* tedge,
+                          uint8_t* flag, PipeLaunch L);
+void launch_mark_tri_edges(int64_t nt, const uint32_t* ecol,
+                           const uint32_t* eleft, const uint32_t* tk,
+                           const uint32_t* tedge, const uint8_t* tflag,
+                           const uint64_t* erow, uint8_t* edge_in,
+                           PipeLaunch L);
+void launch_edge_ac2(const Stage& S, int64_t ne, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint8_t* edge_in,
+                     uint8_t* flag, QEntry* q, unsigned int* qn,
+                     uint32_t qcap, PipeLaunch L);
+void launch_edge_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                           const uint32_t* ecol, const uint32_t* eleft,
+                           uint8_t* flag, PipeLaunch L);
+/* final outputs in original ids (flag-compacted) */
+void launch_emit_tets(int64_t nk, const uint4* tets, const uint8_t* flag,
+                      const uint64_t* pos, const uint32_t* perm, int32_t* out,
+                      PipeLaunch L);
+void launch_emit_tris(int64_t nt, const uint32_t* ecol, const uint32_t* eleft,
+                      const uint32_t* tk, const uint32_t* tedge,
+                      const uint8_t* flag, const uint64_t* pos,
+                      const uint32_t* perm, int32_t* out, PipeLaunch L);
+void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+                       const uint8_t* flag, const uint64_t* pos,
+                       const uint32_t* perm, int32_t* out, PipeLaunch L);
+
+/* SoS table as compiled into the device code (for tests) */
+int sos_table_host(int32_t* out3, int cap);
+
+}  // namespace pac
+#endif

@@ -1,0 +1,282 @@
+"""Python binding of libpac (the MI355X weighted alpha complex engine).
+
+Thin ctypes layer over the C ABI in include/pac.h.  The reference
+(forest113/ParallelAC-xmp) exposes only the `parallelac` CLI
+(AlphaComplexParallel.cu:1053-1057); `alpha_complex()` is the in-process
+equivalent: balls in (x, y, z, r_raw), solvent and alpha, sorted edge /
+triangle / tetrahedron lists out in the reference's row order.
+
+There is no CPU fallback: if libpac.so is missing or no GPU is visible the
+call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpac.so")
+
+PAC_OK = 0
+PAC_ERR_IO, PAC_ERR_INVALID, PAC_ERR_DEGENERATE = -1, -2, -3
+PAC_ERR_NOMEM, PAC_ERR_DEVICE, PAC_ERR_OVERFLOW, PAC_ERR_RANGE = -4, -5, -6, -7
+CELL_EXACT, CELL_REFERENCE_RAW = 0, 1
+PREC_PLAIN_FP64, PREC_STAGED = 0, 1
+ORDER_REFERENCE, ORDER_CANONICAL = 0, 1
+
+PHASE_LABELS = ("Data transfer", "Grid computation", "Potential edges",
+                "Potential triangles", "Potential tetrahedra",
+                "AC2 test for tetrahedra", "AC2 test for triangles",
+                "AC2 test for edges")
+FAMILIES = ("edge_filter", "tri_filter", "tet_filter", "tet_ac2", "tri_ac2",
+            "edge_ac2")
+STAGES = ("fp32", "fp64", "dd", "wide_int")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("cell_policy", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("skip_tet_ac2", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("device_inputs", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64),
+                ("edge_candidates", ctypes.c_int64), ("edge_alpha", ctypes.c_int64),
+                ("edges_final", ctypes.c_int64),
+                ("tri_candidates", ctypes.c_int64), ("tri_alpha", ctypes.c_int64),
+                ("tris_final", ctypes.c_int64),
+                ("tet_candidates", ctypes.c_int64), ("tet_alpha", ctypes.c_int64),
+                ("tets_final", ctypes.c_int64),
+                ("stage_evals", (ctypes.c_int64 * 4) * 6),
+                ("sos_calls", ctypes.c_int64),
+                ("ac2_pairs", ctypes.c_int64 * 3),
+                ("k_enum", ctypes.c_int32), ("k_ac2", ctypes.c_int32),
+                ("scale_bits", ctypes.c_int32), ("grid_cells", ctypes.c_int32),
+                ("phase_ms", ctypes.c_double * 8), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_
+             if f not in ("stage_evals", "phase_ms", "ac2_pairs")}
+        d["stage_evals"] = {FAMILIES[f]: {STAGES[s]: int(self.stage_evals[f][s])
+                                          for s in range(4)} for f in range(6)}
+        d["phase_ms"] = {PHASE_LABELS[i]: float(self.phase_ms[i]) for i in range(8)}
+        d["ac2_pairs"] = [int(x) for x in self.ac2_pairs]
+        return d
+
+
+# every symbol include/pac.h declares
+EXPORTS = (
+    "pac_options_init", "pac_compute", "pac_compute_f64", "pac_result_counts",
+    "pac_result_copy_edges", "pac_result_copy_triangles", "pac_result_copy_tets",
+    "pac_result_device_list", "pac_result_copy_alpha", "pac_result_stats",
+    "pac_result_free", "pac_compute_batch", "pac_batch_counts", "pac_batch_copy",
+    "pac_batch_stats", "pac_batch_free", "pac_alpha_adjoint", "pac_read_crd",
+    "pac_write_output", "pac_free", "pac_synth_globule", "pac_synth_globule_f64",
+    "pac_synth_batch", "pac_synth_stress", "pac_last_error", "pac_version",
+)
+
+_lib = None
+
+
+def lib():
+    """Load libpac.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libpac.so not built: run __graft_entry__.build() "
+                           "(or make -C parallelac-xmp_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    L.pac_options_init.argtypes = [ctypes.POINTER(Options)]
+    L.pac_compute.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float,
+                              ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.pac_compute_f64.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.pac_result_counts.argtypes = [vp, vp]
+    for f in ("pac_result_copy_edges", "pac_result_copy_triangles",
+              "pac_result_copy_tets"):
+        getattr(L, f).argtypes = [vp, vp]
+    L.pac_result_device_list.argtypes = [vp, ctypes.c_int]
+    L.pac_result_device_list.restype = vp
+    L.pac_result_copy_alpha.argtypes = [vp, ctypes.c_int, vp]
+    L.pac_result_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.pac_result_free.argtypes = [vp]
+    L.pac_compute_batch.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float,
+                                    ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.pac_batch_counts.argtypes = [vp, vp]
+    L.pac_batch_copy.argtypes = [vp, i64, ctypes.c_int, vp]
+    L.pac_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.pac_batch_free.argtypes = [vp]
+    L.pac_alpha_adjoint.argtypes = [vp, vp, vp, vp, vp]
+    L.pac_read_crd.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp),
+                               ctypes.POINTER(i64)]
+    L.pac_write_output.argtypes = [ctypes.c_char_p, vp, i64, ctypes.c_float, vp]
+    L.pac_free.argtypes = [vp]
+    L.pac_synth_globule.argtypes = [i64, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_uint64, ctypes.c_int, vp]
+    L.pac_synth_globule_f64.argtypes = [i64, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, ctypes.c_double, vp]
+    L.pac_synth_batch.argtypes = [i64, i64, i64, ctypes.c_uint64, vp, vp]
+    L.pac_synth_stress.argtypes = [i64, ctypes.c_double, ctypes.c_uint64,
+                                   ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, vp]
+    L.pac_last_error.restype = ctypes.c_char_p
+    L.pac_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+class PacError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pac error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _check(rc):
+    if rc != PAC_OK:
+        raise PacError(rc, lib().pac_last_error().decode(errors="replace"))
+
+
+def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
+            order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None):
+    o = Options()
+    lib().pac_options_init(ctypes.byref(o))
+    o.cell_policy = cell_policy
+    o.precision = precision
+    o.skip_tet_ac2 = int(bool(skip_tet_ac2))
+    o.order = order
+    o.device_inputs = int(bool(device_inputs))
+    o.device = device
+    o.stream = stream
+    return o
+
+
+class Result:
+    """Owned handle to a pac_result; lists are copied out on demand."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.pac_result_free(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def counts(self):
+        c = np.zeros(4, dtype=np.int64)
+        _check(lib().pac_result_counts(self._h, c.ctypes.data))
+        return c
+
+    def lists(self):
+        c = self.counts()
+        out = {}
+        fns = (lib().pac_result_copy_edges, lib().pac_result_copy_triangles,
+               lib().pac_result_copy_tets)
+        for d in (1, 2, 3):
+            a = np.zeros((int(c[d]), d + 1), dtype=np.int32)
+            _check(fns[d - 1](self._h, a.ctypes.data))
+            out[d] = a
+        return out
+
+    def device_list(self, dim):
+        return lib().pac_result_device_list(self._h, dim)
+
+    def stats(self):
+        s = Stats()
+        _check(lib().pac_result_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def alpha_values(self, dim):
+        n = int(self.counts()[dim])
+        rho = np.zeros(n, dtype=np.float64)
+        _check(lib().pac_result_copy_alpha(self._h, dim, rho.ctypes.data))
+        return rho
+
+
+def compute(xyzr, solvent, alpha, **kw):
+    """Run the pipeline on host FP32 balls (n x 4: x, y, z, r_raw)."""
+    a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+    h = ctypes.c_void_p()
+    o = options(**kw)
+    _check(lib().pac_compute(a.ctypes.data, a.shape[0], solvent, alpha,
+                             ctypes.byref(o), ctypes.byref(h)))
+    return Result(h.value)
+
+
+def compute_device(ptr, n, solvent, alpha, **kw):
+    """Run on a device-resident FP32 n x 4 array (e.g. a torch tensor)."""
+    h = ctypes.c_void_p()
+    o = options(device_inputs=True, **kw)
+    _check(lib().pac_compute(ctypes.c_void_p(ptr), n, solvent, alpha,
+                             ctypes.byref(o), ctypes.byref(h)))
+    return Result(h.value)
+
+
+def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):
+    """Convenience: returns (edges, triangles, tets) int32 arrays."""
+    r = compute(xyzr, solvent, alpha, **kw)
+    L = r.lists()
+    return L[1], L[2], L[3]
+
+
+def read_crd(path):
+    p = ctypes.c_void_p()
+    n = ctypes.c_int64()
+    _check(lib().pac_read_crd(path.encode(), ctypes.byref(p), ctypes.byref(n)))
+    try:
+        buf = (ctypes.c_float * (4 * n.value)).from_address(p.value) if n.value else []
+        a = np.array(buf, dtype=np.float32).reshape(-1, 4)
+    finally:
+        lib().pac_free(p)
+    return a
+
+
+def write_output(path, xyzr, solvent, result):
+    a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+    _check(lib().pac_write_output(path.encode(), a.ctypes.data, a.shape[0],
+                                  solvent, result.handle))
+
+
+# ---------------------------------------------------------------- synth
+def synth_globule(n, sphere_radius=None, min_sep=1.3, seed=1, decimals=3):
+    if sphere_radius is None:
+        sphere_radius = (3.0 * n / (4.0 * np.pi * 0.05)) ** (1.0 / 3.0)
+    out = np.zeros((n, 4), dtype=np.float32)
+    _check(lib().pac_synth_globule(n, sphere_radius, min_sep, seed, decimals,
+                                   out.ctypes.data))
+    return out
+
+
+def synth_globule_f64(n, sphere_radius=None, min_sep=1.3, seed=4, sigma=0.01):
+    if sphere_radius is None:
+        sphere_radius = (3.0 * n / (4.0 * np.pi * 0.05)) ** (1.0 / 3.0)
+    out = np.zeros((n, 4), dtype=np.float64)
+    _check(lib().pac_synth_globule_f64(n, sphere_radius, min_sep, seed, sigma,
+                                       out.ctypes.data))
+    return out
+
+
+def synth_batch(n_mol, nmin=200, nmax=20000, seed=3):
+    off = np.zeros(n_mol + 1, dtype=np.int64)
+    _check(lib().pac_synth_batch(n_mol, nmin, nmax, seed, off.ctypes.data, None))
+    out = np.zeros((int(off[-1]), 4), dtype=np.float32)
+    _check(lib().pac_synth_batch(n_mol, nmin, nmax, seed, off.ctypes.data,
+                                 out.ctypes.data))
+    return out, off
+
+
+def synth_stress(n, sphere_radius=None, seed=5, lattice_frac=0.1, spacing=2.0,
+                 lattice_r=1.70):
+    if sphere_radius is None:
+        sphere_radius = (3.0 * n / (4.0 * np.pi * 0.05)) ** (1.0 / 3.0)
+    out = np.zeros((n, 4), dtype=np.float32)
+    _check(lib().pac_synth_stress(n, sphere_radius, seed, lattice_frac, spacing,
+                                  lattice_r, out.ctypes.data))
+    return out
