@@ -47,7 +47,8 @@ typedef struct pac_options {
   int32_t device;        /* HIP device ordinal                            */
   void* stream;          /* hipStream_t to run on, NULL = library stream  */
   int32_t block;         /* threads per block override (0 = tuned)        */
-  int32_t reserved[7];
+  int32_t profile;       /* 1: time the main kernels with hipEvents       */
+  int32_t reserved[6];
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -71,6 +72,10 @@ typedef struct pac_stats {
    * AC2 test for tetrahedra, AC2 test for triangles, AC2 test for edges */
   double phase_ms[8];
   double total_ms;
+  /* with pac_options.profile: device time (ms) of the main kernels, in
+   * order edge_fill, tri_fill, tet_fill, tet_ac2, tri_ac2, edge_ac2,
+   * radix_sort, tet_ac2_exact (hipEvents on the run's stream) */
+  double kernel_ms[8];
 } pac_stats;
 
 void pac_options_init(pac_options* opt);

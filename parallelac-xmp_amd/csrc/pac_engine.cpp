@@ -251,6 +251,43 @@ struct Timer {
   }
 };
 
+/* optional per-kernel hipEvent timing (pac_options.profile) */
+struct KProf {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  void begin(int k) {
+    if (!on) return;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, s));
+    ev.push_back({k, {a, b}});
+  }
+  void end() {
+    if (!on) return;
+    HIPCHK(hipEventRecord(ev.back().second.second, s));
+  }
+  void collect(double* out) {
+    for (auto& e : ev) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, e.second.first, e.second.second) == hipSuccess)
+        out[e.first] += t;
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+    ev.clear();
+  }
+  ~KProf() {
+    for (auto& e : ev) {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  }
+};
+enum { KP_EDGE_FILL, KP_TRI_FILL, KP_TET_FILL, KP_TET_AC2, KP_TRI_AC2,
+       KP_EDGE_AC2, KP_SORT, KP_TET_AC2_EXACT };
+
 template <class T>
 static T readback(Engine* E, hipStream_t s, const T* dptr) {
   HIPCHK(hipMemcpyAsync(E->pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, s));
@@ -286,7 +323,7 @@ static void zero_u32(unsigned* p, int n, hipStream_t s) {
 /* ------------------------------------------------------------------ */
 static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                          float alpha, const pac_options& opt, pac_result* R,
-                         hipStream_t s, Timer& T) {
+                         hipStream_t s, Timer& T, KProf& KP) {
   Pool* P = &E->pool;
   PipeLaunch L{s, kBlock};
   pac_stats& st = R->st;
@@ -371,8 +408,10 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   size_t rtb = radix_tmp_bytes(n);
   DBuf<uint8_t> rtmp(P, rtb);
   bool alt = false;
+  KP.begin(KP_SORT);
   radix_sort_pairs(key.get(), idx.get(), key2.get(), idx2.get(), n, kbits,
                    rtmp.get(), rtb, s, &alt);
+  KP.end();
   KCHK();
   const uint32_t* skey = alt ? key2.get() : key.get();
   const uint32_t* sidx = alt ? idx2.get() : idx.get();
@@ -413,8 +452,10 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     DBuf<uint8_t> flag(P, Ecand);
     DBuf<QEntry> q(P, std::max<int64_t>(Ecand, 1));
     zero_u32(qn.get(), 1, s);
+    KP.begin(KP_EDGE_FILL);
     launch_edge_fill(SG, off.get(), cand.get(), flag.get(), keep.get(), q.get(),
                      qn.get(), (uint32_t)std::max<int64_t>(Ecand, 1), L);
+    KP.end();
     KCHK();
     uint32_t nq = readback(E, s, qn.get());
     launch_edge_exact(SG, q.get(), nq, cand.get(), flag.get(), keep.get(), L);
@@ -460,10 +501,12 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     DBuf<uint8_t> flag(P, Tcand);
     DBuf<QEntry> q(P, std::max<int64_t>(Tcand, 1));
     zero_u32(qn.get(), 1, s);
+    KP.begin(KP_TRI_FILL);
     if (Ealpha)
       launch_tri_fill(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
                       off.get(), cand.get(), flag.get(), keep.get(), q.get(),
                       qn.get(), (uint32_t)std::max<int64_t>(Tcand, 1), L);
+    KP.end();
     KCHK();
     uint32_t nq = readback(E, s, qn.get());
     launch_tri_exact(SG, q.get(), nq, ecol.get(), eleft.get(), cand.get(),
@@ -505,11 +548,13 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     DBuf<uint8_t> flag(P, Kcand);
     DBuf<QEntry> q(P, std::max<int64_t>(Kcand, 1));
     zero_u32(qn.get(), 1, s);
+    KP.begin(KP_TET_FILL);
     if (Talpha)
       launch_tet_fill(SG, Talpha, erow.get(), ecol.get(), eleft.get(),
                       trow.get(), tk.get(), tedge.get(), off.get(), cand.get(),
                       flag.get(), keep.get(), q.get(), qn.get(),
                       (uint32_t)std::max<int64_t>(Kcand, 1), L);
+    KP.end();
     KCHK();
     uint32_t nq = readback(E, s, qn.get());
     launch_tet_exact(SG, q.get(), nq, ecol.get(), eleft.get(), tk.get(),
@@ -533,17 +578,22 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   T.mark(5);
 
   /* ---- emptiness with retry on queue overflow ---- */
-  auto run_queued = [&](int64_t items, auto&& scan_kernel, auto&& exact_kernel) {
+  auto run_queued = [&](int64_t items, int kp_scan, int kp_exact,
+                        auto&& scan_kernel, auto&& exact_kernel) {
     uint32_t cap = (uint32_t)std::min<int64_t>(
         std::max<int64_t>(1 << 20, 2 * items), (int64_t)UINT32_MAX - 1);
     for (;;) {
       DBuf<QEntry> q(P, cap);
       zero_u32(qn.get(), 1, s);
+      KP.begin(kp_scan);
       scan_kernel(q.get(), cap);
+      KP.end();
       KCHK();
       uint32_t nq = readback(E, s, qn.get());
       if (nq <= cap) {
+        if (kp_exact >= 0) KP.begin(kp_exact);
         exact_kernel(q.get(), nq);
+        if (kp_exact >= 0) KP.end();
         KCHK();
         return;
       }
@@ -557,7 +607,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     fill_u8(kflag.get(), Kalpha, 1, s);
   } else if (Kalpha) {
     run_queued(
-        Kalpha,
+        Kalpha, KP_TET_AC2, KP_TET_AC2_EXACT,
         [&](QEntry* q, uint32_t cap) {
           launch_tet_ac2(SG, Kalpha, tets.get(), perm.get(), kflag.get(), q,
                          qn.get(), cap, L);
@@ -575,7 +625,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                         trow.get(), tk.get(), tri_in.get(), L);
   if (Talpha)
     run_queued(
-        Talpha,
+        Talpha, KP_TRI_AC2, -1,
         [&](QEntry* q, uint32_t cap) {
           launch_tri_ac2(SG, Talpha, ecol.get(), eleft.get(), tk.get(),
                          tedge.get(), tri_in.get(), tflag.get(), q, qn.get(),
@@ -594,7 +644,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                         tflag.get(), erow.get(), edge_in.get(), L);
   if (Ealpha)
     run_queued(
-        Ealpha,
+        Ealpha, KP_EDGE_AC2, -1,
         [&](QEntry* q, uint32_t cap) {
           launch_edge_ac2(SG, Ealpha, ecol.get(), eleft.get(), edge_in.get(),
                           eflag.get(), q, qn.get(), cap, L);
@@ -674,6 +724,9 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
   std::memset(&R->st, 0, sizeof(R->st));
   R->counts[0] = n;
   Timer T(s);
+  KProf KP;
+  KP.on = opt.profile != 0;
+  KP.s = s;
   T.mark(0);
   auto t_host0 = std::chrono::steady_clock::now();
   DBuf<float4> raw_own;
@@ -690,7 +743,7 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
     hipLaunchKernelGGL(k_balls_orig, dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, s, raw, n, solvent, R->balls.get());
     KCHK();
-    run_pipeline(E, raw, n, solvent, alpha, opt, R.get(), s, T);
+    run_pipeline(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
   } else {
     for (int i = 2; i <= 8; ++i) T.mark(i);
   }
@@ -707,6 +760,7 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
   st.phase_ms[7] = T.ms(7, 8);
   st.total_ms =
       std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
+  KP.collect(st.kernel_ms);
   *out = R.release();
 }
 

@@ -31,6 +31,8 @@ PHASE_LABELS = ("Data transfer", "Grid computation", "Potential edges",
 FAMILIES = ("edge_filter", "tri_filter", "tet_filter", "tet_ac2", "tri_ac2",
             "edge_ac2")
 STAGES = ("fp32", "fp64", "dd", "wide_int")
+KERNELS = ("edge_fill", "tri_fill", "tet_fill", "tet_ac2", "tri_ac2",
+           "edge_ac2", "radix_sort", "tet_ac2_exact")
 
 
 class Options(ctypes.Structure):
@@ -38,7 +40,7 @@ class Options(ctypes.Structure):
                 ("skip_tet_ac2", ctypes.c_int32), ("order", ctypes.c_int32),
                 ("device_inputs", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 7)]
+                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
 class Stats(ctypes.Structure):
@@ -54,15 +56,17 @@ class Stats(ctypes.Structure):
                 ("ac2_pairs", ctypes.c_int64 * 3),
                 ("k_enum", ctypes.c_int32), ("k_ac2", ctypes.c_int32),
                 ("scale_bits", ctypes.c_int32), ("grid_cells", ctypes.c_int32),
-                ("phase_ms", ctypes.c_double * 8), ("total_ms", ctypes.c_double)]
+                ("phase_ms", ctypes.c_double * 8), ("total_ms", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double * 8)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_
-             if f not in ("stage_evals", "phase_ms", "ac2_pairs")}
+             if f not in ("stage_evals", "phase_ms", "ac2_pairs", "kernel_ms")}
         d["stage_evals"] = {FAMILIES[f]: {STAGES[s]: int(self.stage_evals[f][s])
                                           for s in range(4)} for f in range(6)}
         d["phase_ms"] = {PHASE_LABELS[i]: float(self.phase_ms[i]) for i in range(8)}
         d["ac2_pairs"] = [int(x) for x in self.ac2_pairs]
+        d["kernel_ms"] = {KERNELS[i]: float(self.kernel_ms[i]) for i in range(8)}
         return d
 
 
@@ -141,7 +145,8 @@ def _check(rc):
 
 
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
-            order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None):
+            order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
+            profile=False):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -151,6 +156,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.device_inputs = int(bool(device_inputs))
     o.device = device
     o.stream = stream
+    o.profile = int(bool(profile))
     return o
 
 
