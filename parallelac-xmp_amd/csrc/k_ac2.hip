@@ -1,0 +1,726 @@
+/*
+ * k_ac2.hip -- emptiness ("AC2") tests of alpha tetrahedra, dangling
+ * triangles and dangling edges.
+ *
+ * Hot kernels (one thread per simplex) compute the orthocentre once with a
+ * certified error, build the cell box that must contain every possible
+ * violator, and scan it centre cell first with the semi-static power-gap
+ * test in one precision (FP32 staged / FP64 plain).  The first certain
+ * violator ends the scan; undecided (simplex, ball) pairs, or simplices
+ * whose centre could not be certified, are queued.  The escalation kernel
+ * retries them in FP64 then double-double, and the exact kernel settles the
+ * rest in 512-bit integers, breaking exact tet ties by Edelsbrunner-Muecke
+ * simulation of simplicity over the original input order.
+ *
+ * Reference: checkTetsOrthosphere AlphaComplexLargeData.cu:597-735 (with
+ * orient3d / insphere_w / orientation4SoS_w, predicates.h:2015, :4250,
+ * :4669), checkDanglingTris :807-881, checkDanglingEdges :928-1015.
+ */
+#include "pac_dev.h"
+#include "pac_sos_table.h"
+
+namespace pac {
+
+/* exact lifted row for SoS */
+__device__ void lifted_row(const float4& b, const LdWI& L, WI out[4]) {
+  V3<WI> p = L.p(b);
+  out[0] = p.x;
+  out[1] = p.y;
+  out[2] = p.z;
+  out[3] = p.x * p.x + p.y * p.y + p.z * p.z - L.w(b);
+}
+__device__ __forceinline__ WI ent(const WI M[5][4], int r, int c) {
+  return c == 4 ? wi_from_i64(1) : M[r][c];
+}
+__device__ WI det2(const WI M[5][4], const int* r, const int* c) {
+  return ent(M, r[0], c[0]) * ent(M, r[1], c[1]) -
+         ent(M, r[0], c[1]) * ent(M, r[1], c[0]);
+}
+__device__ WI det3(const WI M[5][4], const int* r, const int* c) {
+  WI acc = wi_from_i64(0);
+  for (int q = 0; q < 3; ++q) {
+    int cc[2], m = 0;
+    for (int t = 0; t < 3; ++t)
+      if (t != q) cc[m++] = c[t];
+    WI term = ent(M, r[0], c[q]) * det2(M, r + 1, cc);
+    acc = (q & 1) ? acc - term : acc + term;
+  }
+  return acc;
+}
+__device__ WI det4(const WI M[5][4], const int* r, const int* c) {
+  WI acc = wi_from_i64(0);
+  for (int q = 0; q < 4; ++q) {
+    int cc[3], m = 0;
+    for (int t = 0; t < 4; ++t)
+      if (t != q) cc[m++] = c[t];
+    WI term = ent(M, r[0], c[q]) * det3(M, r + 1, cc);
+    acc = (q & 1) ? acc - term : acc + term;
+  }
+  return acc;
+}
+
+/* Sign of the SoS-perturbed det[x y z l 1] of the five balls, rows in
+ * argument order, perturbation order given by ids (original input ids,
+ * as the reference's Atom.index, AlphaComplexParallel.cu:1087). */
+__device__ int sos_det5(const float4 b[5], const uint32_t ids[5],
+                        const LdWI& L) {
+  int ord[5] = {0, 1, 2, 3, 4};
+  int swaps = 0;
+  for (int a = 0; a < 5; ++a)
+    for (int q = 0; q + 1 < 5 - a; ++q)
+      if (ids[ord[q + 1]] < ids[ord[q]]) {
+        int t = ord[q];
+        ord[q] = ord[q + 1];
+        ord[q + 1] = t;
+        ++swaps;
+      }
+  WI M[5][4];
+  for (int r = 0; r < 5; ++r) lifted_row(b[ord[r]], L, M[r]);
+  int parity = (swaps & 1) ? -1 : 1;
+  for (int t = 0; t < PAC_SOS_TERMS; ++t) {
+    int rows[5], cols[5], m = 0, mc = 0;
+    int rm = pac_sos_table[t][0], cm = pac_sos_table[t][1];
+    for (int r = 0; r < 5; ++r)
+      if (rm & (1 << r)) rows[m++] = r;
+    for (int c = 0; c < 5; ++c)
+      if (cm & (1 << c)) cols[mc++] = c;
+    int s;
+    if (m == 1)
+      s = wi_sign(ent(M, rows[0], cols[0]));
+    else if (m == 2)
+      s = wi_sign(det2(M, rows, cols));
+    else if (m == 3)
+      s = wi_sign(det3(M, rows, cols));
+    else
+      s = wi_sign(det4(M, rows, cols));
+    if (s != 0) return parity * pac_sos_table[t][2] * s;
+  }
+  return parity;
+}
+
+/* WI -> double (relative error < 2^-50), scaled back by 2^-S */
+__device__ double wi_to_double(const WI& a, int S) {
+  bool neg = (a.l[WL - 1] >> 31) != 0;
+  WI m = neg ? wi_neg(a) : a;
+  int top = WL - 1;
+  while (top > 0 && m.l[top] == 0) --top;
+  double v = 0;
+  for (int q = top; q >= 0 && q >= top - 2; --q)
+    v += __builtin_ldexp((double)m.l[q], 32 * q);
+  v = __builtin_ldexp(v, -S);
+  return neg ? -v : v;
+}
+
+/* ---------------- tet emptiness ---------------- */
+template <class T>
+struct TetQ {
+  T D, wa;
+  V3<T> pa, N;
+};
+template <class T>
+__device__ __forceinline__ void tetq_init(TetQ<T>& t, const float4 v[4]) {
+  t.pa = Ld<T>::p(v[0]);
+  t.wa = Ld<T>::w(v[0]);
+  tet_DN(t.pa, t.wa, Ld<T>::p(v[1]), Ld<T>::w(v[1]), Ld<T>::p(v[2]),
+         Ld<T>::w(v[2]), Ld<T>::p(v[3]), Ld<T>::w(v[3]), &t.D, &t.N);
+}
+/* 1 violated, 0 not violated, 2 undecided (needs exact) */
+template <class T>
+__device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
+                                             const float4& e) {
+  T g = power_gap(t.pa, t.wa, Ld<T>::p(e), Ld<T>::w(e), t.D, t.N);
+  int s = fe_sign(g);
+  if (s == 0) return 2;
+  return (s * sD) < 0 ? 1 : 0;
+}
+
+
+/* per-simplex setup at precision R: certified centre box and the
+ * semi-static pair constants. */
+template <class R>
+__device__ __forceinline__ bool tet_setup(const Stage& S, const float4 v[4],
+                                          PairSS<R>& P, CellBox& box) {
+  TetQ<FE<R>> t;
+  tetq_init(t, v);
+  if (!pairss_init(P, t.pa, t.wa, t.D, t.N)) return false;
+  double D2 = 2.0 * (double)t.D.v;
+  double cx = (double)v[0].x + (double)t.N.x.v / D2;
+  double cy = (double)v[0].y + (double)t.N.y.v / D2;
+  double cz = (double)v[0].z + (double)t.N.z.v / D2;
+  return make_box(S.g, S.reach, S.margin, cx, cy, cz,
+                  quot_err(t.N.x, t.D, 2.0), quot_err(t.N.y, t.D, 2.0),
+                  quot_err(t.N.z, t.D, 2.0), box);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
+    Stage S, int64_t nk, const uint4* tets, uint8_t* flag, Queue q1) {
+  int64_t id = gtid();
+  unsigned evals = 0, pairs = 0;
+  if (id < nk) {
+    uint4 t = tets[id];
+    float4 v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
+    PairSS<R> P;
+    CellBox box;
+    uint8_t res;
+    if (!tet_setup(S, v, P, box)) {
+      push_q(q1, (uint32_t)id, 0xffffffffu);
+      res = 2;
+    } else {
+      bool pending = false, dead = false;
+      scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+        if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
+        ++pairs;
+        ++evals;
+        int s = pairss_sign(P, S.ball[p]);
+        if (s == 0) {
+          pending = true;
+          push_q(q1, (uint32_t)id, p);
+          return false;
+        }
+        if (s * P.sden < 0) { /* pi_e < rho: violator */
+          dead = true;
+          return true;
+        }
+        return false;
+      });
+      res = dead ? 0 : (pending ? 2 : 1);
+    }
+    flag[id] = res;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
+  flush_ctr(S.ctr, CTR_PAIRS + 0, pairs);
+}
+
+/* FP64 then double-double for queued pairs / uncertified tets */
+template <class T>
+__device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
+                                               const float4& e) {
+  return tet_pair_code(t, sD, e);
+}
+__global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
+    Stage S, const QEntry* q, uint32_t nq, const uint4* tets, uint8_t* flag,
+    Queue q2) {
+  int64_t t = gtid();
+  StageCnt sc;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint4 tt = tets[e.item];
+    float4 v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
+    TetQ<FE<double>> t64;
+    tetq_init(t64, v);
+    int sD = fe_sign(t64.D);
+    TetQ<FEDD> tdd;
+    bool have_dd = false;
+    if (!sD) {
+      tetq_init(tdd, v);
+      have_dd = true;
+      sD = fe_sign(tdd.D);
+    }
+    auto pair = [&](uint32_t p) -> int { /* 1 violated, 0 clear, 2 undecided */
+      float4 E = S.ball[p];
+      int c = 2;
+      if (fe_sign(t64.D) != 0) {
+        sc.c[1]++;
+        c = tet_pair_code(t64, sD, E);
+      }
+      if (c == 2) {
+        if (!have_dd) {
+          tetq_init(tdd, v);
+          have_dd = true;
+        }
+        if (fe_sign(tdd.D) != 0) {
+          sc.c[2]++;
+          c = tet_pair_code(tdd, sD, E);
+        }
+      }
+      return c;
+    };
+    if (e.aux != 0xffffffffu) {
+      int c = sD ? pair(e.aux) : 2;
+      if (c == 1) flag[e.item] = 0;
+      else if (c == 2) push_q(q2, e.item, e.aux);
+    } else if (!sD) {
+      push_q(q2, e.item, 0xffffffffu);
+    } else {
+      /* full scan with the best certified centre */
+      double cx, cy, cz, ex, ey, ez;
+      if (fe_sign(t64.D) != 0) {
+        double D2 = 2.0 * t64.D.v;
+        cx = (double)v[0].x + t64.N.x.v / D2;
+        cy = (double)v[0].y + t64.N.y.v / D2;
+        cz = (double)v[0].z + t64.N.z.v / D2;
+        ex = quot_err(t64.N.x, t64.D, 2.0);
+        ey = quot_err(t64.N.y, t64.D, 2.0);
+        ez = quot_err(t64.N.z, t64.D, 2.0);
+      } else {
+        double D2 = 2.0 * tdd.D.v.hi;
+        cx = (double)v[0].x + tdd.N.x.v.hi / D2;
+        cy = (double)v[0].y + tdd.N.y.v.hi / D2;
+        cz = (double)v[0].z + tdd.N.z.v.hi / D2;
+        ex = quot_err(tdd.N.x, tdd.D, 2.0);
+        ey = quot_err(tdd.N.y, tdd.D, 2.0);
+        ez = quot_err(tdd.N.z, tdd.D, 2.0);
+      }
+      CellBox box;
+      if (!make_box(S.g, S.reach, S.margin, cx, cy, cz, ex, ey, ez, box)) {
+        push_q(q2, e.item, 0xffffffffu);
+      } else {
+        bool dead = false;
+        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+          if (p == tt.x || p == tt.y || p == tt.z || p == tt.w) return false;
+          int c = pair(p);
+          if (c == 1) {
+            dead = true;
+            return true;
+          }
+          if (c == 2) push_q(q2, e.item, p);
+          return false;
+        });
+        if (dead) flag[e.item] = 0;
+      }
+    }
+  }
+  sc.flush(S.ctr, FAM_TET_AC2);
+}
+
+/* exact resolution of one (tet, ball) pair, or a full exact scan when
+ * aux == ~0 (tet whose orientation no float stage could certify) */
+__device__ int tet_pair_exact(const Stage& S, const float4 v[4],
+                              const uint32_t vid[4], const uint32_t* perm,
+                              const WI& D, const V3<WI>& N, const V3<WI>& pa,
+                              const WI& wa, uint32_t p, unsigned* sos) {
+  LdWI L{S.ex.S};
+  float4 E = S.ball[p];
+  WI g = power_gap(pa, wa, L.p(E), L.w(E), D, N);
+  int s = wi_sign(g), sD = wi_sign(D);
+  if (s != 0) return (s * sD) < 0;
+  /* exact tie: insphere_w = det[x y z l 1](a,b,c,d,e) with SoS; the tet
+   * is violated iff SoS * orient3d(a,b,c,d) > 0, orient3d = -sign(D)
+   * (AlphaComplexParallel.cu:653-683). */
+  ++*sos;
+  float4 bb[5] = {v[0], v[1], v[2], v[3], E};
+  uint32_t ids[5] = {perm[vid[0]], perm[vid[1]], perm[vid[2]], perm[vid[3]],
+                     perm[p]};
+  int ss = sos_det5(bb, ids, L);
+  return (ss * -sD) > 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
+    Stage S, const QEntry* q, uint32_t nq, const uint4* tets,
+    const uint32_t* perm, uint8_t* flag, double reach, double margin) {
+  int64_t t = gtid();
+  unsigned cnt = 0, sos = 0;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint4 tt = tets[e.item];
+    uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
+    float4 v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
+    LdWI L{S.ex.S};
+    V3<WI> pa = L.p(v[0]);
+    WI wa = L.w(v[0]), D;
+    V3<WI> N;
+    tet_DN(pa, wa, L.p(v[1]), L.w(v[1]), L.p(v[2]), L.w(v[2]), L.p(v[3]),
+           L.w(v[3]), &D, &N);
+    if (e.aux != 0xffffffffu) {
+      cnt = 1;
+      if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, e.aux, &sos))
+        flag[e.item] = 0;
+    } else if (wi_sign(D) != 0) {
+      double D2 = 2.0 * wi_to_double(D, 3 * S.ex.S);
+      double cx = (double)v[0].x + wi_to_double(N.x, 4 * S.ex.S) / D2;
+      double cy = (double)v[0].y + wi_to_double(N.y, 4 * S.ex.S) / D2;
+      double cz = (double)v[0].z + wi_to_double(N.z, 4 * S.ex.S) / D2;
+      double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
+      CellBox box;
+      bool viol = false;
+      if (make_box(S.g, reach, margin, cx, cy, cz, err, err, err, box)) {
+        for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
+          for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
+            int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+            uint32_t b = S.cstart[row + box.lo[0]],
+                     en = S.cstart[row + box.hi[0] + 1];
+            for (uint32_t p = b; p < en; ++p) {
+              if (p == tt.x || p == tt.y || p == tt.z || p == tt.w) continue;
+              ++cnt;
+              if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, p, &sos)) {
+                viol = true;
+                break;
+              }
+            }
+          }
+      }
+      flag[e.item] = viol ? 0 : 2;
+    }
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, cnt);
+  flush_ctr(S.ctr, CTR_SOS, sos);
+}
+
+
+/* ---------------- triangle / edge emptiness ---------------- */
+template <class T>
+struct LowQ {   /* triangle or edge: power gap with den, num */
+  T den, wa;
+  V3<T> pa, num;
+};
+template <class T>
+__device__ __forceinline__ void triq_init(LowQ<T>& t, const float4& a,
+                                          const float4& b, const float4& c) {
+  t.pa = Ld<T>::p(a);
+  t.wa = Ld<T>::w(a);
+  tri_X(t.pa, t.wa, Ld<T>::p(b), Ld<T>::w(b), Ld<T>::p(c), Ld<T>::w(c),
+        &t.den, &t.num);
+}
+template <class T>
+__device__ __forceinline__ void edgeq_init(LowQ<T>& t, const float4& a,
+                                           const float4& b) {
+  t.pa = Ld<T>::p(a);
+  t.wa = Ld<T>::w(a);
+  V3<T> u = vsub(Ld<T>::p(b), t.pa);
+  t.den = vdot(u, u);
+  T A = t.den + t.wa - Ld<T>::w(b);
+  t.num = vscale(A, u);
+}
+/* 1 violated (gap <= 0), 0 clear, 2 undecided */
+template <class T>
+__device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
+                                             const float4& e) {
+  T g = power_gap(t.pa, t.wa, Ld<T>::p(e), Ld<T>::w(e), t.den, t.num);
+  int s = fe_sign(g);
+  return s == 0 ? 2 : (s < 0 ? 1 : 0);
+}
+
+
+template <class R, int KIND>
+__device__ __forceinline__ bool low_setup(const Stage& S, const float4* v,
+                                          PairSS<R>& P, CellBox& box) {
+  LowQ<FE<R>> t;
+  if (KIND == 0)
+    triq_init(t, v[0], v[1], v[2]);
+  else
+    edgeq_init(t, v[0], v[1]);
+  if (!pairss_init(P, t.pa, t.wa, t.den, t.num) || P.sden < 0) return false;
+  double d2 = 2.0 * (double)t.den.v;
+  double cx = (double)v[0].x + (double)t.num.x.v / d2;
+  double cy = (double)v[0].y + (double)t.num.y.v / d2;
+  double cz = (double)v[0].z + (double)t.num.z.v / d2;
+  return make_box(S.g, S.reach, S.margin, cx, cy, cz,
+                  quot_err(t.num.x, t.den, 2.0), quot_err(t.num.y, t.den, 2.0),
+                  quot_err(t.num.z, t.den, 2.0), box);
+}
+
+/* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
+ * Simplices already in the closure (in_flag) pass without a test. */
+template <class R, int KIND>
+__global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
+    Stage S, int64_t nitems, const uint32_t* ecol, const uint32_t* eleft,
+    const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
+    uint8_t* flag, Queue q1) {
+  int64_t id = gtid();
+  unsigned evals = 0, pairs = 0;
+  if (id < nitems) {
+    uint8_t res = 1;
+    if (!in_flag[id]) {
+      uint32_t vid[3];
+      if (KIND == 0) {
+        uint32_t e = tedge[id];
+        vid[0] = eleft[e];
+        vid[1] = ecol[e];
+        vid[2] = tk[id];
+      } else {
+        vid[0] = eleft[id];
+        vid[1] = ecol[id];
+        vid[2] = vid[1];
+      }
+      float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+      PairSS<R> P;
+      CellBox box;
+      if (!low_setup<R, KIND>(S, v, P, box)) {
+        push_q(q1, (uint32_t)id, 0xffffffffu);
+        res = 2;
+      } else {
+        bool pending = false, dead = false;
+        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+          if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
+          ++pairs;
+          ++evals;
+          int s = pairss_sign(P, S.ball[p]);
+          if (s < 0) { /* power <= rho strictly: violator */
+            dead = true;
+            return true;
+          }
+          if (s == 0) {
+            pending = true;
+            push_q(q1, (uint32_t)id, p);
+          }
+          return false;
+        });
+        res = dead ? 0 : (pending ? 2 : 1);
+      }
+    }
+    flag[id] = res;
+  }
+  const int fam = KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2;
+  flush_ctr(S.ctr, CTR_STAGE + fam * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
+  flush_ctr(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    uint8_t* flag, Queue q2) {
+  int64_t t = gtid();
+  StageCnt sc;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint32_t vid[3];
+    if (KIND == 0) {
+      uint32_t ed = tedge[e.item];
+      vid[0] = eleft[ed];
+      vid[1] = ecol[ed];
+      vid[2] = tk[e.item];
+    } else {
+      vid[0] = eleft[e.item];
+      vid[1] = ecol[e.item];
+      vid[2] = vid[1];
+    }
+    float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+    LowQ<FE<double>> q64;
+    LowQ<FEDD> qdd;
+    bool have_dd = false;
+    if (KIND == 0) triq_init(q64, v[0], v[1], v[2]); else edgeq_init(q64, v[0], v[1]);
+    auto init_dd = [&]() {
+      if (!have_dd) {
+        if (KIND == 0) triq_init(qdd, v[0], v[1], v[2]); else edgeq_init(qdd, v[0], v[1]);
+        have_dd = true;
+      }
+    };
+    auto pair = [&](uint32_t p) -> int { /* 1 violated, 0 clear, 2 undecided */
+      float4 E = S.ball[p];
+      int c = 2;
+      if (fe_sign(q64.den) > 0) {
+        sc.c[1]++;
+        c = low_pair_code(q64, E);
+      }
+      if (c == 2) {
+        init_dd();
+        if (fe_sign(qdd.den) > 0) {
+          sc.c[2]++;
+          c = low_pair_code(qdd, E);
+        }
+      }
+      return c;
+    };
+    if (e.aux != 0xffffffffu) {
+      int c = pair(e.aux);
+      if (c == 1) flag[e.item] = 0;
+      else if (c == 2) push_q(q2, e.item, e.aux);
+    } else {
+      double cx = 0, cy = 0, cz = 0, ex = INFINITY, ey = INFINITY, ez = INFINITY;
+      bool ok = false;
+      if (fe_sign(q64.den) > 0) {
+        double d2 = 2.0 * q64.den.v;
+        cx = (double)v[0].x + q64.num.x.v / d2;
+        cy = (double)v[0].y + q64.num.y.v / d2;
+        cz = (double)v[0].z + q64.num.z.v / d2;
+        ex = quot_err(q64.num.x, q64.den, 2.0);
+        ey = quot_err(q64.num.y, q64.den, 2.0);
+        ez = quot_err(q64.num.z, q64.den, 2.0);
+        ok = true;
+      } else {
+        init_dd();
+        if (fe_sign(qdd.den) > 0) {
+          double d2 = 2.0 * qdd.den.v.hi;
+          cx = (double)v[0].x + qdd.num.x.v.hi / d2;
+          cy = (double)v[0].y + qdd.num.y.v.hi / d2;
+          cz = (double)v[0].z + qdd.num.z.v.hi / d2;
+          ex = quot_err(qdd.num.x, qdd.den, 2.0);
+          ey = quot_err(qdd.num.y, qdd.den, 2.0);
+          ez = quot_err(qdd.num.z, qdd.den, 2.0);
+          ok = true;
+        }
+      }
+      CellBox box;
+      if (!ok || !make_box(S.g, S.reach, S.margin, cx, cy, cz, ex, ey, ez, box)) {
+        push_q(q2, e.item, 0xffffffffu);
+      } else {
+        bool dead = false;
+        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+          if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
+          int c = pair(p);
+          if (c == 1) {
+            dead = true;
+            return true;
+          }
+          if (c == 2) push_q(q2, e.item, p);
+          return false;
+        });
+        if (dead) flag[e.item] = 0;
+      }
+    }
+  }
+  sc.flush(S.ctr, KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2);
+}
+
+/* exact triangle / edge pair (or full scan when aux == ~0) */
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    uint8_t* flag, double reach, double margin) {
+  int64_t t = gtid();
+  unsigned cnt = 0;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint32_t vid[3];
+    if (KIND == 0) {
+      uint32_t ed = tedge[e.item];
+      vid[0] = eleft[ed];
+      vid[1] = ecol[ed];
+      vid[2] = tk[e.item];
+    } else {
+      vid[0] = eleft[e.item];
+      vid[1] = ecol[e.item];
+      vid[2] = 0xffffffffu;
+    }
+    LdWI L{S.ex.S};
+    float4 a = S.ball[vid[0]], b = S.ball[vid[1]];
+    V3<WI> pa = L.p(a), num;
+    WI wa = L.w(a), den;
+    if (KIND == 0) {
+      tri_X(pa, wa, L.p(b), L.w(b), L.p(S.ball[vid[2]]), L.w(S.ball[vid[2]]),
+            &den, &num);
+    } else {
+      V3<WI> u = vsub(L.p(b), pa);
+      den = vdot(u, u);
+      WI A = den + wa - L.w(b);
+      num = vscale(A, u);
+    }
+    if (e.aux != 0xffffffffu) {
+      cnt = 1;
+      float4 E = S.ball[e.aux];
+      if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0)
+        flag[e.item] = 0;
+    } else if (wi_sign(den) > 0) {
+      double d2 = 2.0 * wi_to_double(den, (KIND == 0 ? 4 : 2) * S.ex.S);
+      int sh = (KIND == 0 ? 5 : 3) * S.ex.S;
+      double cx = (double)a.x + wi_to_double(num.x, sh) / d2;
+      double cy = (double)a.y + wi_to_double(num.y, sh) / d2;
+      double cz = (double)a.z + wi_to_double(num.z, sh) / d2;
+      double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
+      CellBox box;
+      bool viol = false;
+      if (make_box(S.g, reach, margin, cx, cy, cz, err, err, err, box)) {
+        for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
+          for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
+            int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+            uint32_t bb = S.cstart[row + box.lo[0]],
+                     en = S.cstart[row + box.hi[0] + 1];
+            for (uint32_t p = bb; p < en; ++p) {
+              if (p == vid[0] || p == vid[1] || (KIND == 0 && p == vid[2]))
+                continue;
+              ++cnt;
+              float4 E = S.ball[p];
+              if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0) {
+                viol = true;
+                break;
+              }
+            }
+          }
+      }
+      flag[e.item] = viol ? 0 : 2;
+    } else {
+      flag[e.item] = 0; /* degenerate simplex: cannot be an alpha simplex */
+    }
+  }
+  flush_ctr(S.ctr, CTR_STAGE + (KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2) * 4 + 3,
+            cnt);
+}
+
+
+/* ---------------- launchers ---------------- */
+void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
+                        uint8_t* flag, Queue q1, PipeLaunch L) {
+  if (nk <= 0) return;
+  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_tet_ac2_hot<float>, gr, dim3(kBlock), 0, L.s, S, nk,
+                       tets, flag, q1);
+  else
+    hipLaunchKernelGGL(k_tet_ac2_hot<double>, gr, dim3(kBlock), 0, L.s, S, nk,
+                       tets, flag, q1);
+}
+void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                        const uint4* tets, uint8_t* flag, Queue q2,
+                        PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((nq + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, nq, tets, flag, q2);
+}
+void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                          const uint4* tets, const uint32_t* perm,
+                          uint8_t* flag, PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((nq + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, nq, tets, perm, flag, S.reach,
+                     S.margin);
+}
+void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        const uint8_t* in_flag, uint8_t* flag, Queue q1,
+                        PipeLaunch L) {
+  if (nitems <= 0) return;
+  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock));
+  if (kind == 0) {
+    if (S.staged)
+      hipLaunchKernelGGL((k_low_ac2_hot<float, 0>), gr, dim3(kBlock), 0, L.s, S,
+                         nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+    else
+      hipLaunchKernelGGL((k_low_ac2_hot<double, 0>), gr, dim3(kBlock), 0, L.s,
+                         S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+  } else {
+    if (S.staged)
+      hipLaunchKernelGGL((k_low_ac2_hot<float, 1>), gr, dim3(kBlock), 0, L.s, S,
+                         nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+    else
+      hipLaunchKernelGGL((k_low_ac2_hot<double, 1>), gr, dim3(kBlock), 0, L.s,
+                         S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+  }
+}
+void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        uint8_t* flag, Queue q2, PipeLaunch L) {
+  if (!nq) return;
+  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_low_ac2_esc<0>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+                       ecol, eleft, tk, tedge, flag, q2);
+  else
+    hipLaunchKernelGGL(k_low_ac2_esc<1>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+                       ecol, eleft, tk, tedge, flag, q2);
+}
+void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
+                          uint32_t nq, const uint32_t* ecol,
+                          const uint32_t* eleft, const uint32_t* tk,
+                          const uint32_t* tedge, uint8_t* flag, PipeLaunch L) {
+  if (!nq) return;
+  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (kind == 0)
+    hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+                       ecol, eleft, tk, tedge, flag, S.reach, S.margin);
+  else
+    hipLaunchKernelGGL(k_low_ac2_exact<1>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+                       ecol, eleft, tk, tedge, flag, S.reach, S.margin);
+}
+
+int sos_table_host(int32_t* out3, int cap) {
+  for (int t = 0; t < PAC_SOS_TERMS && t < cap; ++t)
+    for (int c = 0; c < 3; ++c) out3[3 * t + c] = pac_sos_table[t][c];
+  return PAC_SOS_TERMS;
+}
+
+}  // namespace pac

@@ -1,0 +1,152 @@
+/*
+ * k_closure.hip -- closure of the complex and output emission.
+ *
+ * The reference builds the faces of the final tets and the edges of the
+ * final triangles, sorts and de-duplicates them and merges them with the
+ * surviving dangling simplices (AlphaComplexParallel.cu:1490-1632,
+ * :1646-1788; generateTrisInTets / markTrisForDeletion / generateEdgesInTris
+ * / markEdgesForDeletion, AlphaComplexLargeData.cu:737-926).  Every such
+ * face is itself an alpha simplex, so here each final tet (triangle) marks
+ * its faces (edges) in the already sorted alpha lists by CSR lookup; the
+ * final lists are then a stable compaction of the alpha lists and come out
+ * in the reference's lexicographic order with no sort or merge.
+ */
+#include "pac_dev.h"
+
+namespace pac {
+
+__global__ __launch_bounds__(kBlock) void k_compact_u4(int64_t n,
+                                                       const uint4* in,
+                                                       const uint8_t* flag,
+                                                       const uint64_t* pos,
+                                                       uint4* out) {
+  int64_t i = gtid();
+  if (i >= n || !flag[i]) return;
+  out[pos[i]] = in[i];
+}
+void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
+                         const uint64_t* pos, uint4* out, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, in, flag, pos, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_ids(int64_t n,
+                                                        const uint8_t* flag,
+                                                        const uint64_t* pos,
+                                                        uint32_t* out) {
+  int64_t i = gtid();
+  if (i >= n || !flag[i]) return;
+  out[pos[i]] = (uint32_t)i;
+}
+void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
+                        uint32_t* out, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, flag, pos, out);
+}
+
+/* faces of the final tets -> tri_in */
+__global__ __launch_bounds__(kBlock) void k_mark_faces(
+    int64_t nk, const uint4* ftets, const uint64_t* erow, const uint32_t* ecol,
+    const uint64_t* trow, const uint32_t* tk, uint8_t* tri_in) {
+  int64_t id = gtid();
+  if (id >= nk) return;
+  uint4 t = ftets[id];
+  int64_t f;
+  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.y, t.z)) >= 0) tri_in[f] = 1;
+  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.y, t.w)) >= 0) tri_in[f] = 1;
+  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.z, t.w)) >= 0) tri_in[f] = 1;
+  if ((f = find_tri(erow, ecol, trow, tk, t.y, t.z, t.w)) >= 0) tri_in[f] = 1;
+}
+void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
+                       const uint32_t* ecol, const uint64_t* trow,
+                       const uint32_t* tk, uint8_t* tri_in, PipeLaunch L) {
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(k_mark_faces, dim3((unsigned)((nk + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nk, ftets, erow, ecol, trow, tk,
+                     tri_in);
+}
+
+/* edges of the final triangles -> edge_in */
+__global__ __launch_bounds__(kBlock) void k_mark_edges(
+    int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    const uint64_t* erow, uint8_t* edge_in) {
+  int64_t id = gtid();
+  if (id >= nt) return;
+  uint32_t t = ftri[id];
+  uint32_t e = tedge[t];
+  uint32_t i = eleft[e], j = ecol[e], k = tk[t];
+  edge_in[e] = 1;
+  int64_t f;
+  if ((f = find_edge(erow, ecol, i, k)) >= 0) edge_in[f] = 1;
+  if ((f = find_edge(erow, ecol, j, k)) >= 0) edge_in[f] = 1;
+}
+void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+                       const uint32_t* eleft, const uint32_t* tk,
+                       const uint32_t* tedge, const uint64_t* erow,
+                       uint8_t* edge_in, PipeLaunch L) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_mark_edges, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, ftri, ecol, eleft, tk, tedge,
+                     erow, edge_in);
+}
+
+/* ---- emission in original ids ---- */
+__global__ __launch_bounds__(kBlock) void k_emit_tets(int64_t nk,
+                                                      const uint4* ftets,
+                                                      const uint32_t* perm,
+                                                      int4* out) {
+  int64_t id = gtid();
+  if (id >= nk) return;
+  uint4 t = ftets[id];
+  out[id] = make_int4((int)perm[t.x], (int)perm[t.y], (int)perm[t.z],
+                      (int)perm[t.w]);
+}
+void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L) {
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(k_emit_tets, dim3((unsigned)((nk + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nk, ftets, perm, (int4*)out);
+}
+__global__ __launch_bounds__(kBlock) void k_emit_tris(
+    int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    const uint32_t* perm, int32_t* out) {
+  int64_t id = gtid();
+  if (id >= nt) return;
+  uint32_t t = ftri[id];
+  uint32_t e = tedge[t];
+  uint64_t w = (uint64_t)id * 3;
+  out[w] = (int32_t)perm[eleft[e]];
+  out[w + 1] = (int32_t)perm[ecol[e]];
+  out[w + 2] = (int32_t)perm[tk[t]];
+}
+void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+                      const uint32_t* eleft, const uint32_t* tk,
+                      const uint32_t* tedge, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_emit_tris, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, ftri, ecol, eleft, tk, tedge,
+                     perm, out);
+}
+__global__ __launch_bounds__(kBlock) void k_emit_edges(
+    int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+    const uint8_t* flag, const uint64_t* pos, const uint32_t* perm,
+    int2* out) {
+  int64_t id = gtid();
+  if (id >= ne || !flag[id]) return;
+  out[pos[id]] = make_int2((int)perm[eleft[id]], (int)perm[ecol[id]]);
+}
+void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+                       const uint8_t* flag, const uint64_t* pos,
+                       const uint32_t* perm, int32_t* out, PipeLaunch L) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, ne, ecol, eleft, flag, pos, perm,
+                     (int2*)out);
+}
+
+}  // namespace pac

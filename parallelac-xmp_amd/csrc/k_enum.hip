@@ -1,0 +1,783 @@
+/*
+ * k_enum.hip -- binning, candidate enumeration and the alpha filters.
+ *
+ * One thread per ball / alpha edge / alpha triangle enumerates its
+ * candidates (cells of the stencil, sorted-row intersections, three-way
+ * merges), classifies each with one precision stage (FP32 on the staged
+ * path, FP64 on the plain path) and appends undecided candidates to a
+ * queue.  The escalation kernel re-decides them in FP64 then double-double;
+ * what is still undecided goes to the wide-integer kernel.
+ *
+ * Reference kernels restated (AlphaComplexLargeData.cu unless noted):
+ *   k_cell_key           computeCellIndex :65-78
+ *   k_edge_count/fill    computePossibleEdgeCount / fillPotentialEdges
+ *                        :80-165 + markRealEdges :167-190 (fused)
+ *   k_tri_count/fill     computePossibleTriangleCount / fillPossibleTriangles
+ *                        :192-296 + markRealTriangles :298-348 (fused)
+ *   k_tet_count/fill     computePossibleTetCount / fillPossibleTets :350-529
+ *                        + markRealTets :531-591 (fused)
+ */
+#include "pac_dev.h"
+
+namespace pac {
+
+/* ------------------------------------------------------------------ */
+/* bounds / keys / gather                                              */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ int neg_lsb_exp(float v) {
+  uint32_t b = __float_as_uint(v) & 0x7fffffffu;
+  if (b == 0) return 0;
+  uint32_t E = b >> 23, m = b & 0x7fffffu;
+  int e;
+  if (E == 0) {
+    e = -149;
+  } else {
+    m |= 0x800000u;
+    e = (int)E - 150;
+  }
+  int lsb = e + __builtin_ctz(m);
+  return lsb < 0 ? -lsb : 0;
+}
+
+struct BRed {
+  float v[9];  /* minx miny minz -maxx -maxy -maxz -maxrawr -maxr -maxabs */
+  int shift, nonfinite;
+};
+__device__ void bred_init(BRed& b) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) b.v[q] = INFINITY;
+  b.shift = 0;
+  b.nonfinite = 0;
+}
+__device__ void bred_merge(BRed& a, const BRed& b) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) a.v[q] = fminf(a.v[q], b.v[q]);
+  a.shift = max(a.shift, b.shift);
+  a.nonfinite += b.nonfinite;
+}
+__device__ void bred_block(BRed& mine, BRed* sh) {
+  sh[threadIdx.x] = mine;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) bred_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  mine = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_part(const float4* raw,
+                                                        int64_t n,
+                                                        float solvent,
+                                                        BRed* part) {
+  __shared__ BRed sh[kBlock];
+  BRed b;
+  bred_init(b);
+  for (int64_t i = gtid(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float4 a = raw[i];
+    if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z) || !isfinite(a.w)) {
+      b.nonfinite++;
+      continue;
+    }
+    float r = __fadd_rn(a.w, solvent);
+    b.v[0] = fminf(b.v[0], a.x);
+    b.v[1] = fminf(b.v[1], a.y);
+    b.v[2] = fminf(b.v[2], a.z);
+    b.v[3] = fminf(b.v[3], -a.x);
+    b.v[4] = fminf(b.v[4], -a.y);
+    b.v[5] = fminf(b.v[5], -a.z);
+    b.v[6] = fminf(b.v[6], -a.w);
+    b.v[7] = fminf(b.v[7], -fabsf(r));
+    b.v[8] = fminf(b.v[8], -fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))));
+    int s = max(max(neg_lsb_exp(a.x), neg_lsb_exp(a.y)),
+                max(neg_lsb_exp(a.z), neg_lsb_exp(r)));
+    b.shift = max(b.shift, s);
+  }
+  bred_block(b, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const BRed* part,
+                                                         int np, Bounds* out) {
+  __shared__ BRed sh[kBlock];
+  BRed b;
+  bred_init(b);
+  for (int i = threadIdx.x; i < np; i += blockDim.x) bred_merge(b, part[i]);
+  bred_block(b, sh);
+  if (threadIdx.x == 0) {
+    Bounds o;
+    o.minx = b.v[0];
+    o.miny = b.v[1];
+    o.minz = b.v[2];
+    o.maxx = -b.v[3];
+    o.maxy = -b.v[4];
+    o.maxz = -b.v[5];
+    o.max_raw_r = -b.v[6];
+    o.max_r = -b.v[7];
+    o.max_abs = -b.v[8];
+    o.shift = b.shift;
+    o.nonfinite = b.nonfinite;
+    o.pad = 0;
+    *out = o;
+  }
+}
+void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
+                   Bounds* out, PipeLaunch L) {
+  int nb = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_bounds_part, dim3(nb), dim3(kBlock), 0, L.s, raw, n,
+                     solvent, (BRed*)part);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, L.s,
+                     (const BRed*)part, nb, out);
+}
+
+/* computeCellIndex (LD:65-78): FP32 subtract, IEEE divide, truncate. */
+__global__ __launch_bounds__(kBlock) void k_cell_key(const float4* raw,
+                                                     int64_t n, GridP g,
+                                                     uint32_t* key,
+                                                     uint32_t* idx) {
+  int64_t i = gtid();
+  if (i >= n) return;
+  float4 a = raw[i];
+  int cx = (int)__fdiv_rn(__fsub_rn(a.x, g.minx), g.step);
+  int cy = (int)__fdiv_rn(__fsub_rn(a.y, g.miny), g.step);
+  int cz = (int)__fdiv_rn(__fsub_rn(a.z, g.minz), g.step);
+  key[i] = (uint32_t)(cx + g.ex * cy + g.ex * g.ey * cz);
+  idx[i] = (uint32_t)i;
+}
+void launch_cell_key(const float4* raw, int64_t n, GridP g, uint32_t* key,
+                     uint32_t* idx, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cell_key, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, raw, n, g, key, idx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const float4* raw,
+                                                   const uint32_t* sidx,
+                                                   const uint32_t* skey,
+                                                   int64_t n, float solvent,
+                                                   float4* ball, uint32_t* perm,
+                                                   uint32_t* cellk) {
+  int64_t s = gtid();
+  if (s >= n) return;
+  uint32_t o = sidx[s];
+  float4 a = raw[o];
+  ball[s] = make_float4(a.x, a.y, a.z, __fadd_rn(a.w, solvent));
+  perm[s] = o;
+  cellk[s] = skey[s];
+}
+void launch_gather(const float4* raw, const uint32_t* sidx,
+                   const uint32_t* skey, int64_t n, float solvent,
+                   float4* ball, uint32_t* perm, uint32_t* cellk,
+                   PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, raw, sidx, skey, n, solvent, ball,
+                     perm, cellk);
+}
+
+/* cstart[c] = first sorted ball with key >= c, for c in [0, G] */
+__global__ __launch_bounds__(kBlock) void k_cell_starts(const uint32_t* cellk,
+                                                        int64_t n, int64_t G,
+                                                        uint32_t* cstart) {
+  int64_t s = gtid();
+  if (s > n) return;
+  int64_t lo = (s == 0) ? 0 : (int64_t)cellk[s - 1] + 1;
+  int64_t hi = (s == n) ? G : (int64_t)cellk[s];
+  for (int64_t c = lo; c <= hi; ++c) cstart[c] = (uint32_t)s;
+}
+void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
+                        uint32_t* cstart, PipeLaunch L) {
+  hipLaunchKernelGGL(k_cell_starts,
+                     dim3((unsigned)((n + 1 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, cellk, n, G, cstart);
+}
+
+/* ------------------------------------------------------------------ */
+/* filter decisions (one stage) and exact decisions                    */
+/* ------------------------------------------------------------------ */
+template <class T>
+__device__ int edge_code(const float4& a, const float4& b, float alpha) {
+  T v = edge_val(Ld<T>::p(a), Ld<T>::w(a), Ld<T>::p(b), Ld<T>::w(b),
+                 Al<T>::c(alpha));
+  return code_of(fe_sign(v));
+}
+__device__ int edge_exact(const float4& a, const float4& b,
+                          const ExactScale& X) {
+  if (same_pos(a, b)) return REJECT;
+  LdWI L{X.S};
+  WI v = edge_val(L.p(a), L.w(a), L.p(b), L.w(b), X.alpha);
+  return wi_sign(v) >= 0 ? KEEP : REJECT;
+}
+
+template <class T>
+__device__ int tri_code(const float4& a, const float4& b, const float4& c,
+                        float alpha) {
+  T nn;
+  T v = tri_val(Ld<T>::p(a), Ld<T>::w(a), Ld<T>::p(b), Ld<T>::w(b),
+                Ld<T>::p(c), Ld<T>::w(c), Al<T>::c(alpha), &nn);
+  if (fe_sign(nn) <= 0) return UNDEC;
+  return code_of(fe_sign(v));
+}
+__device__ int tri_exact(const float4& a, const float4& b, const float4& c,
+                         const ExactScale& X) {
+  LdWI L{X.S};
+  WI nn;
+  WI v = tri_val(L.p(a), L.w(a), L.p(b), L.w(b), L.p(c), L.w(c), X.alpha, &nn);
+  if (wi_sign(nn) == 0) return REJECT;
+  return wi_sign(v) >= 0 ? KEEP : REJECT;
+}
+
+template <class T>
+__device__ int tet_code(const float4& a, const float4& b, const float4& c,
+                        const float4& d, float alpha) {
+  T D;
+  V3<T> N;
+  T wa = Ld<T>::w(a);
+  tet_DN(Ld<T>::p(a), wa, Ld<T>::p(b), Ld<T>::w(b), Ld<T>::p(c), Ld<T>::w(c),
+         Ld<T>::p(d), Ld<T>::w(d), &D, &N);
+  if (fe_sign(D) == 0) return UNDEC;
+  return code_of(fe_sign(tet_val(D, N, wa, Al<T>::c(alpha))));
+}
+__device__ int tet_exact(const float4& a, const float4& b, const float4& c,
+                         const float4& d, const ExactScale& X) {
+  LdWI L{X.S};
+  WI D;
+  V3<WI> N;
+  WI wa = L.w(a);
+  tet_DN(L.p(a), wa, L.p(b), L.w(b), L.p(c), L.w(c), L.p(d), L.w(d), &D, &N);
+  if (wi_sign(D) == 0) return REJECT;
+  return wi_sign(tet_val(D, N, wa, X.alpha)) >= 0 ? KEEP : REJECT;
+}
+
+/* ------------------------------------------------------------------ */
+/* edges                                                               */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void cell_xyz(uint32_t c, const GridP& g, int& x,
+                                         int& y, int& z) {
+  x = (int)(c % (uint32_t)g.ex);
+  uint32_t t = c / (uint32_t)g.ex;
+  y = (int)(t % (uint32_t)g.ey);
+  z = (int)(t / (uint32_t)g.ey);
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_count(Stage S, uint32_t* cnt) {
+  int64_t i = gtid();
+  if (i >= S.n) return;
+  uint32_t c = S.cellk[i];
+  int gx, gy, gz;
+  cell_xyz(c, S.g, gx, gy, gz);
+  const int k = S.g.k_enum;
+  uint32_t total = 0;
+  for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
+    for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
+      int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+      int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
+      int64_t c0 = row + x0, c1 = row + x1;
+      if (c1 < (int64_t)c) continue;
+      uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
+      if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
+      if (e > b) total += e - b;
+    }
+  cnt[i] = total;
+}
+void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
+  if (S.n <= 0) return;
+  hipLaunchKernelGGL(k_edge_count, dim3((unsigned)((S.n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, S, cnt);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_edge_fill(
+    Stage S, const uint64_t* off, uint32_t* cand, uint8_t* flag,
+    uint32_t* keep, Queue q1) {
+  int64_t i = gtid();
+  unsigned evals = 0;
+  if (i < S.n) {
+    uint32_t c = S.cellk[i];
+    int gx, gy, gz;
+    cell_xyz(c, S.g, gx, gy, gz);
+    const int k = S.g.k_enum;
+    float4 a = S.ball[i];
+    uint64_t o = off[i];
+    uint32_t kept = 0;
+    for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
+      for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
+        int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+        int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
+        int64_t c0 = row + x0, c1 = row + x1;
+        if (c1 < (int64_t)c) continue;
+        uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
+        if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
+        for (uint32_t j = b; j < e; ++j) {
+          float4 bj = S.ball[j];
+          int r = same_pos(a, bj) ? REJECT : edge_code<FE<R>>(a, bj, S.alpha);
+          ++evals;
+          cand[o] = j;
+          flag[o] = (uint8_t)r;
+          kept += (r == KEEP);
+          if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)i);
+          ++o;
+        }
+      }
+    keep[i] = kept;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_EDGE * 4 + StageOf<FE<R>>::v, evals);
+}
+void launch_edge_fill(const Stage& S, const uint64_t* off, uint32_t* cand,
+                      uint8_t* flag, uint32_t* keep, Queue q1, PipeLaunch L) {
+  if (S.n <= 0) return;
+  dim3 gr((unsigned)((S.n + kBlock - 1) / kBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_edge_fill<float>, gr, dim3(kBlock), 0, L.s, S, off,
+                       cand, flag, keep, q1);
+  else
+    hipLaunchKernelGGL(k_edge_fill<double>, gr, dim3(kBlock), 0, L.s, S, off,
+                       cand, flag, keep, q1);
+}
+
+/* escalation FP64 -> double-double for queued candidates (staged path) */
+__global__ __launch_bounds__(kBlock) void k_edge_esc(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* cand,
+    uint8_t* flag, uint32_t* keep, Queue q2) {
+  int64_t t = gtid();
+  StageCnt sc;
+  if (t < nq) {
+    QEntry e = q[t];
+    float4 a = S.ball[e.aux], b = S.ball[cand[e.item]];
+    sc.c[1]++;
+    int r = edge_code<FE<double>>(a, b, S.alpha);
+    if (r == UNDEC) {
+      sc.c[2]++;
+      r = edge_code<FEDD>(a, b, S.alpha);
+    }
+    if (r == UNDEC) {
+      push_q(q2, e.item, e.aux);
+    } else {
+      flag[e.item] = (uint8_t)r;
+      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    }
+  }
+  sc.flush(S.ctr, FAM_EDGE);
+}
+void launch_edge_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                     const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                     Queue q2, PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_edge_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
+                     0, L.s, S, q, nq, cand, flag, keep, q2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_exact(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* cand,
+    uint8_t* flag, uint32_t* keep) {
+  int64_t t = gtid();
+  unsigned cnt = 0;
+  if (t < nq) {
+    QEntry e = q[t];
+    int r = edge_exact(S.ball[e.aux], S.ball[cand[e.item]], S.ex);
+    flag[e.item] = (uint8_t)r;
+    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    cnt = 1;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_EDGE * 4 + 3, cnt);
+}
+void launch_edge_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                       const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                       PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_edge_exact, dim3((nq + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, nq, cand, flag, keep);
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_compact(
+    int64_t n, const uint64_t* off, const uint32_t* cand, const uint8_t* flag,
+    const uint64_t* erow, uint32_t* ecol, uint32_t* eleft) {
+  int64_t i = gtid();
+  if (i >= n) return;
+  uint64_t w = erow[i];
+  for (uint64_t p = off[i]; p < off[i + 1]; ++p)
+    if (flag[p] == KEEP) {
+      ecol[w] = cand[p];
+      eleft[w] = (uint32_t)i;
+      ++w;
+    }
+}
+void launch_edge_compact(int64_t n, const uint64_t* off, const uint32_t* cand,
+                         const uint8_t* flag, const uint64_t* erow,
+                         uint32_t* ecol, uint32_t* eleft, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_edge_compact, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, off, cand, flag, erow, ecol,
+                     eleft);
+}
+
+/* ------------------------------------------------------------------ */
+/* triangles: k in row(i) after j, intersected with row(j)             */
+/* ------------------------------------------------------------------ */
+__global__ __launch_bounds__(kBlock) void k_tri_count(
+    int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, uint32_t* cnt) {
+  int64_t q = gtid();
+  if (q >= ne) return;
+  uint32_t i = eleft[q], j = ecol[q];
+  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+  uint64_t b = erow[j], be = erow[j + 1];
+  uint32_t c = 0;
+  while (a < ae && b < be) {
+    uint32_t x = ecol[a], y = ecol[b];
+    c += (x == y);
+    a += (x <= y);
+    b += (y <= x);
+  }
+  cnt[q] = c;
+}
+void launch_tri_count(const Stage&, int64_t ne, const uint64_t* erow,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      uint32_t* cnt, PipeLaunch L) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tri_fill(
+    Stage S, int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* off, uint32_t* cand, uint8_t* flag,
+    uint32_t* keep, Queue q1) {
+  int64_t q = gtid();
+  unsigned evals = 0;
+  if (q < ne) {
+    uint32_t i = eleft[q], j = ecol[q];
+    float4 A = S.ball[i], B = S.ball[j];
+    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+    uint64_t b = erow[j], be = erow[j + 1];
+    uint64_t o = off[q];
+    uint32_t kept = 0;
+    while (a < ae && b < be) {
+      uint32_t x = ecol[a], y = ecol[b];
+      if (x == y) {
+        int r = tri_code<FE<R>>(A, B, S.ball[x], S.alpha);
+        ++evals;
+        cand[o] = x;
+        flag[o] = (uint8_t)r;
+        kept += (r == KEEP);
+        if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)q);
+        ++o;
+      }
+      a += (x <= y);
+      b += (y <= x);
+    }
+    keep[q] = kept;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TRI * 4 + StageOf<FE<R>>::v, evals);
+}
+void launch_tri_fill(const Stage& S, int64_t ne, const uint64_t* erow,
+                     const uint32_t* ecol, const uint32_t* eleft,
+                     const uint64_t* off, uint32_t* cand, uint8_t* flag,
+                     uint32_t* keep, Queue q1, PipeLaunch L) {
+  if (ne <= 0) return;
+  dim3 gr((unsigned)((ne + kBlock - 1) / kBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_tri_fill<float>, gr, dim3(kBlock), 0, L.s, S, ne, erow,
+                       ecol, eleft, off, cand, flag, keep, q1);
+  else
+    hipLaunchKernelGGL(k_tri_fill<double>, gr, dim3(kBlock), 0, L.s, S, ne,
+                       erow, ecol, eleft, off, cand, flag, keep, q1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_esc(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* cand, uint8_t* flag,
+    uint32_t* keep, Queue q2) {
+  int64_t t = gtid();
+  StageCnt sc;
+  if (t < nq) {
+    QEntry e = q[t];
+    float4 a = S.ball[eleft[e.aux]], b = S.ball[ecol[e.aux]],
+           c = S.ball[cand[e.item]];
+    sc.c[1]++;
+    int r = tri_code<FE<double>>(a, b, c, S.alpha);
+    if (r == UNDEC) {
+      sc.c[2]++;
+      r = tri_code<FEDD>(a, b, c, S.alpha);
+    }
+    if (r == UNDEC) {
+      push_q(q2, e.item, e.aux);
+    } else {
+      flag[e.item] = (uint8_t)r;
+      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    }
+  }
+  sc.flush(S.ctr, FAM_TRI);
+}
+void launch_tri_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                    const uint32_t* ecol, const uint32_t* eleft,
+                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                    Queue q2, PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tri_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
+                     0, L.s, S, q, nq, ecol, eleft, cand, flag, keep, q2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_exact(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* cand, uint8_t* flag,
+    uint32_t* keep) {
+  int64_t t = gtid();
+  unsigned cnt = 0;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint32_t i = eleft[e.aux], j = ecol[e.aux], k = cand[e.item];
+    int r = tri_exact(S.ball[i], S.ball[j], S.ball[k], S.ex);
+    flag[e.item] = (uint8_t)r;
+    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    cnt = 1;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TRI * 4 + 3, cnt);
+}
+void launch_tri_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                      PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tri_exact, dim3((nq + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, nq, ecol, eleft, cand, flag,
+                     keep);
+}
+
+/* generic segmented compaction: segment s owns candidates [off[s],off[s+1]) */
+__global__ __launch_bounds__(kBlock) void k_seg_compact(
+    int64_t nseg, const uint64_t* off, const uint32_t* cand,
+    const uint8_t* flag, const uint64_t* orow, uint32_t* ocol,
+    uint32_t* oseg) {
+  int64_t s = gtid();
+  if (s >= nseg) return;
+  uint64_t w = orow[s];
+  for (uint64_t p = off[s]; p < off[s + 1]; ++p)
+    if (flag[p] == KEEP) {
+      ocol[w] = cand[p];
+      if (oseg) oseg[w] = (uint32_t)s;
+      ++w;
+    }
+}
+void launch_seg_compact(int64_t nseg, const uint64_t* off,
+                        const uint32_t* cand, const uint8_t* flag,
+                        const uint64_t* orow, uint32_t* ocol, uint32_t* oseg,
+                        PipeLaunch L) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_seg_compact,
+                     dim3((unsigned)((nseg + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nseg, off, cand, flag, orow, ocol,
+                     oseg);
+}
+
+/* ------------------------------------------------------------------ */
+/* tets: l in T(i,j) after k, T(i,k), T(j,k)                           */
+/* ------------------------------------------------------------------ */
+struct TetWalk {
+  uint64_t a, ae, b, be, c, ce;
+};
+__device__ __forceinline__ bool tet_walk_init(
+    int64_t q, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tedge,
+    const uint32_t* tk, uint32_t& i, uint32_t& j, uint32_t& k, TetWalk& w) {
+  uint32_t eij = tedge[q];
+  i = eleft[eij];
+  j = ecol[eij];
+  k = tk[q];
+  w.a = (uint64_t)q + 1;
+  w.ae = trow[eij + 1];
+  if (w.a >= w.ae) return false;
+  int64_t eik = find_edge(erow, ecol, i, k), ejk = find_edge(erow, ecol, j, k);
+  if (eik < 0 || ejk < 0) return false;
+  w.b = trow[eik];
+  w.be = trow[eik + 1];
+  w.c = trow[ejk];
+  w.ce = trow[ejk + 1];
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_count(
+    int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
+    const uint32_t* tedge, uint32_t* cnt) {
+  int64_t q = gtid();
+  if (q >= nt) return;
+  uint32_t i, j, k;
+  TetWalk w;
+  uint32_t c = 0;
+  if (tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) {
+    while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
+      uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
+      uint32_t m = max(x, max(y, z));
+      if (x == m && y == m && z == m) {
+        ++c;
+        ++w.a;
+        ++w.b;
+        ++w.c;
+      } else {
+        w.a += (x < m);
+        w.b += (y < m);
+        w.c += (z < m);
+      }
+    }
+  }
+  cnt[q] = c;
+}
+void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+                      const uint32_t* eleft, const uint64_t* trow,
+                      const uint32_t* tk, const uint32_t* tedge,
+                      uint32_t* cnt, PipeLaunch L) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_tet_count, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, erow, ecol, eleft, trow, tk,
+                     tedge, cnt);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tet_fill(
+    Stage S, int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
+    const uint32_t* tedge, const uint64_t* off, uint32_t* cand, uint8_t* flag,
+    uint32_t* keep, Queue q1) {
+  int64_t q = gtid();
+  unsigned evals = 0;
+  if (q < nt) {
+    uint32_t i, j, k;
+    TetWalk w;
+    uint64_t o = off[q];
+    uint32_t kept = 0;
+    if (tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) {
+      float4 A = S.ball[i], B = S.ball[j], C = S.ball[k];
+      while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
+        uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
+        uint32_t m = max(x, max(y, z));
+        if (x == m && y == m && z == m) {
+          int r = tet_code<FE<R>>(A, B, C, S.ball[m], S.alpha);
+          ++evals;
+          cand[o] = m;
+          flag[o] = (uint8_t)r;
+          kept += (r == KEEP);
+          if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)q);
+          ++o;
+          ++w.a;
+          ++w.b;
+          ++w.c;
+        } else {
+          w.a += (x < m);
+          w.b += (y < m);
+          w.c += (z < m);
+        }
+      }
+    }
+    keep[q] = kept;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET * 4 + StageOf<FE<R>>::v, evals);
+}
+void launch_tet_fill(const Stage& S, int64_t nt, const uint64_t* erow,
+                     const uint32_t* ecol, const uint32_t* eleft,
+                     const uint64_t* trow, const uint32_t* tk,
+                     const uint32_t* tedge, const uint64_t* off,
+                     uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q1,
+                     PipeLaunch L) {
+  if (nt <= 0) return;
+  dim3 gr((unsigned)((nt + kBlock - 1) / kBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_tet_fill<float>, gr, dim3(kBlock), 0, L.s, S, nt, erow,
+                       ecol, eleft, trow, tk, tedge, off, cand, flag, keep, q1);
+  else
+    hipLaunchKernelGGL(k_tet_fill<double>, gr, dim3(kBlock), 0, L.s, S, nt,
+                       erow, ecol, eleft, trow, tk, tedge, off, cand, flag,
+                       keep, q1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_esc(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    const uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q2) {
+  int64_t t = gtid();
+  StageCnt sc;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint32_t eij = tedge[e.aux];
+    float4 a = S.ball[eleft[eij]], b = S.ball[ecol[eij]], c = S.ball[tk[e.aux]],
+           d = S.ball[cand[e.item]];
+    sc.c[1]++;
+    int r = tet_code<FE<double>>(a, b, c, d, S.alpha);
+    if (r == UNDEC) {
+      sc.c[2]++;
+      r = tet_code<FEDD>(a, b, c, d, S.alpha);
+    }
+    if (r == UNDEC) {
+      push_q(q2, e.item, e.aux);
+    } else {
+      flag[e.item] = (uint8_t)r;
+      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    }
+  }
+  sc.flush(S.ctr, FAM_TET);
+}
+void launch_tet_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                    const uint32_t* ecol, const uint32_t* eleft,
+                    const uint32_t* tk, const uint32_t* tedge,
+                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                    Queue q2, PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tet_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
+                     0, L.s, S, q, nq, ecol, eleft, tk, tedge, cand, flag, keep,
+                     q2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_exact(
+    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
+    const uint32_t* cand, uint8_t* flag, uint32_t* keep) {
+  int64_t t = gtid();
+  unsigned cnt = 0;
+  if (t < nq) {
+    QEntry e = q[t];
+    uint32_t eij = tedge[e.aux];
+    uint32_t i = eleft[eij], j = ecol[eij], k = tk[e.aux], l = cand[e.item];
+    int r = tet_exact(S.ball[i], S.ball[j], S.ball[k], S.ball[l], S.ex);
+    flag[e.item] = (uint8_t)r;
+    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    cnt = 1;
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET * 4 + 3, cnt);
+}
+void launch_tet_exact(const Stage& S, const QEntry* q, uint32_t nq,
+                      const uint32_t* ecol, const uint32_t* eleft,
+                      const uint32_t* tk, const uint32_t* tedge,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                      PipeLaunch L) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_tet_exact, dim3((nq + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, nq, ecol, eleft, tk, tedge,
+                     cand, flag, keep);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_compact(
+    int64_t nt, const uint64_t* off, const uint32_t* cand, const uint8_t* flag,
+    const uint64_t* orow, const uint32_t* ecol, const uint32_t* eleft,
+    const uint32_t* tk, const uint32_t* tedge, uint4* tets) {
+  int64_t q = gtid();
+  if (q >= nt) return;
+  uint64_t w = orow[q];
+  uint64_t p = off[q], pe = off[q + 1];
+  if (p == pe) return;
+  uint32_t eij = tedge[q];
+  uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
+  for (; p < pe; ++p)
+    if (flag[p] == KEEP) tets[w++] = make_uint4(i, j, k, cand[p]);
+}
+void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
+                        const uint8_t* flag, const uint64_t* orow,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        uint4* tets, PipeLaunch L) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_tet_compact, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, off, cand, flag, orow, ecol,
+                     eleft, tk, tedge, tets);
+}
+
+}  // namespace pac

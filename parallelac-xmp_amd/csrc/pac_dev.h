@@ -1,0 +1,254 @@
+/*
+ * pac_dev.h -- device helpers shared by the kernel translation units:
+ * wave-aggregated counters, queue appends, CSR lookups, the emptiness
+ * search box, and the semi-static power-gap test used by the hot
+ * emptiness kernels.
+ */
+#ifndef PAC_DEV_H
+#define PAC_DEV_H
+
+#include "pac_internal.h"
+
+namespace pac {
+
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ __forceinline__ void flush_ctr(unsigned long long* ctr, int idx,
+                                          unsigned v) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&ctr[idx], (unsigned long long)v);
+}
+struct StageCnt {
+  unsigned c[4];
+  __device__ StageCnt() { c[0] = c[1] = c[2] = c[3] = 0; }
+  __device__ void flush(unsigned long long* ctr, int fam) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) flush_ctr(ctr, CTR_STAGE + fam * 4 + s, c[s]);
+  }
+};
+
+__device__ __forceinline__ int64_t gtid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ bool same_pos(const float4& a, const float4& b) {
+  return a.x == b.x && a.y == b.y && a.z == b.z;
+}
+__device__ __forceinline__ void push_q(const Queue& q, uint32_t item,
+                                       uint32_t aux) {
+  unsigned p = atomicAdd(q.n, 1u);
+  if (p < q.cap) q.q[p] = QEntry{item, aux};
+}
+
+template <class T>
+struct Al;
+template <class R>
+struct Al<FE<R>> {
+  static __device__ FE<R> c(float a) { return FE<R>{(R)a, (R)0}; }
+};
+template <>
+struct Al<FEDD> {
+  static __device__ FEDD c(float a) { return FEDD{dd{(double)a, 0.0}, 0.0}; }
+};
+
+enum { REJECT = 0, KEEP = 1, UNDEC = 2 };
+__device__ __forceinline__ int code_of(int s) {
+  return s > 0 ? KEEP : (s < 0 ? REJECT : UNDEC);
+}
+/* stage index of the precision type */
+template <class T>
+struct StageOf;
+template <>
+struct StageOf<FE<float>> {
+  static constexpr int v = 0;
+};
+template <>
+struct StageOf<FE<double>> {
+  static constexpr int v = 1;
+};
+template <>
+struct StageOf<FEDD> {
+  static constexpr int v = 2;
+};
+
+/* ---- CSR lookups ---- */
+__device__ __forceinline__ int64_t find_in(const uint32_t* a, uint64_t lo,
+                                           uint64_t hi, uint32_t v) {
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    uint32_t x = a[mid];
+    if (x < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (int64_t)lo;
+}
+__device__ __forceinline__ int64_t find_edge(const uint64_t* erow,
+                                             const uint32_t* ecol, uint32_t a,
+                                             uint32_t b) {
+  uint64_t lo = erow[a], hi = erow[a + 1];
+  int64_t p = find_in(ecol, lo, hi, b);
+  return ((uint64_t)p < hi && ecol[p] == b) ? p : -1;
+}
+__device__ __forceinline__ int64_t find_tri(const uint64_t* erow,
+                                            const uint32_t* ecol,
+                                            const uint64_t* trow,
+                                            const uint32_t* tk, uint32_t a,
+                                            uint32_t b, uint32_t c) {
+  int64_t e = find_edge(erow, ecol, a, b);
+  if (e < 0) return -1;
+  uint64_t lo = trow[e], hi = trow[e + 1];
+  int64_t p = find_in(tk, lo, hi, c);
+  return ((uint64_t)p < hi && tk[p] == c) ? p : -1;
+}
+
+/* ---- emptiness search box ----
+ * Every ball that can violate a simplex lies within `reach` of its centre
+ * c.  The centre is known to within err per axis and the FP32 cell keys of
+ * the balls carry at most `margin` of rounding (length units), so the box
+ * [c - reach - margin - err, c + reach + margin + err] in cells covers all
+ * of them. */
+struct CellBox {
+  int lo[3], hi[3];
+  int cc[3]; /* cell of the centre, clamped into the box */
+};
+__device__ __forceinline__ bool make_box(const GridP& g, double reach,
+                                         double margin, double cx, double cy,
+                                         double cz, double ex, double ey,
+                                         double ez, CellBox& b) {
+  double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
+  double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
+  int ext[3] = {g.ex, g.ey, g.ez};
+  double inv = 1.0 / (double)g.step;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double r = reach + margin + e[a];
+    double ql = (c[a] - r - mn[a]) * inv, qh = (c[a] + r - mn[a]) * inv;
+    double qc = (c[a] - mn[a]) * inv;
+    if (!(ql == ql) || !(qh == qh) || !(r < 1e30)) return false;
+    double lo = floor(ql - 1e-9 * (fabs(ql) + 1.0));
+    double hi = floor(qh + 1e-9 * (fabs(qh) + 1.0));
+    if (lo < 0) lo = 0;
+    if (hi > ext[a] - 1) hi = ext[a] - 1;
+    if (lo > hi) {
+      b.lo[a] = 1;
+      b.hi[a] = 0;
+      b.cc[a] = 0;
+      continue;
+    }
+    b.lo[a] = (int)lo;
+    b.hi[a] = (int)hi;
+    double f = floor(qc);
+    b.cc[a] = f < lo ? (int)lo : (f > hi ? (int)hi : (int)f);
+  }
+  return true;
+}
+/* Visit every ball of the box, the centre cell first (the most likely
+ * violators, as in AlphaComplexParallel.cu:632-643), then the rest in
+ * row order.  visit(p) returns true to stop. */
+template <class F>
+__device__ __forceinline__ void scan_box(const GridP& g,
+                                         const uint32_t* cstart,
+                                         const CellBox& b, F&& visit) {
+  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
+  {
+    int64_t c = b.cc[0] + (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
+    for (uint32_t p = cstart[c], e = cstart[c + 1]; p < e; ++p)
+      if (visit(p)) return;
+  }
+  for (int z = b.lo[2]; z <= b.hi[2]; ++z)
+    for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
+      int64_t row = (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      bool crow = (z == b.cc[2] && y == b.cc[1]);
+      if (!crow) {
+        for (uint32_t p = cstart[row + b.lo[0]], e = cstart[row + b.hi[0] + 1];
+             p < e; ++p)
+          if (visit(p)) return;
+      } else {
+        for (uint32_t p = cstart[row + b.lo[0]], e = cstart[row + b.cc[0]];
+             p < e; ++p)
+          if (visit(p)) return;
+        for (uint32_t p = cstart[row + b.cc[0] + 1],
+                      e = cstart[row + b.hi[0] + 1];
+             p < e; ++p)
+          if (visit(p)) return;
+      }
+    }
+}
+
+/* per-axis error of x = num/(k den) from certified errors of num, den */
+template <class R>
+__device__ __forceinline__ double quot_err(const FE<R>& num, const FE<R>& den,
+                                           double k) {
+  double D = fabs((double)den.v), eD = (double)den.e;
+  double N = fabs((double)num.v), eN = (double)num.e;
+  if (!(D > eD * 1.01)) return INFINITY;
+  return ((eN * D + N * eD) / ((D - eD) * D)) / k * 1.001 + (N / D) / k * 1e-14;
+}
+__device__ __forceinline__ double quot_err(const FEDD& num, const FEDD& den,
+                                           double k) {
+  double D = fabs(den.v.hi), eD = den.e;
+  double N = fabs(num.v.hi), eN = num.e;
+  if (!(D > eD * 1.01)) return INFINITY;
+  return ((eN * D + N * eD) / ((D - eD) * D)) / k * 1.001 + (N / D) / k * 1e-14;
+}
+
+/* ---- semi-static power-gap test ----
+ * For a simplex with vertex a, certified den and num (value + error bound)
+ * and a ball e, evaluates g = L*den - y.num with y = p_e - p_a,
+ * L = |y|^2 - w_e + w_a in precision R with the a-priori bound
+ *   |g - g_exact| <= Labs*(c u |den| + e_den) + sum|y_i|(c u |num_i| + e_num_i)
+ *                    + e_wa |den| + tiny,     Labs = |y|^2 + w_e + w_a,
+ * c = 12 covering the <= 8 roundings on any path of the expression (y, the
+ * squares and their sums, w_e = r_e^2, the products with den and num, the
+ * final difference) plus second-order terms.  Returns +1 / -1 when the sign
+ * is certified, 0 otherwise. */
+template <class R>
+struct PairSS {
+  R ax, ay, az, wa;
+  R den, nx, ny, nz;
+  R K1, K2, Mx, My, Mz;
+  int sden;
+};
+template <class R>
+__device__ __forceinline__ bool pairss_init(PairSS<R>& P, const V3<FE<R>>& pa,
+                                            const FE<R>& wa, const FE<R>& den,
+                                            const V3<FE<R>>& num) {
+  P.sden = fe_sign(den);
+  if (!P.sden) return false;
+  const R cu = (R)12 * FEc<R>::u();
+  P.ax = pa.x.v;
+  P.ay = pa.y.v;
+  P.az = pa.z.v;
+  P.wa = wa.v;
+  P.den = den.v;
+  P.nx = num.x.v;
+  P.ny = num.y.v;
+  P.nz = num.z.v;
+  R sl = FEc<R>::slack();
+  P.K1 = (cu * fabs(den.v) + den.e) * sl;
+  P.K2 = (wa.e * fabs(den.v)) * sl + (R)64 * FEc<R>::tiny();
+  P.Mx = (cu * fabs(num.x.v) + num.x.e) * sl;
+  P.My = (cu * fabs(num.y.v) + num.y.e) * sl;
+  P.Mz = (cu * fabs(num.z.v) + num.z.e) * sl;
+  return true;
+}
+template <class R>
+__device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
+                                           const float4& e) {
+  R yx = (R)e.x - P.ax, yy = (R)e.y - P.ay, yz = (R)e.z - P.az;
+  R q = yx * yx + yy * yy + yz * yz;
+  R we = (R)e.w * (R)e.w;
+  R L = (q - we) + P.wa;
+  R g = L * P.den - (yx * P.nx + yy * P.ny + yz * P.nz);
+  R Labs = (q + we) + P.wa;
+  R B = Labs * P.K1 + (fabs(yx) * P.Mx + fabs(yy) * P.My + fabs(yz) * P.Mz) +
+        P.K2;
+  return g > B ? 1 : (g < -B ? -1 : 0);
+}
+
+}  // namespace pac
+#endif

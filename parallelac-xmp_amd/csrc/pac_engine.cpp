@@ -397,7 +397,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   DBuf<unsigned long long> ctr(P, CTR_COUNT);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ctr.get(), CTR_COUNT);
   SG.ctr = ctr.get();
-  DBuf<unsigned> qn(P, 4);
+  DBuf<unsigned> qn(P, 4); /* [0] hot queue, [1] escalation queue */
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
@@ -432,6 +432,55 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   T.mark(2);
 
   auto scan_tmp = [&](int64_t m) { return DBuf<uint8_t>(P, scan_tmp_bytes(m) + 64); };
+  const bool staged = SG.staged != 0;
+
+  /* precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
+   * double-double (q1 -> q2)] -> exact wide-integer kernel.  Queue
+   * overflows rerun the producing kernel with a larger queue; only the
+   * emptiness kernels can overflow, and they are idempotent. */
+  auto ladder = [&](uint32_t cap1, int kp_hot, int kp_exact, auto&& hot,
+                    auto&& esc, auto&& exact) {
+    for (;;) {
+      DBuf<QEntry> q1(P, cap1);
+      zero_u32(qn.get(), 2, s);
+      if (kp_hot >= 0) KP.begin(kp_hot);
+      hot(Queue{q1.get(), qn.get(), cap1});
+      if (kp_hot >= 0) KP.end();
+      KCHK();
+      uint32_t n1 = readback(E, s, qn.get());
+      if (n1 > cap1) {
+        cap1 = (uint32_t)std::min<int64_t>((int64_t)n1 + n1 / 4 + 1024,
+                                           (int64_t)UINT32_MAX - 1);
+        continue;
+      }
+      const QEntry* qx = q1.get();
+      uint32_t nx = n1;
+      DBuf<QEntry> q2;
+      if (staged && n1) {
+        uint32_t cap2 = (uint32_t)std::min<int64_t>(4 * (int64_t)n1 + 4096,
+                                                    (int64_t)UINT32_MAX - 1);
+        for (;;) {
+          q2 = DBuf<QEntry>(P, cap2);
+          zero_u32(qn.get() + 1, 1, s);
+          esc(q1.get(), n1, Queue{q2.get(), qn.get() + 1, cap2});
+          KCHK();
+          uint32_t n2 = readback(E, s, qn.get() + 1);
+          if (n2 <= cap2) {
+            nx = n2;
+            break;
+          }
+          cap2 = (uint32_t)std::min<int64_t>((int64_t)n2 + n2 / 4 + 1024,
+                                             (int64_t)UINT32_MAX - 1);
+        }
+        qx = q2.get();
+      }
+      if (kp_exact >= 0) KP.begin(kp_exact);
+      exact(qx, nx);
+      if (kp_exact >= 0) KP.end();
+      KCHK();
+      return;
+    }
+  };
 
   /* ---- edges ---- */
   int64_t Ecand = 0, Ealpha = 0;
@@ -450,15 +499,18 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     if (Ecand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "edge candidates exceed 2^32");
     DBuf<uint32_t> cand(P, Ecand), keep(P, n);
     DBuf<uint8_t> flag(P, Ecand);
-    DBuf<QEntry> q(P, std::max<int64_t>(Ecand, 1));
-    zero_u32(qn.get(), 1, s);
-    KP.begin(KP_EDGE_FILL);
-    launch_edge_fill(SG, off.get(), cand.get(), flag.get(), keep.get(), q.get(),
-                     qn.get(), (uint32_t)std::max<int64_t>(Ecand, 1), L);
-    KP.end();
-    KCHK();
-    uint32_t nq = readback(E, s, qn.get());
-    launch_edge_exact(SG, q.get(), nq, cand.get(), flag.get(), keep.get(), L);
+    uint32_t cap = (uint32_t)std::max<int64_t>(Ecand, 1);
+    ladder(
+        cap, KP_EDGE_FILL, -1,
+        [&](Queue q) {
+          launch_edge_fill(SG, off.get(), cand.get(), flag.get(), keep.get(), q, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_edge_esc(SG, q, nq, cand.get(), flag.get(), keep.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_edge_exact(SG, q, nq, cand.get(), flag.get(), keep.get(), L);
+        });
     {
       auto tmp = scan_tmp(n);
       scan_u32_to_u64(keep.get(), erow.get(), n, tmp.get(), 0, s);
@@ -499,18 +551,20 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     if (Tcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "triangle candidates exceed 2^32");
     DBuf<uint32_t> cand(P, Tcand), keep(P, Ealpha);
     DBuf<uint8_t> flag(P, Tcand);
-    DBuf<QEntry> q(P, std::max<int64_t>(Tcand, 1));
-    zero_u32(qn.get(), 1, s);
-    KP.begin(KP_TRI_FILL);
-    if (Ealpha)
-      launch_tri_fill(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
-                      off.get(), cand.get(), flag.get(), keep.get(), q.get(),
-                      qn.get(), (uint32_t)std::max<int64_t>(Tcand, 1), L);
-    KP.end();
-    KCHK();
-    uint32_t nq = readback(E, s, qn.get());
-    launch_tri_exact(SG, q.get(), nq, ecol.get(), eleft.get(), cand.get(),
-                     flag.get(), keep.get(), L);
+    ladder(
+        (uint32_t)std::max<int64_t>(Tcand, 1), KP_TRI_FILL, -1,
+        [&](Queue q) {
+          launch_tri_fill(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
+                          off.get(), cand.get(), flag.get(), keep.get(), q, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_tri_esc(SG, q, nq, ecol.get(), eleft.get(), cand.get(),
+                         flag.get(), keep.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_tri_exact(SG, q, nq, ecol.get(), eleft.get(), cand.get(),
+                           flag.get(), keep.get(), L);
+        });
     {
       auto tmp = scan_tmp(Ealpha);
       scan_u32_to_u64(keep.get(), trow.get(), Ealpha, tmp.get(), 0, s);
@@ -519,9 +573,8 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     Talpha = (int64_t)readback(E, s, trow.get() + Ealpha);
     tk = DBuf<uint32_t>(P, Talpha);
     tedge = DBuf<uint32_t>(P, Talpha);
-    if (Ealpha)
-      launch_seg_compact(Ealpha, off.get(), cand.get(), flag.get(), trow.get(),
-                         tk.get(), tedge.get(), L);
+    launch_seg_compact(Ealpha, off.get(), cand.get(), flag.get(), trow.get(),
+                       tk.get(), tedge.get(), L);
     KCHK();
   }
   st.tri_candidates = Tcand;
@@ -534,9 +587,8 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   {
     DBuf<uint32_t> cnt(P, Talpha);
     DBuf<uint64_t> off(P, Talpha + 1);
-    if (Talpha)
-      launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
-                       tk.get(), tedge.get(), cnt.get(), L);
+    launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
+                     tk.get(), tedge.get(), cnt.get(), L);
     {
       auto tmp = scan_tmp(Talpha);
       scan_u32_to_u64(cnt.get(), off.get(), Talpha, tmp.get(), 0, s);
@@ -546,19 +598,21 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     if (Kcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "tet candidates exceed 2^32");
     DBuf<uint32_t> cand(P, Kcand), keep(P, Talpha);
     DBuf<uint8_t> flag(P, Kcand);
-    DBuf<QEntry> q(P, std::max<int64_t>(Kcand, 1));
-    zero_u32(qn.get(), 1, s);
-    KP.begin(KP_TET_FILL);
-    if (Talpha)
-      launch_tet_fill(SG, Talpha, erow.get(), ecol.get(), eleft.get(),
-                      trow.get(), tk.get(), tedge.get(), off.get(), cand.get(),
-                      flag.get(), keep.get(), q.get(), qn.get(),
-                      (uint32_t)std::max<int64_t>(Kcand, 1), L);
-    KP.end();
-    KCHK();
-    uint32_t nq = readback(E, s, qn.get());
-    launch_tet_exact(SG, q.get(), nq, ecol.get(), eleft.get(), tk.get(),
-                     tedge.get(), cand.get(), flag.get(), keep.get(), L);
+    ladder(
+        (uint32_t)std::max<int64_t>(Kcand, 1), KP_TET_FILL, -1,
+        [&](Queue q) {
+          launch_tet_fill(SG, Talpha, erow.get(), ecol.get(), eleft.get(),
+                          trow.get(), tk.get(), tedge.get(), off.get(),
+                          cand.get(), flag.get(), keep.get(), q, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_tet_esc(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
+                         tedge.get(), cand.get(), flag.get(), keep.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_tet_exact(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
+                           tedge.get(), cand.get(), flag.get(), keep.get(), L);
+        });
     DBuf<uint64_t> orow(P, Talpha + 1);
     {
       auto tmp = scan_tmp(Talpha);
@@ -567,115 +621,122 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     KCHK();
     Kalpha = (int64_t)readback(E, s, orow.get() + Talpha);
     tets = DBuf<uint4>(P, Kalpha);
-    if (Talpha)
-      launch_tet_compact(Talpha, off.get(), cand.get(), flag.get(), orow.get(),
-                         ecol.get(), eleft.get(), tk.get(), tedge.get(),
-                         tets.get(), L);
+    launch_tet_compact(Talpha, off.get(), cand.get(), flag.get(), orow.get(),
+                       ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                       tets.get(), L);
     KCHK();
   }
   st.tet_candidates = Kcand;
   st.tet_alpha = Kalpha;
   T.mark(5);
 
-  /* ---- emptiness with retry on queue overflow ---- */
-  auto run_queued = [&](int64_t items, int kp_scan, int kp_exact,
-                        auto&& scan_kernel, auto&& exact_kernel) {
-    uint32_t cap = (uint32_t)std::min<int64_t>(
-        std::max<int64_t>(1 << 20, 2 * items), (int64_t)UINT32_MAX - 1);
-    for (;;) {
-      DBuf<QEntry> q(P, cap);
-      zero_u32(qn.get(), 1, s);
-      KP.begin(kp_scan);
-      scan_kernel(q.get(), cap);
-      KP.end();
-      KCHK();
-      uint32_t nq = readback(E, s, qn.get());
-      if (nq <= cap) {
-        if (kp_exact >= 0) KP.begin(kp_exact);
-        exact_kernel(q.get(), nq);
-        if (kp_exact >= 0) KP.end();
-        KCHK();
-        return;
-      }
-      cap = (uint32_t)std::min<int64_t>((int64_t)nq + nq / 4 + 1024,
-                                        (int64_t)UINT32_MAX - 1);
-    }
+  auto ac2_cap = [](int64_t items) {
+    return (uint32_t)std::min<int64_t>(std::max<int64_t>(1 << 20, items / 8),
+                                       (int64_t)UINT32_MAX - 1);
   };
 
+  /* ---- tet emptiness ---- */
   DBuf<uint8_t> kflag(P, Kalpha);
   if (opt.skip_tet_ac2) {
     fill_u8(kflag.get(), Kalpha, 1, s);
   } else if (Kalpha) {
-    run_queued(
-        Kalpha, KP_TET_AC2, KP_TET_AC2_EXACT,
-        [&](QEntry* q, uint32_t cap) {
-          launch_tet_ac2(SG, Kalpha, tets.get(), perm.get(), kflag.get(), q,
-                         qn.get(), cap, L);
+    ladder(
+        ac2_cap(Kalpha), KP_TET_AC2, KP_TET_AC2_EXACT,
+        [&](Queue q) { launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, L); },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_tet_ac2_esc(SG, q, nq, tets.get(), kflag.get(), q2, L);
         },
-        [&](QEntry* q, uint32_t nq) {
+        [&](const QEntry* q, uint32_t nq) {
           launch_tet_ac2_exact(SG, q, nq, tets.get(), perm.get(), kflag.get(), L);
         });
   }
+  int64_t Kf = 0;
+  DBuf<uint4> ftets;
+  {
+    DBuf<uint64_t> pos(P, Kalpha + 1);
+    auto tmp = scan_tmp(Kalpha);
+    scan_u8flag_to_u64(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s);
+    KCHK();
+    Kf = (int64_t)readback(E, s, pos.get() + Kalpha);
+    ftets = DBuf<uint4>(P, Kf);
+    launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(), L);
+    KCHK();
+  }
+  tets.release();
   T.mark(6);
 
   /* ---- triangle closure + dangling emptiness ---- */
   DBuf<uint8_t> tri_in(P, Talpha), tflag(P, Talpha);
   fill_u8(tri_in.get(), Talpha, 0, s);
-  launch_mark_tet_faces(Kalpha, tets.get(), kflag.get(), erow.get(), ecol.get(),
-                        trow.get(), tk.get(), tri_in.get(), L);
+  launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
+                    tk.get(), tri_in.get(), L);
   if (Talpha)
-    run_queued(
-        Talpha, KP_TRI_AC2, -1,
-        [&](QEntry* q, uint32_t cap) {
-          launch_tri_ac2(SG, Talpha, ecol.get(), eleft.get(), tk.get(),
-                         tedge.get(), tri_in.get(), tflag.get(), q, qn.get(),
-                         cap, L);
+    ladder(
+        ac2_cap(Talpha), KP_TRI_AC2, -1,
+        [&](Queue q) {
+          launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
+                             tedge.get(), tri_in.get(), tflag.get(), q, L);
         },
-        [&](QEntry* q, uint32_t nq) {
-          launch_tri_ac2_exact(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_low_ac2_esc(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
+                             tedge.get(), tflag.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_low_ac2_exact(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
                                tedge.get(), tflag.get(), L);
         });
+  int64_t Tf = 0;
+  DBuf<uint32_t> ftri;
+  {
+    DBuf<uint64_t> pos(P, Talpha + 1);
+    auto tmp = scan_tmp(Talpha);
+    scan_u8flag_to_u64(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s);
+    KCHK();
+    Tf = (int64_t)readback(E, s, pos.get() + Talpha);
+    ftri = DBuf<uint32_t>(P, Tf);
+    launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L);
+    KCHK();
+  }
   T.mark(7);
 
   /* ---- edge closure + dangling emptiness ---- */
   DBuf<uint8_t> edge_in(P, Ealpha), eflag(P, Ealpha);
   fill_u8(edge_in.get(), Ealpha, 0, s);
-  launch_mark_tri_edges(Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
-                        tflag.get(), erow.get(), edge_in.get(), L);
+  launch_mark_edges(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
+                    tedge.get(), erow.get(), edge_in.get(), L);
   if (Ealpha)
-    run_queued(
-        Ealpha, KP_EDGE_AC2, -1,
-        [&](QEntry* q, uint32_t cap) {
-          launch_edge_ac2(SG, Ealpha, ecol.get(), eleft.get(), edge_in.get(),
-                          eflag.get(), q, qn.get(), cap, L);
+    ladder(
+        ac2_cap(Ealpha), KP_EDGE_AC2, -1,
+        [&](Queue q) {
+          launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
+                             nullptr, edge_in.get(), eflag.get(), q, L);
         },
-        [&](QEntry* q, uint32_t nq) {
-          launch_edge_ac2_exact(SG, q, nq, ecol.get(), eleft.get(), eflag.get(),
-                                L);
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_low_ac2_esc(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+                             nullptr, eflag.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_low_ac2_exact(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+                               nullptr, eflag.get(), L);
         });
+  int64_t Ef = 0;
+  DBuf<uint64_t> pos_e(P, Ealpha + 1);
+  {
+    auto tmp = scan_tmp(Ealpha);
+    scan_u8flag_to_u64(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
+    KCHK();
+    Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
+  }
 
   /* ---- emit final lists (original ids, reference order) ---- */
-  DBuf<uint64_t> pos_k(P, Kalpha + 1), pos_t(P, Talpha + 1),
-      pos_e(P, Ealpha + 1);
-  {
-    auto tmp = scan_tmp(std::max({Kalpha, Talpha, Ealpha, (int64_t)1}));
-    scan_u8flag_to_u64(kflag.get(), pos_k.get(), Kalpha, tmp.get(), 0, s);
-    scan_u8flag_to_u64(tflag.get(), pos_t.get(), Talpha, tmp.get(), 0, s);
-    scan_u8flag_to_u64(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
-  }
-  KCHK();
-  int64_t Kf = (int64_t)readback(E, s, pos_k.get() + Kalpha);
-  int64_t Tf = (int64_t)readback(E, s, pos_t.get() + Talpha);
-  int64_t Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
   R->lists[0] = DBuf<int32_t>(P, 2 * Ef);
   R->lists[1] = DBuf<int32_t>(P, 3 * Tf);
   R->lists[2] = DBuf<int32_t>(P, 4 * Kf);
   launch_emit_edges(Ealpha, ecol.get(), eleft.get(), eflag.get(), pos_e.get(),
                     perm.get(), R->lists[0].get(), L);
-  launch_emit_tris(Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
-                   tflag.get(), pos_t.get(), perm.get(), R->lists[1].get(), L);
-  launch_emit_tets(Kalpha, tets.get(), kflag.get(), pos_k.get(), perm.get(),
-                   R->lists[2].get(), L);
+  launch_emit_tris(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
+                   tedge.get(), perm.get(), R->lists[1].get(), L);
+  launch_emit_tets(Kf, ftets.get(), perm.get(), R->lists[2].get(), L);
   KCHK();
   R->counts[1] = Ef;
   R->counts[2] = Tf;

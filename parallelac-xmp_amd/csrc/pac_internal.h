@@ -92,26 +92,36 @@ struct Stage {
   double margin;  /* FP32 cell-key rounding, in length units */
 };
 
-/* edges */
+struct Queue {
+  QEntry* q;
+  unsigned* n;
+  uint32_t cap;
+};
+
+/* ---- enumeration + filters (k_enum.hip) ---- */
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
 void launch_edge_fill(const Stage& S, const uint64_t* off, uint32_t* cand,
-                      uint8_t* flag, uint32_t* keep, QEntry* q,
-                      unsigned int* qn, uint32_t qcap, PipeLaunch L);
+                      uint8_t* flag, uint32_t* keep, Queue q1, PipeLaunch L);
+void launch_edge_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                     const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                     Queue q2, PipeLaunch L);
 void launch_edge_exact(const Stage& S, const QEntry* q, uint32_t nq,
                        const uint32_t* cand, uint8_t* flag, uint32_t* keep,
                        PipeLaunch L);
 void launch_edge_compact(int64_t n, const uint64_t* off, const uint32_t* cand,
                          const uint8_t* flag, const uint64_t* erow,
                          uint32_t* ecol, uint32_t* eleft, PipeLaunch L);
-/* triangles */
 void launch_tri_count(const Stage& S, int64_t ne, const uint64_t* erow,
                       const uint32_t* ecol, const uint32_t* eleft,
                       uint32_t* cnt, PipeLaunch L);
 void launch_tri_fill(const Stage& S, int64_t ne, const uint64_t* erow,
                      const uint32_t* ecol, const uint32_t* eleft,
                      const uint64_t* off, uint32_t* cand, uint8_t* flag,
-                     uint32_t* keep, QEntry* q, unsigned int* qn,
-                     uint32_t qcap, PipeLaunch L);
+                     uint32_t* keep, Queue q1, PipeLaunch L);
+void launch_tri_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                    const uint32_t* ecol, const uint32_t* eleft,
+                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                    Queue q2, PipeLaunch L);
 void launch_tri_exact(const Stage& S, const QEntry* q, uint32_t nq,
                       const uint32_t* ecol, const uint32_t* eleft,
                       const uint32_t* cand, uint8_t* flag, uint32_t* keep,
@@ -120,7 +130,6 @@ void launch_seg_compact(int64_t nseg, const uint64_t* off,
                         const uint32_t* cand, const uint8_t* flag,
                         const uint64_t* orow, uint32_t* ocol, uint32_t* oseg,
                         PipeLaunch L);
-/* tets */
 void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, const uint64_t* trow,
                       const uint32_t* tk, const uint32_t* tedge,
@@ -129,8 +138,13 @@ void launch_tet_fill(const Stage& S, int64_t nt, const uint64_t* erow,
                      const uint32_t* ecol, const uint32_t* eleft,
                      const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint64_t* off,
-                     uint32_t* cand, uint8_t* flag, uint32_t* keep, QEntry* q,
-                     unsigned int* qn, uint32_t qcap, PipeLaunch L);
+                     uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q1,
+                     PipeLaunch L);
+void launch_tet_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                    const uint32_t* ecol, const uint32_t* eleft,
+                    const uint32_t* tk, const uint32_t* tedge,
+                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
+                    Queue q2, PipeLaunch L);
 void launch_tet_exact(const Stage& S, const QEntry* q, uint32_t nq,
                       const uint32_t* ecol, const uint32_t* eleft,
                       const uint32_t* tk, const uint32_t* tedge,
@@ -141,46 +155,49 @@ void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint4* tets, PipeLaunch L);
-/* emptiness (AC2) */
-void launch_tet_ac2(const Stage& S, int64_t nk, const uint4* tets,
-                    const uint32_t* perm, uint8_t* flag, QEntry* q,
-                    unsigned int* qn, uint32_t qcap, PipeLaunch L);
+
+/* ---- emptiness (k_ac2.hip) ---- */
+void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
+                        uint8_t* flag, Queue q1, PipeLaunch L);
+void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
+                        const uint4* tets, uint8_t* flag, Queue q2,
+                        PipeLaunch L);
 void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L);
-void launch_mark_tet_faces(int64_t nk, const uint4* tets, const uint8_t* kflag,
-                           const uint64_t* erow, const uint32_t* ecol,
-                           const uint64_t* trow, const uint32_t* tk,
-                           uint8_t* tri_in, PipeLaunch L);
-void launch_tri_ac2(const Stage& S, int64_t nt, const uint32_t* ecol,
-                    const uint32_t* eleft, const uint32_t* tk,
-                    const uint32_t* tedge, const uint8_t* tri_in,
-                    uint8_t* flag, QEntry* q, unsigned int* qn, uint32_t qcap,
-                    PipeLaunch L);
-void launch_tri_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                          const uint32_t* ecol, const uint32_t* eleft,
-                          const uint32_t* tk, const uint32_t* tedge,
-                          uint8_t* flag, PipeLaunch L);
-void launch_mark_tri_edges(int64_t nt, const uint32_t* ecol,
-                           const uint32_t* eleft, const uint32_t* tk,
-                           const uint32_t* tedge, const uint8_t* tflag,
-                           const uint64_t* erow, uint8_t* edge_in,
-                           PipeLaunch L);
-void launch_edge_ac2(const Stage& S, int64_t ne, const uint32_t* ecol,
-                     const uint32_t* eleft, const uint8_t* edge_in,
-                     uint8_t* flag, QEntry* q, unsigned int* qn,
-                     uint32_t qcap, PipeLaunch L);
-void launch_edge_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                           const uint32_t* ecol, const uint32_t* eleft,
-                           uint8_t* flag, PipeLaunch L);
-/* final outputs in original ids (flag-compacted) */
-void launch_emit_tets(int64_t nk, const uint4* tets, const uint8_t* flag,
-                      const uint64_t* pos, const uint32_t* perm, int32_t* out,
-                      PipeLaunch L);
-void launch_emit_tris(int64_t nt, const uint32_t* ecol, const uint32_t* eleft,
-                      const uint32_t* tk, const uint32_t* tedge,
-                      const uint8_t* flag, const uint64_t* pos,
-                      const uint32_t* perm, int32_t* out, PipeLaunch L);
+/* kind 0 triangles, 1 edges */
+void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        const uint8_t* in_flag, uint8_t* flag, Queue q1,
+                        PipeLaunch L);
+void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
+                        const uint32_t* ecol, const uint32_t* eleft,
+                        const uint32_t* tk, const uint32_t* tedge,
+                        uint8_t* flag, Queue q2, PipeLaunch L);
+void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
+                          uint32_t nq, const uint32_t* ecol,
+                          const uint32_t* eleft, const uint32_t* tk,
+                          const uint32_t* tedge, uint8_t* flag, PipeLaunch L);
+
+/* ---- closure + emission (k_closure.hip) ---- */
+void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
+                         const uint64_t* pos, uint4* out, PipeLaunch L);
+void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
+                        uint32_t* out, PipeLaunch L);
+void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
+                       const uint32_t* ecol, const uint64_t* trow,
+                       const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
+void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+                       const uint32_t* eleft, const uint32_t* tk,
+                       const uint32_t* tedge, const uint64_t* erow,
+                       uint8_t* edge_in, PipeLaunch L);
+void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L);
+void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+                      const uint32_t* eleft, const uint32_t* tk,
+                      const uint32_t* tedge, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L);
 void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint64_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L);
