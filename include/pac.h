@@ -48,7 +48,8 @@ typedef struct pac_options {
   void* stream;          /* hipStream_t to run on, NULL = library stream  */
   int32_t block;         /* threads per block override (0 = tuned)        */
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
-  int32_t reserved[6];
+  int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
+  int32_t reserved[5];
 } pac_options;
 
 typedef struct pac_result pac_result;

@@ -152,41 +152,70 @@ __device__ __forceinline__ bool tet_setup(const Stage& S, const float4 v[4],
                   quot_err(t.N.z, t.D, 2.0), box);
 }
 
-template <class R>
+/* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
+ * tets not settled within the budget are appended to `rest`.  Phase B
+ * (grid-stride over `rest`): the remaining pairs of those tets, skipping the
+ * first `budget` ones.  Splitting keeps lanes of a wave busy with similar
+ * amounts of work: most tets are rejected by one of their first few
+ * neighbours, while accepted tets must test every ball of their box. */
+template <class R, bool PHASE_B>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
-    Stage S, int64_t nk, const uint4* tets, uint8_t* flag, Queue q1) {
-  int64_t id = gtid();
+    Stage S, int64_t nk, const uint4* tets, uint8_t* flag, Queue q1,
+    uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  if (id < nk) {
+  int64_t total = PHASE_B ? (int64_t)*nrest : nk;
+  int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
+  for (int64_t w = gtid(); w < total; w += stride) {
+    int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     uint4 t = tets[id];
     float4 v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
     PairSS<R> P;
     CellBox box;
-    uint8_t res;
     if (!tet_setup(S, v, P, box)) {
-      push_q(q1, (uint32_t)id, 0xffffffffu);
-      res = 2;
-    } else {
-      bool pending = false, dead = false;
-      scan_box(S.g, S.cstart, box, [&](uint32_t p) {
-        if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
-        ++pairs;
-        ++evals;
-        int s = pairss_sign(P, S.ball[p]);
-        if (s == 0) {
-          pending = true;
-          push_q(q1, (uint32_t)id, p);
+      if (!PHASE_B) {
+        push_q(q1, (uint32_t)id, 0xffffffffu);
+        flag[id] = 2;
+      }
+      continue;
+    }
+    bool pending = false, dead = false, unfinished = false;
+    int seen = 0;
+    scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+      if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
+      if (PHASE_B) {
+        if (seen < budget) {
+          ++seen;
           return false;
         }
-        if (s * P.sden < 0) { /* pi_e < rho: violator */
-          dead = true;
-          return true;
-        }
+      } else if (budget > 0 && seen == budget) {
+        unfinished = true;
+        return true;
+      } else {
+        ++seen;
+      }
+      ++pairs;
+      ++evals;
+      int s = pairss_sign(P, S.ball[p]);
+      if (s == 0) {
+        pending = true;
+        push_q(q1, (uint32_t)id, p);
         return false;
-      });
-      res = dead ? 0 : (pending ? 2 : 1);
+      }
+      if (s * P.sden < 0) { /* pi_e < rho: violator */
+        dead = true;
+        return true;
+      }
+      return false;
+    });
+    if (PHASE_B) {
+      if (dead)
+        flag[id] = 0;
+      else if (pending)
+        flag[id] = 2;
+    } else {
+      flag[id] = dead ? 0 : (pending ? 2 : 1);
+      if (unfinished && !dead) rest[atomicAdd(nrest, 1u)] = (uint32_t)id;
     }
-    flag[id] = res;
   }
   flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
   flush_ctr(S.ctr, CTR_PAIRS + 0, pairs);
@@ -411,55 +440,80 @@ __device__ __forceinline__ bool low_setup(const Stage& S, const float4* v,
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
- * Simplices already in the closure (in_flag) pass without a test. */
-template <class R, int KIND>
+ * Simplices already in the closure (in_flag) pass without a test.  Same
+ * phase A / phase B split as the tet kernel. */
+template <class R, int KIND, bool PHASE_B>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     Stage S, int64_t nitems, const uint32_t* ecol, const uint32_t* eleft,
     const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
-    uint8_t* flag, Queue q1) {
-  int64_t id = gtid();
+    uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  if (id < nitems) {
-    uint8_t res = 1;
-    if (!in_flag[id]) {
-      uint32_t vid[3];
-      if (KIND == 0) {
-        uint32_t e = tedge[id];
-        vid[0] = eleft[e];
-        vid[1] = ecol[e];
-        vid[2] = tk[id];
-      } else {
-        vid[0] = eleft[id];
-        vid[1] = ecol[id];
-        vid[2] = vid[1];
-      }
-      float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
-      PairSS<R> P;
-      CellBox box;
-      if (!low_setup<R, KIND>(S, v, P, box)) {
-        push_q(q1, (uint32_t)id, 0xffffffffu);
-        res = 2;
-      } else {
-        bool pending = false, dead = false;
-        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
-          if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
-          ++pairs;
-          ++evals;
-          int s = pairss_sign(P, S.ball[p]);
-          if (s < 0) { /* power <= rho strictly: violator */
-            dead = true;
-            return true;
-          }
-          if (s == 0) {
-            pending = true;
-            push_q(q1, (uint32_t)id, p);
-          }
-          return false;
-        });
-        res = dead ? 0 : (pending ? 2 : 1);
-      }
+  int64_t total = PHASE_B ? (int64_t)*nrest : nitems;
+  int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
+  for (int64_t w = gtid(); w < total; w += stride) {
+    int64_t id = PHASE_B ? (int64_t)rest[w] : w;
+    if (!PHASE_B && in_flag[id]) {
+      flag[id] = 1;
+      continue;
     }
-    flag[id] = res;
+    uint32_t vid[3];
+    if (KIND == 0) {
+      uint32_t e = tedge[id];
+      vid[0] = eleft[e];
+      vid[1] = ecol[e];
+      vid[2] = tk[id];
+    } else {
+      vid[0] = eleft[id];
+      vid[1] = ecol[id];
+      vid[2] = vid[1];
+    }
+    float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+    PairSS<R> P;
+    CellBox box;
+    if (!low_setup<R, KIND>(S, v, P, box)) {
+      if (!PHASE_B) {
+        push_q(q1, (uint32_t)id, 0xffffffffu);
+        flag[id] = 2;
+      }
+      continue;
+    }
+    bool pending = false, dead = false, unfinished = false;
+    int seen = 0;
+    scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+      if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
+      if (PHASE_B) {
+        if (seen < budget) {
+          ++seen;
+          return false;
+        }
+      } else if (budget > 0 && seen == budget) {
+        unfinished = true;
+        return true;
+      } else {
+        ++seen;
+      }
+      ++pairs;
+      ++evals;
+      int s = pairss_sign(P, S.ball[p]);
+      if (s < 0) { /* power <= rho strictly: violator */
+        dead = true;
+        return true;
+      }
+      if (s == 0) {
+        pending = true;
+        push_q(q1, (uint32_t)id, p);
+      }
+      return false;
+    });
+    if (PHASE_B) {
+      if (dead)
+        flag[id] = 0;
+      else if (pending)
+        flag[id] = 2;
+    } else {
+      flag[id] = dead ? 0 : (pending ? 2 : 1);
+      if (unfinished && !dead) rest[atomicAdd(nrest, 1u)] = (uint32_t)id;
+    }
   }
   const int fam = KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2;
   flush_ctr(S.ctr, CTR_STAGE + fam * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
@@ -641,16 +695,25 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 
 
 /* ---------------- launchers ---------------- */
+static unsigned phase_b_grid() { return 2048; }
 void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
-                        uint8_t* flag, Queue q1, PipeLaunch L) {
+                        uint8_t* flag, Queue q1, uint32_t* rest,
+                        unsigned* nrest, int budget, PipeLaunch L) {
   if (nk <= 0) return;
-  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock));
-  if (S.staged)
-    hipLaunchKernelGGL(k_tet_ac2_hot<float>, gr, dim3(kBlock), 0, L.s, S, nk,
-                       tets, flag, q1);
-  else
-    hipLaunchKernelGGL(k_tet_ac2_hot<double>, gr, dim3(kBlock), 0, L.s, S, nk,
-                       tets, flag, q1);
+  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock)), gb(phase_b_grid());
+  if (S.staged) {
+    hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0, L.s,
+                       S, nk, tets, flag, q1, rest, nrest, budget);
+    if (budget > 0)
+      hipLaunchKernelGGL((k_tet_ac2_hot<float, true>), gb, dim3(kBlock), 0, L.s,
+                         S, nk, tets, flag, q1, rest, nrest, budget);
+  } else {
+    hipLaunchKernelGGL((k_tet_ac2_hot<double, false>), gr, dim3(kBlock), 0,
+                       L.s, S, nk, tets, flag, q1, rest, nrest, budget);
+    if (budget > 0)
+      hipLaunchKernelGGL((k_tet_ac2_hot<double, true>), gb, dim3(kBlock), 0,
+                         L.s, S, nk, tets, flag, q1, rest, nrest, budget);
+  }
 }
 void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
                         const uint4* tets, uint8_t* flag, Queue q2,
@@ -667,27 +730,42 @@ void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
                      dim3(kBlock), 0, L.s, S, q, nq, tets, perm, flag, S.reach,
                      S.margin);
 }
+template <class R, int KIND>
+static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint32_t* tk,
+                     const uint32_t* tedge, const uint8_t* in_flag,
+                     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
+                     int budget, PipeLaunch L) {
+  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock)), gb(phase_b_grid());
+  hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false>), gr, dim3(kBlock), 0, L.s,
+                     S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1, rest,
+                     nrest, budget);
+  if (budget > 0)
+    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true>), gb, dim3(kBlock), 0,
+                       L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
+                       q1, rest, nrest, budget);
+}
 void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         const uint8_t* in_flag, uint8_t* flag, Queue q1,
+                        uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L) {
   if (nitems <= 0) return;
-  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock));
   if (kind == 0) {
     if (S.staged)
-      hipLaunchKernelGGL((k_low_ac2_hot<float, 0>), gr, dim3(kBlock), 0, L.s, S,
-                         nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+      low_hot2<float, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                         rest, nrest, budget, L);
     else
-      hipLaunchKernelGGL((k_low_ac2_hot<double, 0>), gr, dim3(kBlock), 0, L.s,
-                         S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+      low_hot2<double, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                          rest, nrest, budget, L);
   } else {
     if (S.staged)
-      hipLaunchKernelGGL((k_low_ac2_hot<float, 1>), gr, dim3(kBlock), 0, L.s, S,
-                         nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+      low_hot2<float, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                         rest, nrest, budget, L);
     else
-      hipLaunchKernelGGL((k_low_ac2_hot<double, 1>), gr, dim3(kBlock), 0, L.s,
-                         S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1);
+      low_hot2<double, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                          rest, nrest, budget, L);
   }
 }
 void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,

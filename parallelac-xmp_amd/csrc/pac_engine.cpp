@@ -636,13 +636,19 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   };
 
   /* ---- tet emptiness ---- */
+  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
+  DBuf<uint32_t> rest(P, std::max({Kalpha, Talpha, Ealpha, (int64_t)1}));
   DBuf<uint8_t> kflag(P, Kalpha);
   if (opt.skip_tet_ac2) {
     fill_u8(kflag.get(), Kalpha, 1, s);
   } else if (Kalpha) {
     ladder(
         ac2_cap(Kalpha), KP_TET_AC2, KP_TET_AC2_EXACT,
-        [&](Queue q) { launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, L); },
+        [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
+          launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
+                             qn.get() + 2, budget, L);
+        },
         [&](const QEntry* q, uint32_t nq, Queue q2) {
           launch_tet_ac2_esc(SG, q, nq, tets.get(), kflag.get(), q2, L);
         },
@@ -674,8 +680,10 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     ladder(
         ac2_cap(Talpha), KP_TRI_AC2, -1,
         [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
           launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
-                             tedge.get(), tri_in.get(), tflag.get(), q, L);
+                             tedge.get(), tri_in.get(), tflag.get(), q,
+                             rest.get(), qn.get() + 2, budget, L);
         },
         [&](const QEntry* q, uint32_t nq, Queue q2) {
           launch_low_ac2_esc(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
@@ -708,8 +716,10 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     ladder(
         ac2_cap(Ealpha), KP_EDGE_AC2, -1,
         [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
           launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
-                             nullptr, edge_in.get(), eflag.get(), q, L);
+                             nullptr, edge_in.get(), eflag.get(), q,
+                             rest.get(), qn.get() + 2, budget, L);
         },
         [&](const QEntry* q, uint32_t nq, Queue q2) {
           launch_low_ac2_esc(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,

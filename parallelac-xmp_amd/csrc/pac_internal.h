@@ -157,8 +157,11 @@ void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
                         uint4* tets, PipeLaunch L);
 
 /* ---- emptiness (k_ac2.hip) ---- */
+/* rest / nrest: scratch list of simplices left for phase B (capacity =
+ * number of items); budget = phase A pair budget (0: single phase) */
 void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
-                        uint8_t* flag, Queue q1, PipeLaunch L);
+                        uint8_t* flag, Queue q1, uint32_t* rest,
+                        unsigned* nrest, int budget, PipeLaunch L);
 void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L);
@@ -170,6 +173,7 @@ void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         const uint8_t* in_flag, uint8_t* flag, Queue q1,
+                        uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L);
 void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
                         const uint32_t* ecol, const uint32_t* eleft,

@@ -40,7 +40,8 @@ class Options(ctypes.Structure):
                 ("skip_tet_ac2", ctypes.c_int32), ("order", ctypes.c_int32),
                 ("device_inputs", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
 
 
 class Stats(ctypes.Structure):
@@ -146,7 +147,7 @@ def _check(rc):
 
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
-            profile=False):
+            profile=False, ac2_budget=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -157,6 +158,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.device = device
     o.stream = stream
     o.profile = int(bool(profile))
+    o.ac2_budget = int(ac2_budget)
     return o
 
 
