@@ -135,21 +135,48 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 }
 
 
-/* per-simplex setup at precision R: certified centre box and the
- * semi-static pair constants. */
+/* per-simplex setup at precision R: D, N with a-priori error bounds, the
+ * certified centre box and the semi-static pair constants. */
 template <class R>
 __device__ __forceinline__ bool tet_setup(const Stage& S, const float4 v[4],
                                           PairSS<R>& P, CellBox& box) {
-  TetQ<FE<R>> t;
-  tetq_init(t, v);
-  if (!pairss_init(P, t.pa, t.wa, t.D, t.N)) return false;
-  double D2 = 2.0 * (double)t.D.v;
-  double cx = (double)v[0].x + (double)t.N.x.v / D2;
-  double cy = (double)v[0].y + (double)t.N.y.v / D2;
-  double cz = (double)v[0].z + (double)t.N.z.v / D2;
-  return make_box(S.g, S.reach, S.margin, cx, cy, cz,
-                  quot_err(t.N.x, t.D, 2.0), quot_err(t.N.y, t.D, 2.0),
-                  quot_err(t.N.z, t.D, 2.0), box);
+  const R ax = v[0].x, ay = v[0].y, az = v[0].z;
+  R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
+  R vx = (R)v[2].x - ax, vy = (R)v[2].y - ay, vz = (R)v[2].z - az;
+  R tx = (R)v[3].x - ax, ty = (R)v[3].y - ay, tz = (R)v[3].z - az;
+  R wa = (R)v[0].w * (R)v[0].w;
+  R w1 = (R)v[1].w * (R)v[1].w, w2 = (R)v[2].w * (R)v[2].w,
+    w3 = (R)v[3].w * (R)v[3].w;
+  SS<R> vt[3], tu[3], uv[3];
+  cross_ss(vx, vy, vz, tx, ty, tz, vt);
+  cross_ss(tx, ty, tz, ux, uy, uz, tu);
+  cross_ss(ux, uy, uz, vx, vy, vz, uv);
+  SS<R> D{ux * vt[0].v + uy * vt[1].v + uz * vt[2].v,
+          fabs(ux) * vt[0].p + fabs(uy) * vt[1].p + fabs(uz) * vt[2].p};
+  R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz,
+    tt = tx * tx + ty * ty + tz * tz;
+  SS<R> A{(uu + wa) - w1, (uu + wa) + w1}, B{(vv + wa) - w2, (vv + wa) + w2},
+      C{(tt + wa) - w3, (tt + wa) + w3};
+  SS<R> N[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    N[q] = SS<R>{A.v * vt[q].v + B.v * tu[q].v + C.v * uv[q].v,
+                 fabs(A.v) * vt[q].p + fabs(B.v) * tu[q].p + fabs(C.v) * uv[q].p};
+  /* A, B, C permanents exceed |A| etc.; use them for the N bound */
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    N[q].p = A.p * vt[q].p + B.p * tu[q].p + C.p * uv[q].p;
+  FE<R> Df = ss_fe(D, 6);
+  V3<FE<R>> Nf{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
+  V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
+  FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
+  if (!pairss_init(P, pa, waf, Df, Nf)) return false;
+  double D2 = 2.0 * (double)Df.v;
+  double cx = (double)v[0].x + (double)Nf.x.v / D2;
+  double cy = (double)v[0].y + (double)Nf.y.v / D2;
+  double cz = (double)v[0].z + (double)Nf.z.v / D2;
+  return make_box(S.g, S.reach, S.margin, cx, cy, cz, quot_err(Nf.x, Df, 2.0),
+                  quot_err(Nf.y, Df, 2.0), quot_err(Nf.z, Df, 2.0), box);
 }
 
 /* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
@@ -180,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+    scan_box4(S.g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
       if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -195,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       }
       ++pairs;
       ++evals;
-      int s = pairss_sign(P, S.ball[p]);
+      int s = pairss_sign(P, E);
       if (s == 0) {
         pending = true;
         push_q(q1, (uint32_t)id, p);
@@ -424,19 +451,56 @@ __device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
 template <class R, int KIND>
 __device__ __forceinline__ bool low_setup(const Stage& S, const float4* v,
                                           PairSS<R>& P, CellBox& box) {
-  LowQ<FE<R>> t;
-  if (KIND == 0)
-    triq_init(t, v[0], v[1], v[2]);
-  else
-    edgeq_init(t, v[0], v[1]);
-  if (!pairss_init(P, t.pa, t.wa, t.den, t.num) || P.sden < 0) return false;
-  double d2 = 2.0 * (double)t.den.v;
-  double cx = (double)v[0].x + (double)t.num.x.v / d2;
-  double cy = (double)v[0].y + (double)t.num.y.v / d2;
-  double cz = (double)v[0].z + (double)t.num.z.v / d2;
+  const R ax = v[0].x, ay = v[0].y, az = v[0].z;
+  R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
+  R wa = (R)v[0].w * (R)v[0].w, w1 = (R)v[1].w * (R)v[1].w;
+  R uu = ux * ux + uy * uy + uz * uz;
+  SS<R> A{(uu + wa) - w1, (uu + wa) + w1};
+  FE<R> den;
+  V3<FE<R>> num;
+  if (KIND == 0) {
+    R vx = (R)v[2].x - ax, vy = (R)v[2].y - ay, vz = (R)v[2].z - az;
+    R w2 = (R)v[2].w * (R)v[2].w;
+    R vv = vx * vx + vy * vy + vz * vz;
+    SS<R> B{(vv + wa) - w2, (vv + wa) + w2};
+    SS<R> n[3];
+    cross_ss(ux, uy, uz, vx, vy, vz, n);
+    SS<R> nn{n[0].v * n[0].v + n[1].v * n[1].v + n[2].v * n[2].v,
+             n[0].p * n[0].p + n[1].p * n[1].p + n[2].p * n[2].p};
+    /* X = A (v x n) - B (u x n) */
+    R nx = n[0].v, ny = n[1].v, nz = n[2].v;
+    R NX = n[0].p, NY = n[1].p, NZ = n[2].p;
+    R vX = fabs(vx), vY = fabs(vy), vZ = fabs(vz), uX = fabs(ux), uY = fabs(uy),
+      uZ = fabs(uz);
+    SS<R> vn[3] = {{vy * nz - vz * ny, vY * NZ + vZ * NY},
+                   {vz * nx - vx * nz, vZ * NX + vX * NZ},
+                   {vx * ny - vy * nx, vX * NY + vY * NX}};
+    SS<R> un[3] = {{uy * nz - uz * ny, uY * NZ + uZ * NY},
+                   {uz * nx - ux * nz, uZ * NX + uX * NZ},
+                   {ux * ny - uy * nx, uX * NY + uY * NX}};
+    SS<R> X[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      X[q] = SS<R>{A.v * vn[q].v - B.v * un[q].v, A.p * vn[q].p + B.p * un[q].p};
+    den = ss_fe(nn, 8);
+    num = V3<FE<R>>{ss_fe(X[0], 12), ss_fe(X[1], 12), ss_fe(X[2], 12)};
+  } else {
+    SS<R> uuS{uu, uu};
+    den = ss_fe(uuS, 4);
+    num = V3<FE<R>>{ss_fe(SS<R>{A.v * ux, A.p * fabs(ux)}, 8),
+                    ss_fe(SS<R>{A.v * uy, A.p * fabs(uy)}, 8),
+                    ss_fe(SS<R>{A.v * uz, A.p * fabs(uz)}, 8)};
+  }
+  V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
+  FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
+  if (!pairss_init(P, pa, waf, den, num) || P.sden < 0) return false;
+  double d2 = 2.0 * (double)den.v;
+  double cx = (double)v[0].x + (double)num.x.v / d2;
+  double cy = (double)v[0].y + (double)num.y.v / d2;
+  double cz = (double)v[0].z + (double)num.z.v / d2;
   return make_box(S.g, S.reach, S.margin, cx, cy, cz,
-                  quot_err(t.num.x, t.den, 2.0), quot_err(t.num.y, t.den, 2.0),
-                  quot_err(t.num.z, t.den, 2.0), box);
+                  quot_err(num.x, den, 2.0), quot_err(num.y, den, 2.0),
+                  quot_err(num.z, den, 2.0), box);
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
@@ -479,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+    scan_box4(S.g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
       if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -494,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       }
       ++pairs;
       ++evals;
-      int s = pairss_sign(P, S.ball[p]);
+      int s = pairss_sign(P, E);
       if (s < 0) { /* power <= rho strictly: violator */
         dead = true;
         return true;

@@ -285,131 +285,6 @@ void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
                      dim3(kBlock), 0, L.s, S, cnt);
 }
 
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_edge_fill(
-    Stage S, const uint64_t* off, uint32_t* cand, uint8_t* flag,
-    uint32_t* keep, Queue q1) {
-  int64_t i = gtid();
-  unsigned evals = 0;
-  if (i < S.n) {
-    uint32_t c = S.cellk[i];
-    int gx, gy, gz;
-    cell_xyz(c, S.g, gx, gy, gz);
-    const int k = S.g.k_enum;
-    float4 a = S.ball[i];
-    uint64_t o = off[i];
-    uint32_t kept = 0;
-    for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
-      for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
-        int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
-        int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
-        int64_t c0 = row + x0, c1 = row + x1;
-        if (c1 < (int64_t)c) continue;
-        uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
-        if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
-        for (uint32_t j = b; j < e; ++j) {
-          float4 bj = S.ball[j];
-          int r = same_pos(a, bj) ? REJECT : edge_code<FE<R>>(a, bj, S.alpha);
-          ++evals;
-          cand[o] = j;
-          flag[o] = (uint8_t)r;
-          kept += (r == KEEP);
-          if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)i);
-          ++o;
-        }
-      }
-    keep[i] = kept;
-  }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_EDGE * 4 + StageOf<FE<R>>::v, evals);
-}
-void launch_edge_fill(const Stage& S, const uint64_t* off, uint32_t* cand,
-                      uint8_t* flag, uint32_t* keep, Queue q1, PipeLaunch L) {
-  if (S.n <= 0) return;
-  dim3 gr((unsigned)((S.n + kBlock - 1) / kBlock));
-  if (S.staged)
-    hipLaunchKernelGGL(k_edge_fill<float>, gr, dim3(kBlock), 0, L.s, S, off,
-                       cand, flag, keep, q1);
-  else
-    hipLaunchKernelGGL(k_edge_fill<double>, gr, dim3(kBlock), 0, L.s, S, off,
-                       cand, flag, keep, q1);
-}
-
-/* escalation FP64 -> double-double for queued candidates (staged path) */
-__global__ __launch_bounds__(kBlock) void k_edge_esc(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* cand,
-    uint8_t* flag, uint32_t* keep, Queue q2) {
-  int64_t t = gtid();
-  StageCnt sc;
-  if (t < nq) {
-    QEntry e = q[t];
-    float4 a = S.ball[e.aux], b = S.ball[cand[e.item]];
-    sc.c[1]++;
-    int r = edge_code<FE<double>>(a, b, S.alpha);
-    if (r == UNDEC) {
-      sc.c[2]++;
-      r = edge_code<FEDD>(a, b, S.alpha);
-    }
-    if (r == UNDEC) {
-      push_q(q2, e.item, e.aux);
-    } else {
-      flag[e.item] = (uint8_t)r;
-      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
-    }
-  }
-  sc.flush(S.ctr, FAM_EDGE);
-}
-void launch_edge_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                     const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                     Queue q2, PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_edge_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
-                     0, L.s, S, q, nq, cand, flag, keep, q2);
-}
-
-__global__ __launch_bounds__(kBlock) void k_edge_exact(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* cand,
-    uint8_t* flag, uint32_t* keep) {
-  int64_t t = gtid();
-  unsigned cnt = 0;
-  if (t < nq) {
-    QEntry e = q[t];
-    int r = edge_exact(S.ball[e.aux], S.ball[cand[e.item]], S.ex);
-    flag[e.item] = (uint8_t)r;
-    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
-    cnt = 1;
-  }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_EDGE * 4 + 3, cnt);
-}
-void launch_edge_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                       const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                       PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_edge_exact, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, cand, flag, keep);
-}
-
-__global__ __launch_bounds__(kBlock) void k_edge_compact(
-    int64_t n, const uint64_t* off, const uint32_t* cand, const uint8_t* flag,
-    const uint64_t* erow, uint32_t* ecol, uint32_t* eleft) {
-  int64_t i = gtid();
-  if (i >= n) return;
-  uint64_t w = erow[i];
-  for (uint64_t p = off[i]; p < off[i + 1]; ++p)
-    if (flag[p] == KEEP) {
-      ecol[w] = cand[p];
-      eleft[w] = (uint32_t)i;
-      ++w;
-    }
-}
-void launch_edge_compact(int64_t n, const uint64_t* off, const uint32_t* cand,
-                         const uint8_t* flag, const uint64_t* erow,
-                         uint32_t* ecol, uint32_t* eleft, PipeLaunch L) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_edge_compact, dim3((unsigned)((n + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, off, cand, flag, erow, ecol,
-                     eleft);
-}
-
 /* ------------------------------------------------------------------ */
 /* triangles: k in row(i) after j, intersected with row(j)             */
 /* ------------------------------------------------------------------ */
@@ -436,138 +311,6 @@ void launch_tri_count(const Stage&, int64_t ne, const uint64_t* erow,
   if (ne <= 0) return;
   hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
-}
-
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_tri_fill(
-    Stage S, int64_t ne, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* off, uint32_t* cand, uint8_t* flag,
-    uint32_t* keep, Queue q1) {
-  int64_t q = gtid();
-  unsigned evals = 0;
-  if (q < ne) {
-    uint32_t i = eleft[q], j = ecol[q];
-    float4 A = S.ball[i], B = S.ball[j];
-    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-    uint64_t b = erow[j], be = erow[j + 1];
-    uint64_t o = off[q];
-    uint32_t kept = 0;
-    while (a < ae && b < be) {
-      uint32_t x = ecol[a], y = ecol[b];
-      if (x == y) {
-        int r = tri_code<FE<R>>(A, B, S.ball[x], S.alpha);
-        ++evals;
-        cand[o] = x;
-        flag[o] = (uint8_t)r;
-        kept += (r == KEEP);
-        if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)q);
-        ++o;
-      }
-      a += (x <= y);
-      b += (y <= x);
-    }
-    keep[q] = kept;
-  }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TRI * 4 + StageOf<FE<R>>::v, evals);
-}
-void launch_tri_fill(const Stage& S, int64_t ne, const uint64_t* erow,
-                     const uint32_t* ecol, const uint32_t* eleft,
-                     const uint64_t* off, uint32_t* cand, uint8_t* flag,
-                     uint32_t* keep, Queue q1, PipeLaunch L) {
-  if (ne <= 0) return;
-  dim3 gr((unsigned)((ne + kBlock - 1) / kBlock));
-  if (S.staged)
-    hipLaunchKernelGGL(k_tri_fill<float>, gr, dim3(kBlock), 0, L.s, S, ne, erow,
-                       ecol, eleft, off, cand, flag, keep, q1);
-  else
-    hipLaunchKernelGGL(k_tri_fill<double>, gr, dim3(kBlock), 0, L.s, S, ne,
-                       erow, ecol, eleft, off, cand, flag, keep, q1);
-}
-
-__global__ __launch_bounds__(kBlock) void k_tri_esc(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
-    const uint32_t* eleft, const uint32_t* cand, uint8_t* flag,
-    uint32_t* keep, Queue q2) {
-  int64_t t = gtid();
-  StageCnt sc;
-  if (t < nq) {
-    QEntry e = q[t];
-    float4 a = S.ball[eleft[e.aux]], b = S.ball[ecol[e.aux]],
-           c = S.ball[cand[e.item]];
-    sc.c[1]++;
-    int r = tri_code<FE<double>>(a, b, c, S.alpha);
-    if (r == UNDEC) {
-      sc.c[2]++;
-      r = tri_code<FEDD>(a, b, c, S.alpha);
-    }
-    if (r == UNDEC) {
-      push_q(q2, e.item, e.aux);
-    } else {
-      flag[e.item] = (uint8_t)r;
-      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
-    }
-  }
-  sc.flush(S.ctr, FAM_TRI);
-}
-void launch_tri_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                    const uint32_t* ecol, const uint32_t* eleft,
-                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                    Queue q2, PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tri_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
-                     0, L.s, S, q, nq, ecol, eleft, cand, flag, keep, q2);
-}
-
-__global__ __launch_bounds__(kBlock) void k_tri_exact(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
-    const uint32_t* eleft, const uint32_t* cand, uint8_t* flag,
-    uint32_t* keep) {
-  int64_t t = gtid();
-  unsigned cnt = 0;
-  if (t < nq) {
-    QEntry e = q[t];
-    uint32_t i = eleft[e.aux], j = ecol[e.aux], k = cand[e.item];
-    int r = tri_exact(S.ball[i], S.ball[j], S.ball[k], S.ex);
-    flag[e.item] = (uint8_t)r;
-    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
-    cnt = 1;
-  }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TRI * 4 + 3, cnt);
-}
-void launch_tri_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                      PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tri_exact, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, ecol, eleft, cand, flag,
-                     keep);
-}
-
-/* generic segmented compaction: segment s owns candidates [off[s],off[s+1]) */
-__global__ __launch_bounds__(kBlock) void k_seg_compact(
-    int64_t nseg, const uint64_t* off, const uint32_t* cand,
-    const uint8_t* flag, const uint64_t* orow, uint32_t* ocol,
-    uint32_t* oseg) {
-  int64_t s = gtid();
-  if (s >= nseg) return;
-  uint64_t w = orow[s];
-  for (uint64_t p = off[s]; p < off[s + 1]; ++p)
-    if (flag[p] == KEEP) {
-      ocol[w] = cand[p];
-      if (oseg) oseg[w] = (uint32_t)s;
-      ++w;
-    }
-}
-void launch_seg_compact(int64_t nseg, const uint64_t* off,
-                        const uint32_t* cand, const uint8_t* flag,
-                        const uint64_t* orow, uint32_t* ocol, uint32_t* oseg,
-                        PipeLaunch L) {
-  if (nseg <= 0) return;
-  hipLaunchKernelGGL(k_seg_compact,
-                     dim3((unsigned)((nseg + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nseg, off, cand, flag, orow, ocol,
-                     oseg);
 }
 
 /* ------------------------------------------------------------------ */
@@ -633,151 +376,318 @@ void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
                      tedge, cnt);
 }
 
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_tet_fill(
-    Stage S, int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
-    const uint32_t* tedge, const uint64_t* off, uint32_t* cand, uint8_t* flag,
-    uint32_t* keep, Queue q1) {
-  int64_t q = gtid();
-  unsigned evals = 0;
-  if (q < nt) {
-    uint32_t i, j, k;
-    TetWalk w;
-    uint64_t o = off[q];
-    uint32_t kept = 0;
-    if (tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) {
-      float4 A = S.ball[i], B = S.ball[j], C = S.ball[k];
-      while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
-        uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
-        uint32_t m = max(x, max(y, z));
-        if (x == m && y == m && z == m) {
-          int r = tet_code<FE<R>>(A, B, C, S.ball[m], S.alpha);
-          ++evals;
-          cand[o] = m;
-          flag[o] = (uint8_t)r;
-          kept += (r == KEEP);
-          if (r == UNDEC) push_q(q1, (uint32_t)o, (uint32_t)q);
-          ++o;
-          ++w.a;
-          ++w.b;
-          ++w.c;
-        } else {
-          w.a += (x < m);
-          w.b += (y < m);
-          w.c += (z < m);
-        }
+/* ------------------------------------------------------------------ */
+/* enumeration: candidates as (segment, new vertex) pairs              */
+/* ------------------------------------------------------------------ */
+__global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
+                                                      const uint64_t* off,
+                                                      uint32_t* cval,
+                                                      uint32_t* cseg) {
+  int64_t i = gtid();
+  if (i >= S.n) return;
+  uint32_t c = S.cellk[i];
+  int gx, gy, gz;
+  cell_xyz(c, S.g, gx, gy, gz);
+  const int k = S.g.k_enum;
+  uint64_t o = off[i];
+  for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
+    for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
+      int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+      int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
+      int64_t c0 = row + x0, c1 = row + x1;
+      if (c1 < (int64_t)c) continue;
+      uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
+      if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
+      for (uint32_t j = b; j < e; ++j, ++o) {
+        cval[o] = j;
+        cseg[o] = (uint32_t)i;
       }
     }
-    keep[q] = kept;
-  }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TET * 4 + StageOf<FE<R>>::v, evals);
 }
-void launch_tet_fill(const Stage& S, int64_t nt, const uint64_t* erow,
-                     const uint32_t* ecol, const uint32_t* eleft,
-                     const uint64_t* trow, const uint32_t* tk,
-                     const uint32_t* tedge, const uint64_t* off,
-                     uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q1,
-                     PipeLaunch L) {
-  if (nt <= 0) return;
-  dim3 gr((unsigned)((nt + kBlock - 1) / kBlock));
-  if (S.staged)
-    hipLaunchKernelGGL(k_tet_fill<float>, gr, dim3(kBlock), 0, L.s, S, nt, erow,
-                       ecol, eleft, trow, tk, tedge, off, cand, flag, keep, q1);
-  else
-    hipLaunchKernelGGL(k_tet_fill<double>, gr, dim3(kBlock), 0, L.s, S, nt,
-                       erow, ecol, eleft, trow, tk, tedge, off, cand, flag,
-                       keep, q1);
+void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
+                      uint32_t* cseg, PipeLaunch L) {
+  if (S.n <= 0) return;
+  hipLaunchKernelGGL(k_edge_enum, dim3((unsigned)((S.n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, S, off, cval, cseg);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tet_esc(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
-    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
-    const uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q2) {
+__global__ __launch_bounds__(kBlock) void k_tri_enum(
+    int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
+    uint32_t* cseg) {
+  int64_t q = gtid();
+  if (q >= ne) return;
+  uint32_t i = eleft[q], j = ecol[q];
+  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+  uint64_t b = erow[j], be = erow[j + 1];
+  uint64_t o = off[q];
+  while (a < ae && b < be) {
+    uint32_t x = ecol[a], y = ecol[b];
+    if (x == y) {
+      cval[o] = x;
+      cseg[o] = (uint32_t)q;
+      ++o;
+    }
+    a += (x <= y);
+    b += (y <= x);
+  }
+}
+void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
+                     uint32_t* cseg, PipeLaunch L) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(k_tri_enum, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, off, cval,
+                     cseg);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tet_enum(
+    int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
+    const uint32_t* tedge, const uint64_t* off, uint32_t* cval,
+    uint32_t* cseg) {
+  int64_t q = gtid();
+  if (q >= nt) return;
+  uint32_t i, j, k;
+  TetWalk w;
+  uint64_t o = off[q];
+  if (!tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) return;
+  while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
+    uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
+    uint32_t m = max(x, max(y, z));
+    if (x == m && y == m && z == m) {
+      cval[o] = m;
+      cseg[o] = (uint32_t)q;
+      ++o;
+      ++w.a;
+      ++w.b;
+      ++w.c;
+    } else {
+      w.a += (x < m);
+      w.b += (y < m);
+      w.c += (z < m);
+    }
+  }
+}
+void launch_tet_enum(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint64_t* trow,
+                     const uint32_t* tk, const uint32_t* tedge,
+                     const uint64_t* off, uint32_t* cval, uint32_t* cseg,
+                     PipeLaunch L) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, erow, ecol, eleft, trow, tk, tedge,
+                     off, cval, cseg);
+}
+
+/* ------------------------------------------------------------------ */
+/* alpha filters, one thread per candidate (uniform work per lane)     */
+/* ------------------------------------------------------------------ */
+/* vertices of candidate (seg, val): KIND 1 edge (seg = ball), 2 triangle
+ * (seg = alpha edge), 3 tet (seg = alpha triangle) */
+template <int KIND>
+__device__ __forceinline__ void cand_balls(const Stage& S, const CandCtx& C,
+                                           uint32_t seg, uint32_t val,
+                                           float4* v) {
+  if (KIND == 1) {
+    v[0] = S.ball[seg];
+    v[1] = S.ball[val];
+  } else if (KIND == 2) {
+    v[0] = S.ball[C.eleft[seg]];
+    v[1] = S.ball[C.ecol[seg]];
+    v[2] = S.ball[val];
+  } else {
+    uint32_t e = C.tedge[seg];
+    v[0] = S.ball[C.eleft[e]];
+    v[1] = S.ball[C.ecol[e]];
+    v[2] = S.ball[C.tk[seg]];
+    v[3] = S.ball[val];
+  }
+}
+template <int KIND, class T>
+__device__ __forceinline__ int filter_code(const float4* v, float alpha) {
+  if (KIND == 1) return same_pos(v[0], v[1]) ? REJECT : edge_code<T>(v[0], v[1], alpha);
+  if (KIND == 2) return tri_code<T>(v[0], v[1], v[2], alpha);
+  return tet_code<T>(v[0], v[1], v[2], v[3], alpha);
+}
+template <int KIND>
+__device__ __forceinline__ int filter_exact(const float4* v,
+                                            const ExactScale& X) {
+  if (KIND == 1) return edge_exact(v[0], v[1], X);
+  if (KIND == 2) return tri_exact(v[0], v[1], v[2], X);
+  return tet_exact(v[0], v[1], v[2], v[3], X);
+}
+constexpr int fam_of(int KIND) {
+  return KIND == 1 ? FAM_EDGE : (KIND == 2 ? FAM_TRI : FAM_TET);
+}
+
+template <int KIND, class R>
+__global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
+                                                   int64_t nc,
+                                                   const uint32_t* cval,
+                                                   const uint32_t* cseg,
+                                                   uint8_t* flag, Queue q1) {
+  int64_t o = gtid();
+  unsigned evals = 0;
+  if (o < nc) {
+    float4 v[4];
+    uint32_t seg = cseg[o];
+    cand_balls<KIND>(S, C, seg, cval[o], v);
+    int r = filter_code<KIND, FE<R>>(v, S.alpha);
+    evals = 1;
+    flag[o] = (uint8_t)r;
+    if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
+  }
+  flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + StageOf<FE<R>>::v, evals);
+}
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
+                                                       const QEntry* q,
+                                                       uint32_t nq,
+                                                       const uint32_t* cval,
+                                                       uint8_t* flag,
+                                                       Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
   if (t < nq) {
     QEntry e = q[t];
-    uint32_t eij = tedge[e.aux];
-    float4 a = S.ball[eleft[eij]], b = S.ball[ecol[eij]], c = S.ball[tk[e.aux]],
-           d = S.ball[cand[e.item]];
+    float4 v[4];
+    cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
     sc.c[1]++;
-    int r = tet_code<FE<double>>(a, b, c, d, S.alpha);
+    int r = filter_code<KIND, FE<double>>(v, S.alpha);
     if (r == UNDEC) {
       sc.c[2]++;
-      r = tet_code<FEDD>(a, b, c, d, S.alpha);
+      r = filter_code<KIND, FEDD>(v, S.alpha);
     }
-    if (r == UNDEC) {
+    if (r == UNDEC)
       push_q(q2, e.item, e.aux);
-    } else {
+    else
       flag[e.item] = (uint8_t)r;
-      if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
-    }
   }
-  sc.flush(S.ctr, FAM_TET);
+  sc.flush(S.ctr, fam_of(KIND));
 }
-void launch_tet_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                    const uint32_t* ecol, const uint32_t* eleft,
-                    const uint32_t* tk, const uint32_t* tedge,
-                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                    Queue q2, PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tet_esc, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock),
-                     0, L.s, S, q, nq, ecol, eleft, tk, tedge, cand, flag, keep,
-                     q2);
-}
-
-__global__ __launch_bounds__(kBlock) void k_tet_exact(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
-    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
-    const uint32_t* cand, uint8_t* flag, uint32_t* keep) {
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
+                                                         const QEntry* q,
+                                                         uint32_t nq,
+                                                         const uint32_t* cval,
+                                                         uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0;
   if (t < nq) {
     QEntry e = q[t];
-    uint32_t eij = tedge[e.aux];
-    uint32_t i = eleft[eij], j = ecol[eij], k = tk[e.aux], l = cand[e.item];
-    int r = tet_exact(S.ball[i], S.ball[j], S.ball[k], S.ball[l], S.ex);
-    flag[e.item] = (uint8_t)r;
-    if (r == KEEP) atomicAdd(&keep[e.aux], 1u);
+    float4 v[4];
+    cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
+    flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
     cnt = 1;
   }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TET * 4 + 3, cnt);
-}
-void launch_tet_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      const uint32_t* tk, const uint32_t* tedge,
-                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                      PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tet_exact, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, ecol, eleft, tk, tedge,
-                     cand, flag, keep);
+  flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tet_compact(
-    int64_t nt, const uint64_t* off, const uint32_t* cand, const uint8_t* flag,
-    const uint64_t* orow, const uint32_t* ecol, const uint32_t* eleft,
-    const uint32_t* tk, const uint32_t* tedge, uint4* tets) {
-  int64_t q = gtid();
-  if (q >= nt) return;
-  uint64_t w = orow[q];
-  uint64_t p = off[q], pe = off[q + 1];
-  if (p == pe) return;
-  uint32_t eij = tedge[q];
-  uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
-  for (; p < pe; ++p)
-    if (flag[p] == KEEP) tets[w++] = make_uint4(i, j, k, cand[p]);
+void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
+                   const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
+                   Queue q1, PipeLaunch L) {
+  if (nc <= 0) return;
+  dim3 gr((unsigned)((nc + kBlock - 1) / kBlock));
+#define PAC_FILTER(K)                                                        \
+  if (S.staged)                                                              \
+    hipLaunchKernelGGL((k_filter<K, float>), gr, dim3(kBlock), 0, L.s, S, C, \
+                       nc, cval, cseg, flag, q1);                            \
+  else                                                                       \
+    hipLaunchKernelGGL((k_filter<K, double>), gr, dim3(kBlock), 0, L.s, S, C, \
+                       nc, cval, cseg, flag, q1);
+  if (kind == 1) {
+    PAC_FILTER(1)
+  } else if (kind == 2) {
+    PAC_FILTER(2)
+  } else {
+    PAC_FILTER(3)
+  }
+#undef PAC_FILTER
 }
-void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
-                        const uint8_t* flag, const uint64_t* orow,
-                        const uint32_t* ecol, const uint32_t* eleft,
-                        const uint32_t* tk, const uint32_t* tedge,
-                        uint4* tets, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_tet_compact, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, off, cand, flag, orow, ecol,
-                     eleft, tk, tedge, tets);
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, const QEntry* q,
+                       uint32_t nq, const uint32_t* cval, uint8_t* flag,
+                       Queue q2, PipeLaunch L) {
+  if (!nq) return;
+  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (kind == 1)
+    hipLaunchKernelGGL(k_filter_esc<1>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag, q2);
+  else if (kind == 2)
+    hipLaunchKernelGGL(k_filter_esc<2>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag, q2);
+  else
+    hipLaunchKernelGGL(k_filter_esc<3>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag, q2);
+}
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, const QEntry* q,
+                         uint32_t nq, const uint32_t* cval, uint8_t* flag,
+                         PipeLaunch L) {
+  if (!nq) return;
+  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (kind == 1)
+    hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag);
+  else if (kind == 2)
+    hipLaunchKernelGGL(k_filter_exact<2>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag);
+  else
+    hipLaunchKernelGGL(k_filter_exact<3>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+                       cval, flag);
+}
+
+/* ------------------------------------------------------------------ */
+/* survivors: segment starts and flat stable compaction                */
+/* ------------------------------------------------------------------ */
+/* orow[s] = number of survivors before segment s = pos[off[s]] */
+__global__ __launch_bounds__(kBlock) void k_seg_starts(int64_t nseg,
+                                                       const uint64_t* off,
+                                                       const uint64_t* pos,
+                                                       uint64_t* orow) {
+  int64_t s = gtid();
+  if (s > nseg) return;
+  orow[s] = pos[off[s]];
+}
+void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
+                       uint64_t* orow, PipeLaunch L) {
+  hipLaunchKernelGGL(k_seg_starts,
+                     dim3((unsigned)((nseg + 1 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nseg, off, pos, orow);
+}
+/* KIND 1: (ecol, eleft) = (val, seg); 2: (tk, tedge) = (val, seg);
+ * 3: tets = (i, j, k, l) */
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_compact_flat(
+    int64_t nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
+    const uint8_t* flag, const uint64_t* pos, uint32_t* out_val,
+    uint32_t* out_seg, uint4* out_tet) {
+  int64_t o = gtid();
+  if (o >= nc || flag[o] != KEEP) return;
+  uint64_t w = pos[o];
+  uint32_t seg = cseg[o], val = cval[o];
+  if (KIND < 3) {
+    out_val[w] = val;
+    out_seg[w] = seg;
+  } else {
+    uint32_t e = C.tedge[seg];
+    out_tet[w] = make_uint4(C.eleft[e], C.ecol[e], C.tk[seg], val);
+  }
+}
+void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
+                         const uint32_t* cseg, const uint8_t* flag,
+                         const uint64_t* pos, uint32_t* out_val,
+                         uint32_t* out_seg, uint4* out_tet, PipeLaunch L) {
+  if (nc <= 0) return;
+  dim3 gr((unsigned)((nc + kBlock - 1) / kBlock));
+  if (kind == 1)
+    hipLaunchKernelGGL(k_compact_flat<1>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
+                       cseg, flag, pos, out_val, out_seg, out_tet);
+  else if (kind == 2)
+    hipLaunchKernelGGL(k_compact_flat<2>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
+                       cseg, flag, pos, out_val, out_seg, out_tet);
+  else
+    hipLaunchKernelGGL(k_compact_flat<3>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
+                       cseg, flag, pos, out_val, out_seg, out_tet);
 }
 
 }  // namespace pac

@@ -179,6 +179,54 @@ __device__ __forceinline__ void scan_box(const GridP& g,
     }
 }
 
+/* Same traversal, balls fetched four at a time so that each row costs one
+ * round trip to memory instead of one per ball; visit(p, ball) returns
+ * true to stop.  Indices past the end of a range are clamped to its last
+ * element (always a valid address) and not visited. */
+template <class F>
+__device__ __forceinline__ bool scan_range4(const float4* ball, uint32_t b,
+                                            uint32_t e, F& visit) {
+  for (uint32_t p = b; p < e; p += 4) {
+    uint32_t l = e - 1;
+    float4 E0 = ball[p];
+    float4 E1 = ball[min(p + 1, l)];
+    float4 E2 = ball[min(p + 2, l)];
+    float4 E3 = ball[min(p + 3, l)];
+    if (visit(p, E0)) return true;
+    if (p + 1 < e && visit(p + 1, E1)) return true;
+    if (p + 2 < e && visit(p + 2, E2)) return true;
+    if (p + 3 < e && visit(p + 3, E3)) return true;
+  }
+  return false;
+}
+template <class F>
+__device__ __forceinline__ void scan_box4(const GridP& g,
+                                          const uint32_t* cstart,
+                                          const float4* ball, const CellBox& b,
+                                          F&& visit) {
+  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
+  {
+    int64_t c = b.cc[0] + (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
+    if (scan_range4(ball, cstart[c], cstart[c + 1], visit)) return;
+  }
+  for (int z = b.lo[2]; z <= b.hi[2]; ++z)
+    for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
+      int64_t row = (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      if (!(z == b.cc[2] && y == b.cc[1])) {
+        if (scan_range4(ball, cstart[row + b.lo[0]], cstart[row + b.hi[0] + 1],
+                        visit))
+          return;
+      } else {
+        if (scan_range4(ball, cstart[row + b.lo[0]], cstart[row + b.cc[0]],
+                        visit))
+          return;
+        if (scan_range4(ball, cstart[row + b.cc[0] + 1],
+                        cstart[row + b.hi[0] + 1], visit))
+          return;
+      }
+    }
+}
+
 /* per-axis error of x = num/(k den) from certified errors of num, den */
 template <class R>
 __device__ __forceinline__ double quot_err(const FE<R>& num, const FE<R>& den,
@@ -236,6 +284,31 @@ __device__ __forceinline__ bool pairss_init(PairSS<R>& P, const V3<FE<R>>& pa,
   P.Mz = (cu * fabs(num.z.v) + num.z.e) * sl;
   return true;
 }
+/* Semi-static setup: D, N (tets) or den, num (triangles, edges) evaluated
+ * once in precision R together with the same expressions on absolute
+ * values (perm); with leaves y = p - p_a and w = r^2 each rounded once, a
+ * tree of height h has |computed - exact| <= gamma_{h} * perm, and
+ * gamma_{h} <= (h + 1) u for h u << 1.  Heights used: D 6, N 10 (tets);
+ * |n|^2 6, X 10 (triangles); |u|^2 4, A u 8 (edges), rounded up. */
+template <class R>
+struct SS {
+  R v, p; /* value, permanent */
+};
+template <class R>
+__device__ __forceinline__ FE<R> ss_fe(const SS<R>& a, int h) {
+  R g = (R)(h + 2) * FEc<R>::u();
+  return FE<R>{a.v, g * a.p * FEc<R>::slack() + (R)64 * FEc<R>::tiny()};
+}
+template <class R>
+__device__ __forceinline__ void cross_ss(R ax, R ay, R az, R bx, R by, R bz,
+                                         SS<R>* c) {
+  R aX = fabs(ax), aY = fabs(ay), aZ = fabs(az), bX = fabs(bx), bY = fabs(by),
+    bZ = fabs(bz);
+  c[0] = SS<R>{ay * bz - az * by, aY * bZ + aZ * bY};
+  c[1] = SS<R>{az * bx - ax * bz, aZ * bX + aX * bZ};
+  c[2] = SS<R>{ax * by - ay * bx, aX * bY + aY * bX};
+}
+
 template <class R>
 __device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
                                            const float4& e) {

@@ -482,45 +482,65 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     }
   };
 
-  /* ---- edges ---- */
+  /* alpha filter of one dimension over its enumerated candidates: the
+   * ladder settles every flag, a flat scan of the flags gives both the
+   * stable compaction positions and the survivors' CSR row starts. */
+  auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
+                          const uint32_t* cseg, uint8_t* flag, int kp,
+                          DBuf<uint64_t>& pos) -> int64_t {
+    ladder(
+        (uint32_t)std::max<int64_t>(nc, 1), kp, -1,
+        [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_filter_esc(SG, kind, C, q, nq, cval, flag, q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_filter_exact(SG, kind, C, q, nq, cval, flag, L);
+        });
+    pos = DBuf<uint64_t>(P, nc + 1);
+    {
+      auto tmp = scan_tmp(nc);
+      scan_u8flag_to_u64(flag, pos.get(), nc, tmp.get(), 0, s);
+    }
+    KCHK();
+    return (int64_t)readback(E, s, pos.get() + nc);
+  };
+  auto count_scan = [&](const uint32_t* cnt, int64_t nseg, uint64_t* off,
+                        const char* what) -> int64_t {
+    {
+      auto tmp = scan_tmp(nseg);
+      scan_u32_to_u64(cnt, off, nseg, tmp.get(), 0, s);
+    }
+    KCHK();
+    int64_t tot = (int64_t)readback(E, s, off + nseg);
+    if (tot >= ((int64_t)1 << 32))
+      fail(PAC_ERR_OVERFLOW, std::string(what) + " candidates exceed 2^32");
+    return tot;
+  };
+
+  /* ---- edges (AlphaComplexParallel.cu:1200-1263) ---- */
   int64_t Ecand = 0, Ealpha = 0;
   DBuf<uint64_t> erow(P, n + 1);
   DBuf<uint32_t> ecol, eleft;
+  CandCtx C0{nullptr, nullptr, nullptr, nullptr};
   if (!no_edges) {
-    DBuf<uint32_t> cnt(P, n);
     DBuf<uint64_t> off(P, n + 1);
-    launch_edge_count(SG, cnt.get(), L);
     {
-      auto tmp = scan_tmp(n);
-      scan_u32_to_u64(cnt.get(), off.get(), n, tmp.get(), 0, s);
+      DBuf<uint32_t> cnt(P, n);
+      launch_edge_count(SG, cnt.get(), L);
+      Ecand = count_scan(cnt.get(), n, off.get(), "edge");
     }
-    KCHK();
-    Ecand = (int64_t)readback(E, s, off.get() + n);
-    if (Ecand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "edge candidates exceed 2^32");
-    DBuf<uint32_t> cand(P, Ecand), keep(P, n);
+    DBuf<uint32_t> cval(P, Ecand), cseg(P, Ecand);
     DBuf<uint8_t> flag(P, Ecand);
-    uint32_t cap = (uint32_t)std::max<int64_t>(Ecand, 1);
-    ladder(
-        cap, KP_EDGE_FILL, -1,
-        [&](Queue q) {
-          launch_edge_fill(SG, off.get(), cand.get(), flag.get(), keep.get(), q, L);
-        },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_edge_esc(SG, q, nq, cand.get(), flag.get(), keep.get(), q2, L);
-        },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_edge_exact(SG, q, nq, cand.get(), flag.get(), keep.get(), L);
-        });
-    {
-      auto tmp = scan_tmp(n);
-      scan_u32_to_u64(keep.get(), erow.get(), n, tmp.get(), 0, s);
-    }
-    KCHK();
-    Ealpha = (int64_t)readback(E, s, erow.get() + n);
+    launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L);
+    DBuf<uint64_t> pos;
+    Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
+                          KP_EDGE_FILL, pos);
+    launch_seg_starts(n, off.get(), pos.get(), erow.get(), L);
     ecol = DBuf<uint32_t>(P, Ealpha);
     eleft = DBuf<uint32_t>(P, Ealpha);
-    launch_edge_compact(n, off.get(), cand.get(), flag.get(), erow.get(),
-                        ecol.get(), eleft.get(), L);
+    launch_compact_flat(1, Ecand, C0, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), ecol.get(), eleft.get(), nullptr, L);
     KCHK();
   } else {
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 1 + 255) / 256)),
@@ -533,97 +553,59 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   st.edge_alpha = Ealpha;
   T.mark(3);
 
-  /* ---- triangles ---- */
+  /* ---- triangles (AlphaComplexParallel.cu:1279-1371) ---- */
   int64_t Tcand = 0, Talpha = 0;
   DBuf<uint64_t> trow(P, Ealpha + 1);
   DBuf<uint32_t> tk, tedge;
+  CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr};
   {
-    DBuf<uint32_t> cnt(P, Ealpha);
     DBuf<uint64_t> off(P, Ealpha + 1);
-    launch_tri_count(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
-                     cnt.get(), L);
     {
-      auto tmp = scan_tmp(Ealpha);
-      scan_u32_to_u64(cnt.get(), off.get(), Ealpha, tmp.get(), 0, s);
+      DBuf<uint32_t> cnt(P, Ealpha);
+      launch_tri_count(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
+                       cnt.get(), L);
+      Tcand = count_scan(cnt.get(), Ealpha, off.get(), "triangle");
     }
-    KCHK();
-    Tcand = (int64_t)readback(E, s, off.get() + Ealpha);
-    if (Tcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "triangle candidates exceed 2^32");
-    DBuf<uint32_t> cand(P, Tcand), keep(P, Ealpha);
+    DBuf<uint32_t> cval(P, Tcand), cseg(P, Tcand);
     DBuf<uint8_t> flag(P, Tcand);
-    ladder(
-        (uint32_t)std::max<int64_t>(Tcand, 1), KP_TRI_FILL, -1,
-        [&](Queue q) {
-          launch_tri_fill(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
-                          off.get(), cand.get(), flag.get(), keep.get(), q, L);
-        },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_tri_esc(SG, q, nq, ecol.get(), eleft.get(), cand.get(),
-                         flag.get(), keep.get(), q2, L);
-        },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_tri_exact(SG, q, nq, ecol.get(), eleft.get(), cand.get(),
-                           flag.get(), keep.get(), L);
-        });
-    {
-      auto tmp = scan_tmp(Ealpha);
-      scan_u32_to_u64(keep.get(), trow.get(), Ealpha, tmp.get(), 0, s);
-    }
-    KCHK();
-    Talpha = (int64_t)readback(E, s, trow.get() + Ealpha);
+    launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
+                    cval.get(), cseg.get(), L);
+    DBuf<uint64_t> pos;
+    Talpha = filter_stage(2, C1, Tcand, cval.get(), cseg.get(), flag.get(),
+                          KP_TRI_FILL, pos);
+    launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L);
     tk = DBuf<uint32_t>(P, Talpha);
     tedge = DBuf<uint32_t>(P, Talpha);
-    launch_seg_compact(Ealpha, off.get(), cand.get(), flag.get(), trow.get(),
-                       tk.get(), tedge.get(), L);
+    launch_compact_flat(2, Tcand, C1, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), tk.get(), tedge.get(), nullptr, L);
     KCHK();
   }
   st.tri_candidates = Tcand;
   st.tri_alpha = Talpha;
   T.mark(4);
 
-  /* ---- tets ---- */
+  /* ---- tets (AlphaComplexParallel.cu:1381-1442) ---- */
   int64_t Kcand = 0, Kalpha = 0;
   DBuf<uint4> tets;
+  CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get()};
   {
-    DBuf<uint32_t> cnt(P, Talpha);
     DBuf<uint64_t> off(P, Talpha + 1);
-    launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
-                     tk.get(), tedge.get(), cnt.get(), L);
     {
-      auto tmp = scan_tmp(Talpha);
-      scan_u32_to_u64(cnt.get(), off.get(), Talpha, tmp.get(), 0, s);
+      DBuf<uint32_t> cnt(P, Talpha);
+      launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
+                       tk.get(), tedge.get(), cnt.get(), L);
+      Kcand = count_scan(cnt.get(), Talpha, off.get(), "tet");
     }
-    KCHK();
-    Kcand = (int64_t)readback(E, s, off.get() + Talpha);
-    if (Kcand >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "tet candidates exceed 2^32");
-    DBuf<uint32_t> cand(P, Kcand), keep(P, Talpha);
+    DBuf<uint32_t> cval(P, Kcand), cseg(P, Kcand);
     DBuf<uint8_t> flag(P, Kcand);
-    ladder(
-        (uint32_t)std::max<int64_t>(Kcand, 1), KP_TET_FILL, -1,
-        [&](Queue q) {
-          launch_tet_fill(SG, Talpha, erow.get(), ecol.get(), eleft.get(),
-                          trow.get(), tk.get(), tedge.get(), off.get(),
-                          cand.get(), flag.get(), keep.get(), q, L);
-        },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_tet_esc(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
-                         tedge.get(), cand.get(), flag.get(), keep.get(), q2, L);
-        },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_tet_exact(SG, q, nq, ecol.get(), eleft.get(), tk.get(),
-                           tedge.get(), cand.get(), flag.get(), keep.get(), L);
-        });
-    DBuf<uint64_t> orow(P, Talpha + 1);
-    {
-      auto tmp = scan_tmp(Talpha);
-      scan_u32_to_u64(keep.get(), orow.get(), Talpha, tmp.get(), 0, s);
-    }
-    KCHK();
-    Kalpha = (int64_t)readback(E, s, orow.get() + Talpha);
+    launch_tet_enum(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
+                    tk.get(), tedge.get(), off.get(), cval.get(), cseg.get(), L);
+    DBuf<uint64_t> pos;
+    Kalpha = filter_stage(3, C2, Kcand, cval.get(), cseg.get(), flag.get(),
+                          KP_TET_FILL, pos);
     tets = DBuf<uint4>(P, Kalpha);
-    launch_tet_compact(Talpha, off.get(), cand.get(), flag.get(), orow.get(),
-                       ecol.get(), eleft.get(), tk.get(), tedge.get(),
-                       tets.get(), L);
+    launch_compact_flat(3, Kcand, C2, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), nullptr, nullptr, tets.get(), L);
     KCHK();
   }
   st.tet_candidates = Kcand;

@@ -98,63 +98,46 @@ struct Queue {
   uint32_t cap;
 };
 
-/* ---- enumeration + filters (k_enum.hip) ---- */
+/* ---- enumeration + filters (k_enum.hip) ----
+ * Candidates of each dimension are (segment, new vertex) pairs: segment =
+ * ball (edges), alpha edge (triangles), alpha triangle (tets). */
+struct CandCtx {
+  const uint32_t *eleft, *ecol, *tk, *tedge;
+};
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
-void launch_edge_fill(const Stage& S, const uint64_t* off, uint32_t* cand,
-                      uint8_t* flag, uint32_t* keep, Queue q1, PipeLaunch L);
-void launch_edge_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                     const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                     Queue q2, PipeLaunch L);
-void launch_edge_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                       const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                       PipeLaunch L);
-void launch_edge_compact(int64_t n, const uint64_t* off, const uint32_t* cand,
-                         const uint8_t* flag, const uint64_t* erow,
-                         uint32_t* ecol, uint32_t* eleft, PipeLaunch L);
+void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
+                      uint32_t* cseg, PipeLaunch L);
 void launch_tri_count(const Stage& S, int64_t ne, const uint64_t* erow,
                       const uint32_t* ecol, const uint32_t* eleft,
                       uint32_t* cnt, PipeLaunch L);
-void launch_tri_fill(const Stage& S, int64_t ne, const uint64_t* erow,
-                     const uint32_t* ecol, const uint32_t* eleft,
-                     const uint64_t* off, uint32_t* cand, uint8_t* flag,
-                     uint32_t* keep, Queue q1, PipeLaunch L);
-void launch_tri_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                    const uint32_t* ecol, const uint32_t* eleft,
-                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                    Queue q2, PipeLaunch L);
-void launch_tri_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                      PipeLaunch L);
-void launch_seg_compact(int64_t nseg, const uint64_t* off,
-                        const uint32_t* cand, const uint8_t* flag,
-                        const uint64_t* orow, uint32_t* ocol, uint32_t* oseg,
-                        PipeLaunch L);
+void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
+                     uint32_t* cseg, PipeLaunch L);
 void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, const uint64_t* trow,
                       const uint32_t* tk, const uint32_t* tedge,
                       uint32_t* cnt, PipeLaunch L);
-void launch_tet_fill(const Stage& S, int64_t nt, const uint64_t* erow,
-                     const uint32_t* ecol, const uint32_t* eleft,
-                     const uint64_t* trow, const uint32_t* tk,
-                     const uint32_t* tedge, const uint64_t* off,
-                     uint32_t* cand, uint8_t* flag, uint32_t* keep, Queue q1,
+void launch_tet_enum(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint64_t* trow,
+                     const uint32_t* tk, const uint32_t* tedge,
+                     const uint64_t* off, uint32_t* cval, uint32_t* cseg,
                      PipeLaunch L);
-void launch_tet_esc(const Stage& S, const QEntry* q, uint32_t nq,
-                    const uint32_t* ecol, const uint32_t* eleft,
-                    const uint32_t* tk, const uint32_t* tedge,
-                    const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                    Queue q2, PipeLaunch L);
-void launch_tet_exact(const Stage& S, const QEntry* q, uint32_t nq,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      const uint32_t* tk, const uint32_t* tedge,
-                      const uint32_t* cand, uint8_t* flag, uint32_t* keep,
-                      PipeLaunch L);
-void launch_tet_compact(int64_t nt, const uint64_t* off, const uint32_t* cand,
-                        const uint8_t* flag, const uint64_t* orow,
-                        const uint32_t* ecol, const uint32_t* eleft,
-                        const uint32_t* tk, const uint32_t* tedge,
-                        uint4* tets, PipeLaunch L);
+/* kind 1 edges, 2 triangles, 3 tets */
+void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
+                   const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
+                   Queue q1, PipeLaunch L);
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, const QEntry* q,
+                       uint32_t nq, const uint32_t* cval, uint8_t* flag,
+                       Queue q2, PipeLaunch L);
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, const QEntry* q,
+                         uint32_t nq, const uint32_t* cval, uint8_t* flag,
+                         PipeLaunch L);
+void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
+                       uint64_t* orow, PipeLaunch L);
+void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
+                         const uint32_t* cseg, const uint8_t* flag,
+                         const uint64_t* pos, uint32_t* out_val,
+                         uint32_t* out_seg, uint4* out_tet, PipeLaunch L);
 
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =
