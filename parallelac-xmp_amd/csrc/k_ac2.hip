@@ -192,6 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
   unsigned evals = 0, pairs = 0;
   int64_t total = PHASE_B ? (int64_t)*nrest : nk;
   int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
+  uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   for (int64_t w = gtid(); w < total; w += stride) {
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     uint4 t = tets[id];
@@ -241,8 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         flag[id] = 2;
     } else {
       flag[id] = dead ? 0 : (pending ? 2 : 1);
-      if (unfinished && !dead) rest[atomicAdd(nrest, 1u)] = (uint32_t)id;
+      if (unfinished && !dead) rest_id = (uint32_t)id;
     }
+  }
+  if (!PHASE_B) {
+    unsigned at = wave_append(nrest, rest_id != 0xffffffffu);
+    if (rest_id != 0xffffffffu) rest[at] = rest_id;
   }
   flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
   flush_ctr(S.ctr, CTR_PAIRS + 0, pairs);
@@ -514,6 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
   unsigned evals = 0, pairs = 0;
   int64_t total = PHASE_B ? (int64_t)*nrest : nitems;
   int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
+  uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   for (int64_t w = gtid(); w < total; w += stride) {
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     if (!PHASE_B && in_flag[id]) {
@@ -576,8 +582,12 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
         flag[id] = 2;
     } else {
       flag[id] = dead ? 0 : (pending ? 2 : 1);
-      if (unfinished && !dead) rest[atomicAdd(nrest, 1u)] = (uint32_t)id;
+      if (unfinished && !dead) rest_id = (uint32_t)id;
     }
+  }
+  if (!PHASE_B) {
+    unsigned at = wave_append(nrest, rest_id != 0xffffffffu);
+    if (rest_id != 0xffffffffu) rest[at] = rest_id;
   }
   const int fam = KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2;
   flush_ctr(S.ctr, CTR_STAGE + fam * 4 + (sizeof(R) == 4 ? 0 : 1), evals);

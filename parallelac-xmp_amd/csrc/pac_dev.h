@@ -16,10 +16,28 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
+/* Counters live in CTR_SLOTS copies; each wave adds into the copy picked
+ * by its block index, so that one-item-per-thread kernels do not serialize
+ * hundreds of thousands of atomics on a single address.  The host sums
+ * the copies. */
 __device__ __forceinline__ void flush_ctr(unsigned long long* ctr, int idx,
                                           unsigned v) {
   v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&ctr[idx], (unsigned long long)v);
+  if ((threadIdx.x & 63) == 0 && v)
+    atomicAdd(&ctr[idx * CTR_SLOTS + (blockIdx.x & (CTR_SLOTS - 1))],
+              (unsigned long long)v);
+}
+/* wave-aggregated append: one atomic per wave for all lanes with want */
+__device__ __forceinline__ unsigned wave_append(unsigned* counter, bool want) {
+  uint64_t m = __ballot(want);
+  if (!m) return 0;
+  const int lane = threadIdx.x & 63;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+  base = __shfl(base, leader, 64);
+  uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return base + (unsigned)__popcll(m & lt);
 }
 struct StageCnt {
   unsigned c[4];

@@ -394,8 +394,9 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   SG.staged = opt.precision == PAC_PREC_STAGED;
   SG.reach = reach;
   SG.margin = margin;
-  DBuf<unsigned long long> ctr(P, CTR_COUNT);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ctr.get(), CTR_COUNT);
+  DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
+  hipLaunchKernelGGL(k_zero_u64, dim3((CTR_COUNT * CTR_SLOTS + 255) / 256),
+                     dim3(256), 0, s, ctr.get(), CTR_COUNT * CTR_SLOTS);
   SG.ctr = ctr.get();
   DBuf<unsigned> qn(P, 4); /* [0] hot queue, [1] escalation queue */
 
@@ -738,9 +739,13 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   st.tets_final = Kf;
   T.mark(8);
 
-  unsigned long long h[CTR_COUNT];
-  HIPCHK(hipMemcpyAsync(h, ctr.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+  unsigned long long hs[CTR_COUNT * CTR_SLOTS], h[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(hs, ctr.get(), sizeof(hs), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  for (int c = 0; c < CTR_COUNT; ++c) {
+    h[c] = 0;
+    for (int k = 0; k < CTR_SLOTS; ++k) h[c] += hs[c * CTR_SLOTS + k];
+  }
   for (int f = 0; f < 6; ++f)
     for (int q = 0; q < 4; ++q)
       st.stage_evals[f][q] = (int64_t)h[CTR_STAGE + 4 * f + q];

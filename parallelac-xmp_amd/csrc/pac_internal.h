@@ -46,6 +46,7 @@ enum {
   CTR_PAIRS = 25,       /* [3] tet, tri, edge pairs */
   CTR_COUNT = 28
 };
+constexpr int CTR_SLOTS = 32;   /* copies of each counter (see flush_ctr) */
 enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
        FAM_TRI_AC2 = 4, FAM_EDGE_AC2 = 5 };
 
