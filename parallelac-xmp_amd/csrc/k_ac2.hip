@@ -246,11 +246,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
   }
   if (!PHASE_B) {
-    unsigned at = wave_append(nrest, rest_id != 0xffffffffu);
+    unsigned at = block_append(nrest, rest_id != 0xffffffffu);
     if (rest_id != 0xffffffffu) rest[at] = rest_id;
   }
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
-  flush_ctr(S.ctr, CTR_PAIRS + 0, pairs);
+  (void)evals;
+  block_flush(S.ctr, CTR_PAIRS + 0, pairs);
 }
 
 /* FP64 then double-double for queued pairs / uncertified tets */
@@ -586,12 +586,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
   }
   if (!PHASE_B) {
-    unsigned at = wave_append(nrest, rest_id != 0xffffffffu);
+    unsigned at = block_append(nrest, rest_id != 0xffffffffu);
     if (rest_id != 0xffffffffu) rest[at] = rest_id;
   }
-  const int fam = KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2;
-  flush_ctr(S.ctr, CTR_STAGE + fam * 4 + (sizeof(R) == 4 ? 0 : 1), evals);
-  flush_ctr(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
+  (void)evals;
+  block_flush(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
 }
 
 template <int KIND>

@@ -527,18 +527,17 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
                                                    const uint32_t* cval,
                                                    const uint32_t* cseg,
                                                    uint8_t* flag, Queue q1) {
+  /* one evaluation per candidate: the host books nc evaluations at this
+   * stage, no counter traffic here */
   int64_t o = gtid();
-  unsigned evals = 0;
   if (o < nc) {
     float4 v[4];
     uint32_t seg = cseg[o];
     cand_balls<KIND>(S, C, seg, cval[o], v);
     int r = filter_code<KIND, FE<R>>(v, S.alpha);
-    evals = 1;
     flag[o] = (uint8_t)r;
     if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
   }
-  flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + StageOf<FE<R>>::v, evals);
 }
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,

@@ -27,6 +27,43 @@ __device__ __forceinline__ void flush_ctr(unsigned long long* ctr, int idx,
     atomicAdd(&ctr[idx * CTR_SLOTS + (blockIdx.x & (CTR_SLOTS - 1))],
               (unsigned long long)v);
 }
+/* Block-level versions (every thread of the block must call them): one
+ * global atomic per block instead of one per wave. */
+__device__ __forceinline__ void block_flush(unsigned long long* ctr, int idx,
+                                            unsigned v) {
+  __shared__ unsigned s_part[kBlock / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+    if (t)
+      atomicAdd(&ctr[idx * CTR_SLOTS + (blockIdx.x & (CTR_SLOTS - 1))],
+                (unsigned long long)t);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ unsigned block_append(unsigned* counter, bool want) {
+  __shared__ unsigned s_cnt[kBlock / 64];
+  __shared__ unsigned s_base;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t m = __ballot(want);
+  uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (lane == 0) s_cnt[w] = (unsigned)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+    for (int q = 0; q < kBlock / 64; ++q) t += s_cnt[q];
+    s_base = t ? atomicAdd(counter, t) : 0u;
+  }
+  __syncthreads();
+  unsigned off = s_base;
+  for (int q = 0; q < w; ++q) off += s_cnt[q];
+  unsigned r = off + (unsigned)__popcll(m & lt);
+  __syncthreads();
+  return r;
+}
 /* wave-aggregated append: one atomic per wave for all lanes with want */
 __device__ __forceinline__ unsigned wave_append(unsigned* counter, bool want) {
   uint64_t m = __ballot(want);

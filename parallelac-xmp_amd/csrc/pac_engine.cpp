@@ -486,9 +486,11 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   /* alpha filter of one dimension over its enumerated candidates: the
    * ladder settles every flag, a flat scan of the flags gives both the
    * stable compaction positions and the survivors' CSR row starts. */
+  int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
   auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
                           const uint32_t* cseg, uint8_t* flag, int kp,
                           DBuf<uint64_t>& pos) -> int64_t {
+    host_evals[kind - 1] += nc;
     ladder(
         (uint32_t)std::max<int64_t>(nc, 1), kp, -1,
         [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
@@ -749,6 +751,11 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   for (int f = 0; f < 6; ++f)
     for (int q = 0; q < 4; ++q)
       st.stage_evals[f][q] = (int64_t)h[CTR_STAGE + 4 * f + q];
+  /* hot-stage evaluations: one per candidate for the filters, one per
+   * (simplex, ball) pair tested for the emptiness scans */
+  const int hot = SG.staged ? 0 : 1;
+  for (int f = 0; f < 3; ++f) st.stage_evals[f][hot] += host_evals[f];
+  for (int q = 0; q < 3; ++q) st.stage_evals[3 + q][hot] += (int64_t)h[CTR_PAIRS + q];
   st.sos_calls = (int64_t)h[CTR_SOS];
   for (int q = 0; q < 3; ++q) st.ac2_pairs[q] = (int64_t)h[CTR_PAIRS + q];
 }
