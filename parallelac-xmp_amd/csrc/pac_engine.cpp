@@ -809,6 +809,24 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
                        dim3(256), 0, s, raw, n, solvent, R->balls.get());
     KCHK();
     run_pipeline(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
+    if (opt.order == PAC_ORDER_CANONICAL) {
+      /* vertices by original id, rows lexicographic */
+      int idbits = 1;
+      while (idbits < 32 && ((int64_t)1 << idbits) < n) ++idbits;
+      for (int d = 1; d <= 3; ++d) {
+        int64_t m = R->counts[d];
+        if (m <= 0) continue;
+        DBuf<uint32_t> a(&E->pool, m), b(&E->pool, m), c(&E->pool, m),
+            e(&E->pool, m);
+        DBuf<int32_t> copy(&E->pool, m * (d + 1));
+        size_t rtb = radix_tmp_bytes(m);
+        DBuf<uint8_t> rt(&E->pool, rtb);
+        canonical_rows(d + 1, m, R->lists[d - 1].get(), idbits, a.get(),
+                       b.get(), c.get(), e.get(), rt.get(), rtb, copy.get(), s);
+        KCHK();
+        HIPCHK(hipStreamSynchronize(s));
+      }
+    }
   } else {
     for (int i = 2; i <= 8; ++i) T.mark(i);
   }
@@ -882,9 +900,22 @@ const int32_t* pac_result_device_list(const pac_result* r, int dim) {
   if (!r || dim < 1 || dim > 3) return nullptr;
   return r->lists[dim - 1].get();
 }
-int pac_result_copy_alpha(const pac_result*, int, double*) {
-  g_err = "pac_result_copy_alpha: not built in this version";
-  return PAC_ERR_INVALID;
+int pac_result_copy_alpha(const pac_result* r, int dim, double* rho) {
+  if (!r || !rho || dim < 1 || dim > 3) return PAC_ERR_INVALID;
+  return guarded([&] {
+    int64_t m = r->counts[dim];
+    if (m <= 0) return;
+    Engine* E = r->eng;
+    std::lock_guard<std::mutex> lk(E->run_mu);
+    HIPCHK(hipSetDevice(E->device));
+    DBuf<double> d(&E->pool, m);
+    launch_rho(dim, m, r->lists[dim - 1].get(), r->balls.get(), d.get(),
+               E->stream);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(rho, d.get(), (size_t)m * sizeof(double),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+  });
 }
 int pac_result_stats(const pac_result* r, pac_stats* st) {
   if (!r || !st) return PAC_ERR_INVALID;

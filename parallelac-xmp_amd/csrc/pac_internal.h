@@ -190,6 +190,14 @@ void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint64_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L);
 
+/* ---- outputs (k_output.hip) ---- */
+void launch_rho(int dim, int64_t m, const int32_t* rows, const float4* balls,
+                double* rho, hipStream_t s);
+void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
+                    uint32_t* idx, uint32_t* key, uint32_t* idx2,
+                    uint32_t* key2, void* rtmp, size_t rtb, int32_t* copy,
+                    hipStream_t s);
+
 /* SoS table as compiled into the device code (for tests) */
 int sos_table_host(int32_t* out3, int cap);
 
