@@ -68,29 +68,25 @@ void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
                      tri_in);
 }
 
-/* edges of the final triangles -> edge_in */
+/* edges of the final triangles -> edge_in (edge ids recorded at
+ * enumeration: (i,j) = tedge, (i,k) = teik, (j,k) = tejk) */
 __global__ __launch_bounds__(kBlock) void k_mark_edges(
-    int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
-    const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
-    const uint64_t* erow, uint8_t* edge_in) {
+    int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+    const uint32_t* teik, const uint32_t* tejk, uint8_t* edge_in) {
   int64_t id = gtid();
   if (id >= nt) return;
   uint32_t t = ftri[id];
-  uint32_t e = tedge[t];
-  uint32_t i = eleft[e], j = ecol[e], k = tk[t];
-  edge_in[e] = 1;
-  int64_t f;
-  if ((f = find_edge(erow, ecol, i, k)) >= 0) edge_in[f] = 1;
-  if ((f = find_edge(erow, ecol, j, k)) >= 0) edge_in[f] = 1;
+  edge_in[tedge[t]] = 1;
+  edge_in[teik[t]] = 1;
+  edge_in[tejk[t]] = 1;
 }
-void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
-                       const uint32_t* eleft, const uint32_t* tk,
-                       const uint32_t* tedge, const uint64_t* erow,
+void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+                       const uint32_t* teik, const uint32_t* tejk,
                        uint8_t* edge_in, PipeLaunch L) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_mark_edges, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, ftri, ecol, eleft, tk, tedge,
-                     erow, edge_in);
+                     dim3(kBlock), 0, L.s, nt, ftri, tedge, teik, tejk,
+                     edge_in);
 }
 
 /* ---- emission in original ids ---- */

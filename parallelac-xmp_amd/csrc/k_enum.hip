@@ -288,94 +288,9 @@ void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
 /* ------------------------------------------------------------------ */
 /* triangles: k in row(i) after j, intersected with row(j)             */
 /* ------------------------------------------------------------------ */
-__global__ __launch_bounds__(kBlock) void k_tri_count(
-    int64_t ne, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, uint32_t* cnt) {
-  int64_t q = gtid();
-  if (q >= ne) return;
-  uint32_t i = eleft[q], j = ecol[q];
-  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-  uint64_t b = erow[j], be = erow[j + 1];
-  uint32_t c = 0;
-  while (a < ae && b < be) {
-    uint32_t x = ecol[a], y = ecol[b];
-    c += (x == y);
-    a += (x <= y);
-    b += (y <= x);
-  }
-  cnt[q] = c;
-}
-void launch_tri_count(const Stage&, int64_t ne, const uint64_t* erow,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      uint32_t* cnt, PipeLaunch L) {
-  if (ne <= 0) return;
-  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
-}
-
 /* ------------------------------------------------------------------ */
 /* tets: l in T(i,j) after k, T(i,k), T(j,k)                           */
 /* ------------------------------------------------------------------ */
-struct TetWalk {
-  uint64_t a, ae, b, be, c, ce;
-};
-__device__ __forceinline__ bool tet_walk_init(
-    int64_t q, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tedge,
-    const uint32_t* tk, uint32_t& i, uint32_t& j, uint32_t& k, TetWalk& w) {
-  uint32_t eij = tedge[q];
-  i = eleft[eij];
-  j = ecol[eij];
-  k = tk[q];
-  w.a = (uint64_t)q + 1;
-  w.ae = trow[eij + 1];
-  if (w.a >= w.ae) return false;
-  int64_t eik = find_edge(erow, ecol, i, k), ejk = find_edge(erow, ecol, j, k);
-  if (eik < 0 || ejk < 0) return false;
-  w.b = trow[eik];
-  w.be = trow[eik + 1];
-  w.c = trow[ejk];
-  w.ce = trow[ejk + 1];
-  return true;
-}
-
-__global__ __launch_bounds__(kBlock) void k_tet_count(
-    int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
-    const uint32_t* tedge, uint32_t* cnt) {
-  int64_t q = gtid();
-  if (q >= nt) return;
-  uint32_t i, j, k;
-  TetWalk w;
-  uint32_t c = 0;
-  if (tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) {
-    while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
-      uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
-      uint32_t m = max(x, max(y, z));
-      if (x == m && y == m && z == m) {
-        ++c;
-        ++w.a;
-        ++w.b;
-        ++w.c;
-      } else {
-        w.a += (x < m);
-        w.b += (y < m);
-        w.c += (z < m);
-      }
-    }
-  }
-  cnt[q] = c;
-}
-void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-                      const uint32_t* eleft, const uint64_t* trow,
-                      const uint32_t* tk, const uint32_t* tedge,
-                      uint32_t* cnt, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_tet_count, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, erow, ecol, eleft, trow, tk,
-                     tedge, cnt);
-}
-
 /* ------------------------------------------------------------------ */
 /* enumeration: candidates as (segment, new vertex) pairs              */
 /* ------------------------------------------------------------------ */
@@ -411,73 +326,108 @@ void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
                      dim3(kBlock), 0, L.s, S, off, cval, cseg);
 }
 
+/* Slot capacities that need no merge: a triangle from alpha edge (i,j) has
+ * its third vertex in row(i) after j, a tet from alpha triangle (i,j,k) has
+ * its fourth vertex in T(i,j) after k.  Unused slots carry seg = ~0. */
+__global__ __launch_bounds__(kBlock) void k_tail_caps(int64_t m,
+                                                      const uint64_t* row,
+                                                      const uint32_t* owner,
+                                                      uint32_t* cap) {
+  int64_t q = gtid();
+  if (q >= m) return;
+  cap[q] = (uint32_t)(row[owner[q] + 1] - (uint64_t)q - 1);
+}
+void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
+                      uint32_t* cap, PipeLaunch L) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_tail_caps, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, m, row, owner, cap);
+}
+
+/* triangles of alpha edge q = (i,j): k in row(i) after j and in row(j); the
+ * positions of k in row(i) and row(j) are the alpha-edge ids of (i,k) and
+ * (j,k), recorded for the tet stage and the closure. */
 __global__ __launch_bounds__(kBlock) void k_tri_enum(
     int64_t ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-    uint32_t* cseg) {
+    uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr) {
   int64_t q = gtid();
-  if (q >= ne) return;
+  unsigned found = 0;
+  if (q < ne) {
   uint32_t i = eleft[q], j = ecol[q];
   uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
   uint64_t b = erow[j], be = erow[j + 1];
-  uint64_t o = off[q];
+  uint64_t o = off[q], oe = off[q + 1];
   while (a < ae && b < be) {
     uint32_t x = ecol[a], y = ecol[b];
     if (x == y) {
       cval[o] = x;
       cseg[o] = (uint32_t)q;
+      cea[o] = (uint32_t)a;
+      ceb[o] = (uint32_t)b;
       ++o;
+      ++found;
     }
     a += (x <= y);
     b += (y <= x);
   }
+  for (; o < oe; ++o) cseg[o] = 0xffffffffu;
+  }
+  block_flush(ctr, CTR_CAND + 1, found);
 }
 void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, PipeLaunch L) {
+                     uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
+                     unsigned long long* ctr, PipeLaunch L) {
   if (ne <= 0) return;
   hipLaunchKernelGGL(k_tri_enum, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, off, cval,
-                     cseg);
+                     cseg, cea, ceb, ctr);
 }
 
+/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k) */
 __global__ __launch_bounds__(kBlock) void k_tet_enum(
-    int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* trow, const uint32_t* tk,
-    const uint32_t* tedge, const uint64_t* off, uint32_t* cval,
-    uint32_t* cseg) {
+    int64_t nt, const uint64_t* trow, const uint32_t* tk,
+    const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
+    const uint64_t* off, uint32_t* cval, uint32_t* cseg,
+    unsigned long long* ctr) {
   int64_t q = gtid();
-  if (q >= nt) return;
-  uint32_t i, j, k;
-  TetWalk w;
-  uint64_t o = off[q];
-  if (!tet_walk_init(q, erow, ecol, eleft, trow, tedge, tk, i, j, k, w)) return;
-  while (w.a < w.ae && w.b < w.be && w.c < w.ce) {
-    uint32_t x = tk[w.a], y = tk[w.b], z = tk[w.c];
+  unsigned found = 0;
+  if (q < nt) {
+  uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+  uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
+  uint64_t b = trow[eik], be = trow[eik + 1];
+  uint64_t c = trow[ejk], ce = trow[ejk + 1];
+  uint64_t o = off[q], oe = off[q + 1];
+  while (a < ae && b < be && c < ce) {
+    uint32_t x = tk[a], y = tk[b], z = tk[c];
     uint32_t m = max(x, max(y, z));
     if (x == m && y == m && z == m) {
       cval[o] = m;
       cseg[o] = (uint32_t)q;
       ++o;
-      ++w.a;
-      ++w.b;
-      ++w.c;
+      ++found;
+      ++a;
+      ++b;
+      ++c;
     } else {
-      w.a += (x < m);
-      w.b += (y < m);
-      w.c += (z < m);
+      a += (x < m);
+      b += (y < m);
+      c += (z < m);
     }
   }
+  for (; o < oe; ++o) cseg[o] = 0xffffffffu;
+  }
+  block_flush(ctr, CTR_CAND + 2, found);
 }
-void launch_tet_enum(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-                     const uint32_t* eleft, const uint64_t* trow,
-                     const uint32_t* tk, const uint32_t* tedge,
-                     const uint64_t* off, uint32_t* cval, uint32_t* cseg,
-                     PipeLaunch L) {
+void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
+                     const uint32_t* tedge, const uint32_t* teik,
+                     const uint32_t* tejk, const uint64_t* off, uint32_t* cval,
+                     uint32_t* cseg, unsigned long long* ctr, PipeLaunch L) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, erow, ecol, eleft, trow, tk, tedge,
-                     off, cval, cseg);
+                     dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk, off,
+                     cval, cseg, ctr);
 }
 
 /* ------------------------------------------------------------------ */
@@ -531,12 +481,16 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
    * stage, no counter traffic here */
   int64_t o = gtid();
   if (o < nc) {
-    float4 v[4];
     uint32_t seg = cseg[o];
-    cand_balls<KIND>(S, C, seg, cval[o], v);
-    int r = filter_code<KIND, FE<R>>(v, S.alpha);
-    flag[o] = (uint8_t)r;
-    if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
+    if (seg == 0xffffffffu) {
+      flag[o] = REJECT; /* unused slot */
+    } else {
+      float4 v[4];
+      cand_balls<KIND>(S, C, seg, cval[o], v);
+      int r = filter_code<KIND, FE<R>>(v, S.alpha);
+      flag[o] = (uint8_t)r;
+      if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
+    }
   }
 }
 template <int KIND>
@@ -659,7 +613,8 @@ template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_compact_flat(
     int64_t nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
     const uint8_t* flag, const uint64_t* pos, uint32_t* out_val,
-    uint32_t* out_seg, uint4* out_tet) {
+    uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
+    const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b) {
   int64_t o = gtid();
   if (o >= nc || flag[o] != KEEP) return;
   uint64_t w = pos[o];
@@ -667,6 +622,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
   if (KIND < 3) {
     out_val[w] = val;
     out_seg[w] = seg;
+    if (KIND == 2) {
+      out_a[w] = cea[o];
+      out_b[w] = ceb[o];
+    }
   } else {
     uint32_t e = C.tedge[seg];
     out_tet[w] = make_uint4(C.eleft[e], C.ecol[e], C.tk[seg], val);
@@ -675,18 +634,23 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
 void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
                          const uint64_t* pos, uint32_t* out_val,
-                         uint32_t* out_seg, uint4* out_tet, PipeLaunch L) {
+                         uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
+                         const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
+                         PipeLaunch L) {
   if (nc <= 0) return;
   dim3 gr((unsigned)((nc + kBlock - 1) / kBlock));
   if (kind == 1)
     hipLaunchKernelGGL(k_compact_flat<1>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet);
+                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
+                       out_a, out_b);
   else if (kind == 2)
     hipLaunchKernelGGL(k_compact_flat<2>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet);
+                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
+                       out_a, out_b);
   else
     hipLaunchKernelGGL(k_compact_flat<3>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet);
+                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
+                       out_a, out_b);
 }
 
 }  // namespace pac

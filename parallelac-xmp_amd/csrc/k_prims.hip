@@ -49,13 +49,17 @@ __device__ __forceinline__ uint64_t load_val(const TIn* in, int64_t i,
   return (uint64_t)in[i];
 }
 
+/* Tiles of kTile items are read and written striped across the block
+ * (coalesced), and transposed through LDS so that each thread scans
+ * kScanItems consecutive items. */
 template <class TIn, bool kFlag>
 __global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, int64_t n,
                                                       uint64_t* sums) {
-  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t base = (int64_t)blockIdx.x * kTile;
   uint64_t s = 0;
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) s += load_val<TIn, kFlag>(in, base + q, n);
+  for (int q = 0; q < kScanItems; ++q)
+    s += load_val<TIn, kFlag>(in, base + (int64_t)q * kBlock + threadIdx.x, n);
   uint64_t tot;
   block_excl_scan_u64(s, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -65,20 +69,33 @@ template <class TIn, bool kFlag>
 __global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
                                                       const uint64_t* tile_off,
                                                       uint64_t* out) {
-  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+  __shared__ uint64_t tile[kTile];
+  int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    int t = q * kBlock + threadIdx.x;
+    tile[t] = load_val<TIn, kFlag>(in, base + t, n);
+  }
+  __syncthreads();
   uint64_t v[kScanItems];
   uint64_t s = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
-    v[q] = load_val<TIn, kFlag>(in, base + q, n);
+    v[q] = tile[threadIdx.x * kScanItems + q];
     s += v[q];
   }
   uint64_t tot;
   uint64_t ex = block_excl_scan_u64(s, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
-    if (base + q < n) out[base + q] = ex;
+    tile[threadIdx.x * kScanItems + q] = ex;
     ex += v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    int t = q * kBlock + threadIdx.x;
+    if (base + t < n) out[base + t] = tile[t];
   }
   /* the last thread of the last tile writes the grand total at out[n] */
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;

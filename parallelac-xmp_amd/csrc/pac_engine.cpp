@@ -490,7 +490,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
                           const uint32_t* cseg, uint8_t* flag, int kp,
                           DBuf<uint64_t>& pos) -> int64_t {
-    host_evals[kind - 1] += nc;
+    if (kind == 1) host_evals[0] += nc; /* tris / tets: slots != candidates */
     ladder(
         (uint32_t)std::max<int64_t>(nc, 1), kp, -1,
         [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
@@ -543,7 +543,8 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     ecol = DBuf<uint32_t>(P, Ealpha);
     eleft = DBuf<uint32_t>(P, Ealpha);
     launch_compact_flat(1, Ecand, C0, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), ecol.get(), eleft.get(), nullptr, L);
+                        pos.get(), ecol.get(), eleft.get(), nullptr, nullptr,
+                        nullptr, nullptr, nullptr, L);
     KCHK();
   } else {
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 1 + 255) / 256)),
@@ -557,61 +558,64 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   T.mark(3);
 
   /* ---- triangles (AlphaComplexParallel.cu:1279-1371) ---- */
-  int64_t Tcand = 0, Talpha = 0;
+  int64_t Tcand = 0, Tslots = 0, Talpha = 0;
   DBuf<uint64_t> trow(P, Ealpha + 1);
-  DBuf<uint32_t> tk, tedge;
+  DBuf<uint32_t> tk, tedge, teik, tejk;
   CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr};
   {
     DBuf<uint64_t> off(P, Ealpha + 1);
     {
-      DBuf<uint32_t> cnt(P, Ealpha);
-      launch_tri_count(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
-                       cnt.get(), L);
-      Tcand = count_scan(cnt.get(), Ealpha, off.get(), "triangle");
+      DBuf<uint32_t> cap(P, Ealpha);
+      launch_tail_caps(Ealpha, erow.get(), eleft.get(), cap.get(), L);
+      Tslots = count_scan(cap.get(), Ealpha, off.get(), "triangle");
     }
-    DBuf<uint32_t> cval(P, Tcand), cseg(P, Tcand);
-    DBuf<uint8_t> flag(P, Tcand);
+    DBuf<uint32_t> cval(P, Tslots), cseg(P, Tslots), cea(P, Tslots),
+        ceb(P, Tslots);
+    DBuf<uint8_t> flag(P, Tslots);
     launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
-                    cval.get(), cseg.get(), L);
+                    cval.get(), cseg.get(), cea.get(), ceb.get(), ctr.get(), L);
     DBuf<uint64_t> pos;
-    Talpha = filter_stage(2, C1, Tcand, cval.get(), cseg.get(), flag.get(),
+    Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
                           KP_TRI_FILL, pos);
     launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L);
     tk = DBuf<uint32_t>(P, Talpha);
     tedge = DBuf<uint32_t>(P, Talpha);
-    launch_compact_flat(2, Tcand, C1, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), tk.get(), tedge.get(), nullptr, L);
+    teik = DBuf<uint32_t>(P, Talpha);
+    tejk = DBuf<uint32_t>(P, Talpha);
+    launch_compact_flat(2, Tslots, C1, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), tk.get(), tedge.get(), nullptr, cea.get(),
+                        ceb.get(), teik.get(), tejk.get(), L);
     KCHK();
   }
-  st.tri_candidates = Tcand;
   st.tri_alpha = Talpha;
   T.mark(4);
 
   /* ---- tets (AlphaComplexParallel.cu:1381-1442) ---- */
-  int64_t Kcand = 0, Kalpha = 0;
+  int64_t Kcand = 0, Kslots = 0, Kalpha = 0;
   DBuf<uint4> tets;
   CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get()};
   {
     DBuf<uint64_t> off(P, Talpha + 1);
     {
-      DBuf<uint32_t> cnt(P, Talpha);
-      launch_tet_count(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
-                       tk.get(), tedge.get(), cnt.get(), L);
-      Kcand = count_scan(cnt.get(), Talpha, off.get(), "tet");
+      DBuf<uint32_t> cap(P, Talpha);
+      launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
+      Kslots = count_scan(cap.get(), Talpha, off.get(), "tet");
     }
-    DBuf<uint32_t> cval(P, Kcand), cseg(P, Kcand);
-    DBuf<uint8_t> flag(P, Kcand);
-    launch_tet_enum(Talpha, erow.get(), ecol.get(), eleft.get(), trow.get(),
-                    tk.get(), tedge.get(), off.get(), cval.get(), cseg.get(), L);
+    DBuf<uint32_t> cval(P, Kslots), cseg(P, Kslots);
+    DBuf<uint8_t> flag(P, Kslots);
+    launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
+                    tejk.get(), off.get(), cval.get(), cseg.get(), ctr.get(), L);
     DBuf<uint64_t> pos;
-    Kalpha = filter_stage(3, C2, Kcand, cval.get(), cseg.get(), flag.get(),
+    Kalpha = filter_stage(3, C2, Kslots, cval.get(), cseg.get(), flag.get(),
                           KP_TET_FILL, pos);
     tets = DBuf<uint4>(P, Kalpha);
-    launch_compact_flat(3, Kcand, C2, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), nullptr, nullptr, tets.get(), L);
+    launch_compact_flat(3, Kslots, C2, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), nullptr, nullptr, tets.get(), nullptr,
+                        nullptr, nullptr, nullptr, L);
     KCHK();
   }
-  st.tet_candidates = Kcand;
+  (void)Tcand;
+  (void)Kcand;
   st.tet_alpha = Kalpha;
   T.mark(5);
 
@@ -695,8 +699,8 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   /* ---- edge closure + dangling emptiness ---- */
   DBuf<uint8_t> edge_in(P, Ealpha), eflag(P, Ealpha);
   fill_u8(edge_in.get(), Ealpha, 0, s);
-  launch_mark_edges(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
-                    tedge.get(), erow.get(), edge_in.get(), L);
+  launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
+                    edge_in.get(), L);
   if (Ealpha)
     ladder(
         ac2_cap(Ealpha), KP_EDGE_AC2, -1,
@@ -754,6 +758,10 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   /* hot-stage evaluations: one per candidate for the filters, one per
    * (simplex, ball) pair tested for the emptiness scans */
   const int hot = SG.staged ? 0 : 1;
+  st.tri_candidates = (int64_t)h[CTR_CAND + 1];
+  st.tet_candidates = (int64_t)h[CTR_CAND + 2];
+  host_evals[1] = st.tri_candidates;
+  host_evals[2] = st.tet_candidates;
   for (int f = 0; f < 3; ++f) st.stage_evals[f][hot] += host_evals[f];
   for (int q = 0; q < 3; ++q) st.stage_evals[3 + q][hot] += (int64_t)h[CTR_PAIRS + q];
   st.sos_calls = (int64_t)h[CTR_SOS];

@@ -44,7 +44,8 @@ enum {
   CTR_STAGE = 0,        /* [6][4] stage evaluations */
   CTR_SOS = 24,
   CTR_PAIRS = 25,       /* [3] tet, tri, edge pairs */
-  CTR_COUNT = 28
+  CTR_CAND = 28,        /* [3] candidates: (unused), triangles, tets */
+  CTR_COUNT = 31
 };
 constexpr int CTR_SLOTS = 32;   /* copies of each counter (see flush_ctr) */
 enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
@@ -108,21 +109,17 @@ struct CandCtx {
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
 void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
                       uint32_t* cseg, PipeLaunch L);
-void launch_tri_count(const Stage& S, int64_t ne, const uint64_t* erow,
-                      const uint32_t* ecol, const uint32_t* eleft,
-                      uint32_t* cnt, PipeLaunch L);
+/* slot capacity of item q = row[owner[q] + 1] - q - 1 */
+void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
+                      uint32_t* cap, PipeLaunch L);
 void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, PipeLaunch L);
-void launch_tet_count(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-                      const uint32_t* eleft, const uint64_t* trow,
-                      const uint32_t* tk, const uint32_t* tedge,
-                      uint32_t* cnt, PipeLaunch L);
-void launch_tet_enum(int64_t nt, const uint64_t* erow, const uint32_t* ecol,
-                     const uint32_t* eleft, const uint64_t* trow,
-                     const uint32_t* tk, const uint32_t* tedge,
-                     const uint64_t* off, uint32_t* cval, uint32_t* cseg,
-                     PipeLaunch L);
+                     uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
+                     unsigned long long* ctr, PipeLaunch L);
+void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
+                     const uint32_t* tedge, const uint32_t* teik,
+                     const uint32_t* tejk, const uint64_t* off, uint32_t* cval,
+                     uint32_t* cseg, unsigned long long* ctr, PipeLaunch L);
 /* kind 1 edges, 2 triangles, 3 tets */
 void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
@@ -138,7 +135,9 @@ void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
 void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
                          const uint64_t* pos, uint32_t* out_val,
-                         uint32_t* out_seg, uint4* out_tet, PipeLaunch L);
+                         uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
+                         const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
+                         PipeLaunch L);
 
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =
@@ -176,9 +175,8 @@ void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
 void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
-void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
-                       const uint32_t* eleft, const uint32_t* tk,
-                       const uint32_t* tedge, const uint64_t* erow,
+void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+                       const uint32_t* teik, const uint32_t* tejk,
                        uint8_t* edge_in, PipeLaunch L);
 void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
                       int32_t* out, PipeLaunch L);
