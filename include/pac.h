@@ -49,7 +49,8 @@ typedef struct pac_options {
   int32_t block;         /* threads per block override (0 = tuned)        */
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
   int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
-  int32_t reserved[5];
+  int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
+  int32_t reserved[4];
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -111,8 +112,11 @@ int pac_result_stats(const pac_result* r, pac_stats* st);
 void pac_result_free(pac_result* r);
 
 /* Batched ensemble: n_mol molecules concatenated, offsets[n_mol+1]
- * (int64) into the ball array.  All molecules are processed by one graph
- * launch on the device; per-molecule lists are returned with offsets.  No
+ * (int64, offsets[0] = 0) into the ball array (n x 4 FP32, device memory
+ * with opt.device_inputs).  Every molecule gets its own reference grid and
+ * its lists equal those of pac_compute on that molecule alone; molecules
+ * are packed into pipeline passes of at most opt.batch_balls balls, each
+ * pass processing all its molecules with one launch per kernel.  No
  * reference counterpart (the reference handles one molecule per process). */
 typedef struct pac_batch_result pac_batch_result;
 int pac_compute_batch(const float* xyzr, const int64_t* offsets, int64_t n_mol,

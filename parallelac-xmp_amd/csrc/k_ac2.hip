@@ -138,7 +138,7 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 /* per-simplex setup at precision R: D, N with a-priori error bounds, the
  * certified centre box and the semi-static pair constants. */
 template <class R>
-__device__ __forceinline__ bool tet_setup(const Stage& S, const float4 v[4],
+__device__ __forceinline__ bool tet_setup(const GridP& g, const float4 v[4],
                                           PairSS<R>& P, CellBox& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
@@ -175,7 +175,7 @@ __device__ __forceinline__ bool tet_setup(const Stage& S, const float4 v[4],
   double cx = (double)v[0].x + (double)Nf.x.v / D2;
   double cy = (double)v[0].y + (double)Nf.y.v / D2;
   double cz = (double)v[0].z + (double)Nf.z.v / D2;
-  return make_box(S.g, S.reach, S.margin, cx, cy, cz, quot_err(Nf.x, Df, 2.0),
+  return make_box(g, cx, cy, cz, quot_err(Nf.x, Df, 2.0),
                   quot_err(Nf.y, Df, 2.0), quot_err(Nf.z, Df, 2.0), box);
 }
 
@@ -197,9 +197,10 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     uint4 t = tets[id];
     float4 v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
+    const GridP g = grid_of(S, t.x);
     PairSS<R> P;
     CellBox box;
-    if (!tet_setup(S, v, P, box)) {
+    if (!tet_setup(g, v, P, box)) {
       if (!PHASE_B) {
         push_q(q1, (uint32_t)id, 0xffffffffu);
         flag[id] = 2;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(S.g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
+    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
       if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -324,11 +325,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
         ez = quot_err(tdd.N.z, tdd.D, 2.0);
       }
       CellBox box;
-      if (!make_box(S.g, S.reach, S.margin, cx, cy, cz, ex, ey, ez, box)) {
+      const GridP g = grid_of(S, tt.x);
+      if (!make_box(g, cx, cy, cz, ex, ey, ez, box)) {
         push_q(q2, e.item, 0xffffffffu);
       } else {
         bool dead = false;
-        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+        scan_box(g, S.cstart, box, [&](uint32_t p) {
           if (p == tt.x || p == tt.y || p == tt.z || p == tt.w) return false;
           int c = pair(p);
           if (c == 1) {
@@ -369,7 +371,7 @@ __device__ int tet_pair_exact(const Stage& S, const float4 v[4],
 
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     Stage S, const QEntry* q, uint32_t nq, const uint4* tets,
-    const uint32_t* perm, uint8_t* flag, double reach, double margin) {
+    const uint32_t* perm, uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0, sos = 0;
   if (t < nq) {
@@ -395,10 +397,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
       double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
       CellBox box;
       bool viol = false;
-      if (make_box(S.g, reach, margin, cx, cy, cz, err, err, err, box)) {
+      const GridP g = grid_of(S, tt.x);
+      if (make_box(g, cx, cy, cz, err, err, err, box)) {
         for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
           for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
-            int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+            int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
             uint32_t b = S.cstart[row + box.lo[0]],
                      en = S.cstart[row + box.hi[0] + 1];
             for (uint32_t p = b; p < en; ++p) {
@@ -454,7 +457,7 @@ __device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
 
 
 template <class R, int KIND>
-__device__ __forceinline__ bool low_setup(const Stage& S, const float4* v,
+__device__ __forceinline__ bool low_setup(const GridP& g, const float4* v,
                                           PairSS<R>& P, CellBox& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
@@ -503,7 +506,7 @@ __device__ __forceinline__ bool low_setup(const Stage& S, const float4* v,
   double cx = (double)v[0].x + (double)num.x.v / d2;
   double cy = (double)v[0].y + (double)num.y.v / d2;
   double cz = (double)v[0].z + (double)num.z.v / d2;
-  return make_box(S.g, S.reach, S.margin, cx, cy, cz,
+  return make_box(g, cx, cy, cz,
                   quot_err(num.x, den, 2.0), quot_err(num.y, den, 2.0),
                   quot_err(num.z, den, 2.0), box);
 }
@@ -538,9 +541,10 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       vid[2] = vid[1];
     }
     float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+    const GridP g = grid_of(S, vid[0]);
     PairSS<R> P;
     CellBox box;
-    if (!low_setup<R, KIND>(S, v, P, box)) {
+    if (!low_setup<R, KIND>(g, v, P, box)) {
       if (!PHASE_B) {
         push_q(q1, (uint32_t)id, 0xffffffffu);
         flag[id] = 2;
@@ -549,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(S.g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
+    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
       if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -670,11 +674,12 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
         }
       }
       CellBox box;
-      if (!ok || !make_box(S.g, S.reach, S.margin, cx, cy, cz, ex, ey, ez, box)) {
+      const GridP g = grid_of(S, vid[0]);
+      if (!ok || !make_box(g, cx, cy, cz, ex, ey, ez, box)) {
         push_q(q2, e.item, 0xffffffffu);
       } else {
         bool dead = false;
-        scan_box(S.g, S.cstart, box, [&](uint32_t p) {
+        scan_box(g, S.cstart, box, [&](uint32_t p) {
           if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
           int c = pair(p);
           if (c == 1) {
@@ -696,7 +701,7 @@ template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
-    uint8_t* flag, double reach, double margin) {
+    uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0;
   if (t < nq) {
@@ -739,10 +744,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
       double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
       CellBox box;
       bool viol = false;
-      if (make_box(S.g, reach, margin, cx, cy, cz, err, err, err, box)) {
+      const GridP g = grid_of(S, vid[0]);
+      if (make_box(g, cx, cy, cz, err, err, err, box)) {
         for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
           for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
-            int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
+            int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
             uint32_t bb = S.cstart[row + box.lo[0]],
                      en = S.cstart[row + box.hi[0] + 1];
             for (uint32_t p = bb; p < en; ++p) {
@@ -800,8 +806,7 @@ void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
                           uint8_t* flag, PipeLaunch L) {
   if (!nq) return;
   hipLaunchKernelGGL(k_tet_ac2_exact, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, tets, perm, flag, S.reach,
-                     S.margin);
+                     dim3(kBlock), 0, L.s, S, q, nq, tets, perm, flag);
 }
 template <class R, int KIND>
 static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
@@ -862,10 +867,10 @@ void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
   dim3 gr((nq + kBlock - 1) / kBlock);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q, nq,
-                       ecol, eleft, tk, tedge, flag, S.reach, S.margin);
+                       ecol, eleft, tk, tedge, flag);
   else
     hipLaunchKernelGGL(k_low_ac2_exact<1>, gr, dim3(kBlock), 0, L.s, S, q, nq,
-                       ecol, eleft, tk, tedge, flag, S.reach, S.margin);
+                       ecol, eleft, tk, tedge, flag);
 }
 
 int sos_table_host(int32_t* out3, int cap) {

@@ -55,6 +55,41 @@ __device__ void bred_merge(BRed& a, const BRed& b) {
   a.shift = max(a.shift, b.shift);
   a.nonfinite += b.nonfinite;
 }
+__device__ void bred_ball(BRed& b, const float4& a, float solvent) {
+  if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z) || !isfinite(a.w)) {
+    b.nonfinite++;
+    return;
+  }
+  float r = __fadd_rn(a.w, solvent);
+  b.v[0] = fminf(b.v[0], a.x);
+  b.v[1] = fminf(b.v[1], a.y);
+  b.v[2] = fminf(b.v[2], a.z);
+  b.v[3] = fminf(b.v[3], -a.x);
+  b.v[4] = fminf(b.v[4], -a.y);
+  b.v[5] = fminf(b.v[5], -a.z);
+  b.v[6] = fminf(b.v[6], -a.w);
+  b.v[7] = fminf(b.v[7], -fabsf(r));
+  b.v[8] = fminf(b.v[8], -fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))));
+  int s = max(max(neg_lsb_exp(a.x), neg_lsb_exp(a.y)),
+              max(neg_lsb_exp(a.z), neg_lsb_exp(r)));
+  b.shift = max(b.shift, s);
+}
+__device__ Bounds bred_out(const BRed& b) {
+  Bounds o;
+  o.minx = b.v[0];
+  o.miny = b.v[1];
+  o.minz = b.v[2];
+  o.maxx = -b.v[3];
+  o.maxy = -b.v[4];
+  o.maxz = -b.v[5];
+  o.max_raw_r = -b.v[6];
+  o.max_r = -b.v[7];
+  o.max_abs = -b.v[8];
+  o.shift = b.shift;
+  o.nonfinite = b.nonfinite;
+  o.pad = 0;
+  return o;
+}
 __device__ void bred_block(BRed& mine, BRed* sh) {
   sh[threadIdx.x] = mine;
   __syncthreads();
@@ -72,26 +107,8 @@ __global__ __launch_bounds__(kBlock) void k_bounds_part(const float4* raw,
   __shared__ BRed sh[kBlock];
   BRed b;
   bred_init(b);
-  for (int64_t i = gtid(); i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    float4 a = raw[i];
-    if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z) || !isfinite(a.w)) {
-      b.nonfinite++;
-      continue;
-    }
-    float r = __fadd_rn(a.w, solvent);
-    b.v[0] = fminf(b.v[0], a.x);
-    b.v[1] = fminf(b.v[1], a.y);
-    b.v[2] = fminf(b.v[2], a.z);
-    b.v[3] = fminf(b.v[3], -a.x);
-    b.v[4] = fminf(b.v[4], -a.y);
-    b.v[5] = fminf(b.v[5], -a.z);
-    b.v[6] = fminf(b.v[6], -a.w);
-    b.v[7] = fminf(b.v[7], -fabsf(r));
-    b.v[8] = fminf(b.v[8], -fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))));
-    int s = max(max(neg_lsb_exp(a.x), neg_lsb_exp(a.y)),
-                max(neg_lsb_exp(a.z), neg_lsb_exp(r)));
-    b.shift = max(b.shift, s);
-  }
+  for (int64_t i = gtid(); i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bred_ball(b, raw[i], solvent);
   bred_block(b, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
@@ -102,22 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const BRed* part,
   bred_init(b);
   for (int i = threadIdx.x; i < np; i += blockDim.x) bred_merge(b, part[i]);
   bred_block(b, sh);
-  if (threadIdx.x == 0) {
-    Bounds o;
-    o.minx = b.v[0];
-    o.miny = b.v[1];
-    o.minz = b.v[2];
-    o.maxx = -b.v[3];
-    o.maxy = -b.v[4];
-    o.maxz = -b.v[5];
-    o.max_raw_r = -b.v[6];
-    o.max_r = -b.v[7];
-    o.max_abs = -b.v[8];
-    o.shift = b.shift;
-    o.nonfinite = b.nonfinite;
-    o.pad = 0;
-    *out = o;
-  }
+  if (threadIdx.x == 0) *out = bred_out(b);
 }
 void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
                    Bounds* out, PipeLaunch L) {
@@ -129,25 +131,73 @@ void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
                      (const BRed*)part, nb, out);
 }
 
-/* computeCellIndex (LD:65-78): FP32 subtract, IEEE divide, truncate. */
+/* Per-molecule bounds of a batch: block m reduces the balls of molecule m
+ * (a few hundred to tens of thousands each). */
+__global__ __launch_bounds__(kBlock) void k_bounds_mol(const float4* raw,
+                                                       const uint32_t* moff,
+                                                       float solvent,
+                                                       Bounds* out) {
+  __shared__ BRed sh[kBlock];
+  const uint32_t m = blockIdx.x, b0 = moff[m], b1 = moff[m + 1];
+  BRed b;
+  bred_init(b);
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+    bred_ball(b, raw[i], solvent);
+  bred_block(b, sh);
+  if (threadIdx.x == 0) out[m] = bred_out(b);
+}
+void launch_bounds_mol(const float4* raw, const uint32_t* moff, int64_t nmol,
+                       float solvent, Bounds* out, PipeLaunch L) {
+  if (nmol <= 0) return;
+  hipLaunchKernelGGL(k_bounds_mol, dim3((unsigned)nmol), dim3(kBlock), 0, L.s,
+                     raw, moff, solvent, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mol_ids(const uint32_t* moff,
+                                                    int64_t nmol, int64_t n,
+                                                    uint32_t* mol) {
+  int64_t i = gtid();
+  if (i >= n) return;
+  /* last m with moff[m] <= i */
+  int64_t lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if ((int64_t)moff[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  mol[i] = (uint32_t)lo;
+}
+void launch_mol_ids(const uint32_t* moff, int64_t nmol, int64_t n,
+                    uint32_t* mol, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mol_ids, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, moff, nmol, n, mol);
+}
+
+/* computeCellIndex (LD:65-78): FP32 subtract, IEEE divide, truncate; in a
+ * batch the molecule's first global cell id is added. */
 __global__ __launch_bounds__(kBlock) void k_cell_key(const float4* raw,
-                                                     int64_t n, GridP g,
+                                                     int64_t n, GridP g1,
+                                                     const GridP* grids,
+                                                     const uint32_t* mol,
                                                      uint32_t* key,
                                                      uint32_t* idx) {
   int64_t i = gtid();
   if (i >= n) return;
+  const GridP g = mol ? grids[mol[i]] : g1;
   float4 a = raw[i];
   int cx = (int)__fdiv_rn(__fsub_rn(a.x, g.minx), g.step);
   int cy = (int)__fdiv_rn(__fsub_rn(a.y, g.miny), g.step);
   int cz = (int)__fdiv_rn(__fsub_rn(a.z, g.minz), g.step);
-  key[i] = (uint32_t)(cx + g.ex * cy + g.ex * g.ey * cz);
+  key[i] = g.cbase + (uint32_t)(cx + g.ex * cy + g.ex * g.ey * cz);
   idx[i] = (uint32_t)i;
 }
-void launch_cell_key(const float4* raw, int64_t n, GridP g, uint32_t* key,
+void launch_cell_key(const float4* raw, int64_t n, GridP g,
+                     const GridP* grids, const uint32_t* mol, uint32_t* key,
                      uint32_t* idx, PipeLaunch L) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_cell_key, dim3((unsigned)((n + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, raw, n, g, key, idx);
+                     dim3(kBlock), 0, L.s, raw, n, g, grids, mol, key, idx);
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather(const float4* raw,
@@ -262,15 +312,16 @@ __device__ __forceinline__ void cell_xyz(uint32_t c, const GridP& g, int& x,
 __global__ __launch_bounds__(kBlock) void k_edge_count(Stage S, uint32_t* cnt) {
   int64_t i = gtid();
   if (i >= S.n) return;
+  const GridP g = grid_of(S, (uint32_t)i);
   uint32_t c = S.cellk[i];
   int gx, gy, gz;
-  cell_xyz(c, S.g, gx, gy, gz);
-  const int k = S.g.k_enum;
+  cell_xyz(c - g.cbase, g, gx, gy, gz);
+  const int k = g.k_enum;
   uint32_t total = 0;
-  for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
-    for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
-      int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
-      int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
+  for (int z = max(gz - k, 0); z <= min(gz + k, g.ez - 1); ++z)
+    for (int y = max(gy - k, 0); y <= min(gy + k, g.ey - 1); ++y) {
+      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      int x0 = max(gx - k, 0), x1 = min(gx + k, g.ex - 1);
       int64_t c0 = row + x0, c1 = row + x1;
       if (c1 < (int64_t)c) continue;
       uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
@@ -300,15 +351,16 @@ __global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
                                                       uint32_t* cseg) {
   int64_t i = gtid();
   if (i >= S.n) return;
+  const GridP g = grid_of(S, (uint32_t)i);
   uint32_t c = S.cellk[i];
   int gx, gy, gz;
-  cell_xyz(c, S.g, gx, gy, gz);
-  const int k = S.g.k_enum;
+  cell_xyz(c - g.cbase, g, gx, gy, gz);
+  const int k = g.k_enum;
   uint64_t o = off[i];
-  for (int z = max(gz - k, 0); z <= min(gz + k, S.g.ez - 1); ++z)
-    for (int y = max(gy - k, 0); y <= min(gy + k, S.g.ey - 1); ++y) {
-      int64_t row = (int64_t)S.g.ex * (y + (int64_t)S.g.ey * z);
-      int x0 = max(gx - k, 0), x1 = min(gx + k, S.g.ex - 1);
+  for (int z = max(gz - k, 0); z <= min(gz + k, g.ez - 1); ++z)
+    for (int y = max(gy - k, 0); y <= min(gy + k, g.ey - 1); ++y) {
+      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      int x0 = max(gx - k, 0), x1 = min(gx + k, g.ex - 1);
       int64_t c0 = row + x0, c1 = row + x1;
       if (c1 < (int64_t)c) continue;
       uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];

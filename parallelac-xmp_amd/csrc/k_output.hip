@@ -144,4 +144,30 @@ void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
                        hipMemcpyDeviceToDevice, s);
 }
 
+/* Rows of a batch's list are grouped by molecule in molecule order and the
+ * first id of a row lies in its molecule's id range, so "first id >=
+ * moff[m]" is monotone over the rows: out[m] = first row of molecule m. */
+__global__ __launch_bounds__(kBlock) void k_mol_rows(const int32_t* rows,
+                                                    int64_t m, int K,
+                                                    const uint32_t* moff,
+                                                    int64_t nmol,
+                                                    uint64_t* out) {
+  int64_t q = gtid();
+  if (q > nmol) return;
+  uint32_t key = moff[q];
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((uint32_t)rows[(int64_t)K * mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  out[q] = (uint64_t)lo;
+}
+void launch_mol_rows(const int32_t* rows, int64_t m, int K,
+                     const uint32_t* moff, int64_t nmol, uint64_t* out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_mol_rows, dim3((unsigned)((nmol + 1 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, rows, m, K, moff, nmol, out);
+}
+
 }  // namespace pac

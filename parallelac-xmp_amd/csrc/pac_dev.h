@@ -170,10 +170,14 @@ struct CellBox {
   int lo[3], hi[3];
   int cc[3]; /* cell of the centre, clamped into the box */
 };
-__device__ __forceinline__ bool make_box(const GridP& g, double reach,
-                                         double margin, double cx, double cy,
+/* grid of the molecule that ball `b` (sorted id) belongs to */
+__device__ __forceinline__ GridP grid_of(const Stage& S, uint32_t b) {
+  return S.mol ? S.grids[S.mol[b]] : S.g;
+}
+__device__ __forceinline__ bool make_box(const GridP& g, double cx, double cy,
                                          double cz, double ex, double ey,
                                          double ez, CellBox& b) {
+  const double reach = g.reach, margin = g.margin;
   double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
   double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
   int ext[3] = {g.ex, g.ey, g.ez};
@@ -210,13 +214,14 @@ __device__ __forceinline__ void scan_box(const GridP& g,
                                          const CellBox& b, F&& visit) {
   if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
   {
-    int64_t c = b.cc[0] + (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
+    int64_t c = (int64_t)g.cbase + b.cc[0] +
+                (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
     for (uint32_t p = cstart[c], e = cstart[c + 1]; p < e; ++p)
       if (visit(p)) return;
   }
   for (int z = b.lo[2]; z <= b.hi[2]; ++z)
     for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
-      int64_t row = (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
       bool crow = (z == b.cc[2] && y == b.cc[1]);
       if (!crow) {
         for (uint32_t p = cstart[row + b.lo[0]], e = cstart[row + b.hi[0] + 1];
@@ -261,12 +266,13 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
                                           F&& visit) {
   if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
   {
-    int64_t c = b.cc[0] + (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
+    int64_t c = (int64_t)g.cbase + b.cc[0] +
+                (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
     if (scan_range4(ball, cstart[c], cstart[c + 1], visit)) return;
   }
   for (int z = b.lo[2]; z <= b.hi[2]; ++z)
     for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
-      int64_t row = (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
       if (!(z == b.cc[2] && y == b.cc[1])) {
         if (scan_range4(ball, cstart[row + b.lo[0]], cstart[row + b.hi[0] + 1],
                         visit))

@@ -321,22 +321,17 @@ static void zero_u32(unsigned* p, int n, hipStream_t s) {
 /* ------------------------------------------------------------------ */
 /* the pipeline                                                        */
 /* ------------------------------------------------------------------ */
-static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
-                         float alpha, const pac_options& opt, pac_result* R,
-                         hipStream_t s, Timer& T, KProf& KP) {
-  Pool* P = &E->pool;
-  PipeLaunch L{s, kBlock};
-  pac_stats& st = R->st;
-  st.n = n;
-  if (n == 0) return;
-  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
-
-  /* ---- bounds, grid (AlphaComplexParallel.cu:1089-1124) ---- */
-  DBuf<Bounds> bpart(P, 1024 * 2 + 4), bout(P, 1);
-  launch_bounds(raw, n, solvent, bpart.get(), bout.get(), L);
-  KCHK();
-  Bounds B = readback(E, s, bout.get());
-  if (B.nonfinite) fail(PAC_ERR_INVALID, "non-finite coordinate or radius");
+/* Grid of one molecule from its bounds (AlphaComplexParallel.cu:1120-1124,
+ * evaluated in FP32 exactly as there), with the stencil half-widths that
+ * make the enumeration and the emptiness boxes complete. */
+struct GridPlan {
+  GridP g;
+  int64_t G = 0;
+};
+static GridPlan plan_grid(const Bounds& B, float alpha, int policy) {
+  GridPlan P;
+  GridP& g = P.g;
+  std::memset(&g, 0, sizeof(g));
   volatile float m2 = B.max_raw_r * B.max_raw_r;
   volatile float m2a = m2 + alpha;
   volatile float pmr = sqrtf(m2a);
@@ -347,9 +342,8 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                  qz = (B.maxz - B.minz) / step;
   int64_t ex = (int64_t)(int)qx + 1, ey = (int64_t)(int)qy + 1,
           ez = (int64_t)(int)qz + 1;
-  int64_t G = ex * ey * ez;
-  if (G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grid has too many cells");
-  GridP g;
+  P.G = ex * ey * ez;
+  if (P.G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grid has too many cells");
   g.minx = B.minx;
   g.miny = B.miny;
   g.minz = B.minz;
@@ -359,12 +353,11 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   g.ez = (int)ez;
   double maxw = (double)B.max_r * (double)B.max_r;
   double reach2 = maxw + (double)alpha;
-  bool no_edges = reach2 < 0;
   double reach = reach2 > 0 ? std::sqrt(reach2) * (1 + 1e-12) + 1e-12 : 0.0;
   double maxext = std::max({(double)B.maxx - B.minx, (double)B.maxy - B.miny,
                             (double)B.maxz - B.minz});
   double margin = (maxext + (double)step) * 3.0 * 0x1p-24 + 1e-9;
-  if (opt.cell_policy == PAC_CELL_REFERENCE_RAW) {
+  if (policy == PAC_CELL_REFERENCE_RAW) {
     g.k_enum = 1;
     g.k_ac2 = 1;
     reach = (double)step * (1 - 1e-6);
@@ -373,27 +366,105 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
     g.k_enum = std::max(1, (int)std::ceil((2 * reach + 2 * margin) / step));
     g.k_ac2 = std::max(1, (int)std::ceil((reach + margin) / step));
   }
-  st.k_enum = g.k_enum;
-  st.k_ac2 = g.k_ac2;
+  g.reach = reach;
+  g.margin = margin;
+  return P;
+}
+
+/* One pipeline pass over n balls.  nmol == 1: one molecule; otherwise the
+ * balls are nmol molecules, moff[0..nmol] (host, relative to raw), each with
+ * its own grid, processed together by every kernel launch. */
+static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
+                         float alpha, const pac_options& opt, pac_result* R,
+                         hipStream_t s, Timer& T, KProf& KP,
+                         const int64_t* moff = nullptr, int64_t nmol = 1) {
+  Pool* P = &E->pool;
+  PipeLaunch L{s, kBlock};
+  pac_stats& st = R->st;
+  st.n = n;
+  if (n == 0) return;
+  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
+  const bool batch = nmol > 1;
+
+  /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
+  std::vector<Bounds> hb(std::max<int64_t>(nmol, 1));
+  DBuf<uint32_t> dmoff, dmol;
+  DBuf<GridP> dgrids;
+  if (!batch) {
+    DBuf<Bounds> bpart(P, 1024 * 2 + 4), bout(P, 1);
+    launch_bounds(raw, n, solvent, bpart.get(), bout.get(), L);
+    KCHK();
+    hb[0] = readback(E, s, bout.get());
+  } else {
+    std::vector<uint32_t> mo(nmol + 1);
+    for (int64_t m = 0; m <= nmol; ++m) mo[m] = (uint32_t)moff[m];
+    dmoff = DBuf<uint32_t>(P, nmol + 1);
+    HIPCHK(hipMemcpyAsync(dmoff.get(), mo.data(), (nmol + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    DBuf<Bounds> bout(P, nmol);
+    launch_bounds_mol(raw, dmoff.get(), nmol, solvent, bout.get(), L);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(hb.data(), bout.get(), nmol * sizeof(Bounds),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  std::vector<GridP> grids(hb.size());
+  int64_t G = 0;
+  int shift = 0;
+  double mag = 1.0;
+  bool no_edges = true;
+  st.k_enum = st.k_ac2 = 0;
+  for (size_t m = 0; m < hb.size(); ++m) {
+    const Bounds& B = hb[m];
+    if (B.nonfinite) fail(PAC_ERR_INVALID, "non-finite coordinate or radius");
+    bool empty = batch && moff[m + 1] == moff[m];
+    GridPlan gp;
+    if (empty) { /* one empty cell, never visited */
+      std::memset(&gp.g, 0, sizeof(gp.g));
+      gp.g.step = 1;
+      gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
+      gp.G = 1;
+    } else {
+      gp = plan_grid(B, alpha, opt.cell_policy);
+      shift = std::max(shift, B.shift);
+      mag = std::max({mag, (double)B.max_abs, (double)B.max_r});
+      if ((double)B.max_r * (double)B.max_r + (double)alpha >= 0) no_edges = false;
+    }
+    gp.g.cbase = (uint32_t)G;
+    G += gp.G;
+    if (G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grids have too many cells");
+    grids[m] = gp.g;
+    st.k_enum = std::max(st.k_enum, gp.g.k_enum);
+    st.k_ac2 = std::max(st.k_ac2, gp.g.k_ac2);
+  }
   st.grid_cells = (int32_t)G;
 
-  /* exact embedding scale */
-  int S = std::max(B.shift, (neg_lsb_exp_host(alpha) + 1) / 2);
-  double mag = std::max({(double)B.max_abs, (double)B.max_r,
-                         std::sqrt(std::fabs((double)alpha)), 1.0});
+  /* exact embedding scale (one for all molecules: any common scale at
+   * which every input is an integer gives the same exact signs) */
+  int S = std::max(shift, (neg_lsb_exp_host(alpha) + 1) / 2);
+  mag = std::max(mag, std::sqrt(std::fabs((double)alpha)));
   double bits = std::log2(mag) + S + 2;
   if (8 * bits + 16 > 32.0 * WL - 2)
     fail(PAC_ERR_RANGE, "input exponent range exceeds the 512-bit exact stage");
   st.scale_bits = S;
   Stage SG;
+  std::memset(&SG, 0, sizeof(SG));
   SG.n = n;
-  SG.g = g;
+  SG.g = grids[0];
+  if (batch) {
+    dgrids = DBuf<GridP>(P, nmol);
+    HIPCHK(hipMemcpyAsync(dgrids.get(), grids.data(), nmol * sizeof(GridP),
+                          hipMemcpyHostToDevice, s));
+    dmol = DBuf<uint32_t>(P, n);
+    launch_mol_ids(dmoff.get(), nmol, n, dmol.get(), L);
+    KCHK();
+    SG.grids = dgrids.get();
+    SG.mol = dmol.get();
+  }
   SG.alpha = alpha;
   SG.ex.S = S;
   SG.ex.alpha = wi_from_scaled_host((double)alpha, 2 * S);
   SG.staged = opt.precision == PAC_PREC_STAGED;
-  SG.reach = reach;
-  SG.margin = margin;
   DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
   hipLaunchKernelGGL(k_zero_u64, dim3((CTR_COUNT * CTR_SLOTS + 255) / 256),
                      dim3(256), 0, s, ctr.get(), CTR_COUNT * CTR_SLOTS);
@@ -402,7 +473,7 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
-  launch_cell_key(raw, n, g, key.get(), idx.get(), L);
+  launch_cell_key(raw, n, SG.g, SG.grids, SG.mol, key.get(), idx.get(), L);
   KCHK();
   int kbits = 0;
   while (kbits < 32 && ((int64_t)1 << kbits) < G) ++kbits;
@@ -778,6 +849,29 @@ __global__ void k_balls_orig(const float4* raw, int64_t n, float solvent,
   out[i] = make_float4(a.x, a.y, a.z, __fadd_rn(a.w, solvent));
 }
 
+/* canonical order: vertices of each row by original id, rows lexicographic */
+static void canonicalize(Engine* E, pac_result* R, int64_t n, hipStream_t s) {
+  int idbits = 1;
+  while (idbits < 32 && ((int64_t)1 << idbits) < n) ++idbits;
+  for (int d = 1; d <= 3; ++d) {
+    int64_t m = R->counts[d];
+    if (m <= 0) continue;
+    DBuf<uint32_t> a(&E->pool, m), b(&E->pool, m), c(&E->pool, m),
+        e(&E->pool, m);
+    DBuf<int32_t> copy(&E->pool, m * (d + 1));
+    size_t rtb = radix_tmp_bytes(m);
+    DBuf<uint8_t> rt(&E->pool, rtb);
+    canonical_rows(d + 1, m, R->lists[d - 1].get(), idbits, a.get(), b.get(),
+                   c.get(), e.get(), rt.get(), rtb, copy.get(), s);
+    KCHK();
+    HIPCHK(hipStreamSynchronize(s));
+  }
+}
+
+static void fill_phase_ms(Timer& T, pac_stats& st) {
+  for (int q = 0; q < 8; ++q) st.phase_ms[q] = T.ms(q, q + 1);
+}
+
 static void compute_f32(const float* xyzr, int64_t n, float solvent,
                         float alpha, const pac_options* o, pac_result** out) {
   if (!out) fail(PAC_ERR_INVALID, "out is NULL");
@@ -817,42 +911,170 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
                        dim3(256), 0, s, raw, n, solvent, R->balls.get());
     KCHK();
     run_pipeline(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
-    if (opt.order == PAC_ORDER_CANONICAL) {
-      /* vertices by original id, rows lexicographic */
-      int idbits = 1;
-      while (idbits < 32 && ((int64_t)1 << idbits) < n) ++idbits;
-      for (int d = 1; d <= 3; ++d) {
-        int64_t m = R->counts[d];
-        if (m <= 0) continue;
-        DBuf<uint32_t> a(&E->pool, m), b(&E->pool, m), c(&E->pool, m),
-            e(&E->pool, m);
-        DBuf<int32_t> copy(&E->pool, m * (d + 1));
-        size_t rtb = radix_tmp_bytes(m);
-        DBuf<uint8_t> rt(&E->pool, rtb);
-        canonical_rows(d + 1, m, R->lists[d - 1].get(), idbits, a.get(),
-                       b.get(), c.get(), e.get(), rt.get(), rtb, copy.get(), s);
-        KCHK();
-        HIPCHK(hipStreamSynchronize(s));
-      }
-    }
+    if (opt.order == PAC_ORDER_CANONICAL) canonicalize(E, R.get(), n, s);
   } else {
     for (int i = 2; i <= 8; ++i) T.mark(i);
   }
   HIPCHK(hipStreamSynchronize(s));
   auto t_host1 = std::chrono::steady_clock::now();
   pac_stats& st = R->st;
-  st.phase_ms[0] = T.ms(0, 1);
-  st.phase_ms[1] = T.ms(1, 2);
-  st.phase_ms[2] = T.ms(2, 3);
-  st.phase_ms[3] = T.ms(3, 4);
-  st.phase_ms[4] = T.ms(4, 5);
-  st.phase_ms[5] = T.ms(5, 6);
-  st.phase_ms[6] = T.ms(6, 7);
-  st.phase_ms[7] = T.ms(7, 8);
+  fill_phase_ms(T, st);
   st.total_ms =
       std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
   KP.collect(st.kernel_ms);
   *out = R.release();
+}
+
+/* ------------------------------------------------------------------ */
+/* batched ensembles                                                   */
+/* ------------------------------------------------------------------ */
+struct pac_batch_result {
+  Engine* eng = nullptr;
+  int64_t nmol = 0;
+  std::vector<int64_t> off;                        /* ball offsets, nmol+1 */
+  std::vector<std::unique_ptr<pac_result>> groups; /* one per pipeline pass */
+  std::vector<int64_t> group_first;                /* first molecule of each group */
+  std::vector<int32_t> mol_group;
+  std::vector<int64_t> counts;                     /* nmol x 4 */
+  std::vector<uint64_t> row0[3];                   /* first row of each molecule */
+  pac_stats st;
+};
+
+static void add_stats(pac_stats& a, const pac_stats& b) {
+  a.n += b.n;
+  a.edge_candidates += b.edge_candidates;
+  a.edge_alpha += b.edge_alpha;
+  a.edges_final += b.edges_final;
+  a.tri_candidates += b.tri_candidates;
+  a.tri_alpha += b.tri_alpha;
+  a.tris_final += b.tris_final;
+  a.tet_candidates += b.tet_candidates;
+  a.tet_alpha += b.tet_alpha;
+  a.tets_final += b.tets_final;
+  for (int f = 0; f < 6; ++f)
+    for (int q = 0; q < 4; ++q) a.stage_evals[f][q] += b.stage_evals[f][q];
+  a.sos_calls += b.sos_calls;
+  for (int q = 0; q < 3; ++q) a.ac2_pairs[q] += b.ac2_pairs[q];
+  a.k_enum = std::max(a.k_enum, b.k_enum);
+  a.k_ac2 = std::max(a.k_ac2, b.k_ac2);
+  a.scale_bits = std::max(a.scale_bits, b.scale_bits);
+  a.grid_cells = (int32_t)std::min<int64_t>((int64_t)a.grid_cells + b.grid_cells,
+                                            INT32_MAX);
+  for (int q = 0; q < 8; ++q) {
+    a.phase_ms[q] += b.phase_ms[q];
+    a.kernel_ms[q] += b.kernel_ms[q];
+  }
+}
+
+/* Molecules are packed, in order, into passes of at most `cap` balls (a
+ * molecule larger than cap gets a pass of its own); each pass runs the
+ * whole pipeline once over its concatenated balls, every molecule with its
+ * own grid. */
+static void compute_batch(const float* xyzr, const int64_t* offsets,
+                          int64_t nmol, float solvent, float alpha,
+                          const pac_options* o, pac_batch_result** out) {
+  if (!out) fail(PAC_ERR_INVALID, "out is NULL");
+  if (nmol < 0 || (nmol > 0 && !offsets)) fail(PAC_ERR_INVALID, "bad offsets");
+  if (!std::isfinite(solvent) || !std::isfinite(alpha))
+    fail(PAC_ERR_INVALID, "non-finite solvent or alpha");
+  if (nmol > 0 && offsets[0] != 0) fail(PAC_ERR_INVALID, "offsets[0] must be 0");
+  for (int64_t m = 0; m < nmol; ++m)
+    if (offsets[m + 1] < offsets[m])
+      fail(PAC_ERR_INVALID, "offsets must be non-decreasing");
+  const int64_t ntot = nmol > 0 ? offsets[nmol] : 0;
+  if (ntot > 0 && !xyzr) fail(PAC_ERR_INVALID, "bad input array");
+  pac_options opt;
+  pac_options_init(&opt);
+  if (o) opt = *o;
+  const int64_t cap = opt.batch_balls > 0 ? opt.batch_balls : (int64_t)4 << 20;
+  Engine* E = engine_for(opt.device);
+  std::lock_guard<std::mutex> lk(E->run_mu);
+  HIPCHK(hipSetDevice(E->device));
+  hipStream_t s = opt.stream ? (hipStream_t)opt.stream : E->stream;
+  std::unique_ptr<pac_batch_result> B(new pac_batch_result());
+  B->eng = E;
+  B->nmol = nmol;
+  B->off.assign(offsets, offsets + nmol + 1);
+  B->counts.assign(4 * nmol, 0);
+  B->mol_group.assign(nmol, 0);
+  for (auto& r : B->row0) r.assign(nmol, 0);
+  std::memset(&B->st, 0, sizeof(B->st));
+  auto t0 = std::chrono::steady_clock::now();
+  DBuf<float4> raw_own;
+  const float4* raw = (const float4*)xyzr;
+  if (ntot > 0 && !opt.device_inputs) {
+    raw_own = DBuf<float4>(&E->pool, ntot);
+    HIPCHK(hipMemcpyAsync(raw_own.get(), xyzr, (size_t)ntot * 16,
+                          hipMemcpyHostToDevice, s));
+    raw = raw_own.get();
+  }
+  for (int64_t m0 = 0; m0 < nmol;) {
+    int64_t m1 = m0 + 1;
+    while (m1 < nmol && offsets[m1 + 1] - offsets[m0] <= cap) ++m1;
+    const int64_t gm = m1 - m0, b0 = offsets[m0], gn = offsets[m1] - b0;
+    std::vector<int64_t> rel(gm + 1);
+    for (int64_t q = 0; q <= gm; ++q) rel[q] = offsets[m0 + q] - b0;
+    std::unique_ptr<pac_result> R(new pac_result());
+    R->eng = E;
+    R->alpha = alpha;
+    std::memset(&R->st, 0, sizeof(R->st));
+    R->counts[0] = gn;
+    Timer T(s);
+    KProf KP;
+    KP.on = opt.profile != 0;
+    KP.s = s;
+    T.mark(0);
+    T.mark(1);
+    if (gn > 0) {
+      run_pipeline(E, raw + b0, gn, solvent, alpha, opt, R.get(), s, T, KP,
+                   rel.data(), gm);
+      if (opt.order == PAC_ORDER_CANONICAL) canonicalize(E, R.get(), gn, s);
+    } else {
+      for (int i = 2; i <= 8; ++i) T.mark(i);
+    }
+    /* per-molecule row ranges of the group's lists */
+    if (gm == 1) {
+      for (int d = 1; d <= 3; ++d) B->row0[d - 1][m0] = 0;
+      for (int d = 1; d <= 3; ++d) B->counts[4 * m0 + d] = R->counts[d];
+    } else {
+      std::vector<uint32_t> mo(gm + 1);
+      for (int64_t q = 0; q <= gm; ++q) mo[q] = (uint32_t)rel[q];
+      DBuf<uint32_t> dmo(&E->pool, gm + 1);
+      DBuf<uint64_t> rows(&E->pool, 3 * (gm + 1));
+      HIPCHK(hipMemcpyAsync(dmo.get(), mo.data(), mo.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s));
+      for (int d = 1; d <= 3; ++d)
+        launch_mol_rows(R->lists[d - 1].get(), R->counts[d], d + 1, dmo.get(),
+                        gm, rows.get() + (d - 1) * (gm + 1), s);
+      KCHK();
+      std::vector<uint64_t> hr(3 * (gm + 1));
+      HIPCHK(hipMemcpyAsync(hr.data(), rows.get(), hr.size() * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      for (int d = 1; d <= 3; ++d)
+        for (int64_t q = 0; q < gm; ++q) {
+          const uint64_t* h = hr.data() + (d - 1) * (gm + 1);
+          B->row0[d - 1][m0 + q] = h[q];
+          B->counts[4 * (m0 + q) + d] = (int64_t)(h[q + 1] - h[q]);
+        }
+    }
+    for (int64_t q = 0; q < gm; ++q) {
+      B->counts[4 * (m0 + q)] = rel[q + 1] - rel[q];
+      B->mol_group[m0 + q] = (int32_t)B->groups.size();
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    fill_phase_ms(T, R->st);
+    KP.collect(R->st.kernel_ms);
+    add_stats(B->st, R->st);
+    B->group_first.push_back(m0);
+    B->groups.push_back(std::move(R));
+    m0 = m1;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  B->st.total_ms = std::chrono::duration<double, std::milli>(
+                       std::chrono::steady_clock::now() - t0)
+                       .count();
+  *out = B.release();
 }
 
 /* ------------------------------------------------------------------ */
@@ -932,17 +1154,39 @@ int pac_result_stats(const pac_result* r, pac_stats* st) {
 }
 void pac_result_free(pac_result* r) { delete r; }
 
-int pac_compute_batch(const float*, const int64_t*, int64_t, float, float,
-                      const pac_options*, pac_batch_result**) {
-  g_err = "pac_compute_batch: not built in this version";
-  return PAC_ERR_INVALID;
+int pac_compute_batch(const float* xyzr, const int64_t* offsets, int64_t n_mol,
+                      float solvent, float alpha, const pac_options* opt,
+                      pac_batch_result** out) {
+  return guarded([&] {
+    compute_batch(xyzr, offsets, n_mol, solvent, alpha, opt, out);
+  });
 }
-int pac_batch_counts(const pac_batch_result*, int64_t*) { return PAC_ERR_INVALID; }
-int pac_batch_copy(const pac_batch_result*, int64_t, int, int32_t*) {
-  return PAC_ERR_INVALID;
+int pac_batch_counts(const pac_batch_result* r, int64_t* out) {
+  if (!r || !out) return PAC_ERR_INVALID;
+  std::memcpy(out, r->counts.data(), r->counts.size() * sizeof(int64_t));
+  return PAC_OK;
 }
-int pac_batch_stats(const pac_batch_result*, pac_stats*) { return PAC_ERR_INVALID; }
-void pac_batch_free(pac_batch_result*) {}
+int pac_batch_copy(const pac_batch_result* r, int64_t m, int dim, int32_t* dst) {
+  if (!r || !dst || dim < 1 || dim > 3 || m < 0 || m >= r->nmol)
+    return PAC_ERR_INVALID;
+  return guarded([&] {
+    int64_t cnt = r->counts[4 * m + dim];
+    if (cnt <= 0) return;
+    const pac_result* g = r->groups[r->mol_group[m]].get();
+    size_t k = (size_t)(dim + 1);
+    HIPCHK(hipSetDevice(r->eng->device));
+    HIPCHK(hipMemcpy(dst, g->lists[dim - 1].get() + k * r->row0[dim - 1][m],
+                     (size_t)cnt * k * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int32_t base = (int32_t)(r->off[m] - r->off[r->group_first[r->mol_group[m]]]);
+    for (size_t q = 0; q < (size_t)cnt * k; ++q) dst[q] -= base;
+  });
+}
+int pac_batch_stats(const pac_batch_result* r, pac_stats* st) {
+  if (!r || !st) return PAC_ERR_INVALID;
+  *st = r->st;
+  return PAC_OK;
+}
+void pac_batch_free(pac_batch_result* r) { delete r; }
 int pac_alpha_adjoint(const pac_result*, const double*, const double*, double*,
                       double*) {
   g_err = "pac_alpha_adjoint: not built in this version";

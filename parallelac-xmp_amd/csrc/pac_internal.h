@@ -16,12 +16,18 @@ constexpr int kBlock = 256;      /* threads per block, 4 wave64s */
 constexpr int kScanItems = 8;    /* items per thread in scans / radix */
 constexpr int kTile = kBlock * kScanItems;
 
-/* Reference grid (AlphaComplexParallel.cu:1120-1124) plus the stencil
- * half-widths used for enumeration and emptiness scans. */
+/* Reference grid (AlphaComplexParallel.cu:1120-1124) of one molecule plus
+ * the stencil half-widths used for enumeration and emptiness scans.  In a
+ * batch every molecule has its own grid; its cells occupy the global cell
+ * ids [cbase, cbase + ex*ey*ez), so one stable sort by global cell id keeps
+ * the molecules contiguous and in input order. */
 struct GridP {
   float minx, miny, minz, step;
   int ex, ey, ez;
   int k_enum, k_ac2;
+  uint32_t cbase;
+  double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
+  double margin;  /* FP32 cell-key rounding, in length units */
 };
 
 /* raw-input bounds reduced on the device */
@@ -71,7 +77,15 @@ struct PipeLaunch {
 };
 void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
                    Bounds* out, PipeLaunch L);
-void launch_cell_key(const float4* raw, int64_t n, GridP g, uint32_t* key,
+/* per-molecule bounds of a batch: one block per molecule, moff[nmol+1] */
+void launch_bounds_mol(const float4* raw, const uint32_t* moff, int64_t nmol,
+                       float solvent, Bounds* out, PipeLaunch L);
+/* mol[i] = molecule of ball i (same for input and cell-sorted order) */
+void launch_mol_ids(const uint32_t* moff, int64_t nmol, int64_t n,
+                    uint32_t* mol, PipeLaunch L);
+/* global cell key; grids/mol == nullptr: the single grid g */
+void launch_cell_key(const float4* raw, int64_t n, GridP g,
+                     const GridP* grids, const uint32_t* mol, uint32_t* key,
                      uint32_t* idx, PipeLaunch L);
 void launch_gather(const float4* raw, const uint32_t* sidx,
                    const uint32_t* skey, int64_t n, float solvent,
@@ -83,15 +97,15 @@ void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
 struct Stage {
   const float4* ball;
   int64_t n;
-  GridP g;
+  GridP g;              /* the grid of a single molecule */
+  const GridP* grids;   /* batch: grid of each molecule (else nullptr) */
+  const uint32_t* mol;  /* batch: molecule of each ball (else nullptr) */
   const uint32_t* cellk;
   const uint32_t* cstart;
   float alpha;
   ExactScale ex;
   unsigned long long* ctr;
   int staged;
-  double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
-  double margin;  /* FP32 cell-key rounding, in length units */
 };
 
 struct Queue {
@@ -195,6 +209,13 @@ void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
                     uint32_t* idx, uint32_t* key, uint32_t* idx2,
                     uint32_t* key2, void* rtmp, size_t rtb, int32_t* copy,
                     hipStream_t s);
+
+/* batch: first row of each molecule in a list of m rows of K original ids
+ * grouped by molecule (moff: first original id of each molecule);
+ * out[nmol+1] */
+void launch_mol_rows(const int32_t* rows, int64_t m, int K,
+                     const uint32_t* moff, int64_t nmol, uint64_t* out,
+                     hipStream_t s);
 
 /* SoS table as compiled into the device code (for tests) */
 int sos_table_host(int32_t* out3, int cap);

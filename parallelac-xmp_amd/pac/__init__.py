@@ -41,7 +41,8 @@ class Options(ctypes.Structure):
                 ("device_inputs", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("batch_balls", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class Stats(ctypes.Structure):
@@ -147,7 +148,7 @@ def _check(rc):
 
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
-            profile=False, ac2_budget=0):
+            profile=False, ac2_budget=0, batch_balls=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -159,6 +160,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.stream = stream
     o.profile = int(bool(profile))
     o.ac2_budget = int(ac2_budget)
+    o.batch_balls = int(batch_balls)
     return o
 
 
@@ -225,6 +227,55 @@ def compute_device(ptr, n, solvent, alpha, **kw):
     _check(lib().pac_compute(ctypes.c_void_p(ptr), n, solvent, alpha,
                              ctypes.byref(o), ctypes.byref(h)))
     return Result(h.value)
+
+
+class BatchResult:
+    """Owned handle to a pac_batch_result (per-molecule lists, local ids)."""
+
+    def __init__(self, handle, n_mol):
+        self._h = ctypes.c_void_p(handle)
+        self.n_mol = n_mol
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.pac_batch_free(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def counts(self):
+        c = np.zeros((self.n_mol, 4), dtype=np.int64)
+        _check(lib().pac_batch_counts(self._h, c.ctypes.data))
+        return c
+
+    def lists(self, m):
+        c = self.counts()[m]
+        out = {}
+        for d in (1, 2, 3):
+            a = np.zeros((int(c[d]), d + 1), dtype=np.int32)
+            _check(lib().pac_batch_copy(self._h, m, d, a.ctypes.data))
+            out[d] = a
+        return out
+
+    def stats(self):
+        s = Stats()
+        _check(lib().pac_batch_stats(self._h, ctypes.byref(s)))
+        return s
+
+
+def compute_batch(xyzr, offsets, solvent, alpha, **kw):
+    """Batched ensemble: xyzr (host n x 4 FP32, or a device pointer with
+    device_inputs=True and n given by offsets[-1]) and int64 offsets."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n_mol = off.shape[0] - 1
+    h = ctypes.c_void_p()
+    o = options(**kw)
+    if kw.get("device_inputs"):
+        ptr = ctypes.c_void_p(int(xyzr))
+    else:
+        a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+        ptr = a.ctypes.data
+    _check(lib().pac_compute_batch(ptr, off.ctypes.data, n_mol, solvent, alpha,
+                                   ctypes.byref(o), ctypes.byref(h)))
+    return BatchResult(h.value, n_mol)
 
 
 def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):

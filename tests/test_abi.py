@@ -68,3 +68,12 @@ def test_synth_batch_and_stress():
     s = pac.synth_stress(20000, None, seed=5)
     assert s.shape == (20000, 4)
     assert (s[:, 3] == np.float32(1.70)).sum() >= 1728
+
+
+def test_batch_argument_errors():
+    """offsets are validated on the host before any device work"""
+    xyzr = np.zeros((4, 4), dtype=np.float32)
+    for bad in ([0, 3, 2, 4], [1, 4]):
+        with pytest.raises(pac.PacError) as e:
+            pac.compute_batch(xyzr, np.array(bad), 1.4, 0.0)
+        assert e.value.code == pac.PAC_ERR_INVALID
