@@ -1,14 +1,13 @@
 /*
- * k_output.hip -- per-simplex alpha values and canonical row order.
+ * k_output.hip -- per-simplex alpha values, their adjoint, and canonical
+ * row order.
  *
- * Alpha values: rho (squared orthoradius minus nothing: the power of the
- * smallest orthosphere, the quantity the reference compares with alpha and
- * stores in float4 .w, AlphaComplexParallel.cu:370, :603) of every output
- * simplex, evaluated in FP64 from the FP32 balls with the closed forms the
- * filters use:
- *   edge      (A^2)/(4|u|^2) - w_i,                A = |u|^2 + w_i - w_j
- *   triangle  (A^2|v|^2 - 2AB u.v + B^2|u|^2)/(4|n|^2) - w_i
- *   tet       |N|^2/(4 D^2) - w_i
+ * Alpha values: rho = |x|^2 - w_0 with x = c - p_0 the orthocentre offset,
+ * evaluated in FP64 with the closed forms the filters use (u, v, t edge
+ * vectors from p_0, A = |u|^2 + w_0 - w_1 and so on):
+ *   edge      x = A u / (2|u|^2)
+ *   triangle  x = (A (v x n) - B (u x n)) / (2|n|^2),  n = u x v
+ *   tet       x = (A (v x t) + B (t x u) + C (u x v)) / (2 D),  D = u.(v x t)
  * Canonical order: vertices of each row sorted by original id, rows sorted
  * lexicographically (LSD radix over the columns).
  */
@@ -16,46 +15,76 @@
 
 namespace pac {
 
+/* Orthocentre of a simplex relative to its first vertex, x = c - p_0, and
+ * the barycentric coordinates lam of c (c = sum lam_q p_q), in FP64. */
+template <int DIM>
+__device__ __forceinline__ void ortho_bary(const double4* b, const int32_t* r,
+                                           double x[3], double lam[4],
+                                           double* rho) {
+  double4 a = b[r[0]];
+  double wa = a.w * a.w;
+  double u[3] = {b[r[1]].x - a.x, b[r[1]].y - a.y, b[r[1]].z - a.z};
+  double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  double A = uu + wa - b[r[1]].w * b[r[1]].w;
+  lam[2] = lam[3] = 0;
+  if (DIM == 1) {
+    double t = A / (2.0 * uu);
+    for (int q = 0; q < 3; ++q) x[q] = t * u[q];
+    lam[1] = t;
+  } else if (DIM == 2) {
+    double v[3] = {b[r[2]].x - a.x, b[r[2]].y - a.y, b[r[2]].z - a.z};
+    double B = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + wa - b[r[2]].w * b[r[2]].w;
+    double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                   u[0] * v[1] - u[1] * v[0]};
+    double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    double vn[3] = {v[1] * n[2] - v[2] * n[1], v[2] * n[0] - v[0] * n[2],
+                    v[0] * n[1] - v[1] * n[0]};
+    double un[3] = {u[1] * n[2] - u[2] * n[1], u[2] * n[0] - u[0] * n[2],
+                    u[0] * n[1] - u[1] * n[0]};
+    for (int q = 0; q < 3; ++q) x[q] = (A * vn[q] - B * un[q]) / (2.0 * nn);
+    /* x = l1 u + l2 v:  l1 = (x x v).n / |n|^2,  l2 = (u x x).n / |n|^2 */
+    double xv[3] = {x[1] * v[2] - x[2] * v[1], x[2] * v[0] - x[0] * v[2],
+                    x[0] * v[1] - x[1] * v[0]};
+    double ux[3] = {u[1] * x[2] - u[2] * x[1], u[2] * x[0] - u[0] * x[2],
+                    u[0] * x[1] - u[1] * x[0]};
+    lam[1] = (xv[0] * n[0] + xv[1] * n[1] + xv[2] * n[2]) / nn;
+    lam[2] = (ux[0] * n[0] + ux[1] * n[1] + ux[2] * n[2]) / nn;
+  } else {
+    double v[3] = {b[r[2]].x - a.x, b[r[2]].y - a.y, b[r[2]].z - a.z};
+    double t[3] = {b[r[3]].x - a.x, b[r[3]].y - a.y, b[r[3]].z - a.z};
+    double B = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + wa - b[r[2]].w * b[r[2]].w;
+    double C = t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + wa - b[r[3]].w * b[r[3]].w;
+    double vt[3] = {v[1] * t[2] - v[2] * t[1], v[2] * t[0] - v[0] * t[2],
+                    v[0] * t[1] - v[1] * t[0]};
+    double tu[3] = {t[1] * u[2] - t[2] * u[1], t[2] * u[0] - t[0] * u[2],
+                    t[0] * u[1] - t[1] * u[0]};
+    double uv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                    u[0] * v[1] - u[1] * v[0]};
+    double D = u[0] * vt[0] + u[1] * vt[1] + u[2] * vt[2];
+    for (int q = 0; q < 3; ++q) x[q] = (A * vt[q] + B * tu[q] + C * uv[q]) / (2.0 * D);
+    /* Cramer: x = l1 u + l2 v + l3 t */
+    lam[1] = (x[0] * vt[0] + x[1] * vt[1] + x[2] * vt[2]) / D;
+    lam[2] = (x[0] * tu[0] + x[1] * tu[1] + x[2] * tu[2]) / D;
+    lam[3] = (x[0] * uv[0] + x[1] * uv[1] + x[2] * uv[2]) / D;
+  }
+  lam[0] = 1.0 - lam[1] - lam[2] - lam[3];
+  *rho = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] - wa;
+}
+
+/* rho (the power of the smallest orthosphere, the quantity the reference
+ * compares with alpha and stores in float4 .w, AlphaComplexParallel.cu:370,
+ * :603) of every row, from the original-order balls */
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_rho(int64_t m, const int32_t* rows,
-                                                const float4* balls,
+                                                const double4* balls,
                                                 double* rho) {
   int64_t id = gtid();
   if (id >= m) return;
-  const int32_t* r = rows + (int64_t)(DIM + 1) * id;
-  float4 a = balls[r[0]], b = balls[r[1]];
-  double ax = a.x, ay = a.y, az = a.z, wa = (double)a.w * a.w;
-  double ux = b.x - ax, uy = b.y - ay, uz = b.z - az;
-  double uu = ux * ux + uy * uy + uz * uz;
-  double A = uu + wa - (double)b.w * b.w;
-  double out;
-  if (DIM == 1) {
-    out = A * A / (4.0 * uu) - wa;
-  } else if (DIM == 2) {
-    float4 c = balls[r[2]];
-    double vx = c.x - ax, vy = c.y - ay, vz = c.z - az;
-    double vv = vx * vx + vy * vy + vz * vz, uv = ux * vx + uy * vy + uz * vz;
-    double B = vv + wa - (double)c.w * c.w;
-    double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
-    double nn = nx * nx + ny * ny + nz * nz;
-    out = (A * A * vv - 2.0 * A * B * uv + B * B * uu) / (4.0 * nn) - wa;
-  } else {
-    float4 c = balls[r[2]], d = balls[r[3]];
-    double vx = c.x - ax, vy = c.y - ay, vz = c.z - az;
-    double tx = d.x - ax, ty = d.y - ay, tz = d.z - az;
-    double B = vx * vx + vy * vy + vz * vz + wa - (double)c.w * c.w;
-    double C = tx * tx + ty * ty + tz * tz + wa - (double)d.w * d.w;
-    double vtx = vy * tz - vz * ty, vty = vz * tx - vx * tz, vtz = vx * ty - vy * tx;
-    double tux = ty * uz - tz * uy, tuy = tz * ux - tx * uz, tuz = tx * uy - ty * ux;
-    double uvx = uy * vz - uz * vy, uvy = uz * vx - ux * vz, uvz = ux * vy - uy * vx;
-    double D = ux * vtx + uy * vty + uz * vtz;
-    double Nx = A * vtx + B * tux + C * uvx, Ny = A * vty + B * tuy + C * uvy,
-           Nz = A * vtz + B * tuz + C * uvz;
-    out = (Nx * Nx + Ny * Ny + Nz * Nz) / (4.0 * D * D) - wa;
-  }
-  rho[id] = out;
+  double x[3], lam[4], r;
+  ortho_bary<DIM>(balls, rows + (int64_t)(DIM + 1) * id, x, lam, &r);
+  rho[id] = r;
 }
-void launch_rho(int dim, int64_t m, const int32_t* rows, const float4* balls,
+void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
                 double* rho, hipStream_t s) {
   if (m <= 0) return;
   dim3 gr((unsigned)((m + kBlock - 1) / kBlock));
@@ -65,6 +94,79 @@ void launch_rho(int dim, int64_t m, const int32_t* rows, const float4* balls,
     hipLaunchKernelGGL(k_rho<2>, gr, dim3(kBlock), 0, s, m, rows, balls, rho);
   else
     hipLaunchKernelGGL(k_rho<3>, gr, dim3(kBlock), 0, s, m, rows, balls, rho);
+}
+
+/* ---- adjoint of rho with the combinatorics fixed ----
+ * Every vertex q of a simplex satisfies rho = |c - p_q|^2 - w_q.  With
+ * c = sum lam_q p_q (sum lam_q = 1, c in the affine hull), summing the
+ * differentials with weights lam_q removes dc:
+ *     drho = sum_q lam_q (2 (p_q - c) . dp_q - dw_q),   dw_q = 2 r_q dr_q,
+ * so d rho / d p_q = 2 lam_q (p_q - c) and d rho / d r_q = -2 lam_q r_q.
+ * Contributions are written per (simplex, vertex) slot and summed per ball
+ * in slot order after a stable sort by ball: the result is deterministic. */
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_adj_contrib(
+    int64_t m, const int32_t* rows, const double4* balls, const double* g,
+    int64_t slot0, uint32_t* key, uint32_t* val, double4* contrib) {
+  int64_t id = gtid();
+  if (id >= m) return;
+  const int32_t* r = rows + (int64_t)(DIM + 1) * id;
+  double x[3], lam[4], rho;
+  ortho_bary<DIM>(balls, r, x, lam, &rho);
+  const double gs = g[id];
+  double4 a = balls[r[0]];
+#pragma unroll
+  for (int q = 0; q <= DIM; ++q) {
+    double4 p = balls[r[q]];
+    /* p_q - c = (p_q - p_0) - x */
+    double dx = (p.x - a.x) - x[0], dy = (p.y - a.y) - x[1], dz = (p.z - a.z) - x[2];
+    double k = 2.0 * lam[q] * gs;
+    int64_t sl = slot0 + (int64_t)(DIM + 1) * id + q;
+    contrib[sl] = make_double4(k * dx, k * dy, k * dz, -k * p.w);
+    key[sl] = (uint32_t)r[q];
+    val[sl] = (uint32_t)sl;
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_adj_gather(
+    int64_t n, const uint32_t* vstart, const uint32_t* sval,
+    const double4* contrib, double* dxyz, double* dr) {
+  int64_t i = gtid();
+  if (i >= n) return;
+  double sx = 0, sy = 0, sz = 0, sr = 0;
+  for (uint32_t q = vstart[i], e = vstart[i + 1]; q < e; ++q) {
+    double4 c = contrib[sval[q]];
+    sx += c.x;
+    sy += c.y;
+    sz += c.z;
+    sr += c.w;
+  }
+  dxyz[3 * i] = sx;
+  dxyz[3 * i + 1] = sy;
+  dxyz[3 * i + 2] = sz;
+  dr[i] = sr;
+}
+void launch_adj_contrib(int dim, int64_t m, const int32_t* rows,
+                        const double4* balls, const double* g, int64_t slot0,
+                        uint32_t* key, uint32_t* val, double4* contrib,
+                        hipStream_t s) {
+  if (m <= 0) return;
+  dim3 gr((unsigned)((m + kBlock - 1) / kBlock));
+  if (dim == 1)
+    hipLaunchKernelGGL(k_adj_contrib<1>, gr, dim3(kBlock), 0, s, m, rows, balls,
+                       g, slot0, key, val, contrib);
+  else if (dim == 2)
+    hipLaunchKernelGGL(k_adj_contrib<2>, gr, dim3(kBlock), 0, s, m, rows, balls,
+                       g, slot0, key, val, contrib);
+  else
+    hipLaunchKernelGGL(k_adj_contrib<3>, gr, dim3(kBlock), 0, s, m, rows, balls,
+                       g, slot0, key, val, contrib);
+}
+void launch_adj_gather(int64_t n, const uint32_t* vstart, const uint32_t* sval,
+                       const double4* contrib, double* dxyz, double* dr,
+                       hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, n, vstart, sval, contrib, dxyz, dr);
 }
 
 /* sort the vertices of each row ascending (rows of k = DIM+1 ids) */

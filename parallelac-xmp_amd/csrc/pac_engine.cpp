@@ -181,7 +181,7 @@ struct pac_result {
   Engine* eng = nullptr;
   int64_t counts[4] = {0, 0, 0, 0};
   DBuf<int32_t> lists[3];     /* edges, tris, tets (original ids) */
-  DBuf<float4> balls;         /* original order, r = fl(r_raw + s) */
+  DBuf<double4> balls;        /* original order, r = r_raw + s (as computed) */
   double alpha = 0;
   pac_stats st;
 };
@@ -842,11 +842,11 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
 /* copy raw input with solvent-added radii, original order (kept for rho /
  * adjoint queries) */
 __global__ void k_balls_orig(const float4* raw, int64_t n, float solvent,
-                             float4* out) {
+                             double4* out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 a = raw[i];
-  out[i] = make_float4(a.x, a.y, a.z, __fadd_rn(a.w, solvent));
+  out[i] = make_double4(a.x, a.y, a.z, (double)__fadd_rn(a.w, solvent));
 }
 
 /* canonical order: vertices of each row by original id, rows lexicographic */
@@ -906,7 +906,7 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
   }
   T.mark(1);
   if (n > 0) {
-    R->balls = DBuf<float4>(&E->pool, n);
+    R->balls = DBuf<double4>(&E->pool, n);
     hipLaunchKernelGGL(k_balls_orig, dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, s, raw, n, solvent, R->balls.get());
     KCHK();
@@ -923,6 +923,78 @@ static void compute_f32(const float* xyzr, int64_t n, float solvent,
       std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
   KP.collect(st.kernel_ms);
   *out = R.release();
+}
+
+/* ------------------------------------------------------------------ */
+/* adjoint of the alpha values                                         */
+/* ------------------------------------------------------------------ */
+static bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+/* dL/d(x,y,z) and dL/dr of every ball (input order) from dL/drho of the
+ * output triangles and tets; either gradient input may be NULL (zero).
+ * Pointers may be host or device memory. */
+static void alpha_adjoint(const pac_result* r, const double* gtri,
+                          const double* gtet, double* dxyz, double* dr) {
+  if (!r || !dxyz || !dr) fail(PAC_ERR_INVALID, "NULL argument");
+  Engine* E = r->eng;
+  std::lock_guard<std::mutex> lk(E->run_mu);
+  HIPCHK(hipSetDevice(E->device));
+  hipStream_t s = E->stream;
+  Pool* P = &E->pool;
+  const int64_t n = r->counts[0], T = r->counts[2], K = r->counts[3];
+  const bool dev_out = is_device_ptr(dxyz);
+  DBuf<double> gx(P, 3 * std::max<int64_t>(n, 1)), gr(P, std::max<int64_t>(n, 1));
+  if (n > 0) {
+    const int64_t F = 3 * T + 4 * K;
+    DBuf<double> g(P, T + K + 1);
+    const double* src[2] = {gtri, gtet};
+    const int64_t cnt[2] = {T, K}, at[2] = {0, T};
+    for (int q = 0; q < 2; ++q) {
+      if (!cnt[q]) continue;
+      if (src[q])
+        HIPCHK(hipMemcpyAsync(g.get() + at[q], src[q], cnt[q] * sizeof(double),
+                              is_device_ptr(src[q]) ? hipMemcpyDeviceToDevice
+                                                    : hipMemcpyHostToDevice, s));
+      else
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((cnt[q] + 255) / 256)),
+                           dim3(256), 0, s,
+                           (unsigned long long*)(g.get() + at[q]), (int)cnt[q]);
+    }
+    DBuf<uint32_t> key(P, F + 1), val(P, F + 1), key2(P, F + 1), val2(P, F + 1);
+    DBuf<double4> contrib(P, F + 1);
+    launch_adj_contrib(2, T, r->lists[1].get(), r->balls.get(), g.get(), 0,
+                       key.get(), val.get(), contrib.get(), s);
+    launch_adj_contrib(3, K, r->lists[2].get(), r->balls.get(), g.get() + T,
+                       3 * T, key.get(), val.get(), contrib.get(), s);
+    KCHK();
+    int idbits = 1;
+    while (idbits < 32 && ((int64_t)1 << idbits) < n) ++idbits;
+    size_t rtb = radix_tmp_bytes(std::max<int64_t>(F, 1));
+    DBuf<uint8_t> rt(P, rtb);
+    bool alt = false;
+    if (F > 0)
+      radix_sort_pairs(key.get(), val.get(), key2.get(), val2.get(), F, idbits,
+                       rt.get(), rtb, s, &alt);
+    const uint32_t* sk = alt ? key2.get() : key.get();
+    const uint32_t* sv = alt ? val2.get() : val.get();
+    DBuf<uint32_t> vstart(P, n + 1);
+    launch_cell_starts(sk, F, n, vstart.get(), PipeLaunch{s, kBlock});
+    launch_adj_gather(n, vstart.get(), sv, contrib.get(), gx.get(), gr.get(), s);
+    KCHK();
+  }
+  HIPCHK(hipMemcpyAsync(dxyz, gx.get(), 3 * n * sizeof(double),
+                        dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(dr, gr.get(), n * sizeof(double),
+                        is_device_ptr(dr) ? hipMemcpyDeviceToDevice
+                                          : hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
 }
 
 /* ------------------------------------------------------------------ */
@@ -1187,10 +1259,10 @@ int pac_batch_stats(const pac_batch_result* r, pac_stats* st) {
   return PAC_OK;
 }
 void pac_batch_free(pac_batch_result* r) { delete r; }
-int pac_alpha_adjoint(const pac_result*, const double*, const double*, double*,
-                      double*) {
-  g_err = "pac_alpha_adjoint: not built in this version";
-  return PAC_ERR_INVALID;
+int pac_alpha_adjoint(const pac_result* r, const double* dl_drho_tri,
+                      const double* dl_drho_tet, double* dl_dxyz,
+                      double* dl_dr) {
+  return guarded([&] { alpha_adjoint(r, dl_drho_tri, dl_drho_tet, dl_dxyz, dl_dr); });
 }
 
 const char* pac_last_error(void) { return g_err.c_str(); }

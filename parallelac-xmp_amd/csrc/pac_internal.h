@@ -203,8 +203,18 @@ void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint32_t* perm, int32_t* out, PipeLaunch L);
 
 /* ---- outputs (k_output.hip) ---- */
-void launch_rho(int dim, int64_t m, const int32_t* rows, const float4* balls,
+void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
                 double* rho, hipStream_t s);
+/* adjoint: per-(simplex, vertex) contributions at slots slot0 + (dim+1)*id + q
+ * with key = vertex, val = slot; then per-ball sums over vstart ranges of
+ * the slots sorted by vertex */
+void launch_adj_contrib(int dim, int64_t m, const int32_t* rows,
+                        const double4* balls, const double* g, int64_t slot0,
+                        uint32_t* key, uint32_t* val, double4* contrib,
+                        hipStream_t s);
+void launch_adj_gather(int64_t n, const uint32_t* vstart, const uint32_t* sval,
+                       const double4* contrib, double* dxyz, double* dr,
+                       hipStream_t s);
 void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
                     uint32_t* idx, uint32_t* key, uint32_t* idx2,
                     uint32_t* key2, void* rtmp, size_t rtb, int32_t* copy,

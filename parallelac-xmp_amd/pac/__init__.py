@@ -203,6 +203,27 @@ class Result:
         _check(lib().pac_result_stats(self._h, ctypes.byref(s)))
         return s
 
+    def adjoint(self, g_tri=None, g_tet=None):
+        """dL/d(x,y,z) (n x 3) and dL/dr (n) from dL/drho of the output
+        triangles and tets (FP64, list order; None = zero)."""
+        c = self.counts()
+        n = int(c[0])
+        args = []
+        for g, m in ((g_tri, c[2]), (g_tet, c[3])):
+            if g is None:
+                args.append(None)
+            else:
+                a = np.ascontiguousarray(g, dtype=np.float64)
+                assert a.shape == (int(m),)
+                args.append(a)
+        dxyz = np.zeros((n, 3), dtype=np.float64)
+        dr = np.zeros(n, dtype=np.float64)
+        _check(lib().pac_alpha_adjoint(
+            self._h, None if args[0] is None else args[0].ctypes.data,
+            None if args[1] is None else args[1].ctypes.data,
+            dxyz.ctypes.data, dr.ctypes.data))
+        return dxyz, dr
+
     def alpha_values(self, dim):
         n = int(self.counts()[dim])
         rho = np.zeros(n, dtype=np.float64)
@@ -217,6 +238,16 @@ def compute(xyzr, solvent, alpha, **kw):
     o = options(**kw)
     _check(lib().pac_compute(a.ctypes.data, a.shape[0], solvent, alpha,
                              ctypes.byref(o), ctypes.byref(h)))
+    return Result(h.value)
+
+
+def compute_f64(xyzr, solvent, alpha, **kw):
+    """Run the pipeline on host FP64 balls (continuous coordinates)."""
+    a = np.ascontiguousarray(xyzr, dtype=np.float64).reshape(-1, 4)
+    h = ctypes.c_void_p()
+    o = options(**kw)
+    _check(lib().pac_compute_f64(a.ctypes.data, a.shape[0], solvent, alpha,
+                                 ctypes.byref(o), ctypes.byref(h)))
     return Result(h.value)
 
 
