@@ -44,6 +44,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle leg (baseline + bit-exact check)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", choices=("globule", "batch", "stress"),
+                    default="globule",
+                    help="globule: the 100k headline (default); batch: the "
+                         "4096-molecule ensemble; stress: 2M balls with a "
+                         "degenerate lattice block")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not "
@@ -53,6 +58,11 @@ def main():
     log("importing torch")
     import torch
     import pac
+
+    if args.workload == "batch":
+        return bench_batch(args, torch, pac)
+    if args.workload == "stress":
+        return bench_stress(args, torch, pac)
 
     n = args.n
     radius = (3.0 * n / (4.0 * np.pi * 0.05)) ** (1.0 / 3.0)
@@ -156,6 +166,145 @@ def main():
         "kernel_ms_staged": {k: round(v, 3) for k, v in kms.items()},
         "kernel_ms_plain": {pac.KERNELS[i]: round(p_pl.kernel_ms[i], 3) for i in range(8)},
         "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def _classified(st):
+    return int(st.edge_candidates + st.tri_candidates + st.tet_candidates)
+
+
+def _esc(pac, st):
+    return {pac.FAMILIES[f]: {pac.STAGES[s]: int(st.stage_evals[f][s]) for s in range(4)}
+            for f in range(6)}
+
+
+def bench_batch(args, torch, pac):
+    """4096 molecules, sizes log-uniform in [200, 20000] (seed 3), each in
+    its own sphere at 0.05 balls/A^3; one pac_compute_batch call per step
+    over the device-resident concatenated array.  A sample of molecules is
+    checked against single-molecule runs (and two against the oracle)."""
+    n_mol = 4096
+    xyzr, off = pac.synth_batch(n_mol, 200, 20000, seed=3)
+    ntot = int(off[-1])
+    dev = torch.from_numpy(xyzr).to("cuda:0")
+    torch.cuda.synchronize()
+    ptr = dev.data_ptr()
+
+    def run(prec):
+        return pac.compute_batch(ptr, off, 1.4, 0.0, precision=prec,
+                                 device_inputs=True)
+
+    log("batch: %d molecules, %d balls; warmup" % (n_mol, ntot))
+    for _ in range(args.warmup):
+        run(pac.PREC_STAGED)
+    res = {}
+    for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = None
+        for _ in range(args.steps):
+            r = None
+            r = run(prec)
+        torch.cuda.synchronize()
+        res[prec] = (r, (time.perf_counter() - t0) / args.steps)
+    r_st, t_st = res[pac.PREC_STAGED]
+    r_pl, t_pl = res[pac.PREC_PLAIN_FP64]
+    st = r_st.stats()
+    rng = np.random.default_rng(3)
+    sample = sorted(rng.choice(n_mol, 24, replace=False).tolist())
+    same_single = True
+    for m in sample:
+        single = pac.compute(xyzr[off[m]:off[m + 1]], 1.4, 0.0).lists()
+        got = r_st.lists(m)
+        got_pl = r_pl.lists(m)
+        same_single &= all(np.array_equal(got[d], single[d]) and
+                           np.array_equal(got_pl[d], single[d]) for d in (1, 2, 3))
+    oracle_ok = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle_ok = True
+        for m in sample[:2]:
+            o, _, _ = oracle.run(xyzr[off[m]:off[m + 1]], 1.4, 0.0,
+                                 nthreads=min(16, os.cpu_count() or 1))
+            got = r_st.lists(m)
+            oracle_ok &= all(np.array_equal(got[d], o[d]) for d in (1, 2, 3))
+    classified = _classified(st)
+    line = {
+        "metric": "classified simplices/s, 4096-molecule ensemble, bit-exact per molecule",
+        "value": classified / t_st, "unit": "classified simplices/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_st * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32->f64->dd->int512", "data": "synthetic",
+        "config": {"workload": "ensemble-4096: sizes log-uniform [200, 20000] seed 3, "
+                               "Poisson-disk min 1.3 A at 0.05 balls/A^3, FP32, "
+                               "solvent 1.4, alpha 0", "molecules": n_mol, "balls": ntot},
+        "molecules_per_s": n_mol / t_st, "balls_per_s": ntot / t_st,
+        "plain_fp64": {"value": classified / t_pl, "ms_per_step": t_pl * 1e3},
+        "check": {"sampled_molecules": len(sample), "eq_single_runs": bool(same_single),
+                  "eq_oracle_2": oracle_ok},
+        "counts": {"classified": classified, "edges": int(st.edges_final),
+                   "triangles": int(st.tris_final), "tets": int(st.tets_final)},
+        "escalations": _esc(pac, st),
+        "phase_ms": {pac.PHASE_LABELS[i]: round(st.phase_ms[i], 3) for i in range(8)},
+    }
+    print(json.dumps(line))
+
+
+def bench_stress(args, torch, pac):
+    """2,000,000 balls in one pass: Poisson-disk in a sphere r = 212.2 A with
+    a central block replaced by a 2.0 A cubic lattice of r = 1.70 (10% of
+    the balls).  Reports the fallback counts; checked against the oracle."""
+    n = 2000000
+    balls = pac.synth_stress(n, 212.2, seed=5, lattice_frac=0.1, spacing=2.0,
+                             lattice_r=1.70)
+    dev = torch.from_numpy(balls).to("cuda:0")
+    torch.cuda.synchronize()
+    ptr = dev.data_ptr()
+    log("stress: warmup")
+    for _ in range(args.warmup):
+        pac.compute_device(ptr, n, 1.4, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = None
+    for _ in range(args.steps):
+        r = None
+        r = pac.compute_device(ptr, n, 1.4, 0.0)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    st = r.stats()
+    oracle_ok = None
+    cpu = None
+    if not args.no_cpu:
+        log("stress: oracle")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        o, ost, _ = oracle.run(balls, 1.4, 0.0, nthreads=threads)
+        tc = time.perf_counter() - t0
+        L = r.lists()
+        oracle_ok = all(np.array_equal(L[d], o[d]) for d in (1, 2, 3))
+        cpu = {"value": (ost["E_cand"] + ost["T_cand"] + ost["K_cand"]) / tc,
+               "unit": "classified simplices/s", "cores": threads, "kind": "port",
+               "sample": "full 2M-ball stress workload, %.1f s" % tc}
+    classified = _classified(st)
+    line = {
+        "metric": "classified simplices/s, 2M-ball degenerate stress, bit-exact",
+        "value": classified / t, "unit": "classified simplices/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32->f64->dd->int512", "data": "synthetic",
+        "config": {"workload": "stress-2M: Poisson-disk in r=212.2 A, 10% central "
+                               "cubic lattice spacing 2.0 A r=1.70, FP32, solvent 1.4, "
+                               "alpha 0, one pass", "n_balls": n},
+        "bit_exact_vs_oracle": oracle_ok,
+        "counts": {"classified": classified, "edges": int(st.edges_final),
+                   "triangles": int(st.tris_final), "tets": int(st.tets_final)},
+        "escalations": _esc(pac, st), "sos_calls": int(st.sos_calls),
+        "phase_ms": {pac.PHASE_LABELS[i]: round(st.phase_ms[i], 3) for i in range(8)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

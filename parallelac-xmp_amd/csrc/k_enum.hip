@@ -396,6 +396,34 @@ void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
                      dim3(kBlock), 0, L.s, m, row, owner, cap);
 }
 
+/* number of triangles of alpha edge q = (i,j): the size of row(i) after j
+ * intersected with row(j) (computePossibleTriangleCount, Parallel:232-266) */
+__global__ __launch_bounds__(kBlock) void k_tri_count(int64_t ne,
+                                                     const uint64_t* erow,
+                                                     const uint32_t* ecol,
+                                                     const uint32_t* eleft,
+                                                     uint32_t* cnt) {
+  int64_t q = gtid();
+  if (q >= ne) return;
+  uint32_t i = eleft[q], j = ecol[q];
+  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+  uint64_t b = erow[j], be = erow[j + 1];
+  uint32_t c = 0;
+  while (a < ae && b < be) {
+    uint32_t x = ecol[a], y = ecol[b];
+    c += (x == y);
+    a += (x <= y);
+    b += (y <= x);
+  }
+  cnt[q] = c;
+}
+void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+                      const uint32_t* eleft, uint32_t* cnt, PipeLaunch L) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
+}
+
 /* triangles of alpha edge q = (i,j): k in row(i) after j and in row(j); the
  * positions of k in row(i) and row(j) are the alpha-edge ids of (i,k) and
  * (j,k), recorded for the tet stage and the closure. */

@@ -636,9 +636,11 @@ static void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
   {
     DBuf<uint64_t> off(P, Ealpha + 1);
     {
-      DBuf<uint32_t> cap(P, Ealpha);
-      launch_tail_caps(Ealpha, erow.get(), eleft.get(), cap.get(), L);
-      Tslots = count_scan(cap.get(), Ealpha, off.get(), "triangle");
+      /* exact counts: a triangle's third vertex must be in both rows, and
+       * the merge that counts it is cheap next to filtering empty slots */
+      DBuf<uint32_t> cnt(P, Ealpha);
+      launch_tri_count(Ealpha, erow.get(), ecol.get(), eleft.get(), cnt.get(), L);
+      Tslots = count_scan(cnt.get(), Ealpha, off.get(), "triangle");
     }
     DBuf<uint32_t> cval(P, Tslots), cseg(P, Tslots), cea(P, Tslots),
         ceb(P, Tslots);

@@ -126,6 +126,9 @@ void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
 /* slot capacity of item q = row[owner[q] + 1] - q - 1 */
 void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L);
+/* exact triangle count per alpha edge */
+void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+                      const uint32_t* eleft, uint32_t* cnt, PipeLaunch L);
 void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,

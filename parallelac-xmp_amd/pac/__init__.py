@@ -94,6 +94,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libpac.so not built: run __graft_entry__.build() "
                            "(or make -C parallelac-xmp_amd)")
+    # PyTorch-ROCm bundles its own HIP runtime under the same soname as
+    # ROCm's.  Loading torch first makes libpac bind to that copy, so the
+    # process holds one HIP runtime; loaded the other way round, torch's
+    # second copy sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
     L.pac_options_init.argtypes = [ctypes.POINTER(Options)]
