@@ -795,28 +795,38 @@ int run_pipeline(const F* xyzr, int64_t n, F solvent, F alpha, int skip_tet,
       maxR = std::max(maxR, r);
     }
   }
-  volatile F m2 = maxR * maxR;
-  volatile F m2a = m2 + alpha;
-  F step = (F)2 * std::sqrt((F)m2a);
+  /* The cell key is the reference's FP32 computation (LD:65-78).  FP64
+   * balls are keyed by their FP32 roundings: the grid values and the key
+   * arithmetic are FP32 for both input types. */
+  const float fminX = (float)minX, fminY = (float)minY, fminZ = (float)minZ;
+  const float fmaxR = (float)maxR;
+  volatile float m2 = fmaxR * fmaxR;
+  volatile float m2a = m2 + (float)alpha;
+  float step = 2.0f * std::sqrt((float)m2a);
   if (!(step > 0) || !std::isfinite(step)) return PAC_ERR_INVALID;
-  volatile F qx = (maxX - minX) / step, qy = (maxY - minY) / step,
-             qz = (maxZ - minZ) / step;
-  int64_t ex = (int64_t)(F)qx + 1, ey = (int64_t)(F)qy + 1,
-          ez = (int64_t)(F)qz + 1;
+  volatile float qx = ((float)maxX - fminX) / step,
+                 qy = ((float)maxY - fminY) / step,
+                 qz = ((float)maxZ - fminZ) / step;
+  int64_t ex = (int64_t)(float)qx + 1, ey = (int64_t)(float)qy + 1,
+          ez = (int64_t)(float)qz + 1;
   if (ex * ey * ez > (int64_t)1 << 31) return PAC_ERR_RANGE;
-  B.minx = minX;
-  B.miny = minY;
-  B.minz = minZ;
+  B.minx = fminX;
+  B.miny = fminY;
+  B.minz = fminZ;
   B.step = step;
   B.ex = (int)ex;
   B.ey = (int)ey;
   B.ez = (int)ez;
   std::vector<uint32_t> key(n);
+  double maxabs = 0;
   for (int64_t i = 0; i < n; ++i) {
-    volatile F dx = rx[i] - minX, dy = ry[i] - minY, dz = rz[i] - minZ;
-    volatile F fx = dx / step, fy = dy / step, fz = dz / step;
-    int cx = (int)(F)fx, cy = (int)(F)fy, cz = (int)(F)fz;
+    volatile float dx = (float)rx[i] - fminX, dy = (float)ry[i] - fminY,
+                   dz = (float)rz[i] - fminZ;
+    volatile float fx = dx / step, fy = dy / step, fz = dz / step;
+    int cx = (int)(float)fx, cy = (int)(float)fy, cz = (int)(float)fz;
     key[i] = (uint32_t)(cx + ex * cy + ex * ey * cz);
+    maxabs = std::max({maxabs, std::fabs((double)rx[i]), std::fabs((double)ry[i]),
+                       std::fabs((double)rz[i])});
   }
   std::vector<int64_t> perm(n);
   for (int64_t i = 0; i < n; ++i) perm[i] = i;
@@ -855,9 +865,11 @@ int run_pipeline(const F* xyzr, int64_t n, F solvent, F alpha, int skip_tet,
     B.k_enum = B.k_ac2 = 1;
   } else {
     double dmax = reach2 > 0 ? 2.0 * std::sqrt(reach2) * (1 + 1e-12) : 0.0;
-    double u = std::is_same<F, float>::value ? 0x1p-24 : 0x1p-53;
+    /* key rounding in cells: FP32 subtract + divide, and for FP64 balls
+     * the rounding of the coordinate to FP32 */
     double extm = (double)std::max(ex, std::max(ey, ez)) + 2;
-    double margin = 8.0 * extm * u + 1e-9;
+    if (!std::is_same<F, float>::value) extm += maxabs / (double)step;
+    double margin = 8.0 * extm * 0x1p-24 + 1e-9;
     B.k_enum = (int)std::ceil(dmax / (double)step + margin);
     B.k_ac2 = (int)std::ceil(0.5 * dmax / (double)step + margin);
     if (B.k_enum < 1) B.k_enum = 1;
@@ -883,7 +895,7 @@ int run_pipeline(const F* xyzr, int64_t n, F solvent, F alpha, int skip_tet,
   upd((double)alpha, 2);
   B.S = S;
   R->stats[16] = S;
-  double maxabs = 0, maxr = 0;
+  double maxr = 0;
   for (int64_t s = 0; s < n; ++s) {
     maxabs = std::max(maxabs, std::fabs((double)B.x[s]));
     maxabs = std::max(maxabs, std::fabs((double)B.y[s]));

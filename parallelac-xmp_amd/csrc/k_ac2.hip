@@ -20,9 +20,10 @@
 #include "pac_sos_table.h"
 
 namespace pac {
+namespace PAC_V {
 
 /* exact lifted row for SoS */
-__device__ void lifted_row(const float4& b, const LdWI& L, WI out[4]) {
+__device__ void lifted_row(const Ball& b, const LdWI& L, WI out[4]) {
   V3<WI> p = L.p(b);
   out[0] = p.x;
   out[1] = p.y;
@@ -62,7 +63,7 @@ __device__ WI det4(const WI M[5][4], const int* r, const int* c) {
 /* Sign of the SoS-perturbed det[x y z l 1] of the five balls, rows in
  * argument order, perturbation order given by ids (original input ids,
  * as the reference's Atom.index, AlphaComplexParallel.cu:1087). */
-__device__ int sos_det5(const float4 b[5], const uint32_t ids[5],
+__device__ int sos_det5(const Ball b[5], const uint32_t ids[5],
                         const LdWI& L) {
   int ord[5] = {0, 1, 2, 3, 4};
   int swaps = 0;
@@ -118,7 +119,7 @@ struct TetQ {
   V3<T> pa, N;
 };
 template <class T>
-__device__ __forceinline__ void tetq_init(TetQ<T>& t, const float4 v[4]) {
+__device__ __forceinline__ void tetq_init(TetQ<T>& t, const Ball v[4]) {
   t.pa = Ld<T>::p(v[0]);
   t.wa = Ld<T>::w(v[0]);
   tet_DN(t.pa, t.wa, Ld<T>::p(v[1]), Ld<T>::w(v[1]), Ld<T>::p(v[2]),
@@ -127,7 +128,7 @@ __device__ __forceinline__ void tetq_init(TetQ<T>& t, const float4 v[4]) {
 /* 1 violated, 0 not violated, 2 undecided (needs exact) */
 template <class T>
 __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
-                                             const float4& e) {
+                                             const Ball& e) {
   T g = power_gap(t.pa, t.wa, Ld<T>::p(e), Ld<T>::w(e), t.D, t.N);
   int s = fe_sign(g);
   if (s == 0) return 2;
@@ -138,7 +139,7 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 /* per-simplex setup at precision R: D, N with a-priori error bounds, the
  * certified centre box and the semi-static pair constants. */
 template <class R>
-__device__ __forceinline__ bool tet_setup(const GridP& g, const float4 v[4],
+__device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
                                           PairSS<R>& P, CellBox& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
   for (int64_t w = gtid(); w < total; w += stride) {
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     uint4 t = tets[id];
-    float4 v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
+    Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
     const GridP g = grid_of(S, t.x);
     PairSS<R> P;
     CellBox box;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
+    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
 /* FP64 then double-double for queued pairs / uncertified tets */
 template <class T>
 __device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
-                                               const float4& e) {
+                                               const Ball& e) {
   return tet_pair_code(t, sD, e);
 }
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
   if (t < nq) {
     QEntry e = q[t];
     uint4 tt = tets[e.item];
-    float4 v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
+    Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
     TetQ<FE<double>> t64;
     tetq_init(t64, v);
     int sD = fe_sign(t64.D);
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
       sD = fe_sign(tdd.D);
     }
     auto pair = [&](uint32_t p) -> int { /* 1 violated, 0 clear, 2 undecided */
-      float4 E = S.ball[p];
+      Ball E = S.ball[p];
       int c = 2;
       if (fe_sign(t64.D) != 0) {
         sc.c[1]++;
@@ -349,12 +350,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
 
 /* exact resolution of one (tet, ball) pair, or a full exact scan when
  * aux == ~0 (tet whose orientation no float stage could certify) */
-__device__ int tet_pair_exact(const Stage& S, const float4 v[4],
+__device__ int tet_pair_exact(const Stage& S, const Ball v[4],
                               const uint32_t vid[4], const uint32_t* perm,
                               const WI& D, const V3<WI>& N, const V3<WI>& pa,
                               const WI& wa, uint32_t p, unsigned* sos) {
   LdWI L{S.ex.S};
-  float4 E = S.ball[p];
+  Ball E = S.ball[p];
   WI g = power_gap(pa, wa, L.p(E), L.w(E), D, N);
   int s = wi_sign(g), sD = wi_sign(D);
   if (s != 0) return (s * sD) < 0;
@@ -362,7 +363,7 @@ __device__ int tet_pair_exact(const Stage& S, const float4 v[4],
    * is violated iff SoS * orient3d(a,b,c,d) > 0, orient3d = -sign(D)
    * (AlphaComplexParallel.cu:653-683). */
   ++*sos;
-  float4 bb[5] = {v[0], v[1], v[2], v[3], E};
+  Ball bb[5] = {v[0], v[1], v[2], v[3], E};
   uint32_t ids[5] = {perm[vid[0]], perm[vid[1]], perm[vid[2]], perm[vid[3]],
                      perm[p]};
   int ss = sos_det5(bb, ids, L);
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     QEntry e = q[t];
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
-    float4 v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
+    Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
     LdWI L{S.ex.S};
     V3<WI> pa = L.p(v[0]);
     WI wa = L.w(v[0]), D;
@@ -429,16 +430,16 @@ struct LowQ {   /* triangle or edge: power gap with den, num */
   V3<T> pa, num;
 };
 template <class T>
-__device__ __forceinline__ void triq_init(LowQ<T>& t, const float4& a,
-                                          const float4& b, const float4& c) {
+__device__ __forceinline__ void triq_init(LowQ<T>& t, const Ball& a,
+                                          const Ball& b, const Ball& c) {
   t.pa = Ld<T>::p(a);
   t.wa = Ld<T>::w(a);
   tri_X(t.pa, t.wa, Ld<T>::p(b), Ld<T>::w(b), Ld<T>::p(c), Ld<T>::w(c),
         &t.den, &t.num);
 }
 template <class T>
-__device__ __forceinline__ void edgeq_init(LowQ<T>& t, const float4& a,
-                                           const float4& b) {
+__device__ __forceinline__ void edgeq_init(LowQ<T>& t, const Ball& a,
+                                           const Ball& b) {
   t.pa = Ld<T>::p(a);
   t.wa = Ld<T>::w(a);
   V3<T> u = vsub(Ld<T>::p(b), t.pa);
@@ -449,7 +450,7 @@ __device__ __forceinline__ void edgeq_init(LowQ<T>& t, const float4& a,
 /* 1 violated (gap <= 0), 0 clear, 2 undecided */
 template <class T>
 __device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
-                                             const float4& e) {
+                                             const Ball& e) {
   T g = power_gap(t.pa, t.wa, Ld<T>::p(e), Ld<T>::w(e), t.den, t.num);
   int s = fe_sign(g);
   return s == 0 ? 2 : (s < 0 ? 1 : 0);
@@ -457,7 +458,7 @@ __device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
 
 
 template <class R, int KIND>
-__device__ __forceinline__ bool low_setup(const GridP& g, const float4* v,
+__device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
                                           PairSS<R>& P, CellBox& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       vid[1] = ecol[id];
       vid[2] = vid[1];
     }
-    float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+    Ball v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
     const GridP g = grid_of(S, vid[0]);
     PairSS<R> P;
     CellBox box;
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const float4& E) {
+    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
       if (PHASE_B) {
         if (seen < budget) {
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
       vid[1] = ecol[e.item];
       vid[2] = vid[1];
     }
-    float4 v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
+    Ball v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
     LowQ<FE<double>> q64;
     LowQ<FEDD> qdd;
     bool have_dd = false;
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
       }
     };
     auto pair = [&](uint32_t p) -> int { /* 1 violated, 0 clear, 2 undecided */
-      float4 E = S.ball[p];
+      Ball E = S.ball[p];
       int c = 2;
       if (fe_sign(q64.den) > 0) {
         sc.c[1]++;
@@ -718,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
       vid[2] = 0xffffffffu;
     }
     LdWI L{S.ex.S};
-    float4 a = S.ball[vid[0]], b = S.ball[vid[1]];
+    Ball a = S.ball[vid[0]], b = S.ball[vid[1]];
     V3<WI> pa = L.p(a), num;
     WI wa = L.w(a), den;
     if (KIND == 0) {
@@ -732,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     }
     if (e.aux != 0xffffffffu) {
       cnt = 1;
-      float4 E = S.ball[e.aux];
+      Ball E = S.ball[e.aux];
       if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0)
         flag[e.item] = 0;
     } else if (wi_sign(den) > 0) {
@@ -755,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
               if (p == vid[0] || p == vid[1] || (KIND == 0 && p == vid[2]))
                 continue;
               ++cnt;
-              float4 E = S.ball[p];
+              Ball E = S.ball[p];
               if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0) {
                 viol = true;
                 break;
@@ -780,13 +781,17 @@ void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
                         unsigned* nrest, int budget, PipeLaunch L) {
   if (nk <= 0) return;
   dim3 gr((unsigned)((nk + kBlock - 1) / kBlock)), gb(phase_b_grid());
-  if (S.staged) {
-    hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0, L.s,
-                       S, nk, tets, flag, q1, rest, nrest, budget);
-    if (budget > 0)
-      hipLaunchKernelGGL((k_tet_ac2_hot<float, true>), gb, dim3(kBlock), 0, L.s,
-                         S, nk, tets, flag, q1, rest, nrest, budget);
-  } else {
+  if constexpr (!kBallF64) {
+    if (S.staged) {
+      hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0,
+                         L.s, S, nk, tets, flag, q1, rest, nrest, budget);
+      if (budget > 0)
+        hipLaunchKernelGGL((k_tet_ac2_hot<float, true>), gb, dim3(kBlock), 0,
+                           L.s, S, nk, tets, flag, q1, rest, nrest, budget);
+      return;
+    }
+  }
+  {
     hipLaunchKernelGGL((k_tet_ac2_hot<double, false>), gr, dim3(kBlock), 0,
                        L.s, S, nk, tets, flag, q1, rest, nrest, budget);
     if (budget > 0)
@@ -830,21 +835,23 @@ void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
                         uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L) {
   if (nitems <= 0) return;
-  if (kind == 0) {
-    if (S.staged)
-      low_hot2<float, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
-                         rest, nrest, budget, L);
-    else
-      low_hot2<double, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
-                          rest, nrest, budget, L);
-  } else {
-    if (S.staged)
-      low_hot2<float, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
-                         rest, nrest, budget, L);
-    else
-      low_hot2<double, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
-                          rest, nrest, budget, L);
+  if constexpr (!kBallF64) {
+    if (S.staged) {
+      if (kind == 0)
+        low_hot2<float, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                           rest, nrest, budget, L);
+      else
+        low_hot2<float, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                           rest, nrest, budget, L);
+      return;
+    }
   }
+  if (kind == 0)
+    low_hot2<double, 0>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                        rest, nrest, budget, L);
+  else
+    low_hot2<double, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                        rest, nrest, budget, L);
 }
 void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
                         const uint32_t* ecol, const uint32_t* eleft,
@@ -879,4 +886,5 @@ int sos_table_host(int32_t* out3, int cap) {
   return PAC_SOS_TERMS;
 }
 
+}  // namespace PAC_V
 }  // namespace pac

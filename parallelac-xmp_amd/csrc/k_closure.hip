@@ -14,6 +14,7 @@
 #include "pac_dev.h"
 
 namespace pac {
+using namespace PAC_V; /* type-independent device helpers */
 
 __global__ __launch_bounds__(kBlock) void k_compact_u4(int64_t n,
                                                        const uint4* in,

@@ -20,6 +20,7 @@
 #include "pac_dev.h"
 
 namespace pac {
+namespace PAC_V {
 
 /* ------------------------------------------------------------------ */
 /* bounds / keys / gather                                              */
@@ -39,8 +40,28 @@ __device__ __forceinline__ int neg_lsb_exp(float v) {
   return lsb < 0 ? -lsb : 0;
 }
 
+__device__ __forceinline__ int neg_lsb_exp(double v) {
+  uint64_t b = (uint64_t)__double_as_longlong(v) & 0x7fffffffffffffffull;
+  if (b == 0) return 0;
+  uint32_t E = (uint32_t)(b >> 52);
+  uint64_t m = b & ((1ull << 52) - 1);
+  int e;
+  if (E == 0) {
+    e = -1074;
+  } else {
+    m |= 1ull << 52;
+    e = (int)E - 1075;
+  }
+  int lsb = e + __builtin_ctzll(m);
+  return lsb < 0 ? -lsb : 0;
+}
+/* r = r_raw + solvent as the reference computes it (FP32 add,
+ * AlphaComplexParallel.cu:1086), or in FP64 for FP64 balls */
+__device__ __forceinline__ float add_r(float a, float s) { return __fadd_rn(a, s); }
+__device__ __forceinline__ double add_r(double a, double s) { return a + s; }
+
 struct BRed {
-  float v[9];  /* minx miny minz -maxx -maxy -maxz -maxrawr -maxr -maxabs */
+  Real v[9];  /* minx miny minz -maxx -maxy -maxz -maxrawr -maxr -maxabs */
   int shift, nonfinite;
 };
 __device__ void bred_init(BRed& b) {
@@ -51,25 +72,25 @@ __device__ void bred_init(BRed& b) {
 }
 __device__ void bred_merge(BRed& a, const BRed& b) {
 #pragma unroll
-  for (int q = 0; q < 9; ++q) a.v[q] = fminf(a.v[q], b.v[q]);
+  for (int q = 0; q < 9; ++q) a.v[q] = fmin(a.v[q], b.v[q]);
   a.shift = max(a.shift, b.shift);
   a.nonfinite += b.nonfinite;
 }
-__device__ void bred_ball(BRed& b, const float4& a, float solvent) {
+__device__ void bred_ball(BRed& b, const Ball& a, Real solvent) {
   if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z) || !isfinite(a.w)) {
     b.nonfinite++;
     return;
   }
-  float r = __fadd_rn(a.w, solvent);
-  b.v[0] = fminf(b.v[0], a.x);
-  b.v[1] = fminf(b.v[1], a.y);
-  b.v[2] = fminf(b.v[2], a.z);
-  b.v[3] = fminf(b.v[3], -a.x);
-  b.v[4] = fminf(b.v[4], -a.y);
-  b.v[5] = fminf(b.v[5], -a.z);
-  b.v[6] = fminf(b.v[6], -a.w);
-  b.v[7] = fminf(b.v[7], -fabsf(r));
-  b.v[8] = fminf(b.v[8], -fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(a.z))));
+  Real r = add_r(a.w, solvent);
+  b.v[0] = fmin(b.v[0], a.x);
+  b.v[1] = fmin(b.v[1], a.y);
+  b.v[2] = fmin(b.v[2], a.z);
+  b.v[3] = fmin(b.v[3], -a.x);
+  b.v[4] = fmin(b.v[4], -a.y);
+  b.v[5] = fmin(b.v[5], -a.z);
+  b.v[6] = fmin(b.v[6], -a.w);
+  b.v[7] = fmin(b.v[7], -fabs(r));
+  b.v[8] = fmin(b.v[8], -fmax(fabs(a.x), fmax(fabs(a.y), fabs(a.z))));
   int s = max(max(neg_lsb_exp(a.x), neg_lsb_exp(a.y)),
               max(neg_lsb_exp(a.z), neg_lsb_exp(r)));
   b.shift = max(b.shift, s);
@@ -87,7 +108,7 @@ __device__ Bounds bred_out(const BRed& b) {
   o.max_abs = -b.v[8];
   o.shift = b.shift;
   o.nonfinite = b.nonfinite;
-  o.pad = 0;
+
   return o;
 }
 __device__ void bred_block(BRed& mine, BRed* sh) {
@@ -100,9 +121,9 @@ __device__ void bred_block(BRed& mine, BRed* sh) {
   mine = sh[0];
 }
 
-__global__ __launch_bounds__(kBlock) void k_bounds_part(const float4* raw,
+__global__ __launch_bounds__(kBlock) void k_bounds_part(const Ball* raw,
                                                         int64_t n,
-                                                        float solvent,
+                                                        Real solvent,
                                                         BRed* part) {
   __shared__ BRed sh[kBlock];
   BRed b;
@@ -121,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const BRed* part,
   bred_block(b, sh);
   if (threadIdx.x == 0) *out = bred_out(b);
 }
-void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
+void launch_bounds(const Ball* raw, int64_t n, Real solvent, Bounds* part,
                    Bounds* out, PipeLaunch L) {
   int nb = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
   if (nb < 1) nb = 1;
@@ -133,9 +154,9 @@ void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
 
 /* Per-molecule bounds of a batch: block m reduces the balls of molecule m
  * (a few hundred to tens of thousands each). */
-__global__ __launch_bounds__(kBlock) void k_bounds_mol(const float4* raw,
+__global__ __launch_bounds__(kBlock) void k_bounds_mol(const Ball* raw,
                                                        const uint32_t* moff,
-                                                       float solvent,
+                                                       Real solvent,
                                                        Bounds* out) {
   __shared__ BRed sh[kBlock];
   const uint32_t m = blockIdx.x, b0 = moff[m], b1 = moff[m + 1];
@@ -146,8 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mol(const float4* raw,
   bred_block(b, sh);
   if (threadIdx.x == 0) out[m] = bred_out(b);
 }
-void launch_bounds_mol(const float4* raw, const uint32_t* moff, int64_t nmol,
-                       float solvent, Bounds* out, PipeLaunch L) {
+void launch_bounds_mol(const Ball* raw, const uint32_t* moff, int64_t nmol,
+                       Real solvent, Bounds* out, PipeLaunch L) {
   if (nmol <= 0) return;
   hipLaunchKernelGGL(k_bounds_mol, dim3((unsigned)nmol), dim3(kBlock), 0, L.s,
                      raw, moff, solvent, out);
@@ -176,7 +197,7 @@ void launch_mol_ids(const uint32_t* moff, int64_t nmol, int64_t n,
 
 /* computeCellIndex (LD:65-78): FP32 subtract, IEEE divide, truncate; in a
  * batch the molecule's first global cell id is added. */
-__global__ __launch_bounds__(kBlock) void k_cell_key(const float4* raw,
+__global__ __launch_bounds__(kBlock) void k_cell_key(const Ball* raw,
                                                      int64_t n, GridP g1,
                                                      const GridP* grids,
                                                      const uint32_t* mol,
@@ -185,14 +206,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_key(const float4* raw,
   int64_t i = gtid();
   if (i >= n) return;
   const GridP g = mol ? grids[mol[i]] : g1;
-  float4 a = raw[i];
-  int cx = (int)__fdiv_rn(__fsub_rn(a.x, g.minx), g.step);
-  int cy = (int)__fdiv_rn(__fsub_rn(a.y, g.miny), g.step);
-  int cz = (int)__fdiv_rn(__fsub_rn(a.z, g.minz), g.step);
+  Ball a = raw[i];
+  /* FP64 balls are keyed by their FP32 roundings (monotone, so every key
+   * stays inside the grid; the host widens the search margin by the
+   * rounding) */
+  int cx = (int)__fdiv_rn(__fsub_rn((float)a.x, g.minx), g.step);
+  int cy = (int)__fdiv_rn(__fsub_rn((float)a.y, g.miny), g.step);
+  int cz = (int)__fdiv_rn(__fsub_rn((float)a.z, g.minz), g.step);
   key[i] = g.cbase + (uint32_t)(cx + g.ex * cy + g.ex * g.ey * cz);
   idx[i] = (uint32_t)i;
 }
-void launch_cell_key(const float4* raw, int64_t n, GridP g,
+void launch_cell_key(const Ball* raw, int64_t n, GridP g,
                      const GridP* grids, const uint32_t* mol, uint32_t* key,
                      uint32_t* idx, PipeLaunch L) {
   if (n <= 0) return;
@@ -200,23 +224,23 @@ void launch_cell_key(const float4* raw, int64_t n, GridP g,
                      dim3(kBlock), 0, L.s, raw, n, g, grids, mol, key, idx);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gather(const float4* raw,
+__global__ __launch_bounds__(kBlock) void k_gather(const Ball* raw,
                                                    const uint32_t* sidx,
                                                    const uint32_t* skey,
-                                                   int64_t n, float solvent,
-                                                   float4* ball, uint32_t* perm,
+                                                   int64_t n, Real solvent,
+                                                   Ball* ball, uint32_t* perm,
                                                    uint32_t* cellk) {
   int64_t s = gtid();
   if (s >= n) return;
   uint32_t o = sidx[s];
-  float4 a = raw[o];
-  ball[s] = make_float4(a.x, a.y, a.z, __fadd_rn(a.w, solvent));
+  Ball a = raw[o];
+  ball[s] = Ball{a.x, a.y, a.z, add_r(a.w, solvent)};
   perm[s] = o;
   cellk[s] = skey[s];
 }
-void launch_gather(const float4* raw, const uint32_t* sidx,
-                   const uint32_t* skey, int64_t n, float solvent,
-                   float4* ball, uint32_t* perm, uint32_t* cellk,
+void launch_gather(const Ball* raw, const uint32_t* sidx,
+                   const uint32_t* skey, int64_t n, Real solvent,
+                   Ball* ball, uint32_t* perm, uint32_t* cellk,
                    PipeLaunch L) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + kBlock - 1) / kBlock)),
@@ -245,12 +269,12 @@ void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
 /* filter decisions (one stage) and exact decisions                    */
 /* ------------------------------------------------------------------ */
 template <class T>
-__device__ int edge_code(const float4& a, const float4& b, float alpha) {
+__device__ int edge_code(const Ball& a, const Ball& b, Real alpha) {
   T v = edge_val(Ld<T>::p(a), Ld<T>::w(a), Ld<T>::p(b), Ld<T>::w(b),
                  Al<T>::c(alpha));
   return code_of(fe_sign(v));
 }
-__device__ int edge_exact(const float4& a, const float4& b,
+__device__ int edge_exact(const Ball& a, const Ball& b,
                           const ExactScale& X) {
   if (same_pos(a, b)) return REJECT;
   LdWI L{X.S};
@@ -259,15 +283,15 @@ __device__ int edge_exact(const float4& a, const float4& b,
 }
 
 template <class T>
-__device__ int tri_code(const float4& a, const float4& b, const float4& c,
-                        float alpha) {
+__device__ int tri_code(const Ball& a, const Ball& b, const Ball& c,
+                        Real alpha) {
   T nn;
   T v = tri_val(Ld<T>::p(a), Ld<T>::w(a), Ld<T>::p(b), Ld<T>::w(b),
                 Ld<T>::p(c), Ld<T>::w(c), Al<T>::c(alpha), &nn);
   if (fe_sign(nn) <= 0) return UNDEC;
   return code_of(fe_sign(v));
 }
-__device__ int tri_exact(const float4& a, const float4& b, const float4& c,
+__device__ int tri_exact(const Ball& a, const Ball& b, const Ball& c,
                          const ExactScale& X) {
   LdWI L{X.S};
   WI nn;
@@ -277,8 +301,8 @@ __device__ int tri_exact(const float4& a, const float4& b, const float4& c,
 }
 
 template <class T>
-__device__ int tet_code(const float4& a, const float4& b, const float4& c,
-                        const float4& d, float alpha) {
+__device__ int tet_code(const Ball& a, const Ball& b, const Ball& c,
+                        const Ball& d, Real alpha) {
   T D;
   V3<T> N;
   T wa = Ld<T>::w(a);
@@ -287,8 +311,8 @@ __device__ int tet_code(const float4& a, const float4& b, const float4& c,
   if (fe_sign(D) == 0) return UNDEC;
   return code_of(fe_sign(tet_val(D, N, wa, Al<T>::c(alpha))));
 }
-__device__ int tet_exact(const float4& a, const float4& b, const float4& c,
-                         const float4& d, const ExactScale& X) {
+__device__ int tet_exact(const Ball& a, const Ball& b, const Ball& c,
+                         const Ball& d, const ExactScale& X) {
   LdWI L{X.S};
   WI D;
   V3<WI> N;
@@ -518,7 +542,7 @@ void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
 template <int KIND>
 __device__ __forceinline__ void cand_balls(const Stage& S, const CandCtx& C,
                                            uint32_t seg, uint32_t val,
-                                           float4* v) {
+                                           Ball* v) {
   if (KIND == 1) {
     v[0] = S.ball[seg];
     v[1] = S.ball[val];
@@ -535,13 +559,13 @@ __device__ __forceinline__ void cand_balls(const Stage& S, const CandCtx& C,
   }
 }
 template <int KIND, class T>
-__device__ __forceinline__ int filter_code(const float4* v, float alpha) {
+__device__ __forceinline__ int filter_code(const Ball* v, Real alpha) {
   if (KIND == 1) return same_pos(v[0], v[1]) ? REJECT : edge_code<T>(v[0], v[1], alpha);
   if (KIND == 2) return tri_code<T>(v[0], v[1], v[2], alpha);
   return tet_code<T>(v[0], v[1], v[2], v[3], alpha);
 }
 template <int KIND>
-__device__ __forceinline__ int filter_exact(const float4* v,
+__device__ __forceinline__ int filter_exact(const Ball* v,
                                             const ExactScale& X) {
   if (KIND == 1) return edge_exact(v[0], v[1], X);
   if (KIND == 2) return tri_exact(v[0], v[1], v[2], X);
@@ -565,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
     if (seg == 0xffffffffu) {
       flag[o] = REJECT; /* unused slot */
     } else {
-      float4 v[4];
+      Ball v[4];
       cand_balls<KIND>(S, C, seg, cval[o], v);
       int r = filter_code<KIND, FE<R>>(v, S.alpha);
       flag[o] = (uint8_t)r;
@@ -584,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
   StageCnt sc;
   if (t < nq) {
     QEntry e = q[t];
-    float4 v[4];
+    Ball v[4];
     cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
     sc.c[1]++;
     int r = filter_code<KIND, FE<double>>(v, S.alpha);
@@ -609,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
   unsigned cnt = 0;
   if (t < nq) {
     QEntry e = q[t];
-    float4 v[4];
+    Ball v[4];
     cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
     flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
     cnt = 1;
@@ -733,4 +757,5 @@ void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
                        out_a, out_b);
 }
 
+}  // namespace PAC_V
 }  // namespace pac

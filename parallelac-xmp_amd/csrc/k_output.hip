@@ -14,6 +14,7 @@
 #include "pac_dev.h"
 
 namespace pac {
+using namespace PAC_V; /* type-independent device helpers */
 
 /* Orthocentre of a simplex relative to its first vertex, x = c - p_0, and
  * the barycentric coordinates lam of c (c = sum lam_q p_q), in FP64. */

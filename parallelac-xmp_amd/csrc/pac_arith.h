@@ -11,8 +11,9 @@
  *                          AccurateDWPlusDW / DWTimesDW3, relative error
  *                          <= 4u^2) with a double error bound charged at
  *                          2^-100 per op.
- *   WI                     512-bit two's-complement integer in 16 u32 limbs
- *                          (exact stage on inputs scaled by 2^S).
+ *   WI                     two's-complement integer in WL u32 limbs (512
+ *                          bits for FP32 balls, 768 for FP64; exact stage
+ *                          on inputs scaled by 2^S).
  *
  * The predicate polynomials are written once over any of these rings; they
  * restate the reference filters (AlphaComplexLargeData.cu:167-190, :298-348,
@@ -25,9 +26,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pac_variant.h"
+
 #define PAC_DEV __device__ __forceinline__
 
 namespace pac {
+namespace PAC_V {
 
 /* ------------------------------------------------------------------ */
 template <class T>
@@ -165,9 +169,8 @@ PAC_DEV int fe_sign(const FEDD& a) {
 }
 
 /* ------------------------------------------------------------------ */
-/* WI: 512-bit two's-complement integer                                */
+/* WI: WL-limb two's-complement integer                                */
 /* ------------------------------------------------------------------ */
-constexpr int WL = 16;
 struct WI {
   uint32_t l[WL];
 };
@@ -260,7 +263,8 @@ struct K<WI> {
 
 /* ------------------------------------------------------------------ */
 /* Ball loaders: position and weight of a ball in each ring.           */
-/* Balls are float4 (x, y, z, r) with r = fl32(r_raw + solvent).       */
+/* Balls are (x, y, z, r) with r = r_raw + solvent as the pipeline     */
+/* computed it; inputs are exact in FP64 and in double-double.          */
 /* ------------------------------------------------------------------ */
 struct ExactScale {
   int S;          /* coordinates scaled by 2^S are integers */
@@ -271,39 +275,73 @@ template <class T>
 struct Ld;
 template <class R>
 struct Ld<FE<R>> {
-  static PAC_DEV V3<FE<R>> p(const float4& b) {
+  static PAC_DEV V3<FE<R>> p(const Ball& b) {
     return V3<FE<R>>{FE<R>{(R)b.x, (R)0}, FE<R>{(R)b.y, (R)0},
                      FE<R>{(R)b.z, (R)0}};
   }
-  static PAC_DEV FE<R> w(const float4& b) {
+  static PAC_DEV FE<R> w(const Ball& b) {
     FE<R> r{(R)b.w, (R)0};
     return r * r;
   }
 };
 template <>
 struct Ld<FEDD> {
-  static PAC_DEV V3<FEDD> p(const float4& b) {
+  static PAC_DEV V3<FEDD> p(const Ball& b) {
     return V3<FEDD>{FEDD{dd{(double)b.x, 0.0}, 0.0},
                     FEDD{dd{(double)b.y, 0.0}, 0.0},
                     FEDD{dd{(double)b.z, 0.0}, 0.0}};
   }
-  static PAC_DEV FEDD w(const float4& b) {
+  static PAC_DEV FEDD w(const Ball& b) {
     FEDD r{dd{(double)b.w, 0.0}, 0.0};
     return r * r;
   }
 };
-PAC_DEV int64_t scaled_i64(float v, int S) {
-  return (int64_t)__builtin_ldexp((double)v, S);
+/* v * 2^S as an integer; S is at least minus the exponent of v's lowest
+ * set bit, so the product is exact.  FP32: |v| 2^S < 2^63 (checked on the
+ * host), one int64 conversion. */
+PAC_DEV WI wi_scaled(float v, int S) {
+  return wi_from_i64((int64_t)__builtin_ldexp((double)v, S));
+}
+/* FP64: the 53-bit significand shifted into place limb by limb. */
+PAC_DEV WI wi_scaled(double v, int S) {
+  WI r;
+  uint64_t bits = (uint64_t)__double_as_longlong(v);
+  int E = (int)((bits >> 52) & 0x7ff);
+  uint64_t m = bits & ((1ull << 52) - 1);
+  int e;
+  if (E == 0) {
+    e = -1074;
+  } else {
+    m |= 1ull << 52;
+    e = E - 1075;
+  }
+  int sh = e + S;
+  if (sh < 0) { /* the shifted-out bits are zero by the choice of S */
+    m >>= -sh;
+    sh = 0;
+  }
+  const int q = sh >> 5, b = sh & 31;
+  const uint64_t lo = m << b;
+  const uint32_t hi = b ? (uint32_t)(m >> (64 - b)) : 0u;
+#pragma unroll
+  for (int i = 0; i < WL; ++i)
+    r.l[i] = i == q ? (uint32_t)lo
+                    : (i == q + 1 ? (uint32_t)(lo >> 32) : (i == q + 2 ? hi : 0u));
+  if (bits >> 63) {
+    WI z;
+#pragma unroll
+    for (int i = 0; i < WL; ++i) z.l[i] = 0;
+    return z - r;
+  }
+  return r;
 }
 struct LdWI {
   int S;
-  PAC_DEV V3<WI> p(const float4& b) const {
-    return V3<WI>{wi_from_i64(scaled_i64(b.x, S)),
-                  wi_from_i64(scaled_i64(b.y, S)),
-                  wi_from_i64(scaled_i64(b.z, S))};
+  PAC_DEV V3<WI> p(const Ball& b) const {
+    return V3<WI>{wi_scaled(b.x, S), wi_scaled(b.y, S), wi_scaled(b.z, S)};
   }
-  PAC_DEV WI w(const float4& b) const {
-    WI r = wi_from_i64(scaled_i64(b.w, S));
+  PAC_DEV WI w(const Ball& b) const {
+    WI r = wi_scaled(b.w, S);
     return r * r;
   }
 };
@@ -381,5 +419,6 @@ PAC_DEV T power_gap(const V3<T>& pi, const T& wi, const V3<T>& pe,
   return Le * den - vdot(y, num);
 }
 
+}  // namespace PAC_V
 }  // namespace pac
 #endif

@@ -10,6 +10,7 @@
 #include "pac_internal.h"
 
 namespace pac {
+namespace PAC_V {
 
 __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 #pragma unroll
@@ -88,7 +89,7 @@ struct StageCnt {
 __device__ __forceinline__ int64_t gtid() {
   return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 }
-__device__ __forceinline__ bool same_pos(const float4& a, const float4& b) {
+__device__ __forceinline__ bool same_pos(const Ball& a, const Ball& b) {
   return a.x == b.x && a.y == b.y && a.z == b.z;
 }
 __device__ __forceinline__ void push_q(const Queue& q, uint32_t item,
@@ -101,11 +102,11 @@ template <class T>
 struct Al;
 template <class R>
 struct Al<FE<R>> {
-  static __device__ FE<R> c(float a) { return FE<R>{(R)a, (R)0}; }
+  static __device__ FE<R> c(Real a) { return FE<R>{(R)a, (R)0}; }
 };
 template <>
 struct Al<FEDD> {
-  static __device__ FEDD c(float a) { return FEDD{dd{(double)a, 0.0}, 0.0}; }
+  static __device__ FEDD c(Real a) { return FEDD{dd{(double)a, 0.0}, 0.0}; }
 };
 
 enum { REJECT = 0, KEEP = 1, UNDEC = 2 };
@@ -244,14 +245,14 @@ __device__ __forceinline__ void scan_box(const GridP& g,
  * true to stop.  Indices past the end of a range are clamped to its last
  * element (always a valid address) and not visited. */
 template <class F>
-__device__ __forceinline__ bool scan_range4(const float4* ball, uint32_t b,
+__device__ __forceinline__ bool scan_range4(const Ball* ball, uint32_t b,
                                             uint32_t e, F& visit) {
   for (uint32_t p = b; p < e; p += 4) {
     uint32_t l = e - 1;
-    float4 E0 = ball[p];
-    float4 E1 = ball[min(p + 1, l)];
-    float4 E2 = ball[min(p + 2, l)];
-    float4 E3 = ball[min(p + 3, l)];
+    Ball E0 = ball[p];
+    Ball E1 = ball[min(p + 1, l)];
+    Ball E2 = ball[min(p + 2, l)];
+    Ball E3 = ball[min(p + 3, l)];
     if (visit(p, E0)) return true;
     if (p + 1 < e && visit(p + 1, E1)) return true;
     if (p + 2 < e && visit(p + 2, E2)) return true;
@@ -262,7 +263,7 @@ __device__ __forceinline__ bool scan_range4(const float4* ball, uint32_t b,
 template <class F>
 __device__ __forceinline__ void scan_box4(const GridP& g,
                                           const uint32_t* cstart,
-                                          const float4* ball, const CellBox& b,
+                                          const Ball* ball, const CellBox& b,
                                           F&& visit) {
   if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
   {
@@ -372,7 +373,7 @@ __device__ __forceinline__ void cross_ss(R ax, R ay, R az, R bx, R by, R bz,
 
 template <class R>
 __device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
-                                           const float4& e) {
+                                           const Ball& e) {
   R yx = (R)e.x - P.ax, yy = (R)e.y - P.ay, yz = (R)e.z - P.az;
   R q = yx * yx + yy * yy + yz * yz;
   R we = (R)e.w * (R)e.w;
@@ -384,5 +385,6 @@ __device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
   return g > B ? 1 : (g < -B ? -1 : 0);
 }
 
+}  // namespace PAC_V
 }  // namespace pac
 #endif

@@ -1,6 +1,8 @@
 /*
- * pac_internal.h -- shared between the host engine (pac_engine.cpp) and the
- * kernel translation units.  Not part of the C ABI.
+ * pac_internal.h -- shared between the host engine and the kernel
+ * translation units.  Not part of the C ABI.  Types and launchers that do
+ * not depend on the ball type live in namespace pac; the rest in the
+ * compilation's ball-type namespace (pac_variant.h).
  */
 #ifndef PAC_INTERNAL_H
 #define PAC_INTERNAL_H
@@ -30,14 +32,13 @@ struct GridP {
   double margin;  /* FP32 cell-key rounding, in length units */
 };
 
-/* raw-input bounds reduced on the device */
+/* raw-input bounds reduced on the device (values of the input type) */
 struct Bounds {
-  float minx, miny, minz, maxx, maxy, maxz;
-  float max_raw_r, max_r;  /* raw radius, and fl(r_raw + solvent) */
+  double minx, miny, minz, maxx, maxy, maxz;
+  double max_raw_r, max_r; /* raw radius, and r_raw + solvent */
+  double max_abs;          /* max |coordinate| */
   int shift;               /* max over inputs of -lsb exponent (>= 0) */
   int nonfinite;           /* count of non-finite inputs */
-  float max_abs;           /* max |coordinate| */
-  float pad;
 };
 
 struct QEntry {
@@ -70,42 +71,9 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                       size_t tmp_bytes, hipStream_t s, bool* result_in_alt);
 size_t radix_tmp_bytes(int64_t n);
 
-/* ---- pipeline (k_pipeline.hip) ---- */
 struct PipeLaunch {
   hipStream_t s;
   int block;
-};
-void launch_bounds(const float4* raw, int64_t n, float solvent, Bounds* part,
-                   Bounds* out, PipeLaunch L);
-/* per-molecule bounds of a batch: one block per molecule, moff[nmol+1] */
-void launch_bounds_mol(const float4* raw, const uint32_t* moff, int64_t nmol,
-                       float solvent, Bounds* out, PipeLaunch L);
-/* mol[i] = molecule of ball i (same for input and cell-sorted order) */
-void launch_mol_ids(const uint32_t* moff, int64_t nmol, int64_t n,
-                    uint32_t* mol, PipeLaunch L);
-/* global cell key; grids/mol == nullptr: the single grid g */
-void launch_cell_key(const float4* raw, int64_t n, GridP g,
-                     const GridP* grids, const uint32_t* mol, uint32_t* key,
-                     uint32_t* idx, PipeLaunch L);
-void launch_gather(const float4* raw, const uint32_t* sidx,
-                   const uint32_t* skey, int64_t n, float solvent,
-                   float4* ball, uint32_t* perm, uint32_t* cellk,
-                   PipeLaunch L);
-void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
-                        uint32_t* cstart, PipeLaunch L);
-
-struct Stage {
-  const float4* ball;
-  int64_t n;
-  GridP g;              /* the grid of a single molecule */
-  const GridP* grids;   /* batch: grid of each molecule (else nullptr) */
-  const uint32_t* mol;  /* batch: molecule of each ball (else nullptr) */
-  const uint32_t* cellk;
-  const uint32_t* cstart;
-  float alpha;
-  ExactScale ex;
-  unsigned long long* ctr;
-  int staged;
 };
 
 struct Queue {
@@ -114,12 +82,103 @@ struct Queue {
   uint32_t cap;
 };
 
-/* ---- enumeration + filters (k_enum.hip) ----
- * Candidates of each dimension are (segment, new vertex) pairs: segment =
- * ball (edges), alpha edge (triangles), alpha triangle (tets). */
 struct CandCtx {
   const uint32_t *eleft, *ecol, *tk, *tedge;
 };
+
+/* ---- closure + emission (k_closure.hip) ---- */
+void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
+                         const uint64_t* pos, uint4* out, PipeLaunch L);
+void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
+                        uint32_t* out, PipeLaunch L);
+void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
+                       const uint32_t* ecol, const uint64_t* trow,
+                       const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
+void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+                       const uint32_t* teik, const uint32_t* tejk,
+                       uint8_t* edge_in, PipeLaunch L);
+void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L);
+void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+                      const uint32_t* eleft, const uint32_t* tk,
+                      const uint32_t* tedge, const uint32_t* perm,
+                      int32_t* out, PipeLaunch L);
+void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+                       const uint8_t* flag, const uint64_t* pos,
+                       const uint32_t* perm, int32_t* out, PipeLaunch L);
+
+/* ---- outputs (k_output.hip) ---- */
+void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
+                double* rho, hipStream_t s);
+/* adjoint: per-(simplex, vertex) contributions at slots slot0 + (dim+1)*id + q
+ * with key = vertex, val = slot; then per-ball sums over vstart ranges of
+ * the slots sorted by vertex */
+void launch_adj_contrib(int dim, int64_t m, const int32_t* rows,
+                        const double4* balls, const double* g, int64_t slot0,
+                        uint32_t* key, uint32_t* val, double4* contrib,
+                        hipStream_t s);
+void launch_adj_gather(int64_t n, const uint32_t* vstart, const uint32_t* sval,
+                       const double4* contrib, double* dxyz, double* dr,
+                       hipStream_t s);
+void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
+                    uint32_t* idx, uint32_t* key, uint32_t* idx2,
+                    uint32_t* key2, void* rtmp, size_t rtb, int32_t* copy,
+                    hipStream_t s);
+
+/* batch: first row of each molecule in a list of m rows of K original ids
+ * grouped by molecule (moff: first original id of each molecule);
+ * out[nmol+1] */
+void launch_mol_rows(const int32_t* rows, int64_t m, int K,
+                     const uint32_t* moff, int64_t nmol, uint64_t* out,
+                     hipStream_t s);
+
+
+
+}  // namespace pac
+
+namespace pac {
+namespace PAC_V {
+/* ---- binning (k_enum.hip) ---- */
+
+void launch_bounds(const Ball* raw, int64_t n, Real solvent, Bounds* part,
+                   Bounds* out, PipeLaunch L);
+/* per-molecule bounds of a batch: one block per molecule, moff[nmol+1] */
+void launch_bounds_mol(const Ball* raw, const uint32_t* moff, int64_t nmol,
+                       Real solvent, Bounds* out, PipeLaunch L);
+/* mol[i] = molecule of ball i (same for input and cell-sorted order) */
+void launch_mol_ids(const uint32_t* moff, int64_t nmol, int64_t n,
+                    uint32_t* mol, PipeLaunch L);
+/* global cell key; grids/mol == nullptr: the single grid g */
+void launch_cell_key(const Ball* raw, int64_t n, GridP g,
+                     const GridP* grids, const uint32_t* mol, uint32_t* key,
+                     uint32_t* idx, PipeLaunch L);
+void launch_gather(const Ball* raw, const uint32_t* sidx,
+                   const uint32_t* skey, int64_t n, Real solvent,
+                   Ball* ball, uint32_t* perm, uint32_t* cellk,
+                   PipeLaunch L);
+void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
+                        uint32_t* cstart, PipeLaunch L);
+
+struct Stage {
+  const Ball* ball;
+  int64_t n;
+  GridP g;              /* the grid of a single molecule */
+  const GridP* grids;   /* batch: grid of each molecule (else nullptr) */
+  const uint32_t* mol;  /* batch: molecule of each ball (else nullptr) */
+  const uint32_t* cellk;
+  const uint32_t* cstart;
+  Real alpha;
+  ExactScale ex;
+  unsigned long long* ctr;
+  int staged;
+};
+
+
+
+/* ---- enumeration + filters (k_enum.hip) ----
+ * Candidates of each dimension are (segment, new vertex) pairs: segment =
+ * ball (edges), alpha edge (triangles), alpha triangle (tets). */
+
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
 void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
                       uint32_t* cseg, PipeLaunch L);
@@ -184,54 +243,10 @@ void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L);
 
-/* ---- closure + emission (k_closure.hip) ---- */
-void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
-                         const uint64_t* pos, uint4* out, PipeLaunch L);
-void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
-                        uint32_t* out, PipeLaunch L);
-void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
-                       const uint32_t* ecol, const uint64_t* trow,
-                       const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
-void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
-                       const uint32_t* teik, const uint32_t* tejk,
-                       uint8_t* edge_in, PipeLaunch L);
-void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
-                      int32_t* out, PipeLaunch L);
-void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
-                      const uint32_t* eleft, const uint32_t* tk,
-                      const uint32_t* tedge, const uint32_t* perm,
-                      int32_t* out, PipeLaunch L);
-void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
-                       const uint8_t* flag, const uint64_t* pos,
-                       const uint32_t* perm, int32_t* out, PipeLaunch L);
-
-/* ---- outputs (k_output.hip) ---- */
-void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
-                double* rho, hipStream_t s);
-/* adjoint: per-(simplex, vertex) contributions at slots slot0 + (dim+1)*id + q
- * with key = vertex, val = slot; then per-ball sums over vstart ranges of
- * the slots sorted by vertex */
-void launch_adj_contrib(int dim, int64_t m, const int32_t* rows,
-                        const double4* balls, const double* g, int64_t slot0,
-                        uint32_t* key, uint32_t* val, double4* contrib,
-                        hipStream_t s);
-void launch_adj_gather(int64_t n, const uint32_t* vstart, const uint32_t* sval,
-                       const double4* contrib, double* dxyz, double* dr,
-                       hipStream_t s);
-void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
-                    uint32_t* idx, uint32_t* key, uint32_t* idx2,
-                    uint32_t* key2, void* rtmp, size_t rtb, int32_t* copy,
-                    hipStream_t s);
-
-/* batch: first row of each molecule in a list of m rows of K original ids
- * grouped by molecule (moff: first original id of each molecule);
- * out[nmol+1] */
-void launch_mol_rows(const int32_t* rows, int64_t m, int K,
-                     const uint32_t* moff, int64_t nmol, uint64_t* out,
-                     hipStream_t s);
 
 /* SoS table as compiled into the device code (for tests) */
 int sos_table_host(int32_t* out3, int cap);
 
+}  // namespace PAC_V
 }  // namespace pac
 #endif

@@ -1,0 +1,578 @@
+/*
+ * pac_pipeline.cpp -- the pipeline driver: one pass of binning,
+ * enumeration + alpha filters, emptiness tests, closure and emission on one
+ * stream (replaces the phase sequence of the reference's main(),
+ * AlphaComplexParallel.cu:1126-1806).  Compiled once per ball type
+ * (pac_variant.h): pac::vf32::run_pipeline, pac::vf64::run_pipeline.
+ */
+#include "pac_host.h"
+
+namespace pac {
+namespace PAC_V {
+
+/* v * 2^shift as a WL-limb two's-complement integer (v*2^shift integral) */
+static WI wi_from_scaled_host(double v, int shift) {
+  WI r;
+  std::memset(&r, 0, sizeof(r));
+  if (v == 0) return r;
+  int e;
+  double m = std::frexp(v, &e);
+  long long mi = (long long)std::ldexp(m, 53);
+  bool neg = mi < 0;
+  unsigned long long mag = (unsigned long long)(neg ? -mi : mi);
+  int ex = e - 53 + shift;
+  while (ex < 0 && (mag & 1ull) == 0) {
+    mag >>= 1;
+    ++ex;
+  }
+  if (ex < 0) fail(PAC_ERR_RANGE, "alpha not representable at the scale");
+  /* place mag << ex into limbs */
+  for (int b = 0; b < 64; ++b)
+    if (mag >> b & 1ull) {
+      int pos = b + ex;
+      if (pos >= 32 * WL - 1) fail(PAC_ERR_RANGE, "alpha too large");
+      r.l[pos / 32] |= 1u << (pos % 32);
+    }
+  if (neg) {
+    /* two's complement negate */
+    uint64_t carry = 1;
+    for (int i = 0; i < WL; ++i) {
+      uint64_t t = (uint64_t)(uint32_t)~r.l[i] + carry;
+      r.l[i] = (uint32_t)t;
+      carry = t >> 32;
+    }
+  }
+  return r;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* the pipeline                                                        */
+/* ------------------------------------------------------------------ */
+/* Grid of one molecule from its bounds (AlphaComplexParallel.cu:1120-1124,
+ * evaluated in FP32 exactly as there), with the stencil half-widths that
+ * make the enumeration and the emptiness boxes complete. */
+struct GridPlan {
+  GridP g;
+  int64_t G = 0;
+};
+static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
+  GridPlan P;
+  GridP& g = P.g;
+  std::memset(&g, 0, sizeof(g));
+  /* FP32 throughout, as the reference (FP64 balls: on their FP32
+   * roundings, the values their cell keys are computed from) */
+  const float mr = (float)B.max_raw_r, fal = (float)alpha;
+  const float fminx = (float)B.minx, fminy = (float)B.miny, fminz = (float)B.minz;
+  volatile float m2 = mr * mr;
+  volatile float m2a = m2 + fal;
+  volatile float pmr = sqrtf(m2a);
+  float step = 2 * pmr;
+  if (!(step > 0) || !std::isfinite(step))
+    fail(PAC_ERR_INVALID, "grid step 2*sqrt(maxR^2+alpha) is not positive");
+  volatile float qx = ((float)B.maxx - fminx) / step,
+                 qy = ((float)B.maxy - fminy) / step,
+                 qz = ((float)B.maxz - fminz) / step;
+  int64_t ex = (int64_t)(int)qx + 1, ey = (int64_t)(int)qy + 1,
+          ez = (int64_t)(int)qz + 1;
+  P.G = ex * ey * ez;
+  if (P.G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grid has too many cells");
+  g.minx = fminx;
+  g.miny = fminy;
+  g.minz = fminz;
+  g.step = step;
+  g.ex = (int)ex;
+  g.ey = (int)ey;
+  g.ez = (int)ez;
+  double maxw = (double)B.max_r * (double)B.max_r;
+  double reach2 = maxw + (double)alpha;
+  double reach = reach2 > 0 ? std::sqrt(reach2) * (1 + 1e-12) + 1e-12 : 0.0;
+  double maxext = std::max({(double)B.maxx - B.minx, (double)B.maxy - B.miny,
+                            (double)B.maxz - B.minz});
+  /* key rounding: FP32 subtract and divide (and for FP64 balls the
+   * rounding of the coordinate itself), in length units */
+  double margin = (maxext + (double)step + (kBallF64 ? B.max_abs : 0.0)) *
+                      3.0 * 0x1p-24 + 1e-9;
+  if (policy == PAC_CELL_REFERENCE_RAW) {
+    g.k_enum = 1;
+    g.k_ac2 = 1;
+    reach = (double)step * (1 - 1e-6);
+    margin = 0;
+  } else {
+    g.k_enum = std::max(1, (int)std::ceil((2 * reach + 2 * margin) / step));
+    g.k_ac2 = std::max(1, (int)std::ceil((reach + margin) / step));
+  }
+  g.reach = reach;
+  g.margin = margin;
+  return P;
+}
+
+/* One pipeline pass over n balls.  nmol == 1: one molecule; otherwise the
+ * balls are nmol molecules, moff[0..nmol] (host, relative to raw), each with
+ * its own grid, processed together by every kernel launch. */
+void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
+                  Real alpha, const pac_options& opt, pac_result* R,
+                  hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
+                  int64_t nmol) {
+  Pool* P = &E->pool;
+  PipeLaunch L{s, kBlock};
+  pac_stats& st = R->st;
+  st.n = n;
+  if (n == 0) return;
+  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
+  const bool batch = nmol > 1;
+
+  /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
+  std::vector<Bounds> hb(std::max<int64_t>(nmol, 1));
+  DBuf<uint32_t> dmoff, dmol;
+  DBuf<GridP> dgrids;
+  if (!batch) {
+    DBuf<Bounds> bpart(P, 1024 * 2 + 4), bout(P, 1);
+    launch_bounds(raw, n, solvent, bpart.get(), bout.get(), L);
+    KCHK();
+    hb[0] = readback(E, s, bout.get());
+  } else {
+    std::vector<uint32_t> mo(nmol + 1);
+    for (int64_t m = 0; m <= nmol; ++m) mo[m] = (uint32_t)moff[m];
+    dmoff = DBuf<uint32_t>(P, nmol + 1);
+    HIPCHK(hipMemcpyAsync(dmoff.get(), mo.data(), (nmol + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    DBuf<Bounds> bout(P, nmol);
+    launch_bounds_mol(raw, dmoff.get(), nmol, solvent, bout.get(), L);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(hb.data(), bout.get(), nmol * sizeof(Bounds),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  std::vector<GridP> grids(hb.size());
+  int64_t G = 0;
+  int shift = 0;
+  double mag = 1.0;
+  bool no_edges = true;
+  st.k_enum = st.k_ac2 = 0;
+  for (size_t m = 0; m < hb.size(); ++m) {
+    const Bounds& B = hb[m];
+    if (B.nonfinite) fail(PAC_ERR_INVALID, "non-finite coordinate or radius");
+    bool empty = batch && moff[m + 1] == moff[m];
+    GridPlan gp;
+    if (empty) { /* one empty cell, never visited */
+      std::memset(&gp.g, 0, sizeof(gp.g));
+      gp.g.step = 1;
+      gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
+      gp.G = 1;
+    } else {
+      gp = plan_grid(B, alpha, opt.cell_policy);
+      shift = std::max(shift, B.shift);
+      mag = std::max({mag, (double)B.max_abs, (double)B.max_r});
+      if ((double)B.max_r * (double)B.max_r + (double)alpha >= 0) no_edges = false;
+    }
+    gp.g.cbase = (uint32_t)G;
+    G += gp.G;
+    if (G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grids have too many cells");
+    grids[m] = gp.g;
+    st.k_enum = std::max(st.k_enum, gp.g.k_enum);
+    st.k_ac2 = std::max(st.k_ac2, gp.g.k_ac2);
+  }
+  st.grid_cells = (int32_t)G;
+
+  /* exact embedding scale (one for all molecules: any common scale at
+   * which every input is an integer gives the same exact signs) */
+  int S = std::max(shift, (neg_lsb_exp_host(alpha) + 1) / 2);
+  mag = std::max(mag, std::sqrt(std::fabs((double)alpha)));
+  double bits = std::log2(mag) + S + 2;
+  if (8 * bits + 16 > 32.0 * WL - 2)
+    fail(PAC_ERR_RANGE, "input exponent range exceeds the 512-bit exact stage");
+  st.scale_bits = S;
+  Stage SG;
+  std::memset(&SG, 0, sizeof(SG));
+  SG.n = n;
+  SG.g = grids[0];
+  if (batch) {
+    dgrids = DBuf<GridP>(P, nmol);
+    HIPCHK(hipMemcpyAsync(dgrids.get(), grids.data(), nmol * sizeof(GridP),
+                          hipMemcpyHostToDevice, s));
+    dmol = DBuf<uint32_t>(P, n);
+    launch_mol_ids(dmoff.get(), nmol, n, dmol.get(), L);
+    KCHK();
+    SG.grids = dgrids.get();
+    SG.mol = dmol.get();
+  }
+  SG.alpha = alpha;
+  SG.ex.S = S;
+  SG.ex.alpha = wi_from_scaled_host((double)alpha, 2 * S);
+  /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
+  SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
+  DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
+  zero_u64(ctr.get(), CTR_COUNT * CTR_SLOTS, s);
+  SG.ctr = ctr.get();
+  DBuf<unsigned> qn(P, 4); /* [0] hot queue, [1] escalation queue */
+
+  /* ---- binning: key, stable radix sort, gather, cell starts ---- */
+  DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
+  launch_cell_key(raw, n, SG.g, SG.grids, SG.mol, key.get(), idx.get(), L);
+  KCHK();
+  int kbits = 0;
+  while (kbits < 32 && ((int64_t)1 << kbits) < G) ++kbits;
+  size_t rtb = radix_tmp_bytes(n);
+  DBuf<uint8_t> rtmp(P, rtb);
+  bool alt = false;
+  KP.begin(KP_SORT);
+  radix_sort_pairs(key.get(), idx.get(), key2.get(), idx2.get(), n, kbits,
+                   rtmp.get(), rtb, s, &alt);
+  KP.end();
+  KCHK();
+  const uint32_t* skey = alt ? key2.get() : key.get();
+  const uint32_t* sidx = alt ? idx2.get() : idx.get();
+  DBuf<Ball> ball(P, n);
+  DBuf<uint32_t> perm(P, n), cellk(P, n), cstart(P, G + 1);
+  launch_gather(raw, sidx, skey, n, solvent, ball.get(), perm.get(),
+                cellk.get(), L);
+  launch_cell_starts(cellk.get(), n, G, cstart.get(), L);
+  KCHK();
+  rtmp.release();
+  key.release();
+  key2.release();
+  idx.release();
+  idx2.release();
+  SG.ball = ball.get();
+  SG.cellk = cellk.get();
+  SG.cstart = cstart.get();
+  T.mark(2);
+
+  auto scan_tmp = [&](int64_t m) { return DBuf<uint8_t>(P, scan_tmp_bytes(m) + 64); };
+  const bool staged = SG.staged != 0;
+
+  /* precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
+   * double-double (q1 -> q2)] -> exact wide-integer kernel.  Queue
+   * overflows rerun the producing kernel with a larger queue; only the
+   * emptiness kernels can overflow, and they are idempotent. */
+  auto ladder = [&](uint32_t cap1, int kp_hot, int kp_exact, auto&& hot,
+                    auto&& esc, auto&& exact) {
+    for (;;) {
+      DBuf<QEntry> q1(P, cap1);
+      zero_u32(qn.get(), 2, s);
+      if (kp_hot >= 0) KP.begin(kp_hot);
+      hot(Queue{q1.get(), qn.get(), cap1});
+      if (kp_hot >= 0) KP.end();
+      KCHK();
+      uint32_t n1 = readback(E, s, qn.get());
+      if (n1 > cap1) {
+        cap1 = (uint32_t)std::min<int64_t>((int64_t)n1 + n1 / 4 + 1024,
+                                           (int64_t)UINT32_MAX - 1);
+        continue;
+      }
+      const QEntry* qx = q1.get();
+      uint32_t nx = n1;
+      DBuf<QEntry> q2;
+      if (staged && n1) {
+        uint32_t cap2 = (uint32_t)std::min<int64_t>(4 * (int64_t)n1 + 4096,
+                                                    (int64_t)UINT32_MAX - 1);
+        for (;;) {
+          q2 = DBuf<QEntry>(P, cap2);
+          zero_u32(qn.get() + 1, 1, s);
+          esc(q1.get(), n1, Queue{q2.get(), qn.get() + 1, cap2});
+          KCHK();
+          uint32_t n2 = readback(E, s, qn.get() + 1);
+          if (n2 <= cap2) {
+            nx = n2;
+            break;
+          }
+          cap2 = (uint32_t)std::min<int64_t>((int64_t)n2 + n2 / 4 + 1024,
+                                             (int64_t)UINT32_MAX - 1);
+        }
+        qx = q2.get();
+      }
+      if (kp_exact >= 0) KP.begin(kp_exact);
+      exact(qx, nx);
+      if (kp_exact >= 0) KP.end();
+      KCHK();
+      return;
+    }
+  };
+
+  /* alpha filter of one dimension over its enumerated candidates: the
+   * ladder settles every flag, a flat scan of the flags gives both the
+   * stable compaction positions and the survivors' CSR row starts. */
+  int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
+  auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
+                          const uint32_t* cseg, uint8_t* flag, int kp,
+                          DBuf<uint64_t>& pos) -> int64_t {
+    if (kind == 1) host_evals[0] += nc; /* tris / tets: slots != candidates */
+    ladder(
+        (uint32_t)std::max<int64_t>(nc, 1), kp, -1,
+        [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_filter_esc(SG, kind, C, q, nq, cval, flag, q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_filter_exact(SG, kind, C, q, nq, cval, flag, L);
+        });
+    pos = DBuf<uint64_t>(P, nc + 1);
+    {
+      auto tmp = scan_tmp(nc);
+      scan_u8flag_to_u64(flag, pos.get(), nc, tmp.get(), 0, s);
+    }
+    KCHK();
+    return (int64_t)readback(E, s, pos.get() + nc);
+  };
+  auto count_scan = [&](const uint32_t* cnt, int64_t nseg, uint64_t* off,
+                        const char* what) -> int64_t {
+    {
+      auto tmp = scan_tmp(nseg);
+      scan_u32_to_u64(cnt, off, nseg, tmp.get(), 0, s);
+    }
+    KCHK();
+    int64_t tot = (int64_t)readback(E, s, off + nseg);
+    if (tot >= ((int64_t)1 << 32))
+      fail(PAC_ERR_OVERFLOW, std::string(what) + " candidates exceed 2^32");
+    return tot;
+  };
+
+  /* ---- edges (AlphaComplexParallel.cu:1200-1263) ---- */
+  int64_t Ecand = 0, Ealpha = 0;
+  DBuf<uint64_t> erow(P, n + 1);
+  DBuf<uint32_t> ecol, eleft;
+  CandCtx C0{nullptr, nullptr, nullptr, nullptr};
+  if (!no_edges) {
+    DBuf<uint64_t> off(P, n + 1);
+    {
+      DBuf<uint32_t> cnt(P, n);
+      launch_edge_count(SG, cnt.get(), L);
+      Ecand = count_scan(cnt.get(), n, off.get(), "edge");
+    }
+    DBuf<uint32_t> cval(P, Ecand), cseg(P, Ecand);
+    DBuf<uint8_t> flag(P, Ecand);
+    launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L);
+    DBuf<uint64_t> pos;
+    Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
+                          KP_EDGE_FILL, pos);
+    launch_seg_starts(n, off.get(), pos.get(), erow.get(), L);
+    ecol = DBuf<uint32_t>(P, Ealpha);
+    eleft = DBuf<uint32_t>(P, Ealpha);
+    launch_compact_flat(1, Ecand, C0, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), ecol.get(), eleft.get(), nullptr, nullptr,
+                        nullptr, nullptr, nullptr, L);
+    KCHK();
+  } else {
+    zero_u64((unsigned long long*)erow.get(), n + 1, s);
+    ecol = DBuf<uint32_t>(P, 1);
+    eleft = DBuf<uint32_t>(P, 1);
+  }
+  st.edge_candidates = Ecand;
+  st.edge_alpha = Ealpha;
+  T.mark(3);
+
+  /* ---- triangles (AlphaComplexParallel.cu:1279-1371) ---- */
+  int64_t Tcand = 0, Tslots = 0, Talpha = 0;
+  DBuf<uint64_t> trow(P, Ealpha + 1);
+  DBuf<uint32_t> tk, tedge, teik, tejk;
+  CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr};
+  {
+    DBuf<uint64_t> off(P, Ealpha + 1);
+    {
+      /* exact counts: a triangle's third vertex must be in both rows, and
+       * the merge that counts it is cheap next to filtering empty slots */
+      DBuf<uint32_t> cnt(P, Ealpha);
+      launch_tri_count(Ealpha, erow.get(), ecol.get(), eleft.get(), cnt.get(), L);
+      Tslots = count_scan(cnt.get(), Ealpha, off.get(), "triangle");
+    }
+    DBuf<uint32_t> cval(P, Tslots), cseg(P, Tslots), cea(P, Tslots),
+        ceb(P, Tslots);
+    DBuf<uint8_t> flag(P, Tslots);
+    launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
+                    cval.get(), cseg.get(), cea.get(), ceb.get(), ctr.get(), L);
+    DBuf<uint64_t> pos;
+    Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
+                          KP_TRI_FILL, pos);
+    launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L);
+    tk = DBuf<uint32_t>(P, Talpha);
+    tedge = DBuf<uint32_t>(P, Talpha);
+    teik = DBuf<uint32_t>(P, Talpha);
+    tejk = DBuf<uint32_t>(P, Talpha);
+    launch_compact_flat(2, Tslots, C1, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), tk.get(), tedge.get(), nullptr, cea.get(),
+                        ceb.get(), teik.get(), tejk.get(), L);
+    KCHK();
+  }
+  st.tri_alpha = Talpha;
+  T.mark(4);
+
+  /* ---- tets (AlphaComplexParallel.cu:1381-1442) ---- */
+  int64_t Kcand = 0, Kslots = 0, Kalpha = 0;
+  DBuf<uint4> tets;
+  CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get()};
+  {
+    DBuf<uint64_t> off(P, Talpha + 1);
+    {
+      DBuf<uint32_t> cap(P, Talpha);
+      launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
+      Kslots = count_scan(cap.get(), Talpha, off.get(), "tet");
+    }
+    DBuf<uint32_t> cval(P, Kslots), cseg(P, Kslots);
+    DBuf<uint8_t> flag(P, Kslots);
+    launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
+                    tejk.get(), off.get(), cval.get(), cseg.get(), ctr.get(), L);
+    DBuf<uint64_t> pos;
+    Kalpha = filter_stage(3, C2, Kslots, cval.get(), cseg.get(), flag.get(),
+                          KP_TET_FILL, pos);
+    tets = DBuf<uint4>(P, Kalpha);
+    launch_compact_flat(3, Kslots, C2, cval.get(), cseg.get(), flag.get(),
+                        pos.get(), nullptr, nullptr, tets.get(), nullptr,
+                        nullptr, nullptr, nullptr, L);
+    KCHK();
+  }
+  (void)Tcand;
+  (void)Kcand;
+  st.tet_alpha = Kalpha;
+  T.mark(5);
+
+  auto ac2_cap = [](int64_t items) {
+    return (uint32_t)std::min<int64_t>(std::max<int64_t>(1 << 20, items / 8),
+                                       (int64_t)UINT32_MAX - 1);
+  };
+
+  /* ---- tet emptiness ---- */
+  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
+  DBuf<uint32_t> rest(P, std::max({Kalpha, Talpha, Ealpha, (int64_t)1}));
+  DBuf<uint8_t> kflag(P, Kalpha);
+  if (opt.skip_tet_ac2) {
+    fill_u8(kflag.get(), Kalpha, 1, s);
+  } else if (Kalpha) {
+    ladder(
+        ac2_cap(Kalpha), KP_TET_AC2, KP_TET_AC2_EXACT,
+        [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
+          launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
+                             qn.get() + 2, budget, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_tet_ac2_esc(SG, q, nq, tets.get(), kflag.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_tet_ac2_exact(SG, q, nq, tets.get(), perm.get(), kflag.get(), L);
+        });
+  }
+  int64_t Kf = 0;
+  DBuf<uint4> ftets;
+  {
+    DBuf<uint64_t> pos(P, Kalpha + 1);
+    auto tmp = scan_tmp(Kalpha);
+    scan_u8flag_to_u64(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s);
+    KCHK();
+    Kf = (int64_t)readback(E, s, pos.get() + Kalpha);
+    ftets = DBuf<uint4>(P, Kf);
+    launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(), L);
+    KCHK();
+  }
+  tets.release();
+  T.mark(6);
+
+  /* ---- triangle closure + dangling emptiness ---- */
+  DBuf<uint8_t> tri_in(P, Talpha), tflag(P, Talpha);
+  fill_u8(tri_in.get(), Talpha, 0, s);
+  launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
+                    tk.get(), tri_in.get(), L);
+  if (Talpha)
+    ladder(
+        ac2_cap(Talpha), KP_TRI_AC2, -1,
+        [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
+          launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
+                             tedge.get(), tri_in.get(), tflag.get(), q,
+                             rest.get(), qn.get() + 2, budget, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_low_ac2_esc(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
+                             tedge.get(), tflag.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_low_ac2_exact(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
+                               tedge.get(), tflag.get(), L);
+        });
+  int64_t Tf = 0;
+  DBuf<uint32_t> ftri;
+  {
+    DBuf<uint64_t> pos(P, Talpha + 1);
+    auto tmp = scan_tmp(Talpha);
+    scan_u8flag_to_u64(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s);
+    KCHK();
+    Tf = (int64_t)readback(E, s, pos.get() + Talpha);
+    ftri = DBuf<uint32_t>(P, Tf);
+    launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L);
+    KCHK();
+  }
+  T.mark(7);
+
+  /* ---- edge closure + dangling emptiness ---- */
+  DBuf<uint8_t> edge_in(P, Ealpha), eflag(P, Ealpha);
+  fill_u8(edge_in.get(), Ealpha, 0, s);
+  launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
+                    edge_in.get(), L);
+  if (Ealpha)
+    ladder(
+        ac2_cap(Ealpha), KP_EDGE_AC2, -1,
+        [&](Queue q) {
+          zero_u32(qn.get() + 2, 1, s);
+          launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
+                             nullptr, edge_in.get(), eflag.get(), q,
+                             rest.get(), qn.get() + 2, budget, L);
+        },
+        [&](const QEntry* q, uint32_t nq, Queue q2) {
+          launch_low_ac2_esc(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+                             nullptr, eflag.get(), q2, L);
+        },
+        [&](const QEntry* q, uint32_t nq) {
+          launch_low_ac2_exact(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+                               nullptr, eflag.get(), L);
+        });
+  int64_t Ef = 0;
+  DBuf<uint64_t> pos_e(P, Ealpha + 1);
+  {
+    auto tmp = scan_tmp(Ealpha);
+    scan_u8flag_to_u64(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
+    KCHK();
+    Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
+  }
+
+  /* ---- emit final lists (original ids, reference order) ---- */
+  R->lists[0] = DBuf<int32_t>(P, 2 * Ef);
+  R->lists[1] = DBuf<int32_t>(P, 3 * Tf);
+  R->lists[2] = DBuf<int32_t>(P, 4 * Kf);
+  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), eflag.get(), pos_e.get(),
+                    perm.get(), R->lists[0].get(), L);
+  launch_emit_tris(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
+                   tedge.get(), perm.get(), R->lists[1].get(), L);
+  launch_emit_tets(Kf, ftets.get(), perm.get(), R->lists[2].get(), L);
+  KCHK();
+  R->counts[1] = Ef;
+  R->counts[2] = Tf;
+  R->counts[3] = Kf;
+  st.edges_final = Ef;
+  st.tris_final = Tf;
+  st.tets_final = Kf;
+  T.mark(8);
+
+  unsigned long long hs[CTR_COUNT * CTR_SLOTS], h[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(hs, ctr.get(), sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int c = 0; c < CTR_COUNT; ++c) {
+    h[c] = 0;
+    for (int k = 0; k < CTR_SLOTS; ++k) h[c] += hs[c * CTR_SLOTS + k];
+  }
+  for (int f = 0; f < 6; ++f)
+    for (int q = 0; q < 4; ++q)
+      st.stage_evals[f][q] = (int64_t)h[CTR_STAGE + 4 * f + q];
+  /* hot-stage evaluations: one per candidate for the filters, one per
+   * (simplex, ball) pair tested for the emptiness scans */
+  const int hot = SG.staged ? 0 : 1;
+  st.tri_candidates = (int64_t)h[CTR_CAND + 1];
+  st.tet_candidates = (int64_t)h[CTR_CAND + 2];
+  host_evals[1] = st.tri_candidates;
+  host_evals[2] = st.tet_candidates;
+  for (int f = 0; f < 3; ++f) st.stage_evals[f][hot] += host_evals[f];
+  for (int q = 0; q < 3; ++q) st.stage_evals[3 + q][hot] += (int64_t)h[CTR_PAIRS + q];
+  st.sos_calls = (int64_t)h[CTR_SOS];
+  for (int q = 0; q < 3; ++q) st.ac2_pairs[q] = (int64_t)h[CTR_PAIRS + q];
+}
+}  // namespace PAC_V
+}  // namespace pac

@@ -112,3 +112,24 @@ def test_torch_autograd():
     scale = max(np.abs(dxyz).max(), 1.0)
     assert np.abs(gx[:, :3] - dxyz).max() <= 1e-5 * scale  # grad stored in FP32
     assert np.abs(gx[:, 3] - dr).max() <= 1e-5 * scale
+
+
+def test_adjoint_c4_fp64():
+    """the differentiation workload: 20,000 balls, continuous FP64 centres
+    with N(0, 0.01 A) jitter, solvent 1.4, alpha 0; FD on 256 coordinates.
+    An FP32-forward variant (the same balls rounded to FP32) reports its
+    gradient error against the FP64 one."""
+    _say("case: adjoint C4 20k FP64")
+    b = pac.synth_globule_f64(20000, 45.7, 1.3, seed=4, sigma=0.01)
+    res, lists, pos, r = _setup(b, 1.4, pac.compute_f64)
+    _check_gradients(res, lists, pos, r, 44, "C4 fp64")
+    g64 = res.adjoint(np.ones(len(lists[2])), np.ones(len(lists[3])))
+    b32 = b.astype(np.float32)
+    r32 = pac.compute(b32, 1.4, 0.0)
+    c32 = r32.counts()
+    g32 = r32.adjoint(np.ones(int(c32[2])), np.ones(int(c32[3])))
+    num = np.sqrt((np.abs(g32[0] - g64[0]) ** 2).sum() + ((g32[1] - g64[1]) ** 2).sum())
+    den = np.sqrt((g64[0] ** 2).sum() + (g64[1] ** 2).sum())
+    _say("C4 FP32-forward gradient relative error vs FP64: %.3e "
+         "(complex sizes fp64 %s fp32 %s)" % (num / den, list(res.counts()), list(c32)))
+    assert num / den < 0.05
