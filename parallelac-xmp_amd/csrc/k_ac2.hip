@@ -210,6 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
+    PendingPairs pp;
     scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
       if (PHASE_B) {
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       int s = pairss_sign(P, E);
       if (s == 0) {
         pending = true;
-        push_q(q1, (uint32_t)id, p);
+        pp.add(q1, (uint32_t)id, p);
         return false;
       }
       if (s * P.sden < 0) { /* pi_e < rho: violator */
@@ -237,6 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       }
       return false;
     });
+    if (!dead) pp.flush(q1, (uint32_t)id);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
     Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
-  if (t < nq) {
+  if (t < nq && live_entry(q[t], flag)) {
     QEntry e = q[t];
     uint4 tt = tets[e.item];
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     const uint32_t* perm, uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0, sos = 0;
-  if (t < nq) {
+  if (t < nq && live_entry(q[t], flag)) {
     QEntry e = q[t];
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
@@ -554,6 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
+    PendingPairs pp;
     scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
       if (PHASE_B) {
@@ -576,10 +579,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       }
       if (s == 0) {
         pending = true;
-        push_q(q1, (uint32_t)id, p);
+        pp.add(q1, (uint32_t)id, p);
       }
       return false;
     });
+    if (!dead) pp.flush(q1, (uint32_t)id);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -605,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
     uint8_t* flag, Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
-  if (t < nq) {
+  if (t < nq && live_entry(q[t], flag)) {
     QEntry e = q[t];
     uint32_t vid[3];
     if (KIND == 0) {
@@ -705,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0;
-  if (t < nq) {
+  if (t < nq && live_entry(q[t], flag)) {
     QEntry e = q[t];
     uint32_t vid[3];
     if (KIND == 0) {

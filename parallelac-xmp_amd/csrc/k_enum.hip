@@ -455,28 +455,32 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(
     int64_t ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
     uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr) {
+  constexpr int CAP = 2048;
+  __shared__ uint32_t s_out[4 * CAP];
+  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t q1 = q0 + kBlock < ne ? q0 + kBlock : ne;
+  BlockOut<4, CAP> B{s_out, off[q0], off[q1] - off[q0]};
+  uint32_t* const g[4] = {cval, cseg, cea, ceb};
   int64_t q = gtid();
   unsigned found = 0;
   if (q < ne) {
-  uint32_t i = eleft[q], j = ecol[q];
-  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-  uint64_t b = erow[j], be = erow[j + 1];
-  uint64_t o = off[q], oe = off[q + 1];
-  while (a < ae && b < be) {
-    uint32_t x = ecol[a], y = ecol[b];
-    if (x == y) {
-      cval[o] = x;
-      cseg[o] = (uint32_t)q;
-      cea[o] = (uint32_t)a;
-      ceb[o] = (uint32_t)b;
-      ++o;
-      ++found;
+    uint32_t i = eleft[q], j = ecol[q];
+    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+    uint64_t b = erow[j], be = erow[j + 1];
+    uint64_t o = off[q];
+    while (a < ae && b < be) {
+      uint32_t x = ecol[a], y = ecol[b];
+      if (x == y) {
+        const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
+        B.put(g, o, v);
+        ++o;
+        ++found;
+      }
+      a += (x <= y);
+      b += (y <= x);
     }
-    a += (x <= y);
-    b += (y <= x);
   }
-  for (; o < oe; ++o) cseg[o] = 0xffffffffu;
-  }
+  B.flush(g);
   block_flush(ctr, CTR_CAND + 1, found);
 }
 void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
@@ -495,33 +499,41 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum(
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
     const uint64_t* off, uint32_t* cval, uint32_t* cseg,
     unsigned long long* ctr) {
+  constexpr int CAP = 4096;
+  __shared__ uint32_t s_out[2 * CAP];
+  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t q1 = q0 + kBlock < nt ? q0 + kBlock : nt;
+  BlockOut<2, CAP> B{s_out, off[q0], off[q1] - off[q0]};
+  uint32_t* const g[2] = {cval, cseg};
   int64_t q = gtid();
   unsigned found = 0;
   if (q < nt) {
-  uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
-  uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
-  uint64_t b = trow[eik], be = trow[eik + 1];
-  uint64_t c = trow[ejk], ce = trow[ejk + 1];
-  uint64_t o = off[q], oe = off[q + 1];
-  while (a < ae && b < be && c < ce) {
-    uint32_t x = tk[a], y = tk[b], z = tk[c];
-    uint32_t m = max(x, max(y, z));
-    if (x == m && y == m && z == m) {
-      cval[o] = m;
-      cseg[o] = (uint32_t)q;
-      ++o;
-      ++found;
-      ++a;
-      ++b;
-      ++c;
-    } else {
-      a += (x < m);
-      b += (y < m);
-      c += (z < m);
+    uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+    uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
+    uint64_t b = trow[eik], be = trow[eik + 1];
+    uint64_t c = trow[ejk], ce = trow[ejk + 1];
+    uint64_t o = off[q], oe = off[q + 1];
+    while (a < ae && b < be && c < ce) {
+      uint32_t x = tk[a], y = tk[b], z = tk[c];
+      uint32_t m = max(x, max(y, z));
+      if (x == m && y == m && z == m) {
+        const uint32_t v[2] = {m, (uint32_t)q};
+        B.put(g, o, v);
+        ++o;
+        ++found;
+        ++a;
+        ++b;
+        ++c;
+      } else {
+        a += (x < m);
+        b += (y < m);
+        c += (z < m);
+      }
     }
+    const uint32_t unused[2] = {0u, 0xffffffffu};
+    for (; o < oe; ++o) B.put(g, o, unused);
   }
-  for (; o < oe; ++o) cseg[o] = 0xffffffffu;
-  }
+  B.flush(g);
   block_flush(ctr, CTR_CAND + 2, found);
 }
 void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,

@@ -65,6 +65,37 @@ __device__ __forceinline__ unsigned block_append(unsigned* counter, bool want) {
   __syncthreads();
   return r;
 }
+/* Block-staged output of a one-thread-per-segment enumeration.  The
+ * segments of a block own the contiguous output range [base, base+total)
+ * (their exclusive-scan offsets); each thread's entries go to LDS at their
+ * offset within that range and the block writes the range out coalesced,
+ * instead of 64 lanes each streaming into its own region.  Entries past
+ * the LDS capacity CAP are stored directly. */
+template <int NARR, int CAP>
+struct BlockOut {
+  uint32_t* s; /* LDS, NARR x CAP */
+  uint64_t base, total;
+  __device__ __forceinline__ void put(uint32_t* const g[NARR], uint64_t o,
+                                      const uint32_t v[NARR]) {
+    uint64_t lo = o - base;
+    if (lo < (uint64_t)CAP) {
+#pragma unroll
+      for (int a = 0; a < NARR; ++a) s[a * CAP + lo] = v[a];
+    } else {
+#pragma unroll
+      for (int a = 0; a < NARR; ++a) g[a][o] = v[a];
+    }
+  }
+  /* every thread of the block calls this once, after its last put */
+  __device__ __forceinline__ void flush(uint32_t* const g[NARR]) {
+    __syncthreads();
+    const uint32_t m = (uint32_t)(total < (uint64_t)CAP ? total : (uint64_t)CAP);
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x)
+#pragma unroll
+      for (int a = 0; a < NARR; ++a) g[a][base + t] = s[a * CAP + t];
+  }
+};
+
 /* wave-aggregated append: one atomic per wave for all lanes with want */
 __device__ __forceinline__ unsigned wave_append(unsigned* counter, bool want) {
   uint64_t m = __ballot(want);
@@ -97,6 +128,39 @@ __device__ __forceinline__ void push_q(const Queue& q, uint32_t item,
   unsigned p = atomicAdd(q.n, 1u);
   if (p < q.cap) q.q[p] = QEntry{item, aux};
 }
+
+/* A queued (simplex, ball) pair whose simplex another pair has already
+ * rejected needs no work (flags only ever go from undecided to 0, so a
+ * stale read merely does the work).  Entries with aux == ~0 are full
+ * scans and always live. */
+__device__ __forceinline__ bool live_entry(const QEntry& e,
+                                           const uint8_t* flag) {
+  return e.aux == 0xffffffffu || flag[e.item] != 0;
+}
+
+/* (simplex, ball) pairs that no float stage decided, held back until the
+ * simplex's scan ends: if a certain violator turns up later they are
+ * dropped instead of queued.  Holds four (in registers); further ones are
+ * queued at once. */
+struct PendingPairs {
+  uint32_t b0, b1, b2, b3;
+  int n = 0;
+  __device__ __forceinline__ void add(const Queue& q, uint32_t item,
+                                      uint32_t ball) {
+    if (n == 0) b0 = ball;
+    else if (n == 1) b1 = ball;
+    else if (n == 2) b2 = ball;
+    else if (n == 3) b3 = ball;
+    else push_q(q, item, ball);
+    ++n;
+  }
+  __device__ __forceinline__ void flush(const Queue& q, uint32_t item) {
+    if (n > 0) push_q(q, item, b0);
+    if (n > 1) push_q(q, item, b1);
+    if (n > 2) push_q(q, item, b2);
+    if (n > 3) push_q(q, item, b3);
+  }
+};
 
 template <class T>
 struct Al;
