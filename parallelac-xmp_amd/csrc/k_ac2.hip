@@ -210,7 +210,6 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    PendingPairs pp;
     scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
       if (PHASE_B) {
@@ -229,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       int s = pairss_sign(P, E);
       if (s == 0) {
         pending = true;
-        pp.add(q1, (uint32_t)id, p);
+        push_q(q1, (uint32_t)id, p);
         return false;
       }
       if (s * P.sden < 0) { /* pi_e < rho: violator */
@@ -238,7 +237,6 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       }
       return false;
     });
-    if (!dead) pp.flush(q1, (uint32_t)id);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -264,12 +262,12 @@ __device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
   return tet_pair_code(t, sD, e);
 }
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
-    Stage S, const QEntry* q, uint32_t nq, const uint4* tets, uint8_t* flag,
+    Stage S, QView q, const uint4* tets, uint8_t* flag,
     Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
-  if (t < nq && live_entry(q[t], flag)) {
-    QEntry e = q[t];
+  if (t < q.size() && live_entry(q.at(t), flag)) {
+    QEntry e = q.at(t);
     uint4 tt = tets[e.item];
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
     TetQ<FE<double>> t64;
@@ -373,12 +371,12 @@ __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
 }
 
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
-    Stage S, const QEntry* q, uint32_t nq, const uint4* tets,
+    Stage S, QView q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0, sos = 0;
-  if (t < nq && live_entry(q[t], flag)) {
-    QEntry e = q[t];
+  if (t < q.size() && live_entry(q.at(t), flag)) {
+    QEntry e = q.at(t);
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
@@ -556,7 +554,6 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    PendingPairs pp;
     scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
       if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
       if (PHASE_B) {
@@ -579,11 +576,10 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       }
       if (s == 0) {
         pending = true;
-        pp.add(q1, (uint32_t)id, p);
+        push_q(q1, (uint32_t)id, p);
       }
       return false;
     });
-    if (!dead) pp.flush(q1, (uint32_t)id);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -604,13 +600,13 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
 
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    Stage S, QView q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag, Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
-  if (t < nq && live_entry(q[t], flag)) {
-    QEntry e = q[t];
+  if (t < q.size() && live_entry(q.at(t), flag)) {
+    QEntry e = q.at(t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -704,13 +700,13 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
 /* exact triangle / edge pair (or full scan when aux == ~0) */
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
-    Stage S, const QEntry* q, uint32_t nq, const uint32_t* ecol,
+    Stage S, QView q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0;
-  if (t < nq && live_entry(q[t], flag)) {
-    QEntry e = q[t];
+  if (t < q.size() && live_entry(q.at(t), flag)) {
+    QEntry e = q.at(t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -803,19 +799,19 @@ void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
                          L.s, S, nk, tets, flag, q1, rest, nrest, budget);
   }
 }
-void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
+void launch_tet_ac2_esc(const Stage& S, QView q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, tets, flag, q2);
+  if (!q.size()) return;
+  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((q.size() + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, tets, flag, q2);
 }
-void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+void launch_tet_ac2_exact(const Stage& S, QView q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L) {
-  if (!nq) return;
-  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((nq + kBlock - 1) / kBlock),
-                     dim3(kBlock), 0, L.s, S, q, nq, tets, perm, flag);
+  if (!q.size()) return;
+  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((q.size() + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
 template <class R, int KIND>
 static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
@@ -857,30 +853,29 @@ void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
     low_hot2<double, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
                         rest, nrest, budget, L);
 }
-void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
+void launch_low_ac2_esc(const Stage& S, int kind, QView q,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint8_t* flag, Queue q2, PipeLaunch L) {
-  if (!nq) return;
-  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (!q.size()) return;
+  dim3 gr((q.size() + kBlock - 1) / kBlock);
   if (kind == 0)
-    hipLaunchKernelGGL(k_low_ac2_esc<0>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+    hipLaunchKernelGGL(k_low_ac2_esc<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag, q2);
   else
-    hipLaunchKernelGGL(k_low_ac2_esc<1>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+    hipLaunchKernelGGL(k_low_ac2_esc<1>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag, q2);
 }
-void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
-                          uint32_t nq, const uint32_t* ecol,
+void launch_low_ac2_exact(const Stage& S, int kind, QView q, const uint32_t* ecol,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L) {
-  if (!nq) return;
-  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (!q.size()) return;
+  dim3 gr((q.size() + kBlock - 1) / kBlock);
   if (kind == 0)
-    hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+    hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);
   else
-    hipLaunchKernelGGL(k_low_ac2_exact<1>, gr, dim3(kBlock), 0, L.s, S, q, nq,
+    hipLaunchKernelGGL(k_low_ac2_exact<1>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);
 }
 

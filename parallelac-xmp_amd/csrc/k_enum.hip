@@ -611,15 +611,14 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
 }
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
-                                                       const QEntry* q,
-                                                       uint32_t nq,
+                                                       QView q,
                                                        const uint32_t* cval,
                                                        uint8_t* flag,
                                                        Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
-  if (t < nq) {
-    QEntry e = q[t];
+  if (t < q.size()) {
+    QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
     sc.c[1]++;
@@ -637,14 +636,13 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
 }
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
-                                                         const QEntry* q,
-                                                         uint32_t nq,
+                                                         QView q,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0;
-  if (t < nq) {
-    QEntry e = q[t];
+  if (t < q.size()) {
+    QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
     flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
@@ -674,34 +672,32 @@ void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
   }
 #undef PAC_FILTER
 }
-void launch_filter_esc(const Stage& S, int kind, CandCtx C, const QEntry* q,
-                       uint32_t nq, const uint32_t* cval, uint8_t* flag,
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L) {
-  if (!nq) return;
-  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (!q.size()) return;
+  dim3 gr((q.size() + kBlock - 1) / kBlock);
   if (kind == 1)
-    hipLaunchKernelGGL(k_filter_esc<1>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_esc<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
   else if (kind == 2)
-    hipLaunchKernelGGL(k_filter_esc<2>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_esc<2>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
   else
-    hipLaunchKernelGGL(k_filter_esc<3>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_esc<3>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
 }
-void launch_filter_exact(const Stage& S, int kind, CandCtx C, const QEntry* q,
-                         uint32_t nq, const uint32_t* cval, uint8_t* flag,
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L) {
-  if (!nq) return;
-  dim3 gr((nq + kBlock - 1) / kBlock);
+  if (!q.size()) return;
+  dim3 gr((q.size() + kBlock - 1) / kBlock);
   if (kind == 1)
-    hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);
   else if (kind == 2)
-    hipLaunchKernelGGL(k_filter_exact<2>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_exact<2>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);
   else
-    hipLaunchKernelGGL(k_filter_exact<3>, gr, dim3(kBlock), 0, L.s, S, C, q, nq,
+    hipLaunchKernelGGL(k_filter_exact<3>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);
 }
 

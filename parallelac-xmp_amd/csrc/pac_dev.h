@@ -125,8 +125,9 @@ __device__ __forceinline__ bool same_pos(const Ball& a, const Ball& b) {
 }
 __device__ __forceinline__ void push_q(const Queue& q, uint32_t item,
                                        uint32_t aux) {
-  unsigned p = atomicAdd(q.n, 1u);
-  if (p < q.cap) q.q[p] = QEntry{item, aux};
+  const uint32_t k = blockIdx.x & (QSLOTS - 1);
+  unsigned p = atomicAdd(q.n + k, 1u);
+  if (p < q.cap) q.q[(uint64_t)k * q.cap + p] = QEntry{item, aux};
 }
 
 /* A queued (simplex, ball) pair whose simplex another pair has already
@@ -137,30 +138,6 @@ __device__ __forceinline__ bool live_entry(const QEntry& e,
                                            const uint8_t* flag) {
   return e.aux == 0xffffffffu || flag[e.item] != 0;
 }
-
-/* (simplex, ball) pairs that no float stage decided, held back until the
- * simplex's scan ends: if a certain violator turns up later they are
- * dropped instead of queued.  Holds four (in registers); further ones are
- * queued at once. */
-struct PendingPairs {
-  uint32_t b0, b1, b2, b3;
-  int n = 0;
-  __device__ __forceinline__ void add(const Queue& q, uint32_t item,
-                                      uint32_t ball) {
-    if (n == 0) b0 = ball;
-    else if (n == 1) b1 = ball;
-    else if (n == 2) b2 = ball;
-    else if (n == 3) b3 = ball;
-    else push_q(q, item, ball);
-    ++n;
-  }
-  __device__ __forceinline__ void flush(const Queue& q, uint32_t item) {
-    if (n > 0) push_q(q, item, b0);
-    if (n > 1) push_q(q, item, b1);
-    if (n > 2) push_q(q, item, b2);
-    if (n > 3) push_q(q, item, b3);
-  }
-};
 
 template <class T>
 struct Al;

@@ -76,10 +76,31 @@ struct PipeLaunch {
   int block;
 };
 
+/* Queues of undecided items between the stages of the precision ladder.
+ * A producer appends to one of QSLOTS sub-queues (picked by its block) so
+ * that many simultaneous appends do not serialize on one counter; sub-queue
+ * k holds entries [k*cap, k*cap + n[k]). */
+constexpr int QSLOTS = 32;
 struct Queue {
   QEntry* q;
-  unsigned* n;
+  unsigned* n;   /* QSLOTS counters */
+  uint32_t cap;  /* per sub-queue */
+};
+/* consumer view: entry i of the concatenated sub-queues */
+struct QView {
+  const QEntry* q;
   uint32_t cap;
+  uint32_t off[QSLOTS + 1]; /* exclusive prefix of the sub-queue counts */
+  __host__ __device__ uint32_t size() const { return off[QSLOTS]; }
+  __device__ QEntry at(uint32_t i) const {
+    int lo = 0, hi = QSLOTS - 1; /* last k with off[k] <= i */
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    return q[(uint64_t)lo * cap + (i - off[lo])];
+  }
 };
 
 struct CandCtx {
@@ -200,11 +221,9 @@ void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
 void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
                    Queue q1, PipeLaunch L);
-void launch_filter_esc(const Stage& S, int kind, CandCtx C, const QEntry* q,
-                       uint32_t nq, const uint32_t* cval, uint8_t* flag,
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L);
-void launch_filter_exact(const Stage& S, int kind, CandCtx C, const QEntry* q,
-                         uint32_t nq, const uint32_t* cval, uint8_t* flag,
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L);
 void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
                        uint64_t* orow, PipeLaunch L);
@@ -221,10 +240,10 @@ void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
 void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, int budget, PipeLaunch L);
-void launch_tet_ac2_esc(const Stage& S, const QEntry* q, uint32_t nq,
+void launch_tet_ac2_esc(const Stage& S, QView q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L);
-void launch_tet_ac2_exact(const Stage& S, const QEntry* q, uint32_t nq,
+void launch_tet_ac2_exact(const Stage& S, QView q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L);
 /* kind 0 triangles, 1 edges */
@@ -234,12 +253,11 @@ void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
                         const uint8_t* in_flag, uint8_t* flag, Queue q1,
                         uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L);
-void launch_low_ac2_esc(const Stage& S, int kind, const QEntry* q, uint32_t nq,
+void launch_low_ac2_esc(const Stage& S, int kind, QView q,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint8_t* flag, Queue q2, PipeLaunch L);
-void launch_low_ac2_exact(const Stage& S, int kind, const QEntry* q,
-                          uint32_t nq, const uint32_t* ecol,
+void launch_low_ac2_exact(const Stage& S, int kind, QView q, const uint32_t* ecol,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L);
 

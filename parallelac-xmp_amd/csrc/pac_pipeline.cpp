@@ -205,7 +205,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
   zero_u64(ctr.get(), CTR_COUNT * CTR_SLOTS, s);
   SG.ctr = ctr.get();
-  DBuf<unsigned> qn(P, 4); /* [0] hot queue, [1] escalation queue */
+  /* [0, QSLOTS) hot queue, [QSLOTS, 2 QSLOTS) escalation queue, then the
+   * count of simplices left for the second emptiness phase */
+  DBuf<unsigned> qn(P, 2 * QSLOTS + 1);
+  unsigned* const nrest = qn.get() + 2 * QSLOTS;
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
@@ -246,44 +249,69 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
    * double-double (q1 -> q2)] -> exact wide-integer kernel.  Queue
    * overflows rerun the producing kernel with a larger queue; only the
    * emptiness kernels can overflow, and they are idempotent. */
-  auto ladder = [&](uint32_t cap1, int kp_hot, int kp_exact, auto&& hot,
+  /* sub-queue capacity for up to `total` entries spread over the slots;
+   * a one-item-per-thread producer of n items never exceeds n/QSLOTS + 512
+   * per slot (its blocks of 256 are dealt round-robin) */
+  auto slot_cap = [](int64_t total) {
+    return (uint32_t)std::min<int64_t>(total / QSLOTS + 512,
+                                       (int64_t)(UINT32_MAX / QSLOTS) - 1);
+  };
+  /* counts of the sub-queues -> consumer view; false on overflow, with
+   * *need = the per-slot capacity that would have sufficed */
+  auto view_of = [&](const QEntry* q, uint32_t cap, const unsigned* dcnt,
+                     QView* v, uint32_t* need) -> bool {
+    unsigned h[QSLOTS];
+    HIPCHK(hipMemcpyAsync(E->pinned, dcnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(h, E->pinned, sizeof(h));
+    v->q = q;
+    v->cap = cap;
+    uint64_t acc = 0;
+    unsigned mx = 0;
+    for (int k = 0; k < QSLOTS; ++k) {
+      v->off[k] = (uint32_t)acc;
+      acc += std::min(h[k], cap);
+      mx = std::max(mx, h[k]);
+    }
+    v->off[QSLOTS] = (uint32_t)acc;
+    *need = mx;
+    return mx <= cap;
+  };
+  auto grow = [](uint32_t need) {
+    return (uint32_t)std::min<int64_t>((int64_t)need + need / 4 + 256,
+                                       (int64_t)(UINT32_MAX / QSLOTS) - 1);
+  };
+  auto ladder = [&](int64_t cap1_total, int kp_hot, int kp_exact, auto&& hot,
                     auto&& esc, auto&& exact) {
+    uint32_t c1 = slot_cap(cap1_total);
     for (;;) {
-      DBuf<QEntry> q1(P, cap1);
-      zero_u32(qn.get(), 2, s);
+      DBuf<QEntry> q1(P, (size_t)c1 * QSLOTS);
+      zero_u32(qn.get(), 2 * QSLOTS, s);
       if (kp_hot >= 0) KP.begin(kp_hot);
-      hot(Queue{q1.get(), qn.get(), cap1});
+      hot(Queue{q1.get(), qn.get(), c1});
       if (kp_hot >= 0) KP.end();
       KCHK();
-      uint32_t n1 = readback(E, s, qn.get());
-      if (n1 > cap1) {
-        cap1 = (uint32_t)std::min<int64_t>((int64_t)n1 + n1 / 4 + 1024,
-                                           (int64_t)UINT32_MAX - 1);
+      QView v1;
+      uint32_t need = 0;
+      if (!view_of(q1.get(), c1, qn.get(), &v1, &need)) {
+        c1 = grow(need);
         continue;
       }
-      const QEntry* qx = q1.get();
-      uint32_t nx = n1;
+      QView vx = v1;
       DBuf<QEntry> q2;
-      if (staged && n1) {
-        uint32_t cap2 = (uint32_t)std::min<int64_t>(4 * (int64_t)n1 + 4096,
-                                                    (int64_t)UINT32_MAX - 1);
+      if (staged && v1.size()) {
+        uint32_t c2 = slot_cap(4 * (int64_t)v1.size() + 4096);
         for (;;) {
-          q2 = DBuf<QEntry>(P, cap2);
-          zero_u32(qn.get() + 1, 1, s);
-          esc(q1.get(), n1, Queue{q2.get(), qn.get() + 1, cap2});
+          q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
+          zero_u32(qn.get() + QSLOTS, QSLOTS, s);
+          esc(v1, Queue{q2.get(), qn.get() + QSLOTS, c2});
           KCHK();
-          uint32_t n2 = readback(E, s, qn.get() + 1);
-          if (n2 <= cap2) {
-            nx = n2;
-            break;
-          }
-          cap2 = (uint32_t)std::min<int64_t>((int64_t)n2 + n2 / 4 + 1024,
-                                             (int64_t)UINT32_MAX - 1);
+          if (view_of(q2.get(), c2, qn.get() + QSLOTS, &vx, &need)) break;
+          c2 = grow(need);
         }
-        qx = q2.get();
       }
       if (kp_exact >= 0) KP.begin(kp_exact);
-      exact(qx, nx);
+      exact(vx);
       if (kp_exact >= 0) KP.end();
       KCHK();
       return;
@@ -298,14 +326,16 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                           const uint32_t* cseg, uint8_t* flag, int kp,
                           DBuf<uint64_t>& pos) -> int64_t {
     if (kind == 1) host_evals[0] += nc; /* tris / tets: slots != candidates */
+    /* few candidates stay undecided in float: start small, a rerun of the
+     * (idempotent) filter sizes the queue exactly if it overflows */
     ladder(
-        (uint32_t)std::max<int64_t>(nc, 1), kp, -1,
+        std::min<int64_t>(nc, nc / 32 + 65536), kp, -1,
         [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_filter_esc(SG, kind, C, q, nq, cval, flag, q2, L);
+        [&](QView q, Queue q2) {
+          launch_filter_esc(SG, kind, C, q, cval, flag, q2, L);
         },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_filter_exact(SG, kind, C, q, nq, cval, flag, L);
+        [&](QView q) {
+          launch_filter_exact(SG, kind, C, q, cval, flag, L);
         });
     pos = DBuf<uint64_t>(P, nc + 1);
     {
@@ -441,15 +471,15 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     ladder(
         ac2_cap(Kalpha), KP_TET_AC2, KP_TET_AC2_EXACT,
         [&](Queue q) {
-          zero_u32(qn.get() + 2, 1, s);
+          zero_u32(nrest, 1, s);
           launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
-                             qn.get() + 2, budget, L);
+                             nrest, budget, L);
         },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_tet_ac2_esc(SG, q, nq, tets.get(), kflag.get(), q2, L);
+        [&](QView q, Queue q2) {
+          launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);
         },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_tet_ac2_exact(SG, q, nq, tets.get(), perm.get(), kflag.get(), L);
+        [&](QView q) {
+          launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
         });
   }
   int64_t Kf = 0;
@@ -476,17 +506,17 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     ladder(
         ac2_cap(Talpha), KP_TRI_AC2, -1,
         [&](Queue q) {
-          zero_u32(qn.get() + 2, 1, s);
+          zero_u32(nrest, 1, s);
           launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
                              tedge.get(), tri_in.get(), tflag.get(), q,
-                             rest.get(), qn.get() + 2, budget, L);
+                             rest.get(), nrest, budget, L);
         },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_low_ac2_esc(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
+        [&](QView q, Queue q2) {
+          launch_low_ac2_esc(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
                              tedge.get(), tflag.get(), q2, L);
         },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_low_ac2_exact(SG, 0, q, nq, ecol.get(), eleft.get(), tk.get(),
+        [&](QView q) {
+          launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
                                tedge.get(), tflag.get(), L);
         });
   int64_t Tf = 0;
@@ -512,17 +542,17 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     ladder(
         ac2_cap(Ealpha), KP_EDGE_AC2, -1,
         [&](Queue q) {
-          zero_u32(qn.get() + 2, 1, s);
+          zero_u32(nrest, 1, s);
           launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
                              nullptr, edge_in.get(), eflag.get(), q,
-                             rest.get(), qn.get() + 2, budget, L);
+                             rest.get(), nrest, budget, L);
         },
-        [&](const QEntry* q, uint32_t nq, Queue q2) {
-          launch_low_ac2_esc(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+        [&](QView q, Queue q2) {
+          launch_low_ac2_esc(SG, 1, q, ecol.get(), eleft.get(), nullptr,
                              nullptr, eflag.get(), q2, L);
         },
-        [&](const QEntry* q, uint32_t nq) {
-          launch_low_ac2_exact(SG, 1, q, nq, ecol.get(), eleft.get(), nullptr,
+        [&](QView q) {
+          launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
                                nullptr, eflag.get(), L);
         });
   int64_t Ef = 0;
