@@ -420,44 +420,8 @@ void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
                      dim3(kBlock), 0, L.s, m, row, owner, cap);
 }
 
-/* triangles of alpha edge q = (i,j): k in row(i) after j and in row(j)
- * (computePossibleTriangleCount / fillPossibleTriangles, Parallel:232-308);
- * emit(k, a, b) with a, b the positions of k in row(i) and row(j), which
- * are the alpha-edge ids of (i,k) and (j,k).  Short rows are loaded in one
- * batch and intersected in registers; longer ones are merged streaming. */
-template <class F>
-__device__ __forceinline__ void tri_cands(int64_t q, const uint64_t* erow,
-                                          const uint32_t* ecol,
-                                          const uint32_t* eleft, F&& emit) {
-  constexpr int NA = 16, NB = 32;
-  const uint32_t i = eleft[q], j = ecol[q];
-  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-  uint64_t b = erow[j], be = erow[j + 1];
-  if (ae - a <= (uint64_t)NA && be - b <= (uint64_t)NB) {
-    uint32_t A[NA], B[NB];
-#pragma unroll
-    for (int u = 0; u < NA; ++u) A[u] = a + u < ae ? ecol[a + u] : 0xffffffffu;
-#pragma unroll
-    for (int v = 0; v < NB; ++v) B[v] = b + v < be ? ecol[b + v] : 0xfffffffeu;
-#pragma unroll
-    for (int u = 0; u < NA; ++u) {
-      const uint32_t x = A[u];
-      int at = -1;
-#pragma unroll
-      for (int v = 0; v < NB; ++v) at = B[v] == x ? v : at;
-      if (at >= 0) emit(x, a + u, b + (uint64_t)at);
-    }
-    return;
-  }
-  while (a < ae && b < be) {
-    uint32_t x = ecol[a], y = ecol[b];
-    if (x == y) emit(x, a, b);
-    a += (x <= y);
-    b += (y <= x);
-  }
-}
-
-/* exact triangle count per alpha edge */
+/* number of triangles of alpha edge q = (i,j): the size of row(i) after j
+ * intersected with row(j) (computePossibleTriangleCount, Parallel:232-266) */
 __global__ __launch_bounds__(kBlock) void k_tri_count(int64_t ne,
                                                      const uint64_t* erow,
                                                      const uint32_t* ecol,
@@ -465,8 +429,16 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int64_t ne,
                                                      uint32_t* cnt) {
   int64_t q = gtid();
   if (q >= ne) return;
+  uint32_t i = eleft[q], j = ecol[q];
+  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+  uint64_t b = erow[j], be = erow[j + 1];
   uint32_t c = 0;
-  tri_cands(q, erow, ecol, eleft, [&](uint32_t, uint64_t, uint64_t) { ++c; });
+  while (a < ae && b < be) {
+    uint32_t x = ecol[a], y = ecol[b];
+    c += (x == y);
+    a += (x <= y);
+    b += (y <= x);
+  }
   cnt[q] = c;
 }
 void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
@@ -476,6 +448,9 @@ void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
 }
 
+/* triangles of alpha edge q = (i,j): k in row(i) after j and in row(j); the
+ * positions of k in row(i) and row(j) are the alpha-edge ids of (i,k) and
+ * (j,k), recorded for the tet stage and the closure. */
 __global__ __launch_bounds__(kBlock) void k_tri_enum(
     int64_t ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
@@ -489,13 +464,21 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(
   int64_t q = gtid();
   unsigned found = 0;
   if (q < ne) {
+    uint32_t i = eleft[q], j = ecol[q];
+    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+    uint64_t b = erow[j], be = erow[j + 1];
     uint64_t o = off[q];
-    tri_cands(q, erow, ecol, eleft, [&](uint32_t k, uint64_t a, uint64_t b) {
-      const uint32_t v[4] = {k, (uint32_t)q, (uint32_t)a, (uint32_t)b};
-      B.put(g, o, v);
-      ++o;
-      ++found;
-    });
+    while (a < ae && b < be) {
+      uint32_t x = ecol[a], y = ecol[b];
+      if (x == y) {
+        const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
+        B.put(g, o, v);
+        ++o;
+        ++found;
+      }
+      a += (x <= y);
+      b += (y <= x);
+    }
   }
   B.flush(g);
   block_flush(ctr, CTR_CAND + 1, found);
@@ -510,79 +493,7 @@ void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      cseg, cea, ceb, ctr);
 }
 
-/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k) and
- * T(j,k) (computePossibleTetCount / fillPossibleTets, Parallel:418-543).
- * The three sorted lists are short (a few entries each), so they are
- * loaded in one batch of independent loads and intersected in registers;
- * the streaming three-way merge, one dependent load per step, is kept for
- * lists longer than the batch.  emit(l) is called in ascending l. */
-template <class F>
-__device__ __forceinline__ void tet_cands(int64_t q, const uint64_t* trow,
-                                          const uint32_t* tk,
-                                          const uint32_t* tedge,
-                                          const uint32_t* teik,
-                                          const uint32_t* tejk, F&& emit) {
-  constexpr int NA = 8, NB = 16;
-  const uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
-  uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
-  uint64_t b = trow[eik], be = trow[eik + 1];
-  uint64_t c = trow[ejk], ce = trow[ejk + 1];
-  if (ae - a <= (uint64_t)NA && be - b <= (uint64_t)NB && ce - c <= (uint64_t)NB) {
-    uint32_t A[NA], B[NB], C[NB];
-#pragma unroll
-    for (int u = 0; u < NA; ++u) A[u] = a + u < ae ? tk[a + u] : 0xffffffffu;
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      B[u] = b + u < be ? tk[b + u] : 0xfffffffeu;
-      C[u] = c + u < ce ? tk[c + u] : 0xfffffffdu;
-    }
-#pragma unroll
-    for (int u = 0; u < NA; ++u) {
-      const uint32_t x = A[u];
-      bool inb = false, inc = false;
-#pragma unroll
-      for (int v = 0; v < NB; ++v) {
-        inb |= B[v] == x;
-        inc |= C[v] == x;
-      }
-      if (inb && inc) emit(x);
-    }
-    return;
-  }
-  while (a < ae && b < be && c < ce) {
-    uint32_t x = tk[a], y = tk[b], z = tk[c];
-    uint32_t m = max(x, max(y, z));
-    if (x == m && y == m && z == m) {
-      emit(m);
-      ++a;
-      ++b;
-      ++c;
-    } else {
-      a += (x < m);
-      b += (y < m);
-      c += (z < m);
-    }
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_tet_count(
-    int64_t nt, const uint64_t* trow, const uint32_t* tk,
-    const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
-    uint32_t* cnt) {
-  int64_t q = gtid();
-  if (q >= nt) return;
-  uint32_t n = 0;
-  tet_cands(q, trow, tk, tedge, teik, tejk, [&](uint32_t) { ++n; });
-  cnt[q] = n;
-}
-void launch_tet_count(int64_t nt, const uint64_t* trow, const uint32_t* tk,
-                      const uint32_t* tedge, const uint32_t* teik,
-                      const uint32_t* tejk, uint32_t* cnt, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_tet_count, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk, cnt);
-}
-
+/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k) */
 __global__ __launch_bounds__(kBlock) void k_tet_enum(
     int64_t nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
@@ -597,13 +508,30 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum(
   int64_t q = gtid();
   unsigned found = 0;
   if (q < nt) {
-    uint64_t o = off[q];
-    tet_cands(q, trow, tk, tedge, teik, tejk, [&](uint32_t l) {
-      const uint32_t v[2] = {l, (uint32_t)q};
-      B.put(g, o, v);
-      ++o;
-      ++found;
-    });
+    uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+    uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
+    uint64_t b = trow[eik], be = trow[eik + 1];
+    uint64_t c = trow[ejk], ce = trow[ejk + 1];
+    uint64_t o = off[q], oe = off[q + 1];
+    while (a < ae && b < be && c < ce) {
+      uint32_t x = tk[a], y = tk[b], z = tk[c];
+      uint32_t m = max(x, max(y, z));
+      if (x == m && y == m && z == m) {
+        const uint32_t v[2] = {m, (uint32_t)q};
+        B.put(g, o, v);
+        ++o;
+        ++found;
+        ++a;
+        ++b;
+        ++c;
+      } else {
+        a += (x < m);
+        b += (y < m);
+        c += (z < m);
+      }
+    }
+    const uint32_t unused[2] = {0u, 0xffffffffu};
+    for (; o < oe; ++o) B.put(g, o, unused);
   }
   B.flush(g);
   block_flush(ctr, CTR_CAND + 2, found);

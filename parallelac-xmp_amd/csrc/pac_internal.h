@@ -213,10 +213,6 @@ void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
                      unsigned long long* ctr, PipeLaunch L);
-/* exact tet count per alpha triangle */
-void launch_tet_count(int64_t nt, const uint64_t* trow, const uint32_t* tk,
-                      const uint32_t* tedge, const uint32_t* teik,
-                      const uint32_t* tejk, uint32_t* cnt, PipeLaunch L);
 void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint32_t* teik,
                      const uint32_t* tejk, const uint64_t* off, uint32_t* cval,

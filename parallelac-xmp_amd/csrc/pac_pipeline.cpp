@@ -435,8 +435,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint64_t> off(P, Talpha + 1);
     {
       DBuf<uint32_t> cap(P, Talpha);
-      launch_tet_count(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
-                       tejk.get(), cap.get(), L);
+      launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
       Kslots = count_scan(cap.get(), Talpha, off.get(), "tet");
     }
     DBuf<uint32_t> cval(P, Kslots), cseg(P, Kslots);
