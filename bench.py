@@ -44,11 +44,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle leg (baseline + bit-exact check)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("globule", "batch", "stress"),
+    ap.add_argument("--workload", choices=("globule", "batch", "stress", "copy"),
                     default="globule",
                     help="globule: the 100k headline (default); batch: the "
                          "4096-molecule ensemble; stress: 2M balls with a "
-                         "degenerate lattice block")
+                         "degenerate lattice block; copy: a 1 GiB device "
+                         "copy, the reference kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not "
@@ -63,6 +64,8 @@ def main():
         return bench_batch(args, torch, pac)
     if args.workload == "stress":
         return bench_stress(args, torch, pac)
+    if args.workload == "copy":
+        return bench_copy(args, torch)
 
     n = args.n
     radius = (3.0 * n / (4.0 * np.pi * 0.05)) ** (1.0 / 3.0)
@@ -251,6 +254,26 @@ def bench_batch(args, torch, pac):
         "phase_ms": {pac.PHASE_LABELS[i]: round(st.phase_ms[i], 3) for i in range(8)},
     }
     print(json.dumps(line))
+
+
+def bench_copy(args, torch):
+    """1 GiB device-to-device copy (torch): a kernel of known traffic
+    (1 GiB read + 1 GiB written) against which the FETCH_SIZE / WRITE_SIZE
+    counters of the pipeline's kernels are scaled."""
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "device copy bandwidth", "value": 2 * 4 * n / t / 1e9,
+                      "unit": "GB/s", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "ms_per_step": t * 1e3, "steps": args.steps}))
 
 
 def bench_stress(args, torch, pac):

@@ -19,14 +19,14 @@ using namespace PAC_V; /* type-independent device helpers */
 __global__ __launch_bounds__(kBlock) void k_compact_u4(int64_t n,
                                                        const uint4* in,
                                                        const uint8_t* flag,
-                                                       const uint64_t* pos,
+                                                       const uint32_t* pos,
                                                        uint4* out) {
   int64_t i = gtid();
   if (i >= n || !flag[i]) return;
   out[pos[i]] = in[i];
 }
 void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
-                         const uint64_t* pos, uint4* out, PipeLaunch L) {
+                         const uint32_t* pos, uint4* out, PipeLaunch L) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, n, in, flag, pos, out);
@@ -34,13 +34,13 @@ void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
 
 __global__ __launch_bounds__(kBlock) void k_compact_ids(int64_t n,
                                                         const uint8_t* flag,
-                                                        const uint64_t* pos,
+                                                        const uint32_t* pos,
                                                         uint32_t* out) {
   int64_t i = gtid();
   if (i >= n || !flag[i]) return;
   out[pos[i]] = (uint32_t)i;
 }
-void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
+void launch_compact_ids(int64_t n, const uint8_t* flag, const uint32_t* pos,
                         uint32_t* out, PipeLaunch L) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n + kBlock - 1) / kBlock)),
@@ -131,14 +131,14 @@ void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
 }
 __global__ __launch_bounds__(kBlock) void k_emit_edges(
     int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
-    const uint8_t* flag, const uint64_t* pos, const uint32_t* perm,
+    const uint8_t* flag, const uint32_t* pos, const uint32_t* perm,
     int2* out) {
   int64_t id = gtid();
   if (id >= ne || !flag[id]) return;
   out[pos[id]] = make_int2((int)perm[eleft[id]], (int)perm[ecol[id]]);
 }
 void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
-                       const uint8_t* flag, const uint64_t* pos,
+                       const uint8_t* flag, const uint32_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L) {
   if (ne <= 0) return;
   hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne + kBlock - 1) / kBlock)),

@@ -707,13 +707,13 @@ void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uin
 /* orow[s] = number of survivors before segment s = pos[off[s]] */
 __global__ __launch_bounds__(kBlock) void k_seg_starts(int64_t nseg,
                                                        const uint64_t* off,
-                                                       const uint64_t* pos,
+                                                       const uint32_t* pos,
                                                        uint64_t* orow) {
   int64_t s = gtid();
   if (s > nseg) return;
   orow[s] = pos[off[s]];
 }
-void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
+void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint32_t* pos,
                        uint64_t* orow, PipeLaunch L) {
   hipLaunchKernelGGL(k_seg_starts,
                      dim3((unsigned)((nseg + 1 + kBlock - 1) / kBlock)),
@@ -724,7 +724,7 @@ void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_compact_flat(
     int64_t nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
-    const uint8_t* flag, const uint64_t* pos, uint32_t* out_val,
+    const uint8_t* flag, const uint32_t* pos, uint32_t* out_val,
     uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
     const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b) {
   int64_t o = gtid();
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
 }
 void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
-                         const uint64_t* pos, uint32_t* out_val,
+                         const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
                          const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
                          PipeLaunch L) {

@@ -65,10 +65,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, int64_t n,
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-template <class TIn, bool kFlag>
+template <class TIn, bool kFlag, class TOut>
 __global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
                                                       const uint64_t* tile_off,
-                                                      uint64_t* out) {
+                                                      TOut* out) {
   __shared__ uint64_t tile[kTile];
   int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
@@ -95,10 +95,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
     int t = q * kBlock + threadIdx.x;
-    if (base + t < n) out[base + t] = tile[t];
+    if (base + t < n) out[base + t] = (TOut)tile[t];
   }
   /* the last thread of the last tile writes the grand total at out[n] */
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = ex;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = (TOut)ex;
 }
 
 size_t scan_tmp_bytes(int64_t n) {
@@ -112,23 +112,24 @@ size_t scan_tmp_bytes(int64_t n) {
   return b + 256;
 }
 
-__global__ void k_store_u64(uint64_t* out, uint64_t v) { *out = v; }
+template <class TOut>
+__global__ void k_store(TOut* out, TOut v) { *out = v; }
 
 /* recursive scan over uint64 tile sums */
 static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
                          uint64_t* tmp, hipStream_t s);
 
-template <class TIn, bool kFlag>
-static void scan_impl(const TIn* in, uint64_t* out, int64_t n, uint64_t* tmp,
+template <class TIn, bool kFlag, class TOut = uint64_t>
+static void scan_impl(const TIn* in, TOut* out, int64_t n, uint64_t* tmp,
                       hipStream_t s) {
   if (n <= 0) {
-    hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, s, out, (uint64_t)0);
+    hipLaunchKernelGGL(k_store<TOut>, dim3(1), dim3(1), 0, s, out, (TOut)0);
     return;
   }
   int64_t tiles = (n + kTile - 1) / kTile;
   if (tiles == 1) {
-    hipLaunchKernelGGL((k_tile_scan<TIn, kFlag>), dim3(1), dim3(kBlock), 0, s,
-                       in, n, (const uint64_t*)nullptr, out);
+    hipLaunchKernelGGL((k_tile_scan<TIn, kFlag, TOut>), dim3(1), dim3(kBlock),
+                       0, s, in, n, (const uint64_t*)nullptr, out);
     return;
   }
   uint64_t* sums = tmp;
@@ -136,7 +137,7 @@ static void scan_impl(const TIn* in, uint64_t* out, int64_t n, uint64_t* tmp,
   hipLaunchKernelGGL((k_tile_sums<TIn, kFlag>), dim3((unsigned)tiles),
                      dim3(kBlock), 0, s, in, n, sums);
   scan_u64_rec(sums, offs, tiles, tmp + 2 * (tiles + 1), s);
-  hipLaunchKernelGGL((k_tile_scan<TIn, kFlag>), dim3((unsigned)tiles),
+  hipLaunchKernelGGL((k_tile_scan<TIn, kFlag, TOut>), dim3((unsigned)tiles),
                      dim3(kBlock), 0, s, in, n, (const uint64_t*)offs, out);
 }
 
@@ -149,9 +150,9 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n, void* tmp,
                      size_t, hipStream_t s) {
   scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s);
 }
-void scan_u8flag_to_u64(const uint8_t* in, uint64_t* out, int64_t n, void* tmp,
+void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, int64_t n, void* tmp,
                         size_t, hipStream_t s) {
-  scan_impl<uint8_t, true>(in, out, n, (uint64_t*)tmp, s);
+  scan_impl<uint8_t, true, uint32_t>(in, out, n, (uint64_t*)tmp, s);
 }
 
 /* ------------------------------------------------------------------ */

@@ -62,7 +62,8 @@ enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
 /* exclusive scan; out has n+1 entries, out[n] = total */
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n,
                      void* tmp, size_t tmp_bytes, hipStream_t s);
-void scan_u8flag_to_u64(const uint8_t* in, uint64_t* out, int64_t n,
+/* flags (nonzero = 1) -> u32 positions; n < 2^32 */
+void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, int64_t n,
                         void* tmp, size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(int64_t n);
 /* stable LSD radix sort of (key, val) by the low `bits` bits of key */
@@ -109,8 +110,8 @@ struct CandCtx {
 
 /* ---- closure + emission (k_closure.hip) ---- */
 void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
-                         const uint64_t* pos, uint4* out, PipeLaunch L);
-void launch_compact_ids(int64_t n, const uint8_t* flag, const uint64_t* pos,
+                         const uint32_t* pos, uint4* out, PipeLaunch L);
+void launch_compact_ids(int64_t n, const uint8_t* flag, const uint32_t* pos,
                         uint32_t* out, PipeLaunch L);
 void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
@@ -125,7 +126,7 @@ void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
                       const uint32_t* tedge, const uint32_t* perm,
                       int32_t* out, PipeLaunch L);
 void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
-                       const uint8_t* flag, const uint64_t* pos,
+                       const uint8_t* flag, const uint32_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L);
 
 /* ---- outputs (k_output.hip) ---- */
@@ -225,11 +226,11 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint3
                        Queue q2, PipeLaunch L);
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L);
-void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint64_t* pos,
+void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint32_t* pos,
                        uint64_t* orow, PipeLaunch L);
 void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
-                         const uint64_t* pos, uint32_t* out_val,
+                         const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
                          const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
                          PipeLaunch L);

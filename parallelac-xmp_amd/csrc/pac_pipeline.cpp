@@ -324,7 +324,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
   auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
                           const uint32_t* cseg, uint8_t* flag, int kp,
-                          DBuf<uint64_t>& pos) -> int64_t {
+                          DBuf<uint32_t>& pos) -> int64_t {
     if (kind == 1) host_evals[0] += nc; /* tris / tets: slots != candidates */
     /* few candidates stay undecided in float: start small, a rerun of the
      * (idempotent) filter sizes the queue exactly if it overflows */
@@ -337,10 +337,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
         [&](QView q) {
           launch_filter_exact(SG, kind, C, q, cval, flag, L);
         });
-    pos = DBuf<uint64_t>(P, nc + 1);
+    pos = DBuf<uint32_t>(P, nc + 1);
     {
       auto tmp = scan_tmp(nc);
-      scan_u8flag_to_u64(flag, pos.get(), nc, tmp.get(), 0, s);
+      scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s);
     }
     KCHK();
     return (int64_t)readback(E, s, pos.get() + nc);
@@ -373,7 +373,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint32_t> cval(P, Ecand), cseg(P, Ecand);
     DBuf<uint8_t> flag(P, Ecand);
     launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L);
-    DBuf<uint64_t> pos;
+    DBuf<uint32_t> pos;
     Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
                           KP_EDGE_FILL, pos);
     launch_seg_starts(n, off.get(), pos.get(), erow.get(), L);
@@ -411,7 +411,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint8_t> flag(P, Tslots);
     launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
                     cval.get(), cseg.get(), cea.get(), ceb.get(), ctr.get(), L);
-    DBuf<uint64_t> pos;
+    DBuf<uint32_t> pos;
     Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
                           KP_TRI_FILL, pos);
     launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L);
@@ -442,7 +442,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint8_t> flag(P, Kslots);
     launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
                     tejk.get(), off.get(), cval.get(), cseg.get(), ctr.get(), L);
-    DBuf<uint64_t> pos;
+    DBuf<uint32_t> pos;
     Kalpha = filter_stage(3, C2, Kslots, cval.get(), cseg.get(), flag.get(),
                           KP_TET_FILL, pos);
     tets = DBuf<uint4>(P, Kalpha);
@@ -485,9 +485,9 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t Kf = 0;
   DBuf<uint4> ftets;
   {
-    DBuf<uint64_t> pos(P, Kalpha + 1);
+    DBuf<uint32_t> pos(P, Kalpha + 1);
     auto tmp = scan_tmp(Kalpha);
-    scan_u8flag_to_u64(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s);
+    scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s);
     KCHK();
     Kf = (int64_t)readback(E, s, pos.get() + Kalpha);
     ftets = DBuf<uint4>(P, Kf);
@@ -522,9 +522,9 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t Tf = 0;
   DBuf<uint32_t> ftri;
   {
-    DBuf<uint64_t> pos(P, Talpha + 1);
+    DBuf<uint32_t> pos(P, Talpha + 1);
     auto tmp = scan_tmp(Talpha);
-    scan_u8flag_to_u64(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s);
+    scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s);
     KCHK();
     Tf = (int64_t)readback(E, s, pos.get() + Talpha);
     ftri = DBuf<uint32_t>(P, Tf);
@@ -556,10 +556,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                                nullptr, eflag.get(), L);
         });
   int64_t Ef = 0;
-  DBuf<uint64_t> pos_e(P, Ealpha + 1);
+  DBuf<uint32_t> pos_e(P, Ealpha + 1);
   {
     auto tmp = scan_tmp(Ealpha);
-    scan_u8flag_to_u64(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
+    scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
     KCHK();
     Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
   }
