@@ -255,48 +255,6 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
   block_flush(S.ctr, CTR_PAIRS + 0, pairs);
 }
 
-/* Second phase, cooperative: kCoop lanes per left-over tet (tets that
- * survived their first `budget` pairs are mostly accepted ones, which
- * must test every ball of their box); same decisions and pair numbering
- * as one lane scanning alone. */
-constexpr int kCoop = 8;
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_tet_ac2_coop(
-    Stage S, const uint4* tets, uint8_t* flag, Queue q1, const uint32_t* rest,
-    const unsigned* nrest, int budget) {
-  unsigned pairs = 0;
-  const int64_t total = (int64_t)*nrest;
-  const int64_t groups = (int64_t)gridDim.x * blockDim.x / kCoop;
-  for (int64_t w = gtid() / kCoop; w < total; w += groups) {
-    const uint32_t id = rest[w];
-    const uint4 t = tets[id];
-    Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
-    const GridP g = grid_of(S, t.x);
-    PairSS<R> P;
-    CellBox box;
-    if (!tet_setup(g, v, P, box)) continue; /* cannot happen: phase one passed */
-    const uint32_t vert[4] = {t.x, t.y, t.z, t.w};
-    CoopOut o = scan_box_coop<kCoop, 4>(g, S.cstart, box, vert, budget,
-                                        [&](uint32_t p) -> int {
-      int sg = pairss_sign(P, S.ball[p]);
-      if (sg == 0) {
-        push_q(q1, id, p);
-        return 2;
-      }
-      return sg * P.sden < 0 ? 1 : 0; /* pi_e < rho: violator */
-    });
-    pairs += o.pairs;
-    const uint64_t gm = ((1ull << kCoop) - 1) << ((threadIdx.x & 63) & ~(kCoop - 1));
-    const bool gdead = (__ballot(o.dead) & gm) != 0;
-    const bool gpend = (__ballot(o.pending) & gm) != 0;
-    if ((threadIdx.x & (kCoop - 1)) == 0) {
-      if (gdead) flag[id] = 0;
-      else if (gpend) flag[id] = 2;
-    }
-  }
-  block_flush(S.ctr, CTR_PAIRS + 0, pairs);
-}
-
 /* FP64 then double-double for queued pairs / uncertified tets */
 template <class T>
 __device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
@@ -640,56 +598,6 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
   block_flush(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
 }
 
-template <class R, int KIND>
-__global__ __launch_bounds__(kBlock) void k_low_ac2_coop(
-    Stage S, const uint32_t* ecol, const uint32_t* eleft, const uint32_t* tk,
-    const uint32_t* tedge, uint8_t* flag, Queue q1, const uint32_t* rest,
-    const unsigned* nrest, int budget) {
-  unsigned pairs = 0;
-  const int64_t total = (int64_t)*nrest;
-  const int64_t groups = (int64_t)gridDim.x * blockDim.x / kCoop;
-  for (int64_t w = gtid() / kCoop; w < total; w += groups) {
-    const uint32_t id = rest[w];
-    uint32_t vid[3];
-    if (KIND == 0) {
-      uint32_t e = tedge[id];
-      vid[0] = eleft[e];
-      vid[1] = ecol[e];
-      vid[2] = tk[id];
-    } else {
-      vid[0] = eleft[id];
-      vid[1] = ecol[id];
-      vid[2] = vid[1];
-    }
-    Ball v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
-    const GridP g = grid_of(S, vid[0]);
-    PairSS<R> P;
-    CellBox box;
-    if (!low_setup<R, KIND>(g, v, P, box)) continue; /* passed in phase one */
-    auto test = [&](uint32_t p) -> int {
-      int sg = pairss_sign(P, S.ball[p]);
-      if (sg < 0) return 1; /* power <= rho strictly: violator */
-      if (sg == 0) {
-        push_q(q1, id, p);
-        return 2;
-      }
-      return 0;
-    };
-    CoopOut o = KIND == 0
-        ? scan_box_coop<kCoop, 3>(g, S.cstart, box, vid, budget, test)
-        : scan_box_coop<kCoop, 2>(g, S.cstart, box, vid, budget, test);
-    pairs += o.pairs;
-    const uint64_t gm = ((1ull << kCoop) - 1) << ((threadIdx.x & 63) & ~(kCoop - 1));
-    const bool gdead = (__ballot(o.dead) & gm) != 0;
-    const bool gpend = (__ballot(o.pending) & gm) != 0;
-    if ((threadIdx.x & (kCoop - 1)) == 0) {
-      if (gdead) flag[id] = 0;
-      else if (gpend) flag[id] = 2;
-    }
-  }
-  block_flush(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
-}
-
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
     Stage S, QView q, const uint32_t* ecol,
@@ -878,8 +786,8 @@ void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
       hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0,
                          L.s, S, nk, tets, flag, q1, rest, nrest, budget);
       if (budget > 0)
-        hipLaunchKernelGGL((k_tet_ac2_coop<float>), gb, dim3(kBlock), 0, L.s, S,
-                           tets, flag, q1, rest, nrest, budget);
+        hipLaunchKernelGGL((k_tet_ac2_hot<float, true>), gb, dim3(kBlock), 0,
+                           L.s, S, nk, tets, flag, q1, rest, nrest, budget);
       return;
     }
   }
@@ -887,8 +795,8 @@ void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
     hipLaunchKernelGGL((k_tet_ac2_hot<double, false>), gr, dim3(kBlock), 0,
                        L.s, S, nk, tets, flag, q1, rest, nrest, budget);
     if (budget > 0)
-      hipLaunchKernelGGL((k_tet_ac2_coop<double>), gb, dim3(kBlock), 0, L.s, S,
-                         tets, flag, q1, rest, nrest, budget);
+      hipLaunchKernelGGL((k_tet_ac2_hot<double, true>), gb, dim3(kBlock), 0,
+                         L.s, S, nk, tets, flag, q1, rest, nrest, budget);
   }
 }
 void launch_tet_ac2_esc(const Stage& S, QView q,
@@ -916,8 +824,9 @@ static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
                      S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1, rest,
                      nrest, budget);
   if (budget > 0)
-    hipLaunchKernelGGL((k_low_ac2_coop<R, KIND>), gb, dim3(kBlock), 0, L.s, S,
-                       ecol, eleft, tk, tedge, flag, q1, rest, nrest, budget);
+    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true>), gb, dim3(kBlock), 0,
+                       L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
+                       q1, rest, nrest, budget);
 }
 void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
                         const uint32_t* ecol, const uint32_t* eleft,

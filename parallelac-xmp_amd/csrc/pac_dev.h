@@ -330,68 +330,6 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
     }
 }
 
-/* Cooperative traversal for the second emptiness phase: a group of G
- * lanes (G divides 64) shares one simplex and visits its box in the order
- * of scan_box, lane gl taking every G-th ball of each cell range.  Pairs
- * are numbered in that order over the balls that are not vertices
- * (vert[0..NV)); the first `skip` pairs were tested by the first phase and
- * are skipped.  test(p) returns 0 (clear), 1 (violator: the group stops
- * after the current range) or 2 (undecided, queued by the caller).  All G
- * lanes of the group call this together; the result is per lane. */
-struct CoopOut {
-  bool dead, pending;
-  unsigned pairs;
-};
-template <int G, int NV, class F>
-__device__ __forceinline__ CoopOut scan_box_coop(const GridP& g,
-                                                 const uint32_t* cstart,
-                                                 const CellBox& b,
-                                                 const uint32_t* vert, int skip,
-                                                 F&& test) {
-  const int lane = threadIdx.x & 63, gl = lane & (G - 1);
-  const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (lane & ~(G - 1));
-  CoopOut o{false, false, 0u};
-  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return o;
-  int64_t base = 0; /* pairs numbered before the current range */
-  auto nv_in = [&](uint32_t lo, uint32_t hi) {
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) c += (vert[k] >= lo && vert[k] < hi);
-    return c;
-  };
-  auto range = [&](uint32_t lo, uint32_t hi) -> bool {
-    for (uint32_t p = lo + gl; p < hi && !o.dead; p += G) {
-      bool is_v = false;
-#pragma unroll
-      for (int k = 0; k < NV; ++k) is_v |= (p == vert[k]);
-      if (is_v) continue;
-      if (base + (int64_t)(p - lo) - nv_in(lo, p) < skip) continue;
-      ++o.pairs;
-      int r = test(p);
-      if (r == 1) o.dead = true;
-      else if (r == 2) o.pending = true;
-    }
-    if (hi > lo) base += (int64_t)(hi - lo) - nv_in(lo, hi);
-    return (__ballot(o.dead) & gmask) != 0;
-  };
-  {
-    int64_t c = (int64_t)g.cbase + b.cc[0] +
-                (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
-    if (range(cstart[c], cstart[c + 1])) return o;
-  }
-  for (int z = b.lo[2]; z <= b.hi[2]; ++z)
-    for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
-      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
-      if (!(z == b.cc[2] && y == b.cc[1])) {
-        if (range(cstart[row + b.lo[0]], cstart[row + b.hi[0] + 1])) return o;
-      } else {
-        if (range(cstart[row + b.lo[0]], cstart[row + b.cc[0]])) return o;
-        if (range(cstart[row + b.cc[0] + 1], cstart[row + b.hi[0] + 1])) return o;
-      }
-    }
-  return o;
-}
-
 /* per-axis error of x = num/(k den) from certified errors of num, den */
 template <class R>
 __device__ __forceinline__ double quot_err(const FE<R>& num, const FE<R>& den,
