@@ -77,9 +77,18 @@ def main():
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
 
+    # the library's autotuner picks the emptiness settings per path
+    # (untimed; outputs do not depend on them)
+    tuned = {}
+    for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+        tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
+                                      n=n, precision=prec)
+    log("autotuned: staged %s plain %s" % (tuned[pac.PREC_STAGED],
+                                            tuned[pac.PREC_PLAIN_FP64]))
+
     def run(prec, profile=False):
         return pac.compute_device(ptr, n, solvent, alpha, precision=prec,
-                                  profile=profile)
+                                  profile=profile, **tuned[prec])
 
     log("warmup")
     for _ in range(args.warmup):
@@ -157,7 +166,8 @@ def main():
                    "n_balls": n, "solvent": solvent, "alpha": alpha, "seed": 2,
                    "cell_policy": "exact", "order": "reference"},
         "plain_fp64": {"value": classified / t_pl, "ms_per_step": t_pl * 1e3,
-                       "dtype": "f64->int512"},
+                       "dtype": "f64->int512", "tuned": tuned[pac.PREC_PLAIN_FP64]},
+        "tuned": tuned[pac.PREC_STAGED],
         "staged_over_plain": t_pl / t_st,
         "bit_exact": {"staged_eq_plain": same_paths, "staged_eq_oracle": oracle_exact},
         "counts": {"classified": classified,
@@ -195,9 +205,15 @@ def bench_batch(args, torch, pac):
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
 
+    tuned = {}
+    for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+        tuned[prec], _ = pac.autotune(ptr, 1.4, 0.0, offsets=off,
+                                      device_inputs=True, precision=prec)
+    log("autotuned on the leading molecules: %s" % tuned)
+
     def run(prec):
         return pac.compute_batch(ptr, off, 1.4, 0.0, precision=prec,
-                                 device_inputs=True)
+                                 device_inputs=True, **tuned[prec])
 
     log("batch: %d molecules, %d balls; warmup" % (n_mol, ntot))
     for _ in range(args.warmup):
@@ -245,6 +261,7 @@ def bench_batch(args, torch, pac):
                                "Poisson-disk min 1.3 A at 0.05 balls/A^3, FP32, "
                                "solvent 1.4, alpha 0", "molecules": n_mol, "balls": ntot},
         "molecules_per_s": n_mol / t_st, "balls_per_s": ntot / t_st,
+        "tuned": tuned[pac.PREC_STAGED],
         "plain_fp64": {"value": classified / t_pl, "ms_per_step": t_pl * 1e3},
         "check": {"sampled_molecules": len(sample), "eq_single_runs": bool(same_single),
                   "eq_oracle_2": oracle_ok},

@@ -50,7 +50,8 @@ typedef struct pac_options {
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
   int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
   int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
-  int32_t reserved[4];
+  int32_t ac2_grid;      /* blocks of the second emptiness phase (0=2048) */
+  int32_t reserved[3];
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -128,6 +129,18 @@ int pac_batch_counts(const pac_batch_result* r, int64_t* out);
 int pac_batch_copy(const pac_batch_result* r, int64_t m, int dim, int32_t* dst);
 int pac_batch_stats(const pac_batch_result* r, pac_stats* st);
 void pac_batch_free(pac_batch_result* r);
+
+/* Autotuner: times the pipeline on the given balls for each candidate
+ * setting of the tunables (emptiness phase-one pair budget, second-phase
+ * grid) and writes the fastest into opt->ac2_budget / opt->ac2_grid; the
+ * other fields of *opt are used as given.  offsets == NULL: the n balls
+ * are one molecule; otherwise they are a batch (offsets[n_mol+1]) and the
+ * leading molecules, up to about 2^20 balls, are the sample.  best_ms (may
+ * be NULL) receives the fastest device time of one run.  Outputs never
+ * depend on these settings.  No reference counterpart. */
+int pac_autotune(const float* xyzr, int64_t n, const int64_t* offsets,
+                 int64_t n_mol, float solvent, float alpha, pac_options* opt,
+                 double* best_ms);
 
 /* Adjoint of the per-simplex alpha values with the combinatorics held
  * fixed: given dL/drho for the output triangles and tets (list order),

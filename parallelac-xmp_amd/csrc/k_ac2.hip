@@ -775,12 +775,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 
 
 /* ---------------- launchers ---------------- */
-static unsigned phase_b_grid() { return 2048; }
 void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, int budget, PipeLaunch L) {
   if (nk <= 0) return;
-  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock)), gb(phase_b_grid());
+  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   if constexpr (!kBallF64) {
     if (S.staged) {
       hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0,
@@ -819,7 +818,7 @@ static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
                      const uint32_t* tedge, const uint8_t* in_flag,
                      uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
                      int budget, PipeLaunch L) {
-  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock)), gb(phase_b_grid());
+  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false>), gr, dim3(kBlock), 0, L.s,
                      S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1, rest,
                      nrest, budget);

@@ -433,6 +433,77 @@ static void compute_batch(const float* xyzr, const int64_t* offsets,
 }
 
 /* ------------------------------------------------------------------ */
+/* autotuner                                                           */
+/* ------------------------------------------------------------------ */
+/* Times each candidate (phase-one pair budget x second-phase grid) on the
+ * sample, best of two runs after a warm-up run, wall clock around the
+ * whole call (what a caller sees), and keeps the fastest. */
+static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
+                     int64_t nmol, float solvent, float alpha, pac_options* o,
+                     double* best_ms) {
+  if (!o) fail(PAC_ERR_INVALID, "opt is NULL");
+  pac_options base = *o;
+  int64_t ns = n, ms = nmol;
+  if (offsets) {
+    if (nmol <= 0 || offsets[0] != 0) fail(PAC_ERR_INVALID, "bad offsets");
+    ms = 1;
+    while (ms < nmol && offsets[ms + 1] <= ((int64_t)1 << 20)) ++ms;
+    ns = offsets[ms];
+  }
+  if (ns <= 0) {
+    if (best_ms) *best_ms = 0;
+    return;
+  }
+  Engine* E = engine_for(base.device);
+  DBuf<float4> dev;
+  const float* src = xyzr;
+  if (!base.device_inputs) {
+    {
+      std::lock_guard<std::mutex> lk(E->run_mu);
+      HIPCHK(hipSetDevice(E->device));
+      dev = DBuf<float4>(&E->pool, ns);
+      HIPCHK(hipMemcpy(dev.get(), xyzr, (size_t)ns * 16, hipMemcpyHostToDevice));
+    }
+    src = (const float*)dev.get();
+    base.device_inputs = 1;
+  }
+  static const int budgets[] = {4, 6, 8, 12, 16, 24};
+  static const int grids[] = {1024, 2048, 4096};
+  double best = 1e300;
+  pac_options pick = base;
+  for (int b : budgets)
+    for (int g : grids) {
+      pac_options t = base;
+      t.ac2_budget = b;
+      t.ac2_grid = g;
+      t.profile = 0;
+      double tmin = 1e300;
+      for (int rep = 0; rep < 3; ++rep) {
+        auto t0 = std::chrono::steady_clock::now();
+        if (offsets) {
+          pac_batch_result* r = nullptr;
+          compute_batch(src, offsets, ms, solvent, alpha, &t, &r);
+          delete r;
+        } else {
+          pac_result* r = nullptr;
+          compute_one<float>(src, ns, solvent, alpha, &t, &r);
+          delete r;
+        }
+        double dt = std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - t0).count();
+        if (rep > 0) tmin = std::min(tmin, dt);
+      }
+      if (tmin < best) {
+        best = tmin;
+        pick = t;
+      }
+    }
+  o->ac2_budget = pick.ac2_budget;
+  o->ac2_grid = pick.ac2_grid;
+  if (best_ms) *best_ms = best;
+}
+
+/* ------------------------------------------------------------------ */
 /* C ABI                                                               */
 /* ------------------------------------------------------------------ */
 extern "C" {
@@ -513,6 +584,13 @@ int pac_compute_batch(const float* xyzr, const int64_t* offsets, int64_t n_mol,
                       pac_batch_result** out) {
   return guarded([&] {
     compute_batch(xyzr, offsets, n_mol, solvent, alpha, opt, out);
+  });
+}
+int pac_autotune(const float* xyzr, int64_t n, const int64_t* offsets,
+                 int64_t n_mol, float solvent, float alpha, pac_options* opt,
+                 double* best_ms) {
+  return guarded([&] {
+    autotune(xyzr, n, offsets, n_mol, solvent, alpha, opt, best_ms);
   });
 }
 int pac_batch_counts(const pac_batch_result* r, int64_t* out) {

@@ -193,6 +193,7 @@ struct Stage {
   ExactScale ex;
   unsigned long long* ctr;
   int staged;
+  int bgrid;     /* blocks of the second emptiness phase */
 };
 
 

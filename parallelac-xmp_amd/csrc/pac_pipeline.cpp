@@ -202,6 +202,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   SG.ex.alpha = wi_from_scaled_host((double)alpha, 2 * S);
   /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
   SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
+  SG.bgrid = opt.ac2_grid > 0 ? std::min(opt.ac2_grid, 1 << 16) : 2048;
   DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
   zero_u64(ctr.get(), CTR_COUNT * CTR_SLOTS, s);
   SG.ctr = ctr.get();

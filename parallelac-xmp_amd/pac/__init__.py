@@ -41,8 +41,8 @@ class Options(ctypes.Structure):
                 ("device_inputs", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
-                ("batch_balls", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("batch_balls", ctypes.c_int32), ("ac2_grid", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -81,6 +81,7 @@ EXPORTS = (
     "pac_batch_stats", "pac_batch_free", "pac_alpha_adjoint", "pac_read_crd",
     "pac_write_output", "pac_free", "pac_synth_globule", "pac_synth_globule_f64",
     "pac_synth_batch", "pac_synth_stress", "pac_last_error", "pac_version",
+    "pac_autotune",
 )
 
 _lib = None
@@ -125,6 +126,8 @@ def lib():
     L.pac_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.pac_batch_free.argtypes = [vp]
     L.pac_alpha_adjoint.argtypes = [vp, vp, vp, vp, vp]
+    L.pac_autotune.argtypes = [vp, i64, vp, i64, ctypes.c_float, ctypes.c_float,
+                               ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]
     L.pac_read_crd.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp),
                                ctypes.POINTER(i64)]
     L.pac_write_output.argtypes = [ctypes.c_char_p, vp, i64, ctypes.c_float, vp]
@@ -156,7 +159,7 @@ def _check(rc):
 
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
-            profile=False, ac2_budget=0, batch_balls=0):
+            profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -169,6 +172,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.profile = int(bool(profile))
     o.ac2_budget = int(ac2_budget)
     o.batch_balls = int(batch_balls)
+    o.ac2_grid = int(ac2_grid)
     return o
 
 
@@ -315,6 +319,28 @@ def compute_batch(xyzr, offsets, solvent, alpha, **kw):
     _check(lib().pac_compute_batch(ptr, off.ctypes.data, n_mol, solvent, alpha,
                                    ctypes.byref(o), ctypes.byref(h)))
     return BatchResult(h.value, n_mol)
+
+
+def autotune(xyzr, solvent, alpha, offsets=None, **kw):
+    """Pick the fastest ac2_budget / ac2_grid for these balls (host FP32
+    n x 4, or a device pointer with device_inputs=True and n given as
+    kw['n']); returns (options dict for compute(), best ms)."""
+    n = kw.pop("n", None)
+    o = options(**kw)
+    if kw.get("device_inputs"):
+        ptr = ctypes.c_void_p(int(xyzr))
+    else:
+        a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+        ptr, n = a.ctypes.data, a.shape[0]
+    best = ctypes.c_double()
+    if offsets is None:
+        _check(lib().pac_autotune(ptr, n, None, 0, solvent, alpha, ctypes.byref(o),
+                                  ctypes.byref(best)))
+    else:
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        _check(lib().pac_autotune(ptr, int(off[-1]), off.ctypes.data, off.shape[0] - 1,
+                                  solvent, alpha, ctypes.byref(o), ctypes.byref(best)))
+    return {"ac2_budget": o.ac2_budget, "ac2_grid": o.ac2_grid}, best.value
 
 
 def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):
