@@ -44,6 +44,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle leg (baseline + bit-exact check)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--budget", type=int, default=0,
+                    help="emptiness phase-one pair budget (with --grid: skip "
+                         "the autotuner, e.g. for profiler runs)")
+    ap.add_argument("--grid", type=int, default=0,
+                    help="second emptiness phase grid (blocks)")
     ap.add_argument("--workload", choices=("globule", "batch", "stress", "copy"),
                     default="globule",
                     help="globule: the 100k headline (default); batch: the "
@@ -81,10 +86,13 @@ def main():
     # (untimed; outputs do not depend on them)
     tuned = {}
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
-        tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
-                                      n=n, precision=prec)
-    log("autotuned: staged %s plain %s" % (tuned[pac.PREC_STAGED],
-                                            tuned[pac.PREC_PLAIN_FP64]))
+        if args.budget and args.grid:
+            tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid}
+        else:
+            tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
+                                          n=n, precision=prec)
+    log("settings: staged %s plain %s" % (tuned[pac.PREC_STAGED],
+                                           tuned[pac.PREC_PLAIN_FP64]))
 
     def run(prec, profile=False):
         return pac.compute_device(ptr, n, solvent, alpha, precision=prec,
@@ -124,9 +132,17 @@ def main():
     dom = max(WORK, key=lambda k: kms[k])
     flops = WORK[dom] * units[dom] + (23 * s_st.tet_alpha if dom == "tet_ac2" else 0)
     achieved = flops / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_globule100k.json")
+    if os.path.exists(tpath) and dom in ("tet_ac2",):
+        tj = json.load(open(tpath))
+        traffic = tj.get("bytes_per_run", {}).get(dom)
+        traffic_src = "profiles/hbm_traffic_globule100k.json (rocprofv3 --pmc " \
+                      "FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy)"
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
                 "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"],
-                "traffic": None, "kernel": dom, "kernel_ms": kms[dom],
+                "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dom, "kernel_ms": kms[dom],
                 "work_units": int(units[dom]), "flops_per_unit": WORK[dom],
                 "note": "vector FP32 ALU (no MFMA on this path); peak = MI355X "
                         "FP32 157.3 TFLOP/s"}
