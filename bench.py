@@ -331,6 +331,14 @@ def bench_stress(args, torch, pac):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / args.steps
     st = r.stats()
+    log("stress: plain FP64 path")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rp = pac.compute_device(ptr, n, 1.4, 0.0, precision=pac.PREC_PLAIN_FP64)
+    torch.cuda.synchronize()
+    t_pl = time.perf_counter() - t0
+    L, Lp = r.lists(), rp.lists()
+    same_paths = all(np.array_equal(L[d], Lp[d]) for d in (1, 2, 3))
     oracle_ok = None
     cpu = None
     if not args.no_cpu:
@@ -341,7 +349,6 @@ def bench_stress(args, torch, pac):
         t0 = time.perf_counter()
         o, ost, _ = oracle.run(balls, 1.4, 0.0, nthreads=threads)
         tc = time.perf_counter() - t0
-        L = r.lists()
         oracle_ok = all(np.array_equal(L[d], o[d]) for d in (1, 2, 3))
         cpu = {"value": (ost["E_cand"] + ost["T_cand"] + ost["K_cand"]) / tc,
                "unit": "classified simplices/s", "cores": threads, "kind": "port",
@@ -356,7 +363,8 @@ def bench_stress(args, torch, pac):
         "config": {"workload": "stress-2M: Poisson-disk in r=212.2 A, 10% central "
                                "cubic lattice spacing 2.0 A r=1.70, FP32, solvent 1.4, "
                                "alpha 0, one pass", "n_balls": n},
-        "bit_exact_vs_oracle": oracle_ok,
+        "bit_exact": {"staged_eq_plain": same_paths, "staged_eq_oracle": oracle_ok},
+        "plain_fp64": {"ms_per_step": t_pl * 1e3},
         "counts": {"classified": classified, "edges": int(st.edges_final),
                    "triangles": int(st.tris_final), "tets": int(st.tets_final)},
         "escalations": _esc(pac, st), "sos_calls": int(st.sos_calls),

@@ -104,6 +104,21 @@ def test_lattice_degenerate():
         assert st.sos_calls > 0
 
 
+def test_stress_lattice_block():
+    """The degenerate stress workload at 60,000 balls: Poisson-disk globule
+    with a central block replaced by a 2.0 A lattice of equal radii (10%).
+    Exact ties reach the wide-integer stage and SoS."""
+    _say("case: stress 60k with lattice block")
+    b = pac.synth_stress(60000, None, seed=5, lattice_frac=0.1, spacing=2.0,
+                         lattice_r=1.70)
+    o, ost, _ = oracle.run(b, 1.4, 0.0)
+    for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+        g, st = _gpu_lists(b, 1.4, 0.0, precision=prec)
+        _assert_same(g, o, "stress")
+        assert st.sos_calls > 0
+        assert st.stage_evals[3][3] > 0  # tet emptiness reached wide integers
+
+
 def test_empty_and_tiny():
     _say("case: empty / tiny inputs")
     r = pac.compute(np.zeros((0, 4), np.float32), 1.4, 0.0)
