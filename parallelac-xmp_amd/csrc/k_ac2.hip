@@ -172,12 +172,9 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
   FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
   if (!pairss_init(P, pa, waf, Df, Nf)) return false;
-  double D2 = 2.0 * (double)Df.v;
-  double cx = (double)v[0].x + (double)Nf.x.v / D2;
-  double cy = (double)v[0].y + (double)Nf.y.v / D2;
-  double cz = (double)v[0].z + (double)Nf.z.v / D2;
-  return make_box(g, cx, cy, cz, quot_err(Nf.x, Df, 2.0),
-                  quot_err(Nf.y, Df, 2.0), quot_err(Nf.z, Df, 2.0), box);
+  double c[3], e[3];
+  if (!centre_err(Df, Nf, v[0], c, e)) return false;
+  return make_box(g, c[0], c[1], c[2], e[0], e[1], e[2], box);
 }
 
 /* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
@@ -503,13 +500,9 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
   FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
   if (!pairss_init(P, pa, waf, den, num) || P.sden < 0) return false;
-  double d2 = 2.0 * (double)den.v;
-  double cx = (double)v[0].x + (double)num.x.v / d2;
-  double cy = (double)v[0].y + (double)num.y.v / d2;
-  double cz = (double)v[0].z + (double)num.z.v / d2;
-  return make_box(g, cx, cy, cz,
-                  quot_err(num.x, den, 2.0), quot_err(num.y, den, 2.0),
-                  quot_err(num.z, den, 2.0), box);
+  double c[3], e[3];
+  if (!centre_err(den, num, v[0], c, e)) return false;
+  return make_box(g, c[0], c[1], c[2], e[0], e[1], e[2], box);
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).

@@ -223,7 +223,7 @@ __device__ __forceinline__ bool make_box(const GridP& g, double cx, double cy,
   double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
   double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
   int ext[3] = {g.ex, g.ey, g.ez};
-  double inv = 1.0 / (double)g.step;
+  const double inv = g.inv_step;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double r = reach + margin + e[a];
@@ -330,6 +330,30 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
     }
 }
 
+/* Centre c = p0 + num / (2 den) and per-axis error bounds from certified
+ * num, den, with two divisions.  With D = |den|, eD its bound (D > eD):
+ * |num/den - num'/den'| <= (eN + |N| eD / (D - eD)) / (D - eD), which is
+ * at least quot_err's bound; 1e-14 |N| / (D - eD) covers the rounding of
+ * the computed quotient.  Returns false when den is not certified. */
+template <class R>
+__device__ __forceinline__ bool centre_err(const FE<R>& den,
+                                           const V3<FE<R>>& num, const Ball& p0,
+                                           double c[3], double e[3]) {
+  const double D = fabs((double)den.v), eD = (double)den.e;
+  if (!(D > eD * 1.01)) return false;
+  const double inv2 = 0.5 / (double)den.v, im = 0.5 / (D - eD);
+  const double n[3] = {(double)num.x.v, (double)num.y.v, (double)num.z.v};
+  const double en[3] = {(double)num.x.e, (double)num.y.e, (double)num.z.e};
+  const double p[3] = {(double)p0.x, (double)p0.y, (double)p0.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c[a] = p[a] + n[a] * inv2;
+    e[a] = (en[a] + fabs(n[a]) * eD * (2.0 * im)) * im * 1.001 +
+           fabs(n[a]) * im * 1e-14;
+  }
+  return true;
+}
+
 /* per-axis error of x = num/(k den) from certified errors of num, den */
 template <class R>
 __device__ __forceinline__ double quot_err(const FE<R>& num, const FE<R>& den,
@@ -357,6 +381,14 @@ __device__ __forceinline__ double quot_err(const FEDD& num, const FEDD& den,
  * squares and their sums, w_e = r_e^2, the products with den and num, the
  * final difference) plus second-order terms.  Returns +1 / -1 when the sign
  * is certified, 0 otherwise. */
+/* fused multiply-add in the precision of R */
+__device__ __forceinline__ float fmaR(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fmaR(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
 template <class R>
 struct PairSS {
   R ax, ay, az, wa;
@@ -415,14 +447,17 @@ __device__ __forceinline__ void cross_ss(R ax, R ay, R az, R bx, R by, R bz,
 template <class R>
 __device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
                                            const Ball& e) {
+  /* fused multiply-adds round once where the bound counts two roundings,
+   * so the bound still holds */
   R yx = (R)e.x - P.ax, yy = (R)e.y - P.ay, yz = (R)e.z - P.az;
-  R q = yx * yx + yy * yy + yz * yz;
+  R q = fmaR(yx, yx, fmaR(yy, yy, yz * yz));
   R we = (R)e.w * (R)e.w;
   R L = (q - we) + P.wa;
-  R g = L * P.den - (yx * P.nx + yy * P.ny + yz * P.nz);
+  R yn = fmaR(yx, P.nx, fmaR(yy, P.ny, yz * P.nz));
+  R g = fmaR(L, P.den, -yn);
   R Labs = (q + we) + P.wa;
-  R B = Labs * P.K1 + (fabs(yx) * P.Mx + fabs(yy) * P.My + fabs(yz) * P.Mz) +
-        P.K2;
+  R B = fmaR(Labs, P.K1,
+             fmaR(fabs(yx), P.Mx, fmaR(fabs(yy), P.My, fmaR(fabs(yz), P.Mz, P.K2))));
   return g > B ? 1 : (g < -B ? -1 : 0);
 }
 

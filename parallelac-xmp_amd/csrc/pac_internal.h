@@ -30,6 +30,7 @@ struct GridP {
   uint32_t cbase;
   double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
   double margin;  /* FP32 cell-key rounding, in length units */
+  double inv_step;
 };
 
 /* raw-input bounds reduced on the device (values of the input type) */

@@ -104,6 +104,7 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
   }
   g.reach = reach;
   g.margin = margin;
+  g.inv_step = 1.0 / (double)step;
   return P;
 }
 
@@ -158,6 +159,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     if (empty) { /* one empty cell, never visited */
       std::memset(&gp.g, 0, sizeof(gp.g));
       gp.g.step = 1;
+      gp.g.inv_step = 1;
       gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
       gp.G = 1;
     } else {
