@@ -301,33 +301,60 @@ __device__ __forceinline__ bool scan_range4(const Ball* ball, uint32_t b,
   }
   return false;
 }
+/* The ranges of scan_box's traversal one at a time (centre cell, then the
+ * rows with the centre cell cut out), so that a scan has a single loop
+ * over ranges and its per-ball code is emitted once. */
+struct BoxIter {
+  int z, y, part; /* part: -1 centre cell, 0 whole row / left of centre, 1 right */
+};
+__device__ __forceinline__ bool box_next(const GridP& g, const uint32_t* cstart,
+                                         const CellBox& b, BoxIter& it,
+                                         uint32_t& lo, uint32_t& hi) {
+  for (;;) {
+    if (it.part == -1) {
+      int64_t c = (int64_t)g.cbase + b.cc[0] +
+                  (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
+      lo = cstart[c];
+      hi = cstart[c + 1];
+      it.part = 0;
+      return true;
+    }
+    if (it.z > b.hi[2]) return false;
+    const int64_t row =
+        (int64_t)g.cbase + (int64_t)g.ex * (it.y + (int64_t)g.ey * it.z);
+    const bool crow = it.z == b.cc[2] && it.y == b.cc[1];
+    if (!crow) {
+      lo = cstart[row + b.lo[0]];
+      hi = cstart[row + b.hi[0] + 1];
+    } else if (it.part == 0) {
+      lo = cstart[row + b.lo[0]];
+      hi = cstart[row + b.cc[0]];
+    } else {
+      lo = cstart[row + b.cc[0] + 1];
+      hi = cstart[row + b.hi[0] + 1];
+    }
+    if (crow && it.part == 0) {
+      it.part = 1;
+    } else {
+      it.part = 0;
+      if (++it.y > b.hi[1]) {
+        it.y = b.lo[1];
+        ++it.z;
+      }
+    }
+    if (hi > lo) return true;
+  }
+}
 template <class F>
 __device__ __forceinline__ void scan_box4(const GridP& g,
                                           const uint32_t* cstart,
                                           const Ball* ball, const CellBox& b,
                                           F&& visit) {
   if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
-  {
-    int64_t c = (int64_t)g.cbase + b.cc[0] +
-                (int64_t)g.ex * (b.cc[1] + (int64_t)g.ey * b.cc[2]);
-    if (scan_range4(ball, cstart[c], cstart[c + 1], visit)) return;
-  }
-  for (int z = b.lo[2]; z <= b.hi[2]; ++z)
-    for (int y = b.lo[1]; y <= b.hi[1]; ++y) {
-      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
-      if (!(z == b.cc[2] && y == b.cc[1])) {
-        if (scan_range4(ball, cstart[row + b.lo[0]], cstart[row + b.hi[0] + 1],
-                        visit))
-          return;
-      } else {
-        if (scan_range4(ball, cstart[row + b.lo[0]], cstart[row + b.cc[0]],
-                        visit))
-          return;
-        if (scan_range4(ball, cstart[row + b.cc[0] + 1],
-                        cstart[row + b.hi[0] + 1], visit))
-          return;
-      }
-    }
+  BoxIter it{b.lo[2], b.lo[1], -1};
+  uint32_t lo, hi;
+  while (box_next(g, cstart, b, it, lo, hi))
+    if (scan_range4(ball, lo, hi, visit)) return;
 }
 
 /* Centre c = p0 + num / (2 den) and per-axis error bounds from certified
