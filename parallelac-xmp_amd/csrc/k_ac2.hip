@@ -172,9 +172,10 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
   FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
   if (!pairss_init(P, pa, waf, Df, Nf)) return false;
-  double c[3], e[3];
-  if (!centre_err(Df, Nf, v[0], c, e)) return false;
-  return make_box(g, c[0], c[1], c[2], e[0], e[1], e[2], box);
+  double c[3], e[3], rho_up;
+  if (!centre_err(Df, Nf, v[0], waf, c, e, &rho_up)) return false;
+  return make_box_r(g, simplex_reach(g, rho_up), c[0], c[1], c[2], e[0], e[1],
+                    e[2], box);
 }
 
 /* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
@@ -500,9 +501,10 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
   FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
   if (!pairss_init(P, pa, waf, den, num) || P.sden < 0) return false;
-  double c[3], e[3];
-  if (!centre_err(den, num, v[0], c, e)) return false;
-  return make_box(g, c[0], c[1], c[2], e[0], e[1], e[2], box);
+  double c[3], e[3], rho_up;
+  if (!centre_err(den, num, v[0], waf, c, e, &rho_up)) return false;
+  return make_box_r(g, simplex_reach(g, rho_up), c[0], c[1], c[2], e[0], e[1],
+                    e[2], box);
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).

@@ -216,10 +216,30 @@ struct CellBox {
 __device__ __forceinline__ GridP grid_of(const Stage& S, uint32_t b) {
   return S.mol ? S.grids[S.mol[b]] : S.g;
 }
+__device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
+                                           double cx, double cy, double cz,
+                                           double ex, double ey, double ez,
+                                           CellBox& b);
 __device__ __forceinline__ bool make_box(const GridP& g, double cx, double cy,
                                          double cz, double ex, double ey,
                                          double ez, CellBox& b) {
-  const double reach = g.reach, margin = g.margin;
+  return make_box_r(g, g.reach, cx, cy, cz, ex, ey, ez, b);
+}
+/* A simplex with power-centre c and alpha value rho is violated only by
+ * balls with |p_e - c|^2 < rho + w_e <= rho + max w, so its own reach
+ * sqrt(rho + max w) (rho certified from above) can replace the global
+ * one (exact cell policy; the reference's stencil keeps its 27 cells). */
+__device__ __forceinline__ double simplex_reach(const GridP& g, double rho_up) {
+  if (!(g.maxw >= 0)) return g.reach;
+  const double r2 = rho_up + g.maxw;
+  const double r = r2 > 0 ? sqrt(r2) * (1 + 1e-12) + 1e-12 : 1e-12;
+  return r < g.reach ? r : g.reach;
+}
+__device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
+                                           double cx, double cy, double cz,
+                                           double ex, double ey, double ez,
+                                           CellBox& b) {
+  const double margin = g.margin;
   double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
   double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
   int ext[3] = {g.ex, g.ey, g.ez};
@@ -365,19 +385,25 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
 template <class R>
 __device__ __forceinline__ bool centre_err(const FE<R>& den,
                                            const V3<FE<R>>& num, const Ball& p0,
-                                           double c[3], double e[3]) {
+                                           const FE<R>& wa, double c[3],
+                                           double e[3], double* rho_up) {
   const double D = fabs((double)den.v), eD = (double)den.e;
   if (!(D > eD * 1.01)) return false;
   const double inv2 = 0.5 / (double)den.v, im = 0.5 / (D - eD);
   const double n[3] = {(double)num.x.v, (double)num.y.v, (double)num.z.v};
   const double en[3] = {(double)num.x.e, (double)num.y.e, (double)num.z.e};
   const double p[3] = {(double)p0.x, (double)p0.y, (double)p0.z};
+  double nn = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     c[a] = p[a] + n[a] * inv2;
     e[a] = (en[a] + fabs(n[a]) * eD * (2.0 * im)) * im * 1.001 +
            fabs(n[a]) * im * 1e-14;
+    const double up = (fabs(n[a]) + en[a]) * im; /* >= |x_a| */
+    nn += up * up;
   }
+  /* rho = |x|^2 - w_a with x = num / (2 den) */
+  *rho_up = nn * (1 + 1e-12) - ((double)wa.v - (double)wa.e) * (1 - 1e-12);
   return true;
 }
 

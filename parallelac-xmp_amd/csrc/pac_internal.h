@@ -31,6 +31,7 @@ struct GridP {
   double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
   double margin;  /* FP32 cell-key rounding, in length units */
   double inv_step;
+  double maxw;    /* max r^2 of the molecule; < 0: keep the global reach */
 };
 
 /* raw-input bounds reduced on the device (values of the input type) */

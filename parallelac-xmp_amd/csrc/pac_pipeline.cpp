@@ -105,6 +105,7 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
   g.reach = reach;
   g.margin = margin;
   g.inv_step = 1.0 / (double)step;
+  g.maxw = policy == PAC_CELL_REFERENCE_RAW ? -1.0 : maxw;
   return P;
 }
 
@@ -160,6 +161,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       std::memset(&gp.g, 0, sizeof(gp.g));
       gp.g.step = 1;
       gp.g.inv_step = 1;
+      gp.g.maxw = -1;
       gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
       gp.G = 1;
     } else {
