@@ -493,22 +493,26 @@ void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      cseg, cea, ceb, ctr);
 }
 
-/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k) */
+/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k)
+ * (computePossibleTetCount / fillPossibleTets, Parallel:418-543); each
+ * candidate is written with its four vertex ids, so that the filter and
+ * the compaction read one uint4 instead of following triangle -> edge ->
+ * vertex links */
 __global__ __launch_bounds__(kBlock) void k_tet_enum(
     int64_t nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
-    const uint64_t* off, uint32_t* cval, uint32_t* cseg,
-    unsigned long long* ctr) {
-  constexpr int CAP = 4096;
-  __shared__ uint32_t s_out[2 * CAP];
+    const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
+    uint4* ctet, unsigned long long* ctr) {
+  constexpr int CAP = 2048;
+  __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   const int64_t q1 = q0 + kBlock < nt ? q0 + kBlock : nt;
-  BlockOut<2, CAP> B{s_out, off[q0], off[q1] - off[q0]};
-  uint32_t* const g[2] = {cval, cseg};
+  BlockOut4<CAP> B{s_out, off[q0], off[q1] - off[q0]};
   int64_t q = gtid();
   unsigned found = 0;
   if (q < nt) {
     uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+    const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
     uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
     uint64_t b = trow[eik], be = trow[eik + 1];
     uint64_t c = trow[ejk], ce = trow[ejk + 1];
@@ -517,8 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum(
       uint32_t x = tk[a], y = tk[b], z = tk[c];
       uint32_t m = max(x, max(y, z));
       if (x == m && y == m && z == m) {
-        const uint32_t v[2] = {m, (uint32_t)q};
-        B.put(g, o, v);
+        B.put(ctet, o, make_uint4(i, j, k, m));
         ++o;
         ++found;
         ++a;
@@ -530,20 +533,20 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum(
         c += (z < m);
       }
     }
-    const uint32_t unused[2] = {0u, 0xffffffffu};
-    for (; o < oe; ++o) B.put(g, o, unused);
+    for (; o < oe; ++o) B.put(ctet, o, make_uint4(0xffffffffu, 0u, 0u, 0u));
   }
-  B.flush(g);
+  B.flush(ctet);
   block_flush(ctr, CTR_CAND + 2, found);
 }
 void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint32_t* teik,
-                     const uint32_t* tejk, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, unsigned long long* ctr, PipeLaunch L) {
+                     const uint32_t* tejk, const uint32_t* eleft,
+                     const uint32_t* ecol, const uint64_t* off, uint4* ctet,
+                     unsigned long long* ctr, PipeLaunch L) {
   if (nt <= 0) return;
   hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk, off,
-                     cval, cseg, ctr);
+                     dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk,
+                     eleft, ecol, off, ctet, ctr);
 }
 
 /* ------------------------------------------------------------------ */
@@ -553,21 +556,21 @@ void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
  * (seg = alpha edge), 3 tet (seg = alpha triangle) */
 template <int KIND>
 __device__ __forceinline__ void cand_balls(const Stage& S, const CandCtx& C,
-                                           uint32_t seg, uint32_t val,
-                                           Ball* v) {
-  if (KIND == 1) {
+                                           int64_t o, uint32_t seg,
+                                           uint32_t val, Ball* v) {
+  if (KIND == 3) {
+    const uint4 t = C.ctet[o];
+    v[0] = S.ball[t.x];
+    v[1] = S.ball[t.y];
+    v[2] = S.ball[t.z];
+    v[3] = S.ball[t.w];
+  } else if (KIND == 1) {
     v[0] = S.ball[seg];
     v[1] = S.ball[val];
   } else if (KIND == 2) {
     v[0] = S.ball[C.eleft[seg]];
     v[1] = S.ball[C.ecol[seg]];
     v[2] = S.ball[val];
-  } else {
-    uint32_t e = C.tedge[seg];
-    v[0] = S.ball[C.eleft[e]];
-    v[1] = S.ball[C.ecol[e]];
-    v[2] = S.ball[C.tk[seg]];
-    v[3] = S.ball[val];
   }
 }
 template <int KIND, class T>
@@ -597,12 +600,12 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
    * stage, no counter traffic here */
   int64_t o = gtid();
   if (o < nc) {
-    uint32_t seg = cseg[o];
+    const uint32_t seg = KIND == 3 ? C.ctet[o].x : cseg[o];
     if (seg == 0xffffffffu) {
       flag[o] = REJECT; /* unused slot */
     } else {
       Ball v[4];
-      cand_balls<KIND>(S, C, seg, cval[o], v);
+      cand_balls<KIND>(S, C, o, seg, KIND == 3 ? 0u : cval[o], v);
       int r = filter_code<KIND, FE<R>>(v, S.alpha);
       flag[o] = (uint8_t)r;
       if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
@@ -620,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
   if (t < q.size()) {
     QEntry e = q.at(t);
     Ball v[4];
-    cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
+    cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     sc.c[1]++;
     int r = filter_code<KIND, FE<double>>(v, S.alpha);
     if (r == UNDEC) {
@@ -644,7 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
   if (t < q.size()) {
     QEntry e = q.at(t);
     Ball v[4];
-    cand_balls<KIND>(S, C, e.aux, cval[e.item], v);
+    cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
     cnt = 1;
   }
@@ -730,17 +733,15 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
   int64_t o = gtid();
   if (o >= nc || flag[o] != KEEP) return;
   uint64_t w = pos[o];
-  uint32_t seg = cseg[o], val = cval[o];
   if (KIND < 3) {
-    out_val[w] = val;
-    out_seg[w] = seg;
+    out_val[w] = cval[o];
+    out_seg[w] = cseg[o];
     if (KIND == 2) {
       out_a[w] = cea[o];
       out_b[w] = ceb[o];
     }
   } else {
-    uint32_t e = C.tedge[seg];
-    out_tet[w] = make_uint4(C.eleft[e], C.ecol[e], C.tk[seg], val);
+    out_tet[w] = C.ctet[o];
   }
 }
 void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,

@@ -96,6 +96,23 @@ struct BlockOut {
   }
 };
 
+/* the same for one array of uint4 entries */
+template <int CAP>
+struct BlockOut4 {
+  uint4* s; /* LDS, CAP */
+  uint64_t base, total;
+  __device__ __forceinline__ void put(uint4* g, uint64_t o, const uint4& v) {
+    uint64_t lo = o - base;
+    if (lo < (uint64_t)CAP) s[lo] = v;
+    else g[o] = v;
+  }
+  __device__ __forceinline__ void flush(uint4* g) {
+    __syncthreads();
+    const uint32_t m = (uint32_t)(total < (uint64_t)CAP ? total : (uint64_t)CAP);
+    for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) g[base + t] = s[t];
+  }
+};
+
 /* wave-aggregated append: one atomic per wave for all lanes with want */
 __device__ __forceinline__ unsigned wave_append(unsigned* counter, bool want) {
   uint64_t m = __ballot(want);

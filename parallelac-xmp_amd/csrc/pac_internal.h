@@ -108,6 +108,7 @@ struct QView {
 
 struct CandCtx {
   const uint32_t *eleft, *ecol, *tk, *tedge;
+  const uint4* ctet; /* tet candidates (i, j, k, l); x = ~0: unused slot */
 };
 
 /* ---- closure + emission (k_closure.hip) ---- */
@@ -219,8 +220,9 @@ void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
                      unsigned long long* ctr, PipeLaunch L);
 void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint32_t* teik,
-                     const uint32_t* tejk, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, unsigned long long* ctr, PipeLaunch L);
+                     const uint32_t* tejk, const uint32_t* eleft,
+                     const uint32_t* ecol, const uint64_t* off, uint4* ctet,
+                     unsigned long long* ctr, PipeLaunch L);
 /* kind 1 edges, 2 triangles, 3 tets */
 void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,

@@ -367,7 +367,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t Ecand = 0, Ealpha = 0;
   DBuf<uint64_t> erow(P, n + 1);
   DBuf<uint32_t> ecol, eleft;
-  CandCtx C0{nullptr, nullptr, nullptr, nullptr};
+  CandCtx C0{nullptr, nullptr, nullptr, nullptr, nullptr};
   if (!no_edges) {
     DBuf<uint64_t> off(P, n + 1);
     {
@@ -401,7 +401,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t Tcand = 0, Tslots = 0, Talpha = 0;
   DBuf<uint64_t> trow(P, Ealpha + 1);
   DBuf<uint32_t> tk, tedge, teik, tejk;
-  CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr};
+  CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr, nullptr};
   {
     DBuf<uint64_t> off(P, Ealpha + 1);
     {
@@ -435,7 +435,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* ---- tets (AlphaComplexParallel.cu:1381-1442) ---- */
   int64_t Kcand = 0, Kslots = 0, Kalpha = 0;
   DBuf<uint4> tets;
-  CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get()};
+  CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get(), nullptr};
   {
     DBuf<uint64_t> off(P, Talpha + 1);
     {
@@ -443,17 +443,19 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
       Kslots = count_scan(cap.get(), Talpha, off.get(), "tet");
     }
-    DBuf<uint32_t> cval(P, Kslots), cseg(P, Kslots);
+    DBuf<uint4> ctet(P, Kslots);
     DBuf<uint8_t> flag(P, Kslots);
     launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
-                    tejk.get(), off.get(), cval.get(), cseg.get(), ctr.get(), L);
+                    tejk.get(), eleft.get(), ecol.get(), off.get(), ctet.get(),
+                    ctr.get(), L);
+    C2.ctet = ctet.get();
     DBuf<uint32_t> pos;
-    Kalpha = filter_stage(3, C2, Kslots, cval.get(), cseg.get(), flag.get(),
+    Kalpha = filter_stage(3, C2, Kslots, nullptr, nullptr, flag.get(),
                           KP_TET_FILL, pos);
     tets = DBuf<uint4>(P, Kalpha);
-    launch_compact_flat(3, Kslots, C2, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), nullptr, nullptr, tets.get(), nullptr,
-                        nullptr, nullptr, nullptr, L);
+    launch_compact_flat(3, Kslots, C2, nullptr, nullptr, flag.get(), pos.get(),
+                        nullptr, nullptr, tets.get(), nullptr, nullptr, nullptr,
+                        nullptr, L);
     KCHK();
   }
   (void)Tcand;
