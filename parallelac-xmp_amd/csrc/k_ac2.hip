@@ -186,10 +186,10 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
  * neighbours, while accepted tets must test every ball of their box. */
 template <class R, bool PHASE_B>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
-    Stage S, int64_t nk, const uint4* tets, uint8_t* flag, Queue q1,
+    Stage S, Cnt c_nk, const uint4* tets, uint8_t* flag, Queue q1,
     uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  int64_t total = PHASE_B ? (int64_t)*nrest : nk;
+  int64_t total = PHASE_B ? (int64_t)*nrest : c_nk.get();
   int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   for (int64_t w = gtid(); w < total; w += stride) {
@@ -260,7 +260,7 @@ __device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
   return tet_pair_code(t, sD, e);
 }
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
-    Stage S, QView q, const uint4* tets, uint8_t* flag,
+    Stage S, QRef q, const uint4* tets, uint8_t* flag,
     Queue q2) {
   int64_t t = gtid();
   StageCnt sc;
@@ -369,7 +369,7 @@ __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
 }
 
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
-    Stage S, QView q, const uint4* tets,
+    Stage S, QRef q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
   int64_t t = gtid();
   unsigned cnt = 0, sos = 0;
@@ -512,11 +512,11 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
  * phase A / phase B split as the tet kernel. */
 template <class R, int KIND, bool PHASE_B>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
-    Stage S, int64_t nitems, const uint32_t* ecol, const uint32_t* eleft,
+    Stage S, Cnt c_nitems, const uint32_t* ecol, const uint32_t* eleft,
     const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  int64_t total = PHASE_B ? (int64_t)*nrest : nitems;
+  int64_t total = PHASE_B ? (int64_t)*nrest : c_nitems.get();
   int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   for (int64_t w = gtid(); w < total; w += stride) {
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
 
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
-    Stage S, QView q, const uint32_t* ecol,
+    Stage S, QRef q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag, Queue q2) {
   int64_t t = gtid();
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
 /* exact triangle / edge pair (or full scan when aux == ~0) */
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
-    Stage S, QView q, const uint32_t* ecol,
+    Stage S, QRef q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag) {
   int64_t t = gtid();
@@ -770,11 +770,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 
 
 /* ---------------- launchers ---------------- */
-void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
+void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, int budget, PipeLaunch L) {
-  if (nk <= 0) return;
-  dim3 gr((unsigned)((nk + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  if (nk.cap <= 0) return;
+  dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   if constexpr (!kBallF64) {
     if (S.staged) {
       hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0,
@@ -793,27 +793,27 @@ void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
                          L.s, S, nk, tets, flag, q1, rest, nrest, budget);
   }
 }
-void launch_tet_ac2_esc(const Stage& S, QView q,
+void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L) {
-  if (!q.size()) return;
-  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((q.size() + kBlock - 1) / kBlock),
+  if (!q.launch_items()) return;
+  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((q.launch_items() + kBlock - 1) / kBlock),
                      dim3(kBlock), 0, L.s, S, q, tets, flag, q2);
 }
-void launch_tet_ac2_exact(const Stage& S, QView q,
+void launch_tet_ac2_exact(const Stage& S, QRef q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L) {
-  if (!q.size()) return;
-  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((q.size() + kBlock - 1) / kBlock),
+  if (!q.launch_items()) return;
+  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((q.launch_items() + kBlock - 1) / kBlock),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
 template <class R, int KIND>
-static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
+static void low_hot2(const Stage& S, Cnt nitems, const uint32_t* ecol,
                      const uint32_t* eleft, const uint32_t* tk,
                      const uint32_t* tedge, const uint8_t* in_flag,
                      uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
                      int budget, PipeLaunch L) {
-  dim3 gr((unsigned)((nitems + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  dim3 gr((unsigned)((nitems.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false>), gr, dim3(kBlock), 0, L.s,
                      S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1, rest,
                      nrest, budget);
@@ -822,13 +822,13 @@ static void low_hot2(const Stage& S, int64_t nitems, const uint32_t* ecol,
                        L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
                        q1, rest, nrest, budget);
 }
-void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
+void launch_low_ac2_hot(const Stage& S, int kind, Cnt nitems,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         const uint8_t* in_flag, uint8_t* flag, Queue q1,
                         uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L) {
-  if (nitems <= 0) return;
+  if (nitems.cap <= 0) return;
   if constexpr (!kBallF64) {
     if (S.staged) {
       if (kind == 0)
@@ -847,12 +847,12 @@ void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
     low_hot2<double, 1>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
                         rest, nrest, budget, L);
 }
-void launch_low_ac2_esc(const Stage& S, int kind, QView q,
+void launch_low_ac2_esc(const Stage& S, int kind, QRef q,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint8_t* flag, Queue q2, PipeLaunch L) {
-  if (!q.size()) return;
-  dim3 gr((q.size() + kBlock - 1) / kBlock);
+  if (!q.launch_items()) return;
+  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_esc<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag, q2);
@@ -860,11 +860,11 @@ void launch_low_ac2_esc(const Stage& S, int kind, QView q,
     hipLaunchKernelGGL(k_low_ac2_esc<1>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag, q2);
 }
-void launch_low_ac2_exact(const Stage& S, int kind, QView q, const uint32_t* ecol,
+void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L) {
-  if (!q.size()) return;
-  dim3 gr((q.size() + kBlock - 1) / kBlock);
+  if (!q.launch_items()) return;
+  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);

@@ -16,41 +16,44 @@
 namespace pac {
 using namespace PAC_V; /* type-independent device helpers */
 
-__global__ __launch_bounds__(kBlock) void k_compact_u4(int64_t n,
+__global__ __launch_bounds__(kBlock) void k_compact_u4(Cnt c_n,
                                                        const uint4* in,
                                                        const uint8_t* flag,
                                                        const uint32_t* pos,
                                                        uint4* out) {
+  const int64_t n = c_n.get();
   int64_t i = gtid();
   if (i >= n || !flag[i]) return;
   out[pos[i]] = in[i];
 }
-void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
+void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
                          const uint32_t* pos, uint4* out, PipeLaunch L) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+  if (n.cap <= 0) return;
+  hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, n, in, flag, pos, out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_ids(int64_t n,
+__global__ __launch_bounds__(kBlock) void k_compact_ids(Cnt c_n,
                                                         const uint8_t* flag,
                                                         const uint32_t* pos,
                                                         uint32_t* out) {
+  const int64_t n = c_n.get();
   int64_t i = gtid();
   if (i >= n || !flag[i]) return;
   out[pos[i]] = (uint32_t)i;
 }
-void launch_compact_ids(int64_t n, const uint8_t* flag, const uint32_t* pos,
+void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
                         uint32_t* out, PipeLaunch L) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+  if (n.cap <= 0) return;
+  hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, n, flag, pos, out);
 }
 
 /* faces of the final tets -> tri_in */
 __global__ __launch_bounds__(kBlock) void k_mark_faces(
-    int64_t nk, const uint4* ftets, const uint64_t* erow, const uint32_t* ecol,
+    Cnt c_nk, const uint4* ftets, const uint64_t* erow, const uint32_t* ecol,
     const uint64_t* trow, const uint32_t* tk, uint8_t* tri_in) {
+  const int64_t nk = c_nk.get();
   int64_t id = gtid();
   if (id >= nk) return;
   uint4 t = ftets[id];
@@ -60,11 +63,11 @@ __global__ __launch_bounds__(kBlock) void k_mark_faces(
   if ((f = find_tri(erow, ecol, trow, tk, t.x, t.z, t.w)) >= 0) tri_in[f] = 1;
   if ((f = find_tri(erow, ecol, trow, tk, t.y, t.z, t.w)) >= 0) tri_in[f] = 1;
 }
-void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
+void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L) {
-  if (nk <= 0) return;
-  hipLaunchKernelGGL(k_mark_faces, dim3((unsigned)((nk + kBlock - 1) / kBlock)),
+  if (nk.cap <= 0) return;
+  hipLaunchKernelGGL(k_mark_faces, dim3((unsigned)((nk.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nk, ftets, erow, ecol, trow, tk,
                      tri_in);
 }
@@ -72,8 +75,9 @@ void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
 /* edges of the final triangles -> edge_in (edge ids recorded at
  * enumeration: (i,j) = tedge, (i,k) = teik, (j,k) = tejk) */
 __global__ __launch_bounds__(kBlock) void k_mark_edges(
-    int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+    Cnt c_nt, const uint32_t* ftri, const uint32_t* tedge,
     const uint32_t* teik, const uint32_t* tejk, uint8_t* edge_in) {
+  const int64_t nt = c_nt.get();
   int64_t id = gtid();
   if (id >= nt) return;
   uint32_t t = ftri[id];
@@ -81,36 +85,38 @@ __global__ __launch_bounds__(kBlock) void k_mark_edges(
   edge_in[teik[t]] = 1;
   edge_in[tejk[t]] = 1;
 }
-void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+void launch_mark_edges(Cnt nt, const uint32_t* ftri, const uint32_t* tedge,
                        const uint32_t* teik, const uint32_t* tejk,
                        uint8_t* edge_in, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_mark_edges, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+  if (nt.cap <= 0) return;
+  hipLaunchKernelGGL(k_mark_edges, dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nt, ftri, tedge, teik, tejk,
                      edge_in);
 }
 
 /* ---- emission in original ids ---- */
-__global__ __launch_bounds__(kBlock) void k_emit_tets(int64_t nk,
+__global__ __launch_bounds__(kBlock) void k_emit_tets(Cnt c_nk,
                                                       const uint4* ftets,
                                                       const uint32_t* perm,
                                                       int4* out) {
+  const int64_t nk = c_nk.get();
   int64_t id = gtid();
   if (id >= nk) return;
   uint4 t = ftets[id];
   out[id] = make_int4((int)perm[t.x], (int)perm[t.y], (int)perm[t.z],
                       (int)perm[t.w]);
 }
-void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
+void launch_emit_tets(Cnt nk, const uint4* ftets, const uint32_t* perm,
                       int32_t* out, PipeLaunch L) {
-  if (nk <= 0) return;
-  hipLaunchKernelGGL(k_emit_tets, dim3((unsigned)((nk + kBlock - 1) / kBlock)),
+  if (nk.cap <= 0) return;
+  hipLaunchKernelGGL(k_emit_tets, dim3((unsigned)((nk.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nk, ftets, perm, (int4*)out);
 }
 __global__ __launch_bounds__(kBlock) void k_emit_tris(
-    int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+    Cnt c_nt, const uint32_t* ftri, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     const uint32_t* perm, int32_t* out) {
+  const int64_t nt = c_nt.get();
   int64_t id = gtid();
   if (id >= nt) return;
   uint32_t t = ftri[id];
@@ -120,28 +126,29 @@ __global__ __launch_bounds__(kBlock) void k_emit_tris(
   out[w + 1] = (int32_t)perm[ecol[e]];
   out[w + 2] = (int32_t)perm[tk[t]];
 }
-void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
                       const uint32_t* eleft, const uint32_t* tk,
                       const uint32_t* tedge, const uint32_t* perm,
                       int32_t* out, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_emit_tris, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+  if (nt.cap <= 0) return;
+  hipLaunchKernelGGL(k_emit_tris, dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nt, ftri, ecol, eleft, tk, tedge,
                      perm, out);
 }
 __global__ __launch_bounds__(kBlock) void k_emit_edges(
-    int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+    Cnt c_ne, const uint32_t* ecol, const uint32_t* eleft,
     const uint8_t* flag, const uint32_t* pos, const uint32_t* perm,
     int2* out) {
+  const int64_t ne = c_ne.get();
   int64_t id = gtid();
   if (id >= ne || !flag[id]) return;
   out[pos[id]] = make_int2((int)perm[eleft[id]], (int)perm[ecol[id]]);
 }
-void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint32_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L) {
-  if (ne <= 0) return;
-  hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+  if (ne.cap <= 0) return;
+  hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, ecol, eleft, flag, pos, perm,
                      (int2*)out);
 }

@@ -372,7 +372,8 @@ void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
 __global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
                                                       const uint64_t* off,
                                                       uint32_t* cval,
-                                                      uint32_t* cseg) {
+                                                      uint32_t* cseg,
+                                                      uint64_t lim) {
   int64_t i = gtid();
   if (i >= S.n) return;
   const GridP g = grid_of(S, (uint32_t)i);
@@ -389,44 +390,46 @@ __global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
       if (c1 < (int64_t)c) continue;
       uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
       if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
-      for (uint32_t j = b; j < e; ++j, ++o) {
+      for (uint32_t j = b; j < e && o < lim; ++j, ++o) {
         cval[o] = j;
         cseg[o] = (uint32_t)i;
       }
     }
 }
 void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
-                      uint32_t* cseg, PipeLaunch L) {
+                      uint32_t* cseg, PipeLaunch L, uint64_t lim) {
   if (S.n <= 0) return;
   hipLaunchKernelGGL(k_edge_enum, dim3((unsigned)((S.n + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, S, off, cval, cseg);
+                     dim3(kBlock), 0, L.s, S, off, cval, cseg, lim);
 }
 
 /* Slot capacities that need no merge: a triangle from alpha edge (i,j) has
  * its third vertex in row(i) after j, a tet from alpha triangle (i,j,k) has
  * its fourth vertex in T(i,j) after k.  Unused slots carry seg = ~0. */
-__global__ __launch_bounds__(kBlock) void k_tail_caps(int64_t m,
+__global__ __launch_bounds__(kBlock) void k_tail_caps(Cnt c_m,
                                                       const uint64_t* row,
                                                       const uint32_t* owner,
                                                       uint32_t* cap) {
+  const int64_t m = c_m.get();
   int64_t q = gtid();
   if (q >= m) return;
   cap[q] = (uint32_t)(row[owner[q] + 1] - (uint64_t)q - 1);
 }
-void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
+void launch_tail_caps(Cnt m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L) {
-  if (m <= 0) return;
-  hipLaunchKernelGGL(k_tail_caps, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+  if (m.cap <= 0) return;
+  hipLaunchKernelGGL(k_tail_caps, dim3((unsigned)((m.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, m, row, owner, cap);
 }
 
 /* number of triangles of alpha edge q = (i,j): the size of row(i) after j
  * intersected with row(j) (computePossibleTriangleCount, Parallel:232-266) */
-__global__ __launch_bounds__(kBlock) void k_tri_count(int64_t ne,
+__global__ __launch_bounds__(kBlock) void k_tri_count(Cnt c_ne,
                                                      const uint64_t* erow,
                                                      const uint32_t* ecol,
                                                      const uint32_t* eleft,
                                                      uint32_t* cnt) {
+  const int64_t ne = c_ne.get();
   int64_t q = gtid();
   if (q >= ne) return;
   uint32_t i = eleft[q], j = ecol[q];
@@ -441,10 +444,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(int64_t ne,
   }
   cnt[q] = c;
 }
-void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, uint32_t* cnt, PipeLaunch L) {
-  if (ne <= 0) return;
-  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+  if (ne.cap <= 0) return;
+  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
 }
 
@@ -452,14 +455,17 @@ void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
  * positions of k in row(i) and row(j) are the alpha-edge ids of (i,k) and
  * (j,k), recorded for the tet stage and the closure. */
 __global__ __launch_bounds__(kBlock) void k_tri_enum(
-    int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+    Cnt c_ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-    uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr) {
+    uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr,
+    uint64_t lim) {
+  const int64_t ne = c_ne.get();
   constexpr int CAP = 2048;
   __shared__ uint32_t s_out[4 * CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  if (q0 >= ne) return; /* whole block past the (device) count */
   const int64_t q1 = q0 + kBlock < ne ? q0 + kBlock : ne;
-  BlockOut<4, CAP> B{s_out, off[q0], off[q1] - off[q0]};
+  BlockOut<4, CAP> B{s_out, off[q0], off[q1] - off[q0], lim};
   uint32_t* const g[4] = {cval, cseg, cea, ceb};
   int64_t q = gtid();
   unsigned found = 0;
@@ -483,14 +489,14 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(
   B.flush(g);
   block_flush(ctr, CTR_CAND + 1, found);
 }
-void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                     unsigned long long* ctr, PipeLaunch L) {
-  if (ne <= 0) return;
-  hipLaunchKernelGGL(k_tri_enum, dim3((unsigned)((ne + kBlock - 1) / kBlock)),
+                     unsigned long long* ctr, PipeLaunch L, uint64_t lim) {
+  if (ne.cap <= 0) return;
+  hipLaunchKernelGGL(k_tri_enum, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, off, cval,
-                     cseg, cea, ceb, ctr);
+                     cseg, cea, ceb, ctr, lim);
 }
 
 /* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k)
@@ -499,15 +505,17 @@ void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
  * the compaction read one uint4 instead of following triangle -> edge ->
  * vertex links */
 __global__ __launch_bounds__(kBlock) void k_tet_enum(
-    int64_t nt, const uint64_t* trow, const uint32_t* tk,
+    Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
-    uint4* ctet, unsigned long long* ctr) {
+    uint4* ctet, unsigned long long* ctr, uint64_t lim) {
+  const int64_t nt = c_nt.get();
   constexpr int CAP = 2048;
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  if (q0 >= nt) return; /* whole block past the (device) count */
   const int64_t q1 = q0 + kBlock < nt ? q0 + kBlock : nt;
-  BlockOut4<CAP> B{s_out, off[q0], off[q1] - off[q0]};
+  BlockOut4<CAP> B{s_out, off[q0], off[q1] - off[q0], lim};
   int64_t q = gtid();
   unsigned found = 0;
   if (q < nt) {
@@ -538,15 +546,15 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum(
   B.flush(ctet);
   block_flush(ctr, CTR_CAND + 2, found);
 }
-void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
+void launch_tet_enum(Cnt nt, const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint32_t* teik,
                      const uint32_t* tejk, const uint32_t* eleft,
                      const uint32_t* ecol, const uint64_t* off, uint4* ctet,
-                     unsigned long long* ctr, PipeLaunch L) {
-  if (nt <= 0) return;
-  hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt + kBlock - 1) / kBlock)),
+                     unsigned long long* ctr, PipeLaunch L, uint64_t lim) {
+  if (nt.cap <= 0) return;
+  hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk,
-                     eleft, ecol, off, ctet, ctr);
+                     eleft, ecol, off, ctet, ctr, lim);
 }
 
 /* ------------------------------------------------------------------ */
@@ -592,12 +600,13 @@ constexpr int fam_of(int KIND) {
 
 template <int KIND, class R>
 __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
-                                                   int64_t nc,
+                                                   Cnt c_nc,
                                                    const uint32_t* cval,
                                                    const uint32_t* cseg,
                                                    uint8_t* flag, Queue q1) {
   /* one evaluation per candidate: the host books nc evaluations at this
    * stage, no counter traffic here */
+  const int64_t nc = c_nc.get();
   int64_t o = gtid();
   if (o < nc) {
     const uint32_t seg = KIND == 3 ? C.ctet[o].x : cseg[o];
@@ -614,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
 }
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
-                                                       QView q,
+                                                       QRef q,
                                                        const uint32_t* cval,
                                                        uint8_t* flag,
                                                        Queue q2) {
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
 }
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
-                                                         QView q,
+                                                         QRef q,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
   int64_t t = gtid();
@@ -654,11 +663,11 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
   flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
 
-void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
+void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
                    Queue q1, PipeLaunch L) {
-  if (nc <= 0) return;
-  dim3 gr((unsigned)((nc + kBlock - 1) / kBlock));
+  if (nc.cap <= 0) return;
+  dim3 gr((unsigned)((nc.cap + kBlock - 1) / kBlock));
 #define PAC_FILTER(K)                                                        \
   if (S.staged)                                                              \
     hipLaunchKernelGGL((k_filter<K, float>), gr, dim3(kBlock), 0, L.s, S, C, \
@@ -675,10 +684,10 @@ void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
   }
 #undef PAC_FILTER
 }
-void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L) {
-  if (!q.size()) return;
-  dim3 gr((q.size() + kBlock - 1) / kBlock);
+  if (!q.launch_items()) return;
+  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_esc<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
@@ -689,10 +698,10 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint3
     hipLaunchKernelGGL(k_filter_esc<3>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
 }
-void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L) {
-  if (!q.size()) return;
-  dim3 gr((q.size() + kBlock - 1) / kBlock);
+  if (!q.launch_items()) return;
+  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);
@@ -708,28 +717,32 @@ void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uin
 /* survivors: segment starts and flat stable compaction                */
 /* ------------------------------------------------------------------ */
 /* orow[s] = number of survivors before segment s = pos[off[s]] */
-__global__ __launch_bounds__(kBlock) void k_seg_starts(int64_t nseg,
+__global__ __launch_bounds__(kBlock) void k_seg_starts(Cnt c_nseg,
                                                        const uint64_t* off,
                                                        const uint32_t* pos,
-                                                       uint64_t* orow) {
+                                                       uint64_t* orow,
+                                                       uint64_t poslim) {
+  const int64_t nseg = c_nseg.get();
   int64_t s = gtid();
   if (s > nseg) return;
-  orow[s] = pos[off[s]];
+  const uint64_t o = off[s];
+  orow[s] = pos[o < poslim ? o : poslim];
 }
-void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L) {
+void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
+                       uint64_t* orow, PipeLaunch L, uint64_t poslim) {
   hipLaunchKernelGGL(k_seg_starts,
-                     dim3((unsigned)((nseg + 1 + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nseg, off, pos, orow);
+                     dim3((unsigned)((nseg.cap + 1 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nseg, off, pos, orow, poslim);
 }
 /* KIND 1: (ecol, eleft) = (val, seg); 2: (tk, tedge) = (val, seg);
  * 3: tets = (i, j, k, l) */
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_compact_flat(
-    int64_t nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
+    Cnt c_nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
     const uint8_t* flag, const uint32_t* pos, uint32_t* out_val,
     uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
     const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b) {
+  const int64_t nc = c_nc.get();
   int64_t o = gtid();
   if (o >= nc || flag[o] != KEEP) return;
   uint64_t w = pos[o];
@@ -744,14 +757,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
     out_tet[w] = C.ctet[o];
   }
 }
-void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
+void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
                          const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
                          const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
                          PipeLaunch L) {
-  if (nc <= 0) return;
-  dim3 gr((unsigned)((nc + kBlock - 1) / kBlock));
+  if (nc.cap <= 0) return;
+  dim3 gr((unsigned)((nc.cap + kBlock - 1) / kBlock));
   if (kind == 1)
     hipLaunchKernelGGL(k_compact_flat<1>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
                        cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,

@@ -251,10 +251,11 @@ void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
  * first id of a row lies in its molecule's id range, so "first id >=
  * moff[m]" is monotone over the rows: out[m] = first row of molecule m. */
 __global__ __launch_bounds__(kBlock) void k_mol_rows(const int32_t* rows,
-                                                    int64_t m, int K,
+                                                    Cnt c_m, int K,
                                                     const uint32_t* moff,
                                                     int64_t nmol,
                                                     uint64_t* out) {
+  const int64_t m = c_m.get();
   int64_t q = gtid();
   if (q > nmol) return;
   uint32_t key = moff[q];
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_mol_rows(const int32_t* rows,
   }
   out[q] = (uint64_t)lo;
 }
-void launch_mol_rows(const int32_t* rows, int64_t m, int K,
+void launch_mol_rows(const int32_t* rows, Cnt m, int K,
                      const uint32_t* moff, int64_t nmol, uint64_t* out,
                      hipStream_t s) {
   hipLaunchKernelGGL(k_mol_rows, dim3((unsigned)((nmol + 1 + kBlock - 1) / kBlock)),

@@ -53,8 +53,9 @@ __device__ __forceinline__ uint64_t load_val(const TIn* in, int64_t i,
  * (coalesced), and transposed through LDS so that each thread scans
  * kScanItems consecutive items. */
 template <class TIn, bool kFlag>
-__global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, int64_t n,
+__global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, Cnt cn,
                                                       uint64_t* sums) {
+  const int64_t n = cn.get();
   int64_t base = (int64_t)blockIdx.x * kTile;
   uint64_t s = 0;
 #pragma unroll
@@ -66,10 +67,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_sums(const TIn* in, int64_t n,
 }
 
 template <class TIn, bool kFlag, class TOut>
-__global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
+__global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, Cnt cn,
                                                       const uint64_t* tile_off,
-                                                      TOut* out) {
+                                                      TOut* out,
+                                                      uint64_t* total64) {
   __shared__ uint64_t tile[kTile];
+  const int64_t n = cn.get();
   int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q) {
@@ -97,8 +100,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_scan(const TIn* in, int64_t n,
     int t = q * kBlock + threadIdx.x;
     if (base + t < n) out[base + t] = (TOut)tile[t];
   }
-  /* the last thread of the last tile writes the grand total at out[n] */
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = (TOut)ex;
+  /* the tile holding position n (the last tile when n ends a tile) writes
+   * the grand total at out[n] */
+  const bool holds = (n >= base && n < base + kTile) ||
+                     (blockIdx.x == gridDim.x - 1 && n == base + kTile);
+  if (holds && threadIdx.x == kBlock - 1) {
+    out[n] = (TOut)ex;
+    if (total64) *total64 = ex;
+  }
 }
 
 size_t scan_tmp_bytes(int64_t n) {
@@ -120,16 +129,19 @@ static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
                          uint64_t* tmp, hipStream_t s);
 
 template <class TIn, bool kFlag, class TOut = uint64_t>
-static void scan_impl(const TIn* in, TOut* out, int64_t n, uint64_t* tmp,
-                      hipStream_t s) {
-  if (n <= 0) {
+static void scan_impl(const TIn* in, TOut* out, Cnt n, uint64_t* tmp,
+                      hipStream_t s, uint64_t* total64 = nullptr) {
+  if (n.cap <= 0) {
     hipLaunchKernelGGL(k_store<TOut>, dim3(1), dim3(1), 0, s, out, (TOut)0);
+    if (total64)
+      hipLaunchKernelGGL(k_store<uint64_t>, dim3(1), dim3(1), 0, s, total64,
+                         (uint64_t)0);
     return;
   }
-  int64_t tiles = (n + kTile - 1) / kTile;
+  int64_t tiles = (n.cap + kTile - 1) / kTile;
   if (tiles == 1) {
     hipLaunchKernelGGL((k_tile_scan<TIn, kFlag, TOut>), dim3(1), dim3(kBlock),
-                       0, s, in, n, (const uint64_t*)nullptr, out);
+                       0, s, in, n, (const uint64_t*)nullptr, out, total64);
     return;
   }
   uint64_t* sums = tmp;
@@ -138,7 +150,8 @@ static void scan_impl(const TIn* in, TOut* out, int64_t n, uint64_t* tmp,
                      dim3(kBlock), 0, s, in, n, sums);
   scan_u64_rec(sums, offs, tiles, tmp + 2 * (tiles + 1), s);
   hipLaunchKernelGGL((k_tile_scan<TIn, kFlag, TOut>), dim3((unsigned)tiles),
-                     dim3(kBlock), 0, s, in, n, (const uint64_t*)offs, out);
+                     dim3(kBlock), 0, s, in, n, (const uint64_t*)offs, out,
+                     total64);
 }
 
 static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
@@ -146,13 +159,13 @@ static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
   scan_impl<uint64_t, false>(in, out, n, tmp, s);
 }
 
-void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n, void* tmp,
-                     size_t, hipStream_t s) {
-  scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s);
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, Cnt n, void* tmp,
+                     size_t, hipStream_t s, uint64_t* total64) {
+  scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s, total64);
 }
-void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, int64_t n, void* tmp,
-                        size_t, hipStream_t s) {
-  scan_impl<uint8_t, true, uint32_t>(in, out, n, (uint64_t*)tmp, s);
+void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, Cnt n, void* tmp,
+                        size_t, hipStream_t s, uint64_t* total64) {
+  scan_impl<uint8_t, true, uint32_t>(in, out, n, (uint64_t*)tmp, s, total64);
 }
 
 /* ------------------------------------------------------------------ */

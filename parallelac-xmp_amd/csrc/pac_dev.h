@@ -75,8 +75,10 @@ template <int NARR, int CAP>
 struct BlockOut {
   uint32_t* s; /* LDS, NARR x CAP */
   uint64_t base, total;
+  uint64_t lim; /* capacity of the output arrays: entries at or past it are dropped */
   __device__ __forceinline__ void put(uint32_t* const g[NARR], uint64_t o,
                                       const uint32_t v[NARR]) {
+    if (o >= lim) return;
     uint64_t lo = o - base;
     if (lo < (uint64_t)CAP) {
 #pragma unroll
@@ -89,7 +91,10 @@ struct BlockOut {
   /* every thread of the block calls this once, after its last put */
   __device__ __forceinline__ void flush(uint32_t* const g[NARR]) {
     __syncthreads();
-    const uint32_t m = (uint32_t)(total < (uint64_t)CAP ? total : (uint64_t)CAP);
+    uint64_t m64 = total < (uint64_t)CAP ? total : (uint64_t)CAP;
+    if (base >= lim) m64 = 0;
+    else if (m64 > lim - base) m64 = lim - base;
+    const uint32_t m = (uint32_t)m64;
     for (uint32_t t = threadIdx.x; t < m; t += blockDim.x)
 #pragma unroll
       for (int a = 0; a < NARR; ++a) g[a][base + t] = s[a * CAP + t];
@@ -100,15 +105,19 @@ struct BlockOut {
 template <int CAP>
 struct BlockOut4 {
   uint4* s; /* LDS, CAP */
-  uint64_t base, total;
+  uint64_t base, total, lim;
   __device__ __forceinline__ void put(uint4* g, uint64_t o, const uint4& v) {
+    if (o >= lim) return;
     uint64_t lo = o - base;
     if (lo < (uint64_t)CAP) s[lo] = v;
     else g[o] = v;
   }
   __device__ __forceinline__ void flush(uint4* g) {
     __syncthreads();
-    const uint32_t m = (uint32_t)(total < (uint64_t)CAP ? total : (uint64_t)CAP);
+    uint64_t m64 = total < (uint64_t)CAP ? total : (uint64_t)CAP;
+    if (base >= lim) m64 = 0;
+    else if (m64 > lim - base) m64 = lim - base;
+    const uint32_t m = (uint32_t)m64;
     for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) g[base + t] = s[t];
   }
 };

@@ -144,6 +144,9 @@ struct pac_result {
   pac::DBuf<double4> balls;        /* original order, r = r_raw + s (as computed) */
   double alpha = 0;
   pac_stats st;
+  /* sizes seen by the run (sizing a recorded graph): candidate / survivor
+   * counts of each stage, final counts, largest sub-queue fills */
+  int64_t sizes[12] = {0};
 };
 
 namespace pac {
@@ -228,7 +231,22 @@ void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void zero_u32(unsigned* p, int n, hipStream_t s);
 void zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 
-struct GridPlan;
+namespace PAC_V {
+/* grids, exact scale and device molecule tables of one pipeline pass */
+struct PassPlan {
+  std::vector<GridP> grids;
+  int64_t G = 0;
+  int S = 0;
+  WI alpha_wi;
+  bool no_edges = true;
+  int k_enum = 0, k_ac2 = 0;
+  DBuf<uint32_t> dmoff, dmol;
+  DBuf<GridP> dgrids;
+};
+void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
+               Real alpha, const pac_options& opt, hipStream_t s,
+               const int64_t* moff, int64_t nmol, PassPlan& pl);
+}  // namespace PAC_V
 namespace vf32 {
 void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                   float alpha, const pac_options& opt, pac_result* R,

@@ -43,6 +43,22 @@ struct Bounds {
   int nonfinite;           /* count of non-finite inputs */
 };
 
+/* Item count of a launch.  The host sizes the grid for `cap` items; when
+ * `d` is set the actual count is *d, read on the device, which lets a whole
+ * pipeline run (and be captured as one graph) without reading counts back.
+ * Converts from a plain count. */
+struct Cnt {
+  int64_t cap;
+  const uint64_t* d;
+  __host__ __device__ Cnt(int64_t n = 0) : cap(n), d(nullptr) {}
+  __host__ __device__ Cnt(int64_t c, const uint64_t* p) : cap(c), d(p) {}
+  __device__ int64_t get() const {
+    if (!d) return cap;
+    const int64_t v = (int64_t)*d;
+    return v < cap ? v : cap;
+  }
+};
+
 struct QEntry {
   uint32_t item;  /* candidate / simplex index */
   uint32_t aux;   /* segment (ball / edge / triangle) or ball index */
@@ -62,11 +78,14 @@ enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
 
 /* ---- primitives (k_prims.hip) ---- */
 /* exclusive scan; out has n+1 entries, out[n] = total */
-void scan_u32_to_u64(const uint32_t* in, uint64_t* out, int64_t n,
-                     void* tmp, size_t tmp_bytes, hipStream_t s);
+/* total64 (optional, device): also receives the total */
+void scan_u32_to_u64(const uint32_t* in, uint64_t* out, Cnt n, void* tmp,
+                     size_t tmp_bytes, hipStream_t s,
+                     uint64_t* total64 = nullptr);
 /* flags (nonzero = 1) -> u32 positions; n < 2^32 */
-void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, int64_t n,
-                        void* tmp, size_t tmp_bytes, hipStream_t s);
+void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, Cnt n, void* tmp,
+                        size_t tmp_bytes, hipStream_t s,
+                        uint64_t* total64 = nullptr);
 size_t scan_tmp_bytes(int64_t n);
 /* stable LSD radix sort of (key, val) by the low `bits` bits of key */
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
@@ -105,6 +124,17 @@ struct QView {
     return q[(uint64_t)lo * cap + (i - off[lo])];
   }
 };
+/* what a consumer kernel gets: the view by value (built on the host), or a
+ * pointer to one built on the device (graph-captured pipelines) and the
+ * number of items its grid covers */
+struct QRef {
+  QView v;
+  const QView* dv;
+  uint32_t bound;
+  __device__ uint32_t size() const { return dv ? dv->size() : v.size(); }
+  __device__ QEntry at(uint32_t i) const { return dv ? dv->at(i) : v.at(i); }
+  uint32_t launch_items() const { return dv ? bound : v.size(); }
+};
 
 struct CandCtx {
   const uint32_t *eleft, *ecol, *tk, *tedge;
@@ -112,23 +142,23 @@ struct CandCtx {
 };
 
 /* ---- closure + emission (k_closure.hip) ---- */
-void launch_compact_tets(int64_t n, const uint4* in, const uint8_t* flag,
+void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
                          const uint32_t* pos, uint4* out, PipeLaunch L);
-void launch_compact_ids(int64_t n, const uint8_t* flag, const uint32_t* pos,
+void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
                         uint32_t* out, PipeLaunch L);
-void launch_mark_faces(int64_t nk, const uint4* ftets, const uint64_t* erow,
+void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
-void launch_mark_edges(int64_t nt, const uint32_t* ftri, const uint32_t* tedge,
+void launch_mark_edges(Cnt nt, const uint32_t* ftri, const uint32_t* tedge,
                        const uint32_t* teik, const uint32_t* tejk,
                        uint8_t* edge_in, PipeLaunch L);
-void launch_emit_tets(int64_t nk, const uint4* ftets, const uint32_t* perm,
+void launch_emit_tets(Cnt nk, const uint4* ftets, const uint32_t* perm,
                       int32_t* out, PipeLaunch L);
-void launch_emit_tris(int64_t nt, const uint32_t* ftri, const uint32_t* ecol,
+void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
                       const uint32_t* eleft, const uint32_t* tk,
                       const uint32_t* tedge, const uint32_t* perm,
                       int32_t* out, PipeLaunch L);
-void launch_emit_edges(int64_t ne, const uint32_t* ecol, const uint32_t* eleft,
+void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint32_t* pos,
                        const uint32_t* perm, int32_t* out, PipeLaunch L);
 
@@ -153,7 +183,7 @@ void canonical_rows(int K, int64_t m, int32_t* rows, int idbits,
 /* batch: first row of each molecule in a list of m rows of K original ids
  * grouped by molecule (moff: first original id of each molecule);
  * out[nmol+1] */
-void launch_mol_rows(const int32_t* rows, int64_t m, int K,
+void launch_mol_rows(const int32_t* rows, Cnt m, int K,
                      const uint32_t* moff, int64_t nmol, uint64_t* out,
                      hipStream_t s);
 
@@ -206,34 +236,35 @@ struct Stage {
  * ball (edges), alpha edge (triangles), alpha triangle (tets). */
 
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
+/* lim: capacity of the candidate arrays (entries past it are dropped) */
 void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
-                      uint32_t* cseg, PipeLaunch L);
+                      uint32_t* cseg, PipeLaunch L, uint64_t lim = ~0ull);
 /* slot capacity of item q = row[owner[q] + 1] - q - 1 */
-void launch_tail_caps(int64_t m, const uint64_t* row, const uint32_t* owner,
+void launch_tail_caps(Cnt m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L);
 /* exact triangle count per alpha edge */
-void launch_tri_count(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, uint32_t* cnt, PipeLaunch L);
-void launch_tri_enum(int64_t ne, const uint64_t* erow, const uint32_t* ecol,
+void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                     unsigned long long* ctr, PipeLaunch L);
-void launch_tet_enum(int64_t nt, const uint64_t* trow, const uint32_t* tk,
+                     unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
+void launch_tet_enum(Cnt nt, const uint64_t* trow, const uint32_t* tk,
                      const uint32_t* tedge, const uint32_t* teik,
                      const uint32_t* tejk, const uint32_t* eleft,
                      const uint32_t* ecol, const uint64_t* off, uint4* ctet,
-                     unsigned long long* ctr, PipeLaunch L);
+                     unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
 /* kind 1 edges, 2 triangles, 3 tets */
-void launch_filter(const Stage& S, int kind, CandCtx C, int64_t nc,
+void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
                    Queue q1, PipeLaunch L);
-void launch_filter_esc(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
+void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L);
-void launch_filter_exact(const Stage& S, int kind, CandCtx C, QView q, const uint32_t* cval, uint8_t* flag,
+void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L);
-void launch_seg_starts(int64_t nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L);
-void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
+void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
+                       uint64_t* orow, PipeLaunch L, uint64_t poslim = ~0ull);
+void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
                          const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
@@ -243,27 +274,27 @@ void launch_compact_flat(int kind, int64_t nc, CandCtx C, const uint32_t* cval,
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =
  * number of items); budget = phase A pair budget (0: single phase) */
-void launch_tet_ac2_hot(const Stage& S, int64_t nk, const uint4* tets,
+void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, int budget, PipeLaunch L);
-void launch_tet_ac2_esc(const Stage& S, QView q,
+void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L);
-void launch_tet_ac2_exact(const Stage& S, QView q,
+void launch_tet_ac2_exact(const Stage& S, QRef q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L);
 /* kind 0 triangles, 1 edges */
-void launch_low_ac2_hot(const Stage& S, int kind, int64_t nitems,
+void launch_low_ac2_hot(const Stage& S, int kind, Cnt nitems,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         const uint8_t* in_flag, uint8_t* flag, Queue q1,
                         uint32_t* rest, unsigned* nrest, int budget,
                         PipeLaunch L);
-void launch_low_ac2_esc(const Stage& S, int kind, QView q,
+void launch_low_ac2_esc(const Stage& S, int kind, QRef q,
                         const uint32_t* ecol, const uint32_t* eleft,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint8_t* flag, Queue q2, PipeLaunch L);
-void launch_low_ac2_exact(const Stage& S, int kind, QView q, const uint32_t* ecol,
+void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L);
 

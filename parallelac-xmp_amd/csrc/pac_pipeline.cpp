@@ -109,25 +109,15 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
   return P;
 }
 
-/* One pipeline pass over n balls.  nmol == 1: one molecule; otherwise the
- * balls are nmol molecules, moff[0..nmol] (host, relative to raw), each with
- * its own grid, processed together by every kernel launch. */
-void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
-                  Real alpha, const pac_options& opt, pac_result* R,
-                  hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
-                  int64_t nmol) {
+/* Grids of one pass (bounds reduced on the device, read back once), the
+ * exact scale and the device-side molecule tables of a batch pass. */
+void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
+               Real alpha, const pac_options& opt, hipStream_t s,
+               const int64_t* moff, int64_t nmol, PassPlan& pl) {
   Pool* P = &E->pool;
   PipeLaunch L{s, kBlock};
-  pac_stats& st = R->st;
-  st.n = n;
-  if (n == 0) return;
-  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
   const bool batch = nmol > 1;
-
-  /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
   std::vector<Bounds> hb(std::max<int64_t>(nmol, 1));
-  DBuf<uint32_t> dmoff, dmol;
-  DBuf<GridP> dgrids;
   if (!batch) {
     DBuf<Bounds> bpart(P, 1024 * 2 + 4), bout(P, 1);
     launch_bounds(raw, n, solvent, bpart.get(), bout.get(), L);
@@ -136,22 +126,22 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   } else {
     std::vector<uint32_t> mo(nmol + 1);
     for (int64_t m = 0; m <= nmol; ++m) mo[m] = (uint32_t)moff[m];
-    dmoff = DBuf<uint32_t>(P, nmol + 1);
-    HIPCHK(hipMemcpyAsync(dmoff.get(), mo.data(), (nmol + 1) * sizeof(uint32_t),
+    pl.dmoff = DBuf<uint32_t>(P, nmol + 1);
+    HIPCHK(hipMemcpyAsync(pl.dmoff.get(), mo.data(), (nmol + 1) * sizeof(uint32_t),
                           hipMemcpyHostToDevice, s));
     DBuf<Bounds> bout(P, nmol);
-    launch_bounds_mol(raw, dmoff.get(), nmol, solvent, bout.get(), L);
+    launch_bounds_mol(raw, pl.dmoff.get(), nmol, solvent, bout.get(), L);
     KCHK();
     HIPCHK(hipMemcpyAsync(hb.data(), bout.get(), nmol * sizeof(Bounds),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   }
-  std::vector<GridP> grids(hb.size());
-  int64_t G = 0;
+  pl.grids.assign(hb.size(), GridP());
+  pl.G = 0;
   int shift = 0;
   double mag = 1.0;
-  bool no_edges = true;
-  st.k_enum = st.k_ac2 = 0;
+  pl.no_edges = true;
+  pl.k_enum = pl.k_ac2 = 0;
   for (size_t m = 0; m < hb.size(); ++m) {
     const Bounds& B = hb[m];
     if (B.nonfinite) fail(PAC_ERR_INVALID, "non-finite coordinate or radius");
@@ -168,42 +158,68 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       gp = plan_grid(B, alpha, opt.cell_policy);
       shift = std::max(shift, B.shift);
       mag = std::max({mag, (double)B.max_abs, (double)B.max_r});
-      if ((double)B.max_r * (double)B.max_r + (double)alpha >= 0) no_edges = false;
+      if ((double)B.max_r * (double)B.max_r + (double)alpha >= 0) pl.no_edges = false;
     }
-    gp.g.cbase = (uint32_t)G;
-    G += gp.G;
-    if (G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grids have too many cells");
-    grids[m] = gp.g;
-    st.k_enum = std::max(st.k_enum, gp.g.k_enum);
-    st.k_ac2 = std::max(st.k_ac2, gp.g.k_ac2);
+    gp.g.cbase = (uint32_t)pl.G;
+    pl.G += gp.G;
+    if (pl.G >= ((int64_t)1 << 31)) fail(PAC_ERR_RANGE, "grids have too many cells");
+    pl.grids[m] = gp.g;
+    pl.k_enum = std::max(pl.k_enum, gp.g.k_enum);
+    pl.k_ac2 = std::max(pl.k_ac2, gp.g.k_ac2);
   }
-  st.grid_cells = (int32_t)G;
-
   /* exact embedding scale (one for all molecules: any common scale at
    * which every input is an integer gives the same exact signs) */
-  int S = std::max(shift, (neg_lsb_exp_host(alpha) + 1) / 2);
+  pl.S = std::max(shift, (neg_lsb_exp_host(alpha) + 1) / 2);
   mag = std::max(mag, std::sqrt(std::fabs((double)alpha)));
-  double bits = std::log2(mag) + S + 2;
+  double bits = std::log2(mag) + pl.S + 2;
   if (8 * bits + 16 > 32.0 * WL - 2)
-    fail(PAC_ERR_RANGE, "input exponent range exceeds the 512-bit exact stage");
+    fail(PAC_ERR_RANGE, "input exponent range exceeds the exact stage");
+  pl.alpha_wi = wi_from_scaled_host((double)alpha, 2 * pl.S);
+  if (batch) {
+    pl.dgrids = DBuf<GridP>(P, nmol);
+    HIPCHK(hipMemcpyAsync(pl.dgrids.get(), pl.grids.data(), nmol * sizeof(GridP),
+                          hipMemcpyHostToDevice, s));
+    pl.dmol = DBuf<uint32_t>(P, n);
+    launch_mol_ids(pl.dmoff.get(), nmol, n, pl.dmol.get(), L);
+    KCHK();
+  }
+}
+
+/* One pipeline pass over n balls.  nmol == 1: one molecule; otherwise the
+ * balls are nmol molecules, moff[0..nmol] (host, relative to raw), each with
+ * its own grid, processed together by every kernel launch. */
+void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
+                  Real alpha, const pac_options& opt, pac_result* R,
+                  hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
+                  int64_t nmol) {
+  Pool* P = &E->pool;
+  PipeLaunch L{s, kBlock};
+  pac_stats& st = R->st;
+  st.n = n;
+  if (n == 0) return;
+  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
+
+  /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
+  PassPlan pl;
+  plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, pl);
+  const int64_t G = pl.G;
+  const int S = pl.S;
+  const bool no_edges = pl.no_edges;
+  st.k_enum = pl.k_enum;
+  st.k_ac2 = pl.k_ac2;
+  st.grid_cells = (int32_t)G;
   st.scale_bits = S;
   Stage SG;
   std::memset(&SG, 0, sizeof(SG));
   SG.n = n;
-  SG.g = grids[0];
-  if (batch) {
-    dgrids = DBuf<GridP>(P, nmol);
-    HIPCHK(hipMemcpyAsync(dgrids.get(), grids.data(), nmol * sizeof(GridP),
-                          hipMemcpyHostToDevice, s));
-    dmol = DBuf<uint32_t>(P, n);
-    launch_mol_ids(dmoff.get(), nmol, n, dmol.get(), L);
-    KCHK();
-    SG.grids = dgrids.get();
-    SG.mol = dmol.get();
+  SG.g = pl.grids[0];
+  if (nmol > 1) {
+    SG.grids = pl.dgrids.get();
+    SG.mol = pl.dmol.get();
   }
   SG.alpha = alpha;
   SG.ex.S = S;
-  SG.ex.alpha = wi_from_scaled_host((double)alpha, 2 * S);
+  SG.ex.alpha = pl.alpha_wi;
   /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
   SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
   SG.bgrid = opt.ac2_grid > 0 ? std::min(opt.ac2_grid, 1 << 16) : 2048;
@@ -280,6 +296,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     }
     v->off[QSLOTS] = (uint32_t)acc;
     *need = mx;
+    int64_t& fill = R->sizes[(dcnt == qn.get()) ? 9 : 10];
+    fill = std::max<int64_t>(fill, mx);
     return mx <= cap;
   };
   auto grow = [](uint32_t need) {
@@ -309,14 +327,14 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
         for (;;) {
           q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
           zero_u32(qn.get() + QSLOTS, QSLOTS, s);
-          esc(v1, Queue{q2.get(), qn.get() + QSLOTS, c2});
+          esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn.get() + QSLOTS, c2});
           KCHK();
           if (view_of(q2.get(), c2, qn.get() + QSLOTS, &vx, &need)) break;
           c2 = grow(need);
         }
       }
       if (kp_exact >= 0) KP.begin(kp_exact);
-      exact(vx);
+      exact(QRef{vx, nullptr, 0});
       if (kp_exact >= 0) KP.end();
       KCHK();
       return;
@@ -336,10 +354,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     ladder(
         std::min<int64_t>(nc, nc / 32 + 65536), kp, -1,
         [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
-        [&](QView q, Queue q2) {
+        [&](QRef q, Queue q2) {
           launch_filter_esc(SG, kind, C, q, cval, flag, q2, L);
         },
-        [&](QView q) {
+        [&](QRef q) {
           launch_filter_exact(SG, kind, C, q, cval, flag, L);
         });
     pos = DBuf<uint32_t>(P, nc + 1);
@@ -482,10 +500,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
                              nrest, budget, L);
         },
-        [&](QView q, Queue q2) {
+        [&](QRef q, Queue q2) {
           launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);
         },
-        [&](QView q) {
+        [&](QRef q) {
           launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
         });
   }
@@ -518,11 +536,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                              tedge.get(), tri_in.get(), tflag.get(), q,
                              rest.get(), nrest, budget, L);
         },
-        [&](QView q, Queue q2) {
+        [&](QRef q, Queue q2) {
           launch_low_ac2_esc(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
                              tedge.get(), tflag.get(), q2, L);
         },
-        [&](QView q) {
+        [&](QRef q) {
           launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
                                tedge.get(), tflag.get(), L);
         });
@@ -554,11 +572,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                              nullptr, edge_in.get(), eflag.get(), q,
                              rest.get(), nrest, budget, L);
         },
-        [&](QView q, Queue q2) {
+        [&](QRef q, Queue q2) {
           launch_low_ac2_esc(SG, 1, q, ecol.get(), eleft.get(), nullptr,
                              nullptr, eflag.get(), q2, L);
         },
-        [&](QView q) {
+        [&](QRef q) {
           launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
                                nullptr, eflag.get(), L);
         });
@@ -584,6 +602,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   R->counts[1] = Ef;
   R->counts[2] = Tf;
   R->counts[3] = Kf;
+  {
+    const int64_t sz[9] = {Ecand, Ealpha, Tslots, Talpha, Kslots, Kalpha, Kf, Tf, Ef};
+    for (int q = 0; q < 9; ++q) R->sizes[q] = sz[q];
+    R->sizes[11] = n;
+  }
   st.edges_final = Ef;
   st.tris_final = Tf;
   st.tets_final = Kf;
