@@ -31,7 +31,8 @@ enum {
   PAC_ERR_NOMEM = -4,       /* host or device allocation failed */
   PAC_ERR_DEVICE = -5,      /* HIP runtime error */
   PAC_ERR_OVERFLOW = -6,    /* a count would exceed the index width */
-  PAC_ERR_RANGE = -7        /* input exponent range exceeds exact capacity */
+  PAC_ERR_RANGE = -7,       /* input exponent range exceeds exact capacity */
+  PAC_ERR_STALE = -8        /* a recorded graph no longer fits its input */
 };
 
 enum { PAC_CELL_EXACT = 0, PAC_CELL_REFERENCE_RAW = 1 };
@@ -129,6 +130,35 @@ int pac_batch_counts(const pac_batch_result* r, int64_t* out);
 int pac_batch_copy(const pac_batch_result* r, int64_t m, int dim, int32_t* dst);
 int pac_batch_stats(const pac_batch_result* r, pac_stats* st);
 void pac_batch_free(pac_batch_result* r);
+
+/* Recorded batch: the whole pipeline of every pass of a batch captured
+ * once as one HIP graph, so that a run is a single hipGraphLaunch with no
+ * host round trip.  pac_graph_create runs the batch eagerly once (this
+ * fixes each pass's grids and measures every count), records each pass
+ * with buffers of those counts plus `headroom` (a fraction, e.g. 0.125;
+ * < 0: the default 0.125) and checks one uncaptured run of the recording
+ * against the eager run.  Each launch recomputes everything from the
+ * input buffer: the graph's own copy of xyzr (replace its contents with
+ * pac_graph_set_input) or, with opt.device_inputs, the caller's device
+ * array itself.  pac_graph_result waits for the last launch and returns
+ * its lists as a batch result (pac_batch_*), equal to pac_compute_batch's.
+ * A launch whose input moved a molecule's bounds (corners, largest radius
+ * or |coordinate|, finer coordinate lsb) reports PAC_ERR_STALE, one whose
+ * counts outgrew the headroom PAC_ERR_OVERFLOW: record again.  The
+ * reference row order only (opt.order).  No reference counterpart. */
+typedef struct pac_graph pac_graph;
+int pac_graph_create(const float* xyzr, const int64_t* offsets, int64_t n_mol,
+                     float solvent, float alpha, const pac_options* opt,
+                     double headroom, pac_graph** out);
+/* replace the graph's copy of the input (host or device pointer, same
+ * ball count), ordered after earlier launches on `stream` */
+int pac_graph_set_input(pac_graph* g, const float* xyzr, void* stream);
+/* one run of the whole batch on `stream` (NULL: the graph's stream) */
+int pac_graph_launch(pac_graph* g, void* stream);
+int pac_graph_result(pac_graph* g, pac_batch_result** out);
+/* kernel nodes in the graph, graph device memory in bytes */
+int pac_graph_info(const pac_graph* g, int64_t* nodes, int64_t* bytes);
+void pac_graph_free(pac_graph* g);
 
 /* Autotuner: times the pipeline on the given balls for each candidate
  * setting of the tunables (emptiness phase-one pair budget, second-phase

@@ -20,33 +20,36 @@ __global__ __launch_bounds__(kBlock) void k_compact_u4(Cnt c_n,
                                                        const uint4* in,
                                                        const uint8_t* flag,
                                                        const uint32_t* pos,
-                                                       uint4* out) {
+                                                       uint4* out,
+                                                       uint64_t ocap) {
   const int64_t n = c_n.get();
   int64_t i = gtid();
-  if (i >= n || !flag[i]) return;
+  if (i >= n || !flag[i] || pos[i] >= ocap) return;
   out[pos[i]] = in[i];
 }
 void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
-                         const uint32_t* pos, uint4* out, PipeLaunch L) {
+                         const uint32_t* pos, uint4* out, PipeLaunch L,
+                         uint64_t ocap) {
   if (n.cap <= 0) return;
   hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, in, flag, pos, out);
+                     dim3(kBlock), 0, L.s, n, in, flag, pos, out, ocap);
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_ids(Cnt c_n,
                                                         const uint8_t* flag,
                                                         const uint32_t* pos,
-                                                        uint32_t* out) {
+                                                        uint32_t* out,
+                                                        uint64_t ocap) {
   const int64_t n = c_n.get();
   int64_t i = gtid();
-  if (i >= n || !flag[i]) return;
+  if (i >= n || !flag[i] || pos[i] >= ocap) return;
   out[pos[i]] = (uint32_t)i;
 }
 void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
-                        uint32_t* out, PipeLaunch L) {
+                        uint32_t* out, PipeLaunch L, uint64_t ocap) {
   if (n.cap <= 0) return;
   hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, flag, pos, out);
+                     dim3(kBlock), 0, L.s, n, flag, pos, out, ocap);
 }
 
 /* faces of the final tets -> tri_in */
@@ -138,19 +141,20 @@ void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
 __global__ __launch_bounds__(kBlock) void k_emit_edges(
     Cnt c_ne, const uint32_t* ecol, const uint32_t* eleft,
     const uint8_t* flag, const uint32_t* pos, const uint32_t* perm,
-    int2* out) {
+    int2* out, uint64_t ocap) {
   const int64_t ne = c_ne.get();
   int64_t id = gtid();
-  if (id >= ne || !flag[id]) return;
+  if (id >= ne || !flag[id] || pos[id] >= ocap) return;
   out[pos[id]] = make_int2((int)perm[eleft[id]], (int)perm[ecol[id]]);
 }
 void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint32_t* pos,
-                       const uint32_t* perm, int32_t* out, PipeLaunch L) {
+                       const uint32_t* perm, int32_t* out, PipeLaunch L,
+                       uint64_t ocap) {
   if (ne.cap <= 0) return;
   hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, ecol, eleft, flag, pos, perm,
-                     (int2*)out);
+                     (int2*)out, ocap);
 }
 
 }  // namespace pac

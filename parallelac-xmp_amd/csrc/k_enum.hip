@@ -213,6 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_key(const Ball* raw,
   int cx = (int)__fdiv_rn(__fsub_rn((float)a.x, g.minx), g.step);
   int cy = (int)__fdiv_rn(__fsub_rn((float)a.y, g.miny), g.step);
   int cz = (int)__fdiv_rn(__fsub_rn((float)a.z, g.minz), g.step);
+  /* balls always key inside the grid planned from their own bounds; the
+   * clamp only keeps a recorded pipeline in bounds after its input moved
+   * (that run is then reported stale) */
+  cx = min(max(cx, 0), g.ex - 1);
+  cy = min(max(cy, 0), g.ey - 1);
+  cz = min(max(cz, 0), g.ez - 1);
   key[i] = g.cbase + (uint32_t)(cx + g.ex * cy + g.ex * g.ey * cz);
   idx[i] = (uint32_t)i;
 }
@@ -721,18 +727,21 @@ __global__ __launch_bounds__(kBlock) void k_seg_starts(Cnt c_nseg,
                                                        const uint64_t* off,
                                                        const uint32_t* pos,
                                                        uint64_t* orow,
-                                                       uint64_t poslim) {
+                                                       uint64_t poslim,
+                                                       uint64_t vlim) {
   const int64_t nseg = c_nseg.get();
   int64_t s = gtid();
   if (s > nseg) return;
   const uint64_t o = off[s];
-  orow[s] = pos[o < poslim ? o : poslim];
+  const uint64_t v = pos[o < poslim ? o : poslim];
+  orow[s] = v < vlim ? v : vlim;
 }
 void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L, uint64_t poslim) {
+                       uint64_t* orow, PipeLaunch L, uint64_t poslim,
+                       uint64_t vlim) {
   hipLaunchKernelGGL(k_seg_starts,
                      dim3((unsigned)((nseg.cap + 1 + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nseg, off, pos, orow, poslim);
+                     dim3(kBlock), 0, L.s, nseg, off, pos, orow, poslim, vlim);
 }
 /* KIND 1: (ecol, eleft) = (val, seg); 2: (tk, tedge) = (val, seg);
  * 3: tets = (i, j, k, l) */
@@ -741,11 +750,12 @@ __global__ __launch_bounds__(kBlock) void k_compact_flat(
     Cnt c_nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
     const uint8_t* flag, const uint32_t* pos, uint32_t* out_val,
     uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
-    const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b) {
+    const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b, uint64_t ocap) {
   const int64_t nc = c_nc.get();
   int64_t o = gtid();
   if (o >= nc || flag[o] != KEEP) return;
   uint64_t w = pos[o];
+  if (w >= ocap) return;
   if (KIND < 3) {
     out_val[w] = cval[o];
     out_seg[w] = cseg[o];
@@ -762,21 +772,21 @@ void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
                          const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
-                         PipeLaunch L) {
+                         PipeLaunch L, uint64_t ocap) {
   if (nc.cap <= 0) return;
   dim3 gr((unsigned)((nc.cap + kBlock - 1) / kBlock));
   if (kind == 1)
     hipLaunchKernelGGL(k_compact_flat<1>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
                        cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b);
+                       out_a, out_b, ocap);
   else if (kind == 2)
     hipLaunchKernelGGL(k_compact_flat<2>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
                        cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b);
+                       out_a, out_b, ocap);
   else
     hipLaunchKernelGGL(k_compact_flat<3>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
                        cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b);
+                       out_a, out_b, ocap);
 }
 
 }  // namespace PAC_V

@@ -276,4 +276,71 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
   *result_in_alt = alt;
 }
 
+/* ------------------------------------------------------------------ */
+/* recorded (fixed-capacity) pipelines                                 */
+/* ------------------------------------------------------------------ */
+/* consumer view of QSLOTS sub-queues from their counters; a counter past
+ * the capacity (entries were dropped) sets `bit` in *status */
+__global__ void k_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
+                            QView* v, unsigned* status, unsigned bit,
+                            unsigned* fill) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0;
+  unsigned mx = 0;
+  for (int k = 0; k < QSLOTS; ++k) {
+    const unsigned c = cnt[k];
+    v->off[k] = acc;
+    acc += c < cap ? c : cap;
+    mx = c > mx ? c : mx;
+  }
+  v->off[QSLOTS] = acc;
+  v->q = q;
+  v->cap = cap;
+  atomicMax(fill, mx);
+  if (mx > cap) atomicOr(status, bit);
+}
+void launch_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
+                      QView* v, unsigned* status, unsigned bit,
+                      unsigned* fill, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_view, dim3(1), dim3(64), 0, s, q, cap, cnt, v,
+                     status, bit, fill);
+}
+
+__global__ void k_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,
+                            unsigned bit) {
+  if (threadIdx.x == 0 && *v > cap) atomicOr(status, bit);
+}
+void launch_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,
+                      unsigned bit, hipStream_t s) {
+  hipLaunchKernelGGL(k_cap_check, dim3(1), dim3(64), 0, s, v, cap, status, bit);
+}
+
+/* bounds of the current input against those the pipeline was planned for:
+ * any difference in the fields the plan depends on (corners, radii,
+ * largest |coordinate|), a non-finite input or a finer input lsb than
+ * the exact scale covers sets `bit` */
+__device__ __forceinline__ bool same_f64(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__global__ void k_plan_check(const Bounds* now, const Bounds* ref, int64_t m,
+                             int scale, unsigned* status, unsigned bit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Bounds a = now[i], b = ref[i];
+  const bool ok = same_f64(a.minx, b.minx) && same_f64(a.miny, b.miny) &&
+                  same_f64(a.minz, b.minz) && same_f64(a.maxx, b.maxx) &&
+                  same_f64(a.maxy, b.maxy) && same_f64(a.maxz, b.maxz) &&
+                  same_f64(a.max_raw_r, b.max_raw_r) &&
+                  same_f64(a.max_r, b.max_r) && same_f64(a.max_abs, b.max_abs) &&
+                  a.nonfinite == 0 && a.shift <= scale;
+  if (!ok) atomicOr(status, bit);
+}
+void launch_plan_check(const Bounds* now, const Bounds* ref, int64_t m,
+                       int scale, unsigned* status, unsigned bit,
+                       hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_plan_check, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, now, ref, m, scale, status, bit);
+}
+
 }  // namespace pac

@@ -34,22 +34,6 @@ using namespace pac;
 static thread_local std::string g_err;
 void pac_set_error(const char* m) { g_err = m ? m : ""; }
 
-template <class F>
-static int guarded(F f) {
-  try {
-    f();
-    return PAC_OK;
-  } catch (const PacError& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
-    return PAC_ERR_NOMEM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return PAC_ERR_INVALID;
-  }
-}
 
 /* ------------------------------------------------------------------ */
 /* engines                                                             */
@@ -283,19 +267,7 @@ static void alpha_adjoint(const pac_result* r, const double* gtri,
 /* ------------------------------------------------------------------ */
 /* batched ensembles                                                   */
 /* ------------------------------------------------------------------ */
-struct pac_batch_result {
-  Engine* eng = nullptr;
-  int64_t nmol = 0;
-  std::vector<int64_t> off;                        /* ball offsets, nmol+1 */
-  std::vector<std::unique_ptr<pac_result>> groups; /* one per pipeline pass */
-  std::vector<int64_t> group_first;                /* first molecule of each group */
-  std::vector<int32_t> mol_group;
-  std::vector<int64_t> counts;                     /* nmol x 4 */
-  std::vector<uint64_t> row0[3];                   /* first row of each molecule */
-  pac_stats st;
-};
-
-static void add_stats(pac_stats& a, const pac_stats& b) {
+void pac::add_stats(pac_stats& a, const pac_stats& b) {
   a.n += b.n;
   a.edge_candidates += b.edge_candidates;
   a.edge_alpha += b.edge_alpha;
@@ -325,9 +297,9 @@ static void add_stats(pac_stats& a, const pac_stats& b) {
  * molecule larger than cap gets a pass of its own); each pass runs the
  * whole pipeline once over its concatenated balls, every molecule with its
  * own grid. */
-static void compute_batch(const float* xyzr, const int64_t* offsets,
-                          int64_t nmol, float solvent, float alpha,
-                          const pac_options* o, pac_batch_result** out) {
+void pac::compute_batch(const float* xyzr, const int64_t* offsets,
+                         int64_t nmol, float solvent, float alpha,
+                         const pac_options* o, pac_batch_result** out) {
   if (!out) fail(PAC_ERR_INVALID, "out is NULL");
   if (nmol < 0 || (nmol > 0 && !offsets)) fail(PAC_ERR_INVALID, "bad offsets");
   if (!std::isfinite(solvent) || !std::isfinite(alpha))

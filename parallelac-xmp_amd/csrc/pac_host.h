@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -149,7 +150,47 @@ struct pac_result {
   int64_t sizes[12] = {0};
 };
 
+/* a batch: one pac_result per pipeline pass, molecules in order */
+struct pac_batch_result {
+  pac::Engine* eng = nullptr;
+  int64_t nmol = 0;
+  std::vector<int64_t> off;                        /* ball offsets, nmol+1 */
+  std::vector<std::unique_ptr<pac_result>> groups; /* one per pipeline pass */
+  std::vector<int64_t> group_first;                /* first molecule of each group */
+  std::vector<int32_t> mol_group;
+  std::vector<int64_t> counts;                     /* nmol x 4 */
+  std::vector<uint64_t> row0[3];                   /* first row of each molecule */
+  pac_stats st;
+};
+
+/* thread's last error message (pac_last_error) */
+void pac_set_error(const char* m);
+
 namespace pac {
+/* runs f, turning exceptions into status codes and the error message */
+template <class F>
+inline int guarded(F f) {
+  try {
+    f();
+    return PAC_OK;
+  } catch (const PacError& e) {
+    pac_set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    pac_set_error("host allocation failed");
+    return PAC_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    pac_set_error(e.what());
+    return PAC_ERR_INVALID;
+  }
+}
+
+/* batches (pac_engine.cpp) */
+void add_stats(pac_stats& a, const pac_stats& b);
+void compute_batch(const float* xyzr, const int64_t* offsets, int64_t nmol,
+                   float solvent, float alpha, const pac_options* o,
+                   pac_batch_result** out);
+
 /* ------------------------------------------------------------------ */
 /* helpers                                                             */
 /* ------------------------------------------------------------------ */
@@ -166,13 +207,16 @@ struct Timer {
   hipEvent_t ev[10];
   int used = 0;
   hipStream_t s;
+  bool on = true; /* off: marks are skipped (recorded pipelines) */
   explicit Timer(hipStream_t st) : s(st) {
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   }
   ~Timer() {
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
-  void mark(int i) { HIPCHK(hipEventRecord(ev[i], s)); }
+  void mark(int i) {
+    if (on) HIPCHK(hipEventRecord(ev[i], s));
+  }
   double ms(int a, int b) {
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, ev[a], ev[b]));
@@ -226,14 +270,64 @@ inline T readback(Engine* E, hipStream_t s, const T* dptr) {
   return v;
 }
 
+/* Statistics from the device counters (CTR_*, CTR_SLOTS copies each):
+ * evaluations per precision stage of each family, SoS calls, emptiness
+ * pairs.  Hot-stage evaluations of the filters are one per candidate (the
+ * edge candidates are counted on the host, the others on the device);
+ * those of the emptiness scans one per (simplex, ball) pair.  Adds to st. */
+inline void stats_from_counters(const unsigned long long* hs, bool staged,
+                                int64_t edge_candidates, pac_stats& st) {
+  unsigned long long h[CTR_COUNT];
+  for (int c = 0; c < CTR_COUNT; ++c) {
+    h[c] = 0;
+    for (int k = 0; k < CTR_SLOTS; ++k) h[c] += hs[c * CTR_SLOTS + k];
+  }
+  for (int f = 0; f < 6; ++f)
+    for (int q = 0; q < 4; ++q)
+      st.stage_evals[f][q] += (int64_t)h[CTR_STAGE + 4 * f + q];
+  const int hot = staged ? 0 : 1;
+  st.tri_candidates += (int64_t)h[CTR_CAND + 1];
+  st.tet_candidates += (int64_t)h[CTR_CAND + 2];
+  st.stage_evals[0][hot] += edge_candidates;
+  st.stage_evals[1][hot] += (int64_t)h[CTR_CAND + 1];
+  st.stage_evals[2][hot] += (int64_t)h[CTR_CAND + 2];
+  for (int q = 0; q < 3; ++q) {
+    st.stage_evals[3 + q][hot] += (int64_t)h[CTR_PAIRS + q];
+    st.ac2_pairs[q] += (int64_t)h[CTR_PAIRS + q];
+  }
+  st.sos_calls += (int64_t)h[CTR_SOS];
+}
+
 /* small fills on a stream (kernels, not hipMemsetAsync) */
 void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void zero_u32(unsigned* p, int n, hipStream_t s);
 void zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 
+/* One pipeline pass recorded with fixed capacities (for capture into a
+ * graph): nothing is read back, every count stays on the device, and
+ * overflows or a stale plan set status bits (REC_*). */
+enum { RC_ECAND, RC_EALPHA, RC_TSLOTS, RC_TALPHA, RC_KSLOTS, RC_KALPHA,
+       RC_KF, RC_TF, RC_EF, RC_COUNT };
+struct PassRec {
+  Pool* pool;              /* the pass's buffers */
+  const void* plan;        /* PassPlan of the pipeline's ball type */
+  const Bounds* bref;      /* device: the bounds the plan was made from */
+  Bounds* bnow;            /* device: bounds of the current input */
+  int64_t cap[RC_COUNT];   /* capacity of each count */
+  uint64_t* dcnt;          /* device [RC_COUNT]: the counts */
+  unsigned* status;        /* device: REC_* bits */
+  unsigned* fill;          /* device [2]: largest sub-queue fills */
+  unsigned long long* ctr; /* device counters (shared by the passes) */
+  unsigned* qn;            /* device [2 QSLOTS + 1] queue counters */
+  QView* views;            /* device [2] */
+  QEntry *q1, *q2;         /* hot / escalation queues */
+  uint32_t c1, c2;         /* their sub-queue capacities */
+};
+
 namespace PAC_V {
 /* grids, exact scale and device molecule tables of one pipeline pass */
 struct PassPlan {
+  std::vector<Bounds> bounds; /* per molecule, as reduced on the device */
   std::vector<GridP> grids;
   int64_t G = 0;
   int S = 0;
@@ -251,13 +345,15 @@ namespace vf32 {
 void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                   float alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP,
-                  const int64_t* moff = nullptr, int64_t nmol = 1);
+                  const int64_t* moff = nullptr, int64_t nmol = 1,
+                  const PassRec* rec = nullptr);
 }
 namespace vf64 {
 void run_pipeline(Engine* E, const double4* raw, int64_t n, double solvent,
                   double alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP,
-                  const int64_t* moff = nullptr, int64_t nmol = 1);
+                  const int64_t* moff = nullptr, int64_t nmol = 1,
+                  const PassRec* rec = nullptr);
 }
 
 }  // namespace pac

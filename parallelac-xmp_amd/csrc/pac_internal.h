@@ -136,16 +136,34 @@ struct QRef {
   uint32_t launch_items() const { return dv ? bound : v.size(); }
 };
 
+/* ---- recorded pipelines (k_prims.hip) ----
+ * Fixed-capacity runs keep every count on the device; overflows and a
+ * stale plan are reported through status bits. */
+enum { REC_CAP = 0,    /* bits 0..8: a count exceeded its capacity */
+       REC_Q1 = 9,     /* hot queue overflow */
+       REC_Q2 = 10,    /* escalation queue overflow */
+       REC_STALE = 11  /* the input no longer matches the recorded plan */ };
+void launch_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
+                      QView* v, unsigned* status, unsigned bit,
+                      unsigned* fill, hipStream_t s);
+void launch_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,
+                      unsigned bit, hipStream_t s);
+void launch_plan_check(const Bounds* now, const Bounds* ref, int64_t m,
+                       int scale, unsigned* status, unsigned bit,
+                       hipStream_t s);
+
 struct CandCtx {
   const uint32_t *eleft, *ecol, *tk, *tedge;
   const uint4* ctet; /* tet candidates (i, j, k, l); x = ~0: unused slot */
 };
 
-/* ---- closure + emission (k_closure.hip) ---- */
+/* ---- closure + emission (k_closure.hip) ----
+ * ocap: capacity of a compaction's output (entries past it are dropped) */
 void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
-                         const uint32_t* pos, uint4* out, PipeLaunch L);
+                         const uint32_t* pos, uint4* out, PipeLaunch L,
+                         uint64_t ocap = ~0ull);
 void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
-                        uint32_t* out, PipeLaunch L);
+                        uint32_t* out, PipeLaunch L, uint64_t ocap = ~0ull);
 void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
@@ -160,7 +178,8 @@ void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
                       int32_t* out, PipeLaunch L);
 void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
                        const uint8_t* flag, const uint32_t* pos,
-                       const uint32_t* perm, int32_t* out, PipeLaunch L);
+                       const uint32_t* perm, int32_t* out, PipeLaunch L,
+                       uint64_t ocap = ~0ull);
 
 /* ---- outputs (k_output.hip) ---- */
 void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
@@ -262,14 +281,16 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32
                        Queue q2, PipeLaunch L);
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L);
+/* orow[s] = pos[min(off[s], poslim)], clamped to vlim */
 void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L, uint64_t poslim = ~0ull);
+                       uint64_t* orow, PipeLaunch L, uint64_t poslim = ~0ull,
+                       uint64_t vlim = ~0ull);
 void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
                          const uint32_t* cseg, const uint8_t* flag,
                          const uint32_t* pos, uint32_t* out_val,
                          uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
                          const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
-                         PipeLaunch L);
+                         PipeLaunch L, uint64_t ocap = ~0ull);
 
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =

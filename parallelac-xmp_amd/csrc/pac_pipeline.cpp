@@ -136,6 +136,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   }
+  pl.bounds = hb;
   pl.grids.assign(hb.size(), GridP());
   pl.G = 0;
   int shift = 0;
@@ -187,12 +188,19 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
 
 /* One pipeline pass over n balls.  nmol == 1: one molecule; otherwise the
  * balls are nmol molecules, moff[0..nmol] (host, relative to raw), each with
- * its own grid, processed together by every kernel launch. */
+ * its own grid, processed together by every kernel launch.
+ *
+ * rec == nullptr: eager; counts are read back as they are produced and
+ * every buffer has its exact size.  rec set: the pass is recorded with the
+ * capacities rec->cap for capture into a graph; nothing is read back, the
+ * counts stay at rec->dcnt, the plan is rec->plan (checked on the device
+ * against the current input) and each precision ladder runs once over
+ * queues of fixed size, overflows setting status bits. */
 void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                   Real alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
-                  int64_t nmol) {
-  Pool* P = &E->pool;
+                  int64_t nmol, const PassRec* rec) {
+  Pool* P = rec ? rec->pool : &E->pool;
   PipeLaunch L{s, kBlock};
   pac_stats& st = R->st;
   st.n = n;
@@ -200,8 +208,29 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
 
   /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
-  PassPlan pl;
-  plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, pl);
+  PassPlan own;
+  if (!rec) plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, own);
+  const PassPlan& pl = rec ? *(const PassPlan*)rec->plan : own;
+  /* counts left on the device by the scans (RC_*) */
+  DBuf<uint64_t> dcnt_own;
+  uint64_t* dcnt = rec ? rec->dcnt : nullptr;
+  if (!rec) {
+    dcnt_own = DBuf<uint64_t>(P, RC_COUNT);
+    dcnt = dcnt_own.get();
+  }
+  if (rec) {
+    zero_u64((unsigned long long*)dcnt, RC_COUNT, s);
+    /* does the current input still fit the recorded plan? */
+    if (nmol > 1) {
+      launch_bounds_mol(raw, pl.dmoff.get(), nmol, solvent, rec->bnow, L);
+    } else {
+      DBuf<Bounds> bpart(P, 1024 * 2 + 4);
+      launch_bounds(raw, n, solvent, bpart.get(), rec->bnow, L);
+    }
+    launch_plan_check(rec->bnow, rec->bref, (int64_t)pl.bounds.size(), pl.S,
+                      rec->status, 1u << REC_STALE, s);
+    KCHK();
+  }
   const int64_t G = pl.G;
   const int S = pl.S;
   const bool no_edges = pl.no_edges;
@@ -223,13 +252,21 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
   SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
   SG.bgrid = opt.ac2_grid > 0 ? std::min(opt.ac2_grid, 1 << 16) : 2048;
-  DBuf<unsigned long long> ctr(P, CTR_COUNT * CTR_SLOTS);
-  zero_u64(ctr.get(), CTR_COUNT * CTR_SLOTS, s);
-  SG.ctr = ctr.get();
+  DBuf<unsigned long long> ctr_own;
+  if (rec) {
+    SG.ctr = rec->ctr;
+  } else {
+    ctr_own = DBuf<unsigned long long>(P, CTR_COUNT * CTR_SLOTS);
+    zero_u64(ctr_own.get(), CTR_COUNT * CTR_SLOTS, s);
+    SG.ctr = ctr_own.get();
+  }
+  unsigned long long* const ctr = SG.ctr;
   /* [0, QSLOTS) hot queue, [QSLOTS, 2 QSLOTS) escalation queue, then the
    * count of simplices left for the second emptiness phase */
-  DBuf<unsigned> qn(P, 2 * QSLOTS + 1);
-  unsigned* const nrest = qn.get() + 2 * QSLOTS;
+  DBuf<unsigned> qn_own;
+  if (!rec) qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 1);
+  unsigned* const qn = rec ? rec->qn : qn_own.get();
+  unsigned* const nrest = qn + 2 * QSLOTS;
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
@@ -266,6 +303,15 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   auto scan_tmp = [&](int64_t m) { return DBuf<uint8_t>(P, scan_tmp_bytes(m) + 64); };
   const bool staged = SG.staged != 0;
 
+  /* the count a scan left at dcnt[q]: read back, or (recorded) kept on the
+   * device under its capacity, an excess setting status bit q */
+  auto settle = [&](int q) -> Cnt {
+    if (!rec) return Cnt((int64_t)readback(E, s, dcnt + q));
+    launch_cap_check(dcnt + q, (uint64_t)rec->cap[q], rec->status,
+                     1u << (REC_CAP + q), s);
+    return Cnt(rec->cap[q], dcnt + q);
+  };
+
   /* precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
    * double-double (q1 -> q2)] -> exact wide-integer kernel.  Queue
    * overflows rerun the producing kernel with a larger queue; only the
@@ -279,10 +325,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   };
   /* counts of the sub-queues -> consumer view; false on overflow, with
    * *need = the per-slot capacity that would have sufficed */
-  auto view_of = [&](const QEntry* q, uint32_t cap, const unsigned* dcnt,
+  auto view_of = [&](const QEntry* q, uint32_t cap, const unsigned* dcq,
                      QView* v, uint32_t* need) -> bool {
     unsigned h[QSLOTS];
-    HIPCHK(hipMemcpyAsync(E->pinned, dcnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(E->pinned, dcq, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::memcpy(h, E->pinned, sizeof(h));
     v->q = q;
@@ -296,7 +342,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     }
     v->off[QSLOTS] = (uint32_t)acc;
     *need = mx;
-    int64_t& fill = R->sizes[(dcnt == qn.get()) ? 9 : 10];
+    int64_t& fill = R->sizes[(dcq == qn) ? 9 : 10];
     fill = std::max<int64_t>(fill, mx);
     return mx <= cap;
   };
@@ -306,17 +352,33 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   };
   auto ladder = [&](int64_t cap1_total, int kp_hot, int kp_exact, auto&& hot,
                     auto&& esc, auto&& exact) {
+    if (rec) { /* once, over the recorded queues; views built on the device */
+      zero_u32(qn, 2 * QSLOTS, s);
+      hot(Queue{rec->q1, qn, rec->c1});
+      launch_make_view(rec->q1, rec->c1, qn, rec->views, rec->status,
+                       1u << REC_Q1, rec->fill, s);
+      QRef r{QView{}, rec->views, QSLOTS * rec->c1};
+      if (staged) {
+        esc(r, Queue{rec->q2, qn + QSLOTS, rec->c2});
+        launch_make_view(rec->q2, rec->c2, qn + QSLOTS, rec->views + 1,
+                         rec->status, 1u << REC_Q2, rec->fill + 1, s);
+        r = QRef{QView{}, rec->views + 1, QSLOTS * rec->c2};
+      }
+      exact(r);
+      KCHK();
+      return;
+    }
     uint32_t c1 = slot_cap(cap1_total);
     for (;;) {
       DBuf<QEntry> q1(P, (size_t)c1 * QSLOTS);
-      zero_u32(qn.get(), 2 * QSLOTS, s);
+      zero_u32(qn, 2 * QSLOTS, s);
       if (kp_hot >= 0) KP.begin(kp_hot);
-      hot(Queue{q1.get(), qn.get(), c1});
+      hot(Queue{q1.get(), qn, c1});
       if (kp_hot >= 0) KP.end();
       KCHK();
       QView v1;
       uint32_t need = 0;
-      if (!view_of(q1.get(), c1, qn.get(), &v1, &need)) {
+      if (!view_of(q1.get(), c1, qn, &v1, &need)) {
         c1 = grow(need);
         continue;
       }
@@ -326,10 +388,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
         uint32_t c2 = slot_cap(4 * (int64_t)v1.size() + 4096);
         for (;;) {
           q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
-          zero_u32(qn.get() + QSLOTS, QSLOTS, s);
-          esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn.get() + QSLOTS, c2});
+          zero_u32(qn + QSLOTS, QSLOTS, s);
+          esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn + QSLOTS, c2});
           KCHK();
-          if (view_of(q2.get(), c2, qn.get() + QSLOTS, &vx, &need)) break;
+          if (view_of(q2.get(), c2, qn + QSLOTS, &vx, &need)) break;
           c2 = grow(need);
         }
       }
@@ -345,44 +407,48 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
    * ladder settles every flag, a flat scan of the flags gives both the
    * stable compaction positions and the survivors' CSR row starts. */
   int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
-  auto filter_stage = [&](int kind, CandCtx C, int64_t nc, const uint32_t* cval,
+  auto filter_stage = [&](int kind, CandCtx C, Cnt nc, const uint32_t* cval,
                           const uint32_t* cseg, uint8_t* flag, int kp,
-                          DBuf<uint32_t>& pos) -> int64_t {
-    if (kind == 1) host_evals[0] += nc; /* tris / tets: slots != candidates */
+                          DBuf<uint32_t>& pos, int q) -> Cnt {
+    if (kind == 1) host_evals[0] += nc.cap; /* tris / tets: slots != candidates */
     /* few candidates stay undecided in float: start small, a rerun of the
      * (idempotent) filter sizes the queue exactly if it overflows */
     ladder(
-        std::min<int64_t>(nc, nc / 32 + 65536), kp, -1,
-        [&](Queue q) { launch_filter(SG, kind, C, nc, cval, cseg, flag, q, L); },
-        [&](QRef q, Queue q2) {
-          launch_filter_esc(SG, kind, C, q, cval, flag, q2, L);
+        std::min<int64_t>(nc.cap, nc.cap / 32 + 65536), kp, -1,
+        [&](Queue qq) { launch_filter(SG, kind, C, nc, cval, cseg, flag, qq, L); },
+        [&](QRef qr, Queue q2) {
+          launch_filter_esc(SG, kind, C, qr, cval, flag, q2, L);
         },
-        [&](QRef q) {
-          launch_filter_exact(SG, kind, C, q, cval, flag, L);
+        [&](QRef qr) {
+          launch_filter_exact(SG, kind, C, qr, cval, flag, L);
         });
-    pos = DBuf<uint32_t>(P, nc + 1);
+    pos = DBuf<uint32_t>(P, nc.cap + 1);
     {
-      auto tmp = scan_tmp(nc);
-      scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s);
+      auto tmp = scan_tmp(nc.cap);
+      scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s, dcnt + q);
     }
     KCHK();
-    return (int64_t)readback(E, s, pos.get() + nc);
+    return settle(q);
   };
-  auto count_scan = [&](const uint32_t* cnt, int64_t nseg, uint64_t* off,
-                        const char* what) -> int64_t {
+  auto count_scan = [&](const uint32_t* cnt, Cnt nseg, uint64_t* off, int q,
+                        const char* what) -> Cnt {
     {
-      auto tmp = scan_tmp(nseg);
-      scan_u32_to_u64(cnt, off, nseg, tmp.get(), 0, s);
+      auto tmp = scan_tmp(nseg.cap);
+      scan_u32_to_u64(cnt, off, nseg, tmp.get(), 0, s, dcnt + q);
     }
     KCHK();
-    int64_t tot = (int64_t)readback(E, s, off + nseg);
-    if (tot >= ((int64_t)1 << 32))
+    Cnt tot = settle(q);
+    if (tot.cap >= ((int64_t)1 << 32))
       fail(PAC_ERR_OVERFLOW, std::string(what) + " candidates exceed 2^32");
     return tot;
   };
+  /* Every buffer below is sized by a count's cap: the exact count (eager)
+   * or its recorded capacity.  Enumerations stop at their arrays' ends,
+   * compactions at their outputs' ends, and CSR row starts are clamped to
+   * the row arrays, so a recorded pass that overflows stays in bounds. */
 
   /* ---- edges (AlphaComplexParallel.cu:1200-1263) ---- */
-  int64_t Ecand = 0, Ealpha = 0;
+  Cnt Ecand(0), Ealpha(0);
   DBuf<uint64_t> erow(P, n + 1);
   DBuf<uint32_t> ecol, eleft;
   CandCtx C0{nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -391,94 +457,93 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     {
       DBuf<uint32_t> cnt(P, n);
       launch_edge_count(SG, cnt.get(), L);
-      Ecand = count_scan(cnt.get(), n, off.get(), "edge");
+      Ecand = count_scan(cnt.get(), n, off.get(), RC_ECAND, "edge");
     }
-    DBuf<uint32_t> cval(P, Ecand), cseg(P, Ecand);
-    DBuf<uint8_t> flag(P, Ecand);
-    launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L);
+    DBuf<uint32_t> cval(P, Ecand.cap), cseg(P, Ecand.cap);
+    DBuf<uint8_t> flag(P, Ecand.cap);
+    launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L,
+                     (uint64_t)Ecand.cap);
     DBuf<uint32_t> pos;
     Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
-                          KP_EDGE_FILL, pos);
-    launch_seg_starts(n, off.get(), pos.get(), erow.get(), L);
-    ecol = DBuf<uint32_t>(P, Ealpha);
-    eleft = DBuf<uint32_t>(P, Ealpha);
+                          KP_EDGE_FILL, pos, RC_EALPHA);
+    launch_seg_starts(n, off.get(), pos.get(), erow.get(), L,
+                      (uint64_t)Ecand.cap, (uint64_t)Ealpha.cap);
+    ecol = DBuf<uint32_t>(P, Ealpha.cap);
+    eleft = DBuf<uint32_t>(P, Ealpha.cap);
     launch_compact_flat(1, Ecand, C0, cval.get(), cseg.get(), flag.get(),
                         pos.get(), ecol.get(), eleft.get(), nullptr, nullptr,
-                        nullptr, nullptr, nullptr, L);
+                        nullptr, nullptr, nullptr, L, (uint64_t)Ealpha.cap);
     KCHK();
   } else {
     zero_u64((unsigned long long*)erow.get(), n + 1, s);
     ecol = DBuf<uint32_t>(P, 1);
     eleft = DBuf<uint32_t>(P, 1);
   }
-  st.edge_candidates = Ecand;
-  st.edge_alpha = Ealpha;
   T.mark(3);
 
   /* ---- triangles (AlphaComplexParallel.cu:1279-1371) ---- */
-  int64_t Tcand = 0, Tslots = 0, Talpha = 0;
-  DBuf<uint64_t> trow(P, Ealpha + 1);
+  Cnt Tslots(0), Talpha(0);
+  DBuf<uint64_t> trow(P, Ealpha.cap + 1);
   DBuf<uint32_t> tk, tedge, teik, tejk;
   CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr, nullptr};
   {
-    DBuf<uint64_t> off(P, Ealpha + 1);
+    DBuf<uint64_t> off(P, Ealpha.cap + 1);
     {
       /* exact counts: a triangle's third vertex must be in both rows, and
        * the merge that counts it is cheap next to filtering empty slots */
-      DBuf<uint32_t> cnt(P, Ealpha);
+      DBuf<uint32_t> cnt(P, Ealpha.cap);
       launch_tri_count(Ealpha, erow.get(), ecol.get(), eleft.get(), cnt.get(), L);
-      Tslots = count_scan(cnt.get(), Ealpha, off.get(), "triangle");
+      Tslots = count_scan(cnt.get(), Ealpha, off.get(), RC_TSLOTS, "triangle");
     }
-    DBuf<uint32_t> cval(P, Tslots), cseg(P, Tslots), cea(P, Tslots),
-        ceb(P, Tslots);
-    DBuf<uint8_t> flag(P, Tslots);
+    DBuf<uint32_t> cval(P, Tslots.cap), cseg(P, Tslots.cap),
+        cea(P, Tslots.cap), ceb(P, Tslots.cap);
+    DBuf<uint8_t> flag(P, Tslots.cap);
     launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
-                    cval.get(), cseg.get(), cea.get(), ceb.get(), ctr.get(), L);
+                    cval.get(), cseg.get(), cea.get(), ceb.get(), ctr, L,
+                    (uint64_t)Tslots.cap);
     DBuf<uint32_t> pos;
     Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
-                          KP_TRI_FILL, pos);
-    launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L);
-    tk = DBuf<uint32_t>(P, Talpha);
-    tedge = DBuf<uint32_t>(P, Talpha);
-    teik = DBuf<uint32_t>(P, Talpha);
-    tejk = DBuf<uint32_t>(P, Talpha);
+                          KP_TRI_FILL, pos, RC_TALPHA);
+    launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L,
+                      (uint64_t)Tslots.cap, (uint64_t)Talpha.cap);
+    tk = DBuf<uint32_t>(P, Talpha.cap);
+    tedge = DBuf<uint32_t>(P, Talpha.cap);
+    teik = DBuf<uint32_t>(P, Talpha.cap);
+    tejk = DBuf<uint32_t>(P, Talpha.cap);
     launch_compact_flat(2, Tslots, C1, cval.get(), cseg.get(), flag.get(),
                         pos.get(), tk.get(), tedge.get(), nullptr, cea.get(),
-                        ceb.get(), teik.get(), tejk.get(), L);
+                        ceb.get(), teik.get(), tejk.get(), L,
+                        (uint64_t)Talpha.cap);
     KCHK();
   }
-  st.tri_alpha = Talpha;
   T.mark(4);
 
   /* ---- tets (AlphaComplexParallel.cu:1381-1442) ---- */
-  int64_t Kcand = 0, Kslots = 0, Kalpha = 0;
+  Cnt Kslots(0), Kalpha(0);
   DBuf<uint4> tets;
   CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get(), nullptr};
   {
-    DBuf<uint64_t> off(P, Talpha + 1);
+    DBuf<uint64_t> off(P, Talpha.cap + 1);
     {
-      DBuf<uint32_t> cap(P, Talpha);
+      DBuf<uint32_t> cap(P, Talpha.cap);
       launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
-      Kslots = count_scan(cap.get(), Talpha, off.get(), "tet");
+      Kslots = count_scan(cap.get(), Talpha, off.get(), RC_KSLOTS, "tet");
     }
-    DBuf<uint4> ctet(P, Kslots);
-    DBuf<uint8_t> flag(P, Kslots);
+    DBuf<uint4> ctet(P, Kslots.cap);
+    DBuf<uint8_t> flag(P, Kslots.cap);
     launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
                     tejk.get(), eleft.get(), ecol.get(), off.get(), ctet.get(),
-                    ctr.get(), L);
+                    ctr, L, (uint64_t)Kslots.cap);
     C2.ctet = ctet.get();
     DBuf<uint32_t> pos;
     Kalpha = filter_stage(3, C2, Kslots, nullptr, nullptr, flag.get(),
-                          KP_TET_FILL, pos);
-    tets = DBuf<uint4>(P, Kalpha);
+                          KP_TET_FILL, pos, RC_KALPHA);
+    tets = DBuf<uint4>(P, Kalpha.cap);
     launch_compact_flat(3, Kslots, C2, nullptr, nullptr, flag.get(), pos.get(),
                         nullptr, nullptr, tets.get(), nullptr, nullptr, nullptr,
-                        nullptr, L);
+                        nullptr, L, (uint64_t)Kalpha.cap);
     KCHK();
   }
-  (void)Tcand;
-  (void)Kcand;
-  st.tet_alpha = Kalpha;
   T.mark(5);
 
   auto ac2_cap = [](int64_t items) {
@@ -488,13 +553,13 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
 
   /* ---- tet emptiness ---- */
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
-  DBuf<uint32_t> rest(P, std::max({Kalpha, Talpha, Ealpha, (int64_t)1}));
-  DBuf<uint8_t> kflag(P, Kalpha);
+  DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
+  DBuf<uint8_t> kflag(P, Kalpha.cap);
   if (opt.skip_tet_ac2) {
-    fill_u8(kflag.get(), Kalpha, 1, s);
-  } else if (Kalpha) {
+    fill_u8(kflag.get(), Kalpha.cap, 1, s);
+  } else if (Kalpha.cap) {
     ladder(
-        ac2_cap(Kalpha), KP_TET_AC2, KP_TET_AC2_EXACT,
+        ac2_cap(Kalpha.cap), KP_TET_AC2, KP_TET_AC2_EXACT,
         [&](Queue q) {
           zero_u32(nrest, 1, s);
           launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
@@ -507,29 +572,31 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
         });
   }
-  int64_t Kf = 0;
+  Cnt Kf(0);
   DBuf<uint4> ftets;
   {
-    DBuf<uint32_t> pos(P, Kalpha + 1);
-    auto tmp = scan_tmp(Kalpha);
-    scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s);
+    DBuf<uint32_t> pos(P, Kalpha.cap + 1);
+    auto tmp = scan_tmp(Kalpha.cap);
+    scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s,
+                       dcnt + RC_KF);
     KCHK();
-    Kf = (int64_t)readback(E, s, pos.get() + Kalpha);
-    ftets = DBuf<uint4>(P, Kf);
-    launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(), L);
+    Kf = settle(RC_KF);
+    ftets = DBuf<uint4>(P, Kf.cap);
+    launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(),
+                        L, (uint64_t)Kf.cap);
     KCHK();
   }
   tets.release();
   T.mark(6);
 
   /* ---- triangle closure + dangling emptiness ---- */
-  DBuf<uint8_t> tri_in(P, Talpha), tflag(P, Talpha);
-  fill_u8(tri_in.get(), Talpha, 0, s);
+  DBuf<uint8_t> tri_in(P, Talpha.cap), tflag(P, Talpha.cap);
+  fill_u8(tri_in.get(), Talpha.cap, 0, s);
   launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
                     tk.get(), tri_in.get(), L);
-  if (Talpha)
+  if (Talpha.cap)
     ladder(
-        ac2_cap(Talpha), KP_TRI_AC2, -1,
+        ac2_cap(Talpha.cap), KP_TRI_AC2, -1,
         [&](Queue q) {
           zero_u32(nrest, 1, s);
           launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
@@ -544,28 +611,30 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
                                tedge.get(), tflag.get(), L);
         });
-  int64_t Tf = 0;
+  Cnt Tf(0);
   DBuf<uint32_t> ftri;
   {
-    DBuf<uint32_t> pos(P, Talpha + 1);
-    auto tmp = scan_tmp(Talpha);
-    scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s);
+    DBuf<uint32_t> pos(P, Talpha.cap + 1);
+    auto tmp = scan_tmp(Talpha.cap);
+    scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s,
+                       dcnt + RC_TF);
     KCHK();
-    Tf = (int64_t)readback(E, s, pos.get() + Talpha);
-    ftri = DBuf<uint32_t>(P, Tf);
-    launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L);
+    Tf = settle(RC_TF);
+    ftri = DBuf<uint32_t>(P, Tf.cap);
+    launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L,
+                       (uint64_t)Tf.cap);
     KCHK();
   }
   T.mark(7);
 
   /* ---- edge closure + dangling emptiness ---- */
-  DBuf<uint8_t> edge_in(P, Ealpha), eflag(P, Ealpha);
-  fill_u8(edge_in.get(), Ealpha, 0, s);
+  DBuf<uint8_t> edge_in(P, Ealpha.cap), eflag(P, Ealpha.cap);
+  fill_u8(edge_in.get(), Ealpha.cap, 0, s);
   launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
                     edge_in.get(), L);
-  if (Ealpha)
+  if (Ealpha.cap)
     ladder(
-        ac2_cap(Ealpha), KP_EDGE_AC2, -1,
+        ac2_cap(Ealpha.cap), KP_EDGE_AC2, -1,
         [&](Queue q) {
           zero_u32(nrest, 1, s);
           launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
@@ -580,59 +649,49 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
                                nullptr, eflag.get(), L);
         });
-  int64_t Ef = 0;
-  DBuf<uint32_t> pos_e(P, Ealpha + 1);
+  Cnt Ef(0);
+  DBuf<uint32_t> pos_e(P, Ealpha.cap + 1);
   {
-    auto tmp = scan_tmp(Ealpha);
-    scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s);
+    auto tmp = scan_tmp(Ealpha.cap);
+    scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s,
+                       dcnt + RC_EF);
     KCHK();
-    Ef = (int64_t)readback(E, s, pos_e.get() + Ealpha);
+    Ef = settle(RC_EF);
   }
 
   /* ---- emit final lists (original ids, reference order) ---- */
-  R->lists[0] = DBuf<int32_t>(P, 2 * Ef);
-  R->lists[1] = DBuf<int32_t>(P, 3 * Tf);
-  R->lists[2] = DBuf<int32_t>(P, 4 * Kf);
+  R->lists[0] = DBuf<int32_t>(P, 2 * Ef.cap);
+  R->lists[1] = DBuf<int32_t>(P, 3 * Tf.cap);
+  R->lists[2] = DBuf<int32_t>(P, 4 * Kf.cap);
   launch_emit_edges(Ealpha, ecol.get(), eleft.get(), eflag.get(), pos_e.get(),
-                    perm.get(), R->lists[0].get(), L);
+                    perm.get(), R->lists[0].get(), L, (uint64_t)Ef.cap);
   launch_emit_tris(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
                    tedge.get(), perm.get(), R->lists[1].get(), L);
   launch_emit_tets(Kf, ftets.get(), perm.get(), R->lists[2].get(), L);
   KCHK();
-  R->counts[1] = Ef;
-  R->counts[2] = Tf;
-  R->counts[3] = Kf;
+  if (rec) return; /* counts and statistics are read after the graph runs */
+  R->counts[1] = Ef.cap;
+  R->counts[2] = Tf.cap;
+  R->counts[3] = Kf.cap;
   {
-    const int64_t sz[9] = {Ecand, Ealpha, Tslots, Talpha, Kslots, Kalpha, Kf, Tf, Ef};
-    for (int q = 0; q < 9; ++q) R->sizes[q] = sz[q];
+    const Cnt* sz[9] = {&Ecand, &Ealpha, &Tslots, &Talpha, &Kslots,
+                        &Kalpha, &Kf, &Tf, &Ef};
+    for (int q = 0; q < 9; ++q) R->sizes[q] = sz[q]->cap;
     R->sizes[11] = n;
   }
-  st.edges_final = Ef;
-  st.tris_final = Tf;
-  st.tets_final = Kf;
+  st.edge_candidates = Ecand.cap;
+  st.edge_alpha = Ealpha.cap;
+  st.tri_alpha = Talpha.cap;
+  st.tet_alpha = Kalpha.cap;
+  st.edges_final = Ef.cap;
+  st.tris_final = Tf.cap;
+  st.tets_final = Kf.cap;
   T.mark(8);
 
-  unsigned long long hs[CTR_COUNT * CTR_SLOTS], h[CTR_COUNT];
-  HIPCHK(hipMemcpyAsync(hs, ctr.get(), sizeof(hs), hipMemcpyDeviceToHost, s));
+  unsigned long long hs[CTR_COUNT * CTR_SLOTS];
+  HIPCHK(hipMemcpyAsync(hs, ctr, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (int c = 0; c < CTR_COUNT; ++c) {
-    h[c] = 0;
-    for (int k = 0; k < CTR_SLOTS; ++k) h[c] += hs[c * CTR_SLOTS + k];
-  }
-  for (int f = 0; f < 6; ++f)
-    for (int q = 0; q < 4; ++q)
-      st.stage_evals[f][q] = (int64_t)h[CTR_STAGE + 4 * f + q];
-  /* hot-stage evaluations: one per candidate for the filters, one per
-   * (simplex, ball) pair tested for the emptiness scans */
-  const int hot = SG.staged ? 0 : 1;
-  st.tri_candidates = (int64_t)h[CTR_CAND + 1];
-  st.tet_candidates = (int64_t)h[CTR_CAND + 2];
-  host_evals[1] = st.tri_candidates;
-  host_evals[2] = st.tet_candidates;
-  for (int f = 0; f < 3; ++f) st.stage_evals[f][hot] += host_evals[f];
-  for (int q = 0; q < 3; ++q) st.stage_evals[3 + q][hot] += (int64_t)h[CTR_PAIRS + q];
-  st.sos_calls = (int64_t)h[CTR_SOS];
-  for (int q = 0; q < 3; ++q) st.ac2_pairs[q] = (int64_t)h[CTR_PAIRS + q];
+  stats_from_counters(hs, SG.staged != 0, host_evals[0], st);
 }
 }  // namespace PAC_V
 }  // namespace pac

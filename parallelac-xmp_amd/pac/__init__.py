@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpac.so")
 PAC_OK = 0
 PAC_ERR_IO, PAC_ERR_INVALID, PAC_ERR_DEGENERATE = -1, -2, -3
 PAC_ERR_NOMEM, PAC_ERR_DEVICE, PAC_ERR_OVERFLOW, PAC_ERR_RANGE = -4, -5, -6, -7
+PAC_ERR_STALE = -8
 CELL_EXACT, CELL_REFERENCE_RAW = 0, 1
 PREC_PLAIN_FP64, PREC_STAGED = 0, 1
 ORDER_REFERENCE, ORDER_CANONICAL = 0, 1
@@ -81,7 +82,8 @@ EXPORTS = (
     "pac_batch_stats", "pac_batch_free", "pac_alpha_adjoint", "pac_read_crd",
     "pac_write_output", "pac_free", "pac_synth_globule", "pac_synth_globule_f64",
     "pac_synth_batch", "pac_synth_stress", "pac_last_error", "pac_version",
-    "pac_autotune",
+    "pac_autotune", "pac_graph_create", "pac_graph_set_input", "pac_graph_launch",
+    "pac_graph_result", "pac_graph_info", "pac_graph_free",
 )
 
 _lib = None
@@ -128,6 +130,14 @@ def lib():
     L.pac_alpha_adjoint.argtypes = [vp, vp, vp, vp, vp]
     L.pac_autotune.argtypes = [vp, i64, vp, i64, ctypes.c_float, ctypes.c_float,
                                ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]
+    L.pac_graph_create.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float,
+                                   ctypes.POINTER(Options), ctypes.c_double,
+                                   ctypes.POINTER(vp)]
+    L.pac_graph_set_input.argtypes = [vp, vp, vp]
+    L.pac_graph_launch.argtypes = [vp, vp]
+    L.pac_graph_result.argtypes = [vp, ctypes.POINTER(vp)]
+    L.pac_graph_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.pac_graph_free.argtypes = [vp]
     L.pac_read_crd.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp),
                                ctypes.POINTER(i64)]
     L.pac_write_output.argtypes = [ctypes.c_char_p, vp, i64, ctypes.c_float, vp]
@@ -319,6 +329,56 @@ def compute_batch(xyzr, offsets, solvent, alpha, **kw):
     _check(lib().pac_compute_batch(ptr, off.ctypes.data, n_mol, solvent, alpha,
                                    ctypes.byref(o), ctypes.byref(h)))
     return BatchResult(h.value, n_mol)
+
+
+class Graph:
+    """A batch recorded as one HIP graph (pac_graph_*): launch() runs the
+    whole batch with one hipGraphLaunch, result() returns a BatchResult."""
+
+    def __init__(self, xyzr, offsets, solvent, alpha, headroom=-1.0, **kw):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.n_mol = off.shape[0] - 1
+        self._n = int(off[-1])
+        self._h = ctypes.c_void_p()
+        o = options(**kw)
+        if kw.get("device_inputs"):
+            ptr = ctypes.c_void_p(int(xyzr))
+        else:
+            a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+            ptr = a.ctypes.data
+        _check(lib().pac_graph_create(ptr, off.ctypes.data, self.n_mol, solvent,
+                                      alpha, ctypes.byref(o), headroom,
+                                      ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.pac_graph_free(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def set_input(self, xyzr, stream=None):
+        """new coordinates (host n x 4 FP32 array, or an int device pointer)"""
+        if isinstance(xyzr, int):
+            ptr = ctypes.c_void_p(xyzr)
+        else:
+            a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+            if a.shape[0] != self._n:
+                raise ValueError("ball count differs from the recorded batch")
+            ptr = a.ctypes.data
+        _check(lib().pac_graph_set_input(self._h, ptr, stream))
+
+    def launch(self, stream=None):
+        _check(lib().pac_graph_launch(self._h, stream))
+
+    def result(self):
+        h = ctypes.c_void_p()
+        _check(lib().pac_graph_result(self._h, ctypes.byref(h)))
+        return BatchResult(h.value, self.n_mol)
+
+    def info(self):
+        """(kernel and other nodes in the graph, device bytes it holds)"""
+        nodes, nbytes = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().pac_graph_info(self._h, ctypes.byref(nodes), ctypes.byref(nbytes)))
+        return nodes.value, nbytes.value
 
 
 def autotune(xyzr, solvent, alpha, offsets=None, **kw):

@@ -77,3 +77,16 @@ def test_batch_argument_errors():
         with pytest.raises(pac.PacError) as e:
             pac.compute_batch(xyzr, np.array(bad), 1.4, 0.0)
         assert e.value.code == pac.PAC_ERR_INVALID
+
+
+def test_graph_argument_errors():
+    """recorded batches validate on the host before any device work"""
+    xyzr = np.zeros((4, 4), dtype=np.float32)
+    off = np.array([0, 4])
+    for kw in (dict(order=pac.ORDER_CANONICAL), dict(headroom=100.0)):
+        with pytest.raises(pac.PacError) as e:
+            pac.Graph(xyzr, off, 1.4, 0.0, **kw)
+        assert e.value.code == pac.PAC_ERR_INVALID
+    with pytest.raises(pac.PacError) as e:
+        pac.Graph(xyzr, np.array([0, 3, 2, 4]), 1.4, 0.0)
+    assert e.value.code == pac.PAC_ERR_INVALID
