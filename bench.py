@@ -266,6 +266,30 @@ def bench_batch(args, torch, pac):
                                  nthreads=min(16, os.cpu_count() or 1))
             got = r_st.lists(m)
             oracle_ok &= all(np.array_equal(got[d], o[d]) for d in (1, 2, 3))
+    # the same batch recorded once as one HIP graph: each step is one
+    # hipGraphLaunch of every pass (recording untimed)
+    log("graph: recording the batch")
+    t0 = time.perf_counter()
+    g = pac.Graph(ptr, off, 1.4, 0.0, precision=pac.PREC_STAGED,
+                  device_inputs=True, **tuned[pac.PREC_STAGED])
+    t_rec = time.perf_counter() - t0
+    nodes, gbytes = g.info()
+    for _ in range(args.warmup):
+        g.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    torch.cuda.synchronize()
+    t_gr = (time.perf_counter() - t0) / args.steps
+    rg = g.result()
+    graph_eq = bool(np.array_equal(rg.counts(), r_st.counts()))
+    for m in sample:
+        got, want = rg.lists(m), r_st.lists(m)
+        graph_eq &= all(np.array_equal(got[d], want[d]) for d in (1, 2, 3))
+    log("graph: %.1f ms per launch (eager %.1f ms), %d nodes, %.1f GB, eq %s"
+        % (t_gr * 1e3, t_st * 1e3, nodes, gbytes / 1e9, graph_eq))
+    del g
     classified = _classified(st)
     line = {
         "metric": "classified simplices/s, 4096-molecule ensemble, bit-exact per molecule",
@@ -279,6 +303,11 @@ def bench_batch(args, torch, pac):
         "molecules_per_s": n_mol / t_st, "balls_per_s": ntot / t_st,
         "tuned": tuned[pac.PREC_STAGED],
         "plain_fp64": {"value": classified / t_pl, "ms_per_step": t_pl * 1e3},
+        "graph": {"value": classified / t_gr, "ms_per_step": t_gr * 1e3,
+                  "launches_per_step": 1, "nodes": nodes,
+                  "device_bytes": gbytes, "record_s": round(t_rec, 2),
+                  "device_ms_last_launch": round(rg.stats().total_ms, 3),
+                  "eq_eager": graph_eq},
         "check": {"sampled_molecules": len(sample), "eq_single_runs": bool(same_single),
                   "eq_oracle_2": oracle_ok},
         "counts": {"classified": classified, "edges": int(st.edges_final),

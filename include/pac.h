@@ -52,7 +52,10 @@ typedef struct pac_options {
   int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
   int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
   int32_t ac2_grid;      /* blocks of the second emptiness phase (0=2048) */
-  int32_t reserved[3];
+  int32_t ac2_fine;      /* exact policy: emptiness scans over the grid's
+                            cells split in ac2_fine (2..8) along x
+                            (0 = 2, < 0 or 1: the grid's own cells)       */
+  int32_t reserved[2];
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -161,13 +164,16 @@ int pac_graph_info(const pac_graph* g, int64_t* nodes, int64_t* bytes);
 void pac_graph_free(pac_graph* g);
 
 /* Autotuner: times the pipeline on the given balls for each candidate
- * setting of the tunables (emptiness phase-one pair budget, second-phase
- * grid) and writes the fastest into opt->ac2_budget / opt->ac2_grid; the
- * other fields of *opt are used as given.  offsets == NULL: the n balls
- * are one molecule; otherwise they are a batch (offsets[n_mol+1]) and the
- * leading molecules, up to about 2^20 balls, are the sample.  best_ms (may
- * be NULL) receives the fastest device time of one run.  Outputs never
- * depend on these settings.  No reference counterpart. */
+ * setting of the tunables and writes the fastest into opt->ac2_fine (the
+ * emptiness scan grid, exact cell policy: x subdivision 2, 4 or the
+ * reference grid, tried first), then opt->ac2_budget / opt->ac2_grid
+ * (emptiness phase-one pair budget x second-phase grid, over the chosen
+ * scan grid); the other fields of *opt are used as given.  offsets ==
+ * NULL: the n balls are one molecule; otherwise they are a batch
+ * (offsets[n_mol+1]) and the leading molecules, up to about 2^20 balls,
+ * are the sample.  best_ms (may be NULL) receives the fastest wall time
+ * of one run.  Outputs never depend on these settings.  No reference
+ * counterpart. */
 int pac_autotune(const float* xyzr, int64_t n, const int64_t* offsets,
                  int64_t n_mol, float solvent, float alpha, pac_options* opt,
                  double* best_ms);

@@ -138,9 +138,9 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 
 /* per-simplex setup at precision R: D, N with a-priori error bounds, the
  * certified centre box and the semi-static pair constants. */
-template <class R>
+template <class R, class Box>
 __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
-                                          PairSS<R>& P, CellBox& box) {
+                                          PairSS<R>& P, Box& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
   R vx = (R)v[2].x - ax, vy = (R)v[2].y - ay, vz = (R)v[2].z - az;
@@ -174,8 +174,7 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
   if (!pairss_init(P, pa, waf, Df, Nf)) return false;
   double c[3], e[3], rho_up;
   if (!centre_err(Df, Nf, v[0], waf, c, e, &rho_up)) return false;
-  return make_box_r(g, simplex_reach(g, rho_up), c[0], c[1], c[2], e[0], e[1],
-                    e[2], box);
+  return place_box(g, simplex_reach(g, rho_up), c, e, box);
 }
 
 /* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
@@ -183,8 +182,12 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
  * (grid-stride over `rest`): the remaining pairs of those tets, skipping the
  * first `budget` ones.  Splitting keeps lanes of a wave busy with similar
  * amounts of work: most tets are rejected by one of their first few
- * neighbours, while accepted tets must test every ball of their box. */
-template <class R, bool PHASE_B>
+ * neighbours, while accepted tets must test every ball of their box.
+ * FINE: the scan runs over the fine grid's cells that meet the sphere
+ * (balls in fine-cell order, fid their ids).  A vertex of the simplex lies
+ * on its orthosphere, so its certified test is always undecided: the
+ * fine scan looks a ball's id up only then, to skip the vertices. */
+template <class R, bool PHASE_B, bool FINE>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     Stage S, Cnt c_nk, const uint4* tets, uint8_t* flag, Queue q1,
     uint32_t* rest, unsigned* nrest, int budget) {
@@ -196,9 +199,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     uint4 t = tets[id];
     Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
-    const GridP g = grid_of(S, t.x);
+    const GridP g = FINE ? sgrid_of(S, t.x) : grid_of(S, t.x);
     PairSS<R> P;
-    CellBox box;
+    typename std::conditional<FINE, SphBox, CellBox>::type box;
     if (!tet_setup(g, v, P, box)) {
       if (!PHASE_B) {
         push_q(q1, (uint32_t)id, 0xffffffffu);
@@ -208,8 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
-      if (p == t.x || p == t.y || p == t.z || p == t.w) return false;
+    auto visit = [&](uint32_t j, const Ball& E) {
+      if (!FINE && (j == t.x || j == t.y || j == t.z || j == t.w)) return false;
       if (PHASE_B) {
         if (seen < budget) {
           ++seen;
@@ -225,6 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       ++evals;
       int s = pairss_sign(P, E);
       if (s == 0) {
+        const uint32_t p = FINE ? S.fid[j] : j;
+        if (FINE && (p == t.x || p == t.y || p == t.z || p == t.w)) return false;
         pending = true;
         push_q(q1, (uint32_t)id, p);
         return false;
@@ -234,7 +239,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         return true;
       }
       return false;
-    });
+    };
+    if constexpr (FINE)
+      scan_sph4(g, S.fstart, S.fball, box, visit);
+    else
+      scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -455,9 +464,9 @@ __device__ __forceinline__ int low_pair_code(const LowQ<T>& t,
 }
 
 
-template <class R, int KIND>
+template <class R, int KIND, class Box>
 __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
-                                          PairSS<R>& P, CellBox& box) {
+                                          PairSS<R>& P, Box& box) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
   R wa = (R)v[0].w * (R)v[0].w, w1 = (R)v[1].w * (R)v[1].w;
@@ -503,14 +512,13 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
   if (!pairss_init(P, pa, waf, den, num) || P.sden < 0) return false;
   double c[3], e[3], rho_up;
   if (!centre_err(den, num, v[0], waf, c, e, &rho_up)) return false;
-  return make_box_r(g, simplex_reach(g, rho_up), c[0], c[1], c[2], e[0], e[1],
-                    e[2], box);
+  return place_box(g, simplex_reach(g, rho_up), c, e, box);
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
  * Simplices already in the closure (in_flag) pass without a test.  Same
  * phase A / phase B split as the tet kernel. */
-template <class R, int KIND, bool PHASE_B>
+template <class R, int KIND, bool PHASE_B, bool FINE>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     Stage S, Cnt c_nitems, const uint32_t* ecol, const uint32_t* eleft,
     const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
@@ -537,9 +545,9 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       vid[2] = vid[1];
     }
     Ball v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
-    const GridP g = grid_of(S, vid[0]);
+    const GridP g = FINE ? sgrid_of(S, vid[0]) : grid_of(S, vid[0]);
     PairSS<R> P;
-    CellBox box;
+    typename std::conditional<FINE, SphBox, CellBox>::type box;
     if (!low_setup<R, KIND>(g, v, P, box)) {
       if (!PHASE_B) {
         push_q(q1, (uint32_t)id, 0xffffffffu);
@@ -549,8 +557,8 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
     bool pending = false, dead = false, unfinished = false;
     int seen = 0;
-    scan_box4(g, S.cstart, S.ball, box, [&](uint32_t p, const Ball& E) {
-      if (p == vid[0] || p == vid[1] || p == vid[2]) return false;
+    auto visit = [&](uint32_t j, const Ball& E) {
+      if (!FINE && (j == vid[0] || j == vid[1] || j == vid[2])) return false;
       if (PHASE_B) {
         if (seen < budget) {
           ++seen;
@@ -570,11 +578,17 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
         return true;
       }
       if (s == 0) {
+        const uint32_t p = FINE ? S.fid[j] : j;
+        if (FINE && (p == vid[0] || p == vid[1] || p == vid[2])) return false;
         pending = true;
         push_q(q1, (uint32_t)id, p);
       }
       return false;
-    });
+    };
+    if constexpr (FINE)
+      scan_sph4(g, S.fstart, S.fball, box, visit);
+    else
+      scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {
       if (dead)
         flag[id] = 0;
@@ -770,28 +784,34 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 
 
 /* ---------------- launchers ---------------- */
+template <class R, bool FINE>
+static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
+                     Queue q1, uint32_t* rest, unsigned* nrest, int budget,
+                     PipeLaunch L) {
+  dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE>), gr, dim3(kBlock), 0, L.s,
+                     S, nk, tets, flag, q1, rest, nrest, budget);
+  if (budget > 0)
+    hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE>), gb, dim3(kBlock), 0, L.s,
+                       S, nk, tets, flag, q1, rest, nrest, budget);
+}
 void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, int budget, PipeLaunch L) {
   if (nk.cap <= 0) return;
-  dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   if constexpr (!kBallF64) {
     if (S.staged) {
-      hipLaunchKernelGGL((k_tet_ac2_hot<float, false>), gr, dim3(kBlock), 0,
-                         L.s, S, nk, tets, flag, q1, rest, nrest, budget);
-      if (budget > 0)
-        hipLaunchKernelGGL((k_tet_ac2_hot<float, true>), gb, dim3(kBlock), 0,
-                           L.s, S, nk, tets, flag, q1, rest, nrest, budget);
+      if (S.fine)
+        tet_hot2<float, true>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+      else
+        tet_hot2<float, false>(S, nk, tets, flag, q1, rest, nrest, budget, L);
       return;
     }
   }
-  {
-    hipLaunchKernelGGL((k_tet_ac2_hot<double, false>), gr, dim3(kBlock), 0,
-                       L.s, S, nk, tets, flag, q1, rest, nrest, budget);
-    if (budget > 0)
-      hipLaunchKernelGGL((k_tet_ac2_hot<double, true>), gb, dim3(kBlock), 0,
-                         L.s, S, nk, tets, flag, q1, rest, nrest, budget);
-  }
+  if (S.fine)
+    tet_hot2<double, true>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+  else
+    tet_hot2<double, false>(S, nk, tets, flag, q1, rest, nrest, budget, L);
 }
 void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,
@@ -807,20 +827,33 @@ void launch_tet_ac2_exact(const Stage& S, QRef q,
   hipLaunchKernelGGL(k_tet_ac2_exact, dim3((q.launch_items() + kBlock - 1) / kBlock),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
+template <class R, int KIND, bool FINE>
+static void low_hot3(const Stage& S, Cnt nitems, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint32_t* tk,
+                     const uint32_t* tedge, const uint8_t* in_flag,
+                     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
+                     int budget, PipeLaunch L) {
+  dim3 gr((unsigned)((nitems.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false, FINE>), gr, dim3(kBlock), 0,
+                     L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
+                     rest, nrest, budget);
+  if (budget > 0)
+    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true, FINE>), gb, dim3(kBlock), 0,
+                       L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
+                       q1, rest, nrest, budget);
+}
 template <class R, int KIND>
 static void low_hot2(const Stage& S, Cnt nitems, const uint32_t* ecol,
                      const uint32_t* eleft, const uint32_t* tk,
                      const uint32_t* tedge, const uint8_t* in_flag,
                      uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
                      int budget, PipeLaunch L) {
-  dim3 gr((unsigned)((nitems.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
-  hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false>), gr, dim3(kBlock), 0, L.s,
-                     S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1, rest,
-                     nrest, budget);
-  if (budget > 0)
-    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true>), gb, dim3(kBlock), 0,
-                       L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
-                       q1, rest, nrest, budget);
+  if (S.fine)
+    low_hot3<R, KIND, true>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
+                            q1, rest, nrest, budget, L);
+  else
+    low_hot3<R, KIND, false>(S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
+                             q1, rest, nrest, budget, L);
 }
 void launch_low_ac2_hot(const Stage& S, int kind, Cnt nitems,
                         const uint32_t* ecol, const uint32_t* eleft,

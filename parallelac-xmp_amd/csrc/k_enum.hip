@@ -271,6 +271,66 @@ void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
                      dim3(kBlock), 0, L.s, cellk, n, G, cstart);
 }
 
+/* Fine scan grid: each reference cell split into kx cells along x.  One
+ * thread per reference cell partitions its balls (in order) by x
+ * sub-cell, (x - minx) * kx / step in double clamped into the cell: a ball
+ * the FP32 key put in a neighbouring cell is off by less than the grid's
+ * margin, which every scan radius includes. */
+__global__ __launch_bounds__(kBlock) void k_fine_split(
+    const Ball* ball, const uint32_t* cstart, int64_t G, int kx, GridP g1,
+    const GridP* grids, const uint32_t* mol, Ball* fball, uint32_t* fid,
+    uint32_t* fstart) {
+  const int64_t c = gtid();
+  if (c >= G) return;
+  const uint32_t b0 = cstart[c], b1 = cstart[c + 1];
+  uint32_t cnt[kMaxFine];
+#pragma unroll
+  for (int q = 0; q < kMaxFine; ++q) cnt[q] = 0;
+  int base = 0;
+  GridP g = g1;
+  if (b1 > b0) {
+    if (mol) g = grids[mol[b0]];
+    /* g is the fine grid: its cell ids and x extent are kx times the
+     * reference grid's */
+    const int64_t local = c - (int64_t)(g.cbase / (uint32_t)kx);
+    base = (int)(local % (g.ex / kx)) * kx; /* first fine x index of the cell */
+    for (uint32_t i = b0; i < b1; ++i) {
+      const double q = ((double)ball[i].x - (double)g.minx) * g.inv_step - base;
+      const int s = q >= 1.0 ? min((int)fmin(q, 64.0), kx - 1) : 0;
+#pragma unroll
+      for (int t = 0; t < kMaxFine; ++t) cnt[t] += (t == s);
+    }
+  }
+  uint32_t off[kMaxFine];
+  uint32_t acc = b0;
+#pragma unroll
+  for (int t = 0; t < kMaxFine; ++t) {
+    off[t] = acc;
+    if (t < kx) fstart[c * kx + t] = acc;
+    acc += cnt[t];
+  }
+  if (c == G - 1) fstart[G * kx] = b1;
+  for (uint32_t i = b0; i < b1; ++i) {
+    const double q = ((double)ball[i].x - (double)g.minx) * g.inv_step - base;
+    const int s = q >= 1.0 ? min((int)fmin(q, 64.0), kx - 1) : 0;
+    uint32_t at = 0;
+#pragma unroll
+    for (int t = 0; t < kMaxFine; ++t)
+      if (t == s) at = off[t]++;
+    fball[at] = ball[i];
+    fid[at] = i;
+  }
+}
+void launch_fine_split(const Ball* ball, const uint32_t* cstart, int64_t G,
+                       int kx, GridP g, const GridP* grids,
+                       const uint32_t* mol, Ball* fball, uint32_t* fid,
+                       uint32_t* fstart, PipeLaunch L) {
+  if (G <= 0) return;
+  hipLaunchKernelGGL(k_fine_split, dim3((unsigned)((G + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, ball, cstart, G, kx, g, grids, mol,
+                     fball, fid, fstart);
+}
+
 /* ------------------------------------------------------------------ */
 /* filter decisions (one stage) and exact decisions                    */
 /* ------------------------------------------------------------------ */

@@ -269,9 +269,9 @@ __device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
   double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
   double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
   int ext[3] = {g.ex, g.ey, g.ez};
-  const double inv = g.inv_step;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
+    const double inv = a == 0 ? g.inv_step : g.inv_yz;
     double r = reach + margin + e[a];
     double ql = (c[a] - r - mn[a]) * inv, qh = (c[a] + r - mn[a]) * inv;
     double qc = (c[a] - mn[a]) * inv;
@@ -401,6 +401,114 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
   uint32_t lo, hi;
   while (box_next(g, cstart, b, it, lo, hi))
     if (scan_range4(ball, lo, hi, visit)) return;
+}
+
+/* ---- emptiness scans over the fine grid ----
+ * The cells visited are those of the certified box that meet the sphere
+ * of radius R = reach + margin + |centre error| about the computed centre
+ * (a violator lies within reach of the exact centre, which lies within the
+ * error of the computed one): the box rows (y, z) whose cell square comes
+ * within R, each cut to the sphere's x-extent over the row's nearest
+ * point, rows nearest the centre first.  Worked in cells relative to the
+ * box's low corner, in FP32, with R inflated far past the rounding (every
+ * ball lies inside its own fine cell: keys are computed in double). */
+struct SphBox {
+  CellBox b;
+  float fx, fy, fz; /* centre minus the box's low corner, in cells */
+  float R2;         /* squared radius in y-z cells, rounded up */
+  float kx;         /* x cells per y-z cell */
+  int cy, cz;       /* the box row nearest the centre */
+};
+__device__ __forceinline__ int clamp_cell(float f, int lo, int hi) {
+  const float q = floorf(fminf(fmaxf(f, -1.0e6f), 1.0e6f));
+  const int c = lo + (int)q;
+  return c < lo ? lo : (c > hi ? hi : c);
+}
+__device__ __forceinline__ bool make_sph(const GridP& g, double reach,
+                                         const double c[3], const double e[3],
+                                         SphBox& s) {
+  if (!make_box_r(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], s.b))
+    return false;
+  const double iyz = g.inv_yz;
+  const double el = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const double R = (reach + g.margin + el) * iyz * (1 + 1e-6) + 1e-3;
+  s.fx = (float)((c[0] - (double)g.minx) * g.inv_step - s.b.lo[0]);
+  s.fy = (float)((c[1] - (double)g.miny) * iyz - s.b.lo[1]);
+  s.fz = (float)((c[2] - (double)g.minz) * iyz - s.b.lo[2]);
+  s.R2 = (float)(R * R) * 1.0001f;
+  s.kx = (float)(g.inv_step / iyz) * 1.0001f;
+  s.cy = clamp_cell(s.fy, s.b.lo[1], s.b.hi[1]);
+  s.cz = clamp_cell(s.fz, s.b.lo[2], s.b.hi[2]);
+  return true;
+}
+/* c, c-1, c+1, c-2, c+2, ... */
+__device__ __forceinline__ int outward(int c, int k) {
+  return c + ((k & 1) ? -((k + 1) >> 1) : (k >> 1));
+}
+/* squared distance from f to the cell [q, q+1) */
+__device__ __forceinline__ float axis_d2(float f, int q) {
+  const float d = fmaxf(0.f, fmaxf((float)q - f, f - (float)(q + 1)));
+  return d * d;
+}
+struct SphIter {
+  int kz, ky, z;
+  float dz2;
+};
+__device__ __forceinline__ bool sph_next(const GridP& g, const uint32_t* cstart,
+                                         const SphBox& s, SphIter& it,
+                                         uint32_t& lo, uint32_t& hi) {
+  const CellBox& b = s.b;
+  const int nzk = 2 * max(s.cz - b.lo[2], b.hi[2] - s.cz);
+  const int nyk = 2 * max(s.cy - b.lo[1], b.hi[1] - s.cy);
+  for (;;) {
+    if (it.ky > nyk) { /* next plane */
+      if (++it.kz > nzk) return false;
+      it.z = outward(s.cz, it.kz);
+      it.ky = nyk + 1;
+      if (it.z < b.lo[2] || it.z > b.hi[2]) continue;
+      it.dz2 = axis_d2(s.fz, it.z - b.lo[2]);
+      if (it.dz2 > s.R2) continue;
+      it.ky = 0;
+    }
+    const int y = outward(s.cy, it.ky++);
+    if (y < b.lo[1] || y > b.hi[1]) continue;
+    const float d2 = it.dz2 + axis_d2(s.fy, y - b.lo[1]);
+    if (d2 > s.R2) continue;
+    const float rx = sqrtf(s.R2 - d2) * s.kx + 1e-3f;
+    const int xl = max(0, (int)floorf(fmaxf(s.fx - rx, -1.0e6f)));
+    const int xh = min(b.hi[0] - b.lo[0], (int)floorf(fminf(s.fx + rx, 1.0e6f)));
+    if (xl > xh) continue;
+    const int64_t row = (int64_t)g.cbase + b.lo[0] +
+                        (int64_t)g.ex * (y + (int64_t)g.ey * it.z);
+    lo = cstart[row + xl];
+    hi = cstart[row + xh + 1];
+    if (hi > lo) return true;
+  }
+}
+template <class F>
+__device__ __forceinline__ void scan_sph4(const GridP& g,
+                                          const uint32_t* fstart,
+                                          const Ball* fball, const SphBox& s,
+                                          F&& visit) {
+  const CellBox& b = s.b;
+  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
+  SphIter it{-1, 1 << 30, 0, 0.f};
+  uint32_t lo, hi;
+  while (sph_next(g, fstart, s, it, lo, hi))
+    if (scan_range4(fball, lo, hi, visit)) return;
+}
+__device__ __forceinline__ bool place_box(const GridP& g, double reach,
+                                          const double c[3], const double e[3],
+                                          CellBox& b) {
+  return make_box_r(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], b);
+}
+__device__ __forceinline__ bool place_box(const GridP& g, double reach,
+                                          const double c[3], const double e[3],
+                                          SphBox& b) {
+  return make_sph(g, reach, c, e, b);
+}
+__device__ __forceinline__ GridP sgrid_of(const Stage& S, uint32_t b) {
+  return S.sgrids ? S.sgrids[S.mol[b]] : S.sg;
 }
 
 /* Centre c = p0 + num / (2 den) and per-axis error bounds from certified

@@ -439,39 +439,60 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
     src = (const float*)dev.get();
     base.device_inputs = 1;
   }
+  /* device time of one run with settings t: best of two after a warm-up */
+  auto time_one = [&](const pac_options& t) {
+    double tmin = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {
+      auto t0 = std::chrono::steady_clock::now();
+      if (offsets) {
+        pac_batch_result* r = nullptr;
+        compute_batch(src, offsets, ms, solvent, alpha, &t, &r);
+        delete r;
+      } else {
+        pac_result* r = nullptr;
+        compute_one<float>(src, ns, solvent, alpha, &t, &r);
+        delete r;
+      }
+      double dt = std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t0).count();
+      if (rep > 0) tmin = std::min(tmin, dt);
+    }
+    return tmin;
+  };
+  /* the scan grid first (at the default budget and grid), then the
+   * phase-one budget x second-phase grid over it */
+  static const int fines[] = {2, 4, -1};
   static const int budgets[] = {4, 6, 8, 12, 16, 24};
   static const int grids[] = {1024, 2048, 4096};
   double best = 1e300;
   pac_options pick = base;
+  pick.profile = 0;
+  if (base.cell_policy == PAC_CELL_EXACT)
+    for (int f : fines) {
+      pac_options t = pick;
+      t.ac2_fine = f;
+      const double tm = time_one(t);
+      if (tm < best) {
+        best = tm;
+        pick.ac2_fine = f;
+      }
+    }
+  const pac_options fixed = pick;
+  best = 1e300;
   for (int b : budgets)
     for (int g : grids) {
-      pac_options t = base;
+      pac_options t = fixed;
       t.ac2_budget = b;
       t.ac2_grid = g;
-      t.profile = 0;
-      double tmin = 1e300;
-      for (int rep = 0; rep < 3; ++rep) {
-        auto t0 = std::chrono::steady_clock::now();
-        if (offsets) {
-          pac_batch_result* r = nullptr;
-          compute_batch(src, offsets, ms, solvent, alpha, &t, &r);
-          delete r;
-        } else {
-          pac_result* r = nullptr;
-          compute_one<float>(src, ns, solvent, alpha, &t, &r);
-          delete r;
-        }
-        double dt = std::chrono::duration<double, std::milli>(
-                        std::chrono::steady_clock::now() - t0).count();
-        if (rep > 0) tmin = std::min(tmin, dt);
-      }
-      if (tmin < best) {
-        best = tmin;
+      const double tm = time_one(t);
+      if (tm < best) {
+        best = tm;
         pick = t;
       }
     }
   o->ac2_budget = pick.ac2_budget;
   o->ac2_grid = pick.ac2_grid;
+  o->ac2_fine = pick.ac2_fine;
   if (best_ms) *best_ms = best;
 }
 

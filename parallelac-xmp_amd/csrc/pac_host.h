@@ -330,6 +330,10 @@ struct PassPlan {
   std::vector<Bounds> bounds; /* per molecule, as reduced on the device */
   std::vector<GridP> grids;
   int64_t G = 0;
+  int kf = 0;                  /* fine scan grid subdivision (0: none) */
+  std::vector<GridP> sgrids;   /* fine scan grids */
+  int64_t Gf = 0;
+  DBuf<GridP> dsgrids;
   int S = 0;
   WI alpha_wi;
   bool no_edges = true;

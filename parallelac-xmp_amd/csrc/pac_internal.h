@@ -17,6 +17,7 @@ namespace pac {
 constexpr int kBlock = 256;      /* threads per block, 4 wave64s */
 constexpr int kScanItems = 8;    /* items per thread in scans / radix */
 constexpr int kTile = kBlock * kScanItems;
+constexpr int kMaxFine = 8;      /* largest x subdivision of a scan grid */
 
 /* Reference grid (AlphaComplexParallel.cu:1120-1124) of one molecule plus
  * the stencil half-widths used for enumeration and emptiness scans.  In a
@@ -30,7 +31,9 @@ struct GridP {
   uint32_t cbase;
   double reach;   /* sqrt(max w + alpha): violators lie this close to the centre */
   double margin;  /* FP32 cell-key rounding, in length units */
-  double inv_step;
+  double inv_step; /* 1 / cell size along x */
+  double inv_yz;   /* 1 / cell size along y and z (a fine scan grid splits
+                      cells along x only; the reference grid is cubic) */
   double maxw;    /* max r^2 of the molecule; < 0: keep the global reach */
 };
 
@@ -232,6 +235,13 @@ void launch_gather(const Ball* raw, const uint32_t* sidx,
                    PipeLaunch L);
 void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
                         uint32_t* cstart, PipeLaunch L);
+/* fine scan grid: reference cells split into kx (<= kMaxFine) cells
+ * along x; fball / fid the balls and their ids in fine-cell order,
+ * fstart[G kx + 1] the fine cell starts; g / grids: the fine grids */
+void launch_fine_split(const Ball* ball, const uint32_t* cstart, int64_t G,
+                       int kx, GridP g, const GridP* grids,
+                       const uint32_t* mol, Ball* fball, uint32_t* fid,
+                       uint32_t* fstart, PipeLaunch L);
 
 struct Stage {
   const Ball* ball;
@@ -246,6 +256,14 @@ struct Stage {
   unsigned long long* ctr;
   int staged;
   int bgrid;     /* blocks of the second emptiness phase */
+  /* exact cell policy: the emptiness scans run over a finer grid of their
+   * own (reference cells split along x), its balls in fine-cell order */
+  int fine;
+  GridP sg;              /* fine grid of a single molecule */
+  const GridP* sgrids;   /* batch: fine grid of each molecule */
+  const Ball* fball;     /* balls in fine-cell order */
+  const uint32_t* fid;   /* their ball (cell-sorted) ids */
+  const uint32_t* fstart;
 };
 
 

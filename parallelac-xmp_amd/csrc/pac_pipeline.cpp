@@ -105,6 +105,7 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
   g.reach = reach;
   g.margin = margin;
   g.inv_step = 1.0 / (double)step;
+  g.inv_yz = g.inv_step;
   g.maxw = policy == PAC_CELL_REFERENCE_RAW ? -1.0 : maxw;
   return P;
 }
@@ -152,6 +153,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
       std::memset(&gp.g, 0, sizeof(gp.g));
       gp.g.step = 1;
       gp.g.inv_step = 1;
+      gp.g.inv_yz = 1;
       gp.g.maxw = -1;
       gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
       gp.G = 1;
@@ -168,6 +170,25 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
     pl.k_enum = std::max(pl.k_enum, gp.g.k_enum);
     pl.k_ac2 = std::max(pl.k_ac2, gp.g.k_ac2);
   }
+  /* Fine scan grids (exact policy): each reference cell split into kx
+   * cells along x (same origin, rows and y, z cells), so that each scan
+   * row is cut to the sphere's x-extent finely at no cost in rows. */
+  pl.kf = 0;
+  pl.Gf = 0;
+  pl.sgrids.clear();
+  const int kx = opt.ac2_fine > 0 ? std::min(opt.ac2_fine, kMaxFine) : 2;
+  if (opt.cell_policy == PAC_CELL_EXACT && opt.ac2_fine >= 0 && kx >= 2 &&
+      pl.G * kx < ((int64_t)1 << 31)) {
+    pl.kf = kx;
+    pl.Gf = pl.G * kx;
+    pl.sgrids = pl.grids;
+    for (GridP& f : pl.sgrids) {
+      f.inv_yz = 1.0 / (double)f.step;
+      f.inv_step = kx / (double)f.step;
+      f.ex *= kx;
+      f.cbase *= (uint32_t)kx;
+    }
+  }
   /* exact embedding scale (one for all molecules: any common scale at
    * which every input is an integer gives the same exact signs) */
   pl.S = std::max(shift, (neg_lsb_exp_host(alpha) + 1) / 2);
@@ -183,6 +204,11 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
     pl.dmol = DBuf<uint32_t>(P, n);
     launch_mol_ids(pl.dmoff.get(), nmol, n, pl.dmol.get(), L);
     KCHK();
+    if (pl.kf) {
+      pl.dsgrids = DBuf<GridP>(P, nmol);
+      HIPCHK(hipMemcpyAsync(pl.dsgrids.get(), pl.sgrids.data(),
+                            nmol * sizeof(GridP), hipMemcpyHostToDevice, s));
+    }
   }
 }
 
@@ -298,6 +324,25 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   SG.ball = ball.get();
   SG.cellk = cellk.get();
   SG.cstart = cstart.get();
+
+  /* ---- fine scan grid: reference cells split along x ---- */
+  DBuf<Ball> fball;
+  DBuf<uint32_t> fid, fstart;
+  if (pl.kf) {
+    const GridP* dsg = nmol > 1 ? pl.dsgrids.get() : nullptr;
+    fball = DBuf<Ball>(P, n);
+    fid = DBuf<uint32_t>(P, n);
+    fstart = DBuf<uint32_t>(P, pl.Gf + 1);
+    launch_fine_split(SG.ball, SG.cstart, G, pl.kf, pl.sgrids[0], dsg, SG.mol,
+                      fball.get(), fid.get(), fstart.get(), L);
+    KCHK();
+    SG.fine = 1;
+    SG.sg = pl.sgrids[0];
+    SG.sgrids = dsg;
+    SG.fball = fball.get();
+    SG.fid = fid.get();
+    SG.fstart = fstart.get();
+  }
   T.mark(2);
 
   auto scan_tmp = [&](int64_t m) { return DBuf<uint8_t>(P, scan_tmp_bytes(m) + 64); };
