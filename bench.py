@@ -49,6 +49,8 @@ def main():
                          "the autotuner, e.g. for profiler runs)")
     ap.add_argument("--grid", type=int, default=0,
                     help="second emptiness phase grid (blocks)")
+    ap.add_argument("--fine", type=int, default=0,
+                    help="emptiness scan grid x split (with --budget and --grid)")
     ap.add_argument("--workload", choices=("globule", "batch", "stress", "copy"),
                     default="globule",
                     help="globule: the 100k headline (default); batch: the "
@@ -87,7 +89,8 @@ def main():
     tuned = {}
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
         if args.budget and args.grid:
-            tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid}
+            tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
+                           "ac2_fine": args.fine}
         else:
             tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
                                           n=n, precision=prec)

@@ -565,64 +565,6 @@ void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      cseg, cea, ceb, ctr, lim);
 }
 
-/* tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k)
- * (computePossibleTetCount / fillPossibleTets, Parallel:418-543); each
- * candidate is written with its four vertex ids, so that the filter and
- * the compaction read one uint4 instead of following triangle -> edge ->
- * vertex links */
-__global__ __launch_bounds__(kBlock) void k_tet_enum(
-    Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
-    const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
-    const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
-    uint4* ctet, unsigned long long* ctr, uint64_t lim) {
-  const int64_t nt = c_nt.get();
-  constexpr int CAP = 2048;
-  __shared__ uint4 s_out[CAP];
-  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
-  if (q0 >= nt) return; /* whole block past the (device) count */
-  const int64_t q1 = q0 + kBlock < nt ? q0 + kBlock : nt;
-  BlockOut4<CAP> B{s_out, off[q0], off[q1] - off[q0], lim};
-  int64_t q = gtid();
-  unsigned found = 0;
-  if (q < nt) {
-    uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
-    const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
-    uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
-    uint64_t b = trow[eik], be = trow[eik + 1];
-    uint64_t c = trow[ejk], ce = trow[ejk + 1];
-    uint64_t o = off[q], oe = off[q + 1];
-    while (a < ae && b < be && c < ce) {
-      uint32_t x = tk[a], y = tk[b], z = tk[c];
-      uint32_t m = max(x, max(y, z));
-      if (x == m && y == m && z == m) {
-        B.put(ctet, o, make_uint4(i, j, k, m));
-        ++o;
-        ++found;
-        ++a;
-        ++b;
-        ++c;
-      } else {
-        a += (x < m);
-        b += (y < m);
-        c += (z < m);
-      }
-    }
-    for (; o < oe; ++o) B.put(ctet, o, make_uint4(0xffffffffu, 0u, 0u, 0u));
-  }
-  B.flush(ctet);
-  block_flush(ctr, CTR_CAND + 2, found);
-}
-void launch_tet_enum(Cnt nt, const uint64_t* trow, const uint32_t* tk,
-                     const uint32_t* tedge, const uint32_t* teik,
-                     const uint32_t* tejk, const uint32_t* eleft,
-                     const uint32_t* ecol, const uint64_t* off, uint4* ctet,
-                     unsigned long long* ctr, PipeLaunch L, uint64_t lim) {
-  if (nt.cap <= 0) return;
-  hipLaunchKernelGGL(k_tet_enum, dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, trow, tk, tedge, teik, tejk,
-                     eleft, ecol, off, ctet, ctr, lim);
-}
-
 /* ------------------------------------------------------------------ */
 /* alpha filters, one thread per candidate (uniform work per lane)     */
 /* ------------------------------------------------------------------ */
@@ -687,6 +629,93 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
     }
   }
 }
+/* Tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k)
+ * (computePossibleTetCount / fillPossibleTets, Parallel:418-543), fused
+ * with the tet filter's first stage (markRealTets, :531-591).  A block's
+ * candidate slots are one contiguous range, staged in LDS as (i, j, k, l)
+ * during the three-list merges; the block then filters the staged slots
+ * round robin (uniform work per lane), writes every slot's flag coalesced
+ * and the four vertex ids only of the slots that are kept or undecided
+ * (the later stages read no others).  Slots past the LDS stage were
+ * written to global memory during the merges and are filtered from there. */
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
+    Stage S, Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
+    const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
+    const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
+    uint4* ctet, uint8_t* flag, Queue q1, uint64_t lim) {
+  const int64_t nt = c_nt.get();
+  constexpr int CAP = 2048;
+  __shared__ uint4 s_out[CAP];
+  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  if (q0 >= nt) return; /* whole block past the (device) count */
+  const int64_t q1e = q0 + kBlock < nt ? q0 + kBlock : nt;
+  BlockOut4<CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
+  int64_t q = gtid();
+  unsigned found = 0;
+  if (q < nt) {
+    uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+    const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
+    uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
+    uint64_t b = trow[eik], be = trow[eik + 1];
+    uint64_t c = trow[ejk], ce = trow[ejk + 1];
+    uint64_t o = off[q], oe = off[q + 1];
+    while (a < ae && b < be && c < ce) {
+      uint32_t x = tk[a], y = tk[b], z = tk[c];
+      uint32_t m = max(x, max(y, z));
+      if (x == m && y == m && z == m) {
+        B.put(ctet, o, make_uint4(i, j, k, m));
+        ++o;
+        ++found;
+        ++a;
+        ++b;
+        ++c;
+      } else {
+        a += (x < m);
+        b += (y < m);
+        c += (z < m);
+      }
+    }
+    for (; o < oe; ++o) B.put(ctet, o, make_uint4(0xffffffffu, 0u, 0u, 0u));
+  }
+  __syncthreads(); /* LDS stage and the overflow slots written */
+  const uint64_t base = B.base;
+  const uint64_t m = base >= lim ? 0 : (B.total < lim - base ? B.total : lim - base);
+  for (uint64_t t = threadIdx.x; t < m; t += kBlock) {
+    const bool staged = t < (uint64_t)CAP;
+    const uint4 v = staged ? s_out[t] : ctet[base + t];
+    int r = REJECT;
+    if (v.x != 0xffffffffu) {
+      const Ball bv[4] = {S.ball[v.x], S.ball[v.y], S.ball[v.z], S.ball[v.w]};
+      r = filter_code<3, FE<R>>(bv, S.alpha);
+    }
+    flag[base + t] = (uint8_t)r;
+    if (r != REJECT && staged) ctet[base + t] = v;
+    if (r == UNDEC) push_q(q1, (uint32_t)(base + t), v.x);
+  }
+  block_flush(S.ctr, CTR_CAND + 2, found);
+}
+void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
+                            const uint32_t* tk, const uint32_t* tedge,
+                            const uint32_t* teik, const uint32_t* tejk,
+                            const uint32_t* eleft, const uint32_t* ecol,
+                            const uint64_t* off, uint4* ctet, uint8_t* flag,
+                            Queue q1, PipeLaunch L, uint64_t lim) {
+  if (nt.cap <= 0) return;
+  dim3 gr((unsigned)((nt.cap + kBlock - 1) / kBlock));
+  if constexpr (!kBallF64) {
+    if (S.staged) {
+      hipLaunchKernelGGL(k_tet_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
+                         nt, trow, tk, tedge, teik, tejk, eleft, ecol, off,
+                         ctet, flag, q1, lim);
+      return;
+    }
+  }
+  hipLaunchKernelGGL(k_tet_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S, nt,
+                     trow, tk, tedge, teik, tejk, eleft, ecol, off, ctet, flag,
+                     q1, lim);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
                                                        QRef q,
@@ -743,10 +772,8 @@ void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
                        nc, cval, cseg, flag, q1);
   if (kind == 1) {
     PAC_FILTER(1)
-  } else if (kind == 2) {
-    PAC_FILTER(2)
   } else {
-    PAC_FILTER(3)
+    PAC_FILTER(2)
   }
 #undef PAC_FILTER
 }

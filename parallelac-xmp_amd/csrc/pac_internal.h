@@ -286,12 +286,17 @@ void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
                      unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
-void launch_tet_enum(Cnt nt, const uint64_t* trow, const uint32_t* tk,
-                     const uint32_t* tedge, const uint32_t* teik,
-                     const uint32_t* tejk, const uint32_t* eleft,
-                     const uint32_t* ecol, const uint64_t* off, uint4* ctet,
-                     unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
-/* kind 1 edges, 2 triangles, 3 tets */
+/* tet candidates of the alpha triangles, each filtered by the first
+ * precision stage in the same kernel: flag per slot, (i, j, k, l) written
+ * for kept / undecided slots, undecided ones queued on q1 */
+void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
+                            const uint32_t* tk, const uint32_t* tedge,
+                            const uint32_t* teik, const uint32_t* tejk,
+                            const uint32_t* eleft, const uint32_t* ecol,
+                            const uint64_t* off, uint4* ctet, uint8_t* flag,
+                            Queue q1, PipeLaunch L, uint64_t lim = ~0ull);
+/* first stage: kind 1 edges, 2 triangles (tets: launch_tet_enum_filter);
+ * escalation and exact stages: kind 1, 2, 3 */
 void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
                    const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
                    Queue q1, PipeLaunch L);

@@ -454,13 +454,12 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
   auto filter_stage = [&](int kind, CandCtx C, Cnt nc, const uint32_t* cval,
                           const uint32_t* cseg, uint8_t* flag, int kp,
-                          DBuf<uint32_t>& pos, int q) -> Cnt {
+                          DBuf<uint32_t>& pos, int q, auto&& hot) -> Cnt {
     if (kind == 1) host_evals[0] += nc.cap; /* tris / tets: slots != candidates */
     /* few candidates stay undecided in float: start small, a rerun of the
      * (idempotent) filter sizes the queue exactly if it overflows */
     ladder(
-        std::min<int64_t>(nc.cap, nc.cap / 32 + 65536), kp, -1,
-        [&](Queue qq) { launch_filter(SG, kind, C, nc, cval, cseg, flag, qq, L); },
+        std::min<int64_t>(nc.cap, nc.cap / 32 + 65536), kp, -1, hot,
         [&](QRef qr, Queue q2) {
           launch_filter_esc(SG, kind, C, qr, cval, flag, q2, L);
         },
@@ -510,7 +509,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                      (uint64_t)Ecand.cap);
     DBuf<uint32_t> pos;
     Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
-                          KP_EDGE_FILL, pos, RC_EALPHA);
+                          KP_EDGE_FILL, pos, RC_EALPHA, [&](Queue qq) {
+                            launch_filter(SG, 1, C0, Ecand, cval.get(),
+                                          cseg.get(), flag.get(), qq, L);
+                          });
     launch_seg_starts(n, off.get(), pos.get(), erow.get(), L,
                       (uint64_t)Ecand.cap, (uint64_t)Ealpha.cap);
     ecol = DBuf<uint32_t>(P, Ealpha.cap);
@@ -548,7 +550,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                     (uint64_t)Tslots.cap);
     DBuf<uint32_t> pos;
     Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
-                          KP_TRI_FILL, pos, RC_TALPHA);
+                          KP_TRI_FILL, pos, RC_TALPHA, [&](Queue qq) {
+                            launch_filter(SG, 2, C1, Tslots, cval.get(),
+                                          cseg.get(), flag.get(), qq, L);
+                          });
     launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L,
                       (uint64_t)Tslots.cap, (uint64_t)Talpha.cap);
     tk = DBuf<uint32_t>(P, Talpha.cap);
@@ -576,13 +581,22 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     }
     DBuf<uint4> ctet(P, Kslots.cap);
     DBuf<uint8_t> flag(P, Kslots.cap);
-    launch_tet_enum(Talpha, trow.get(), tk.get(), tedge.get(), teik.get(),
-                    tejk.get(), eleft.get(), ecol.get(), off.get(), ctet.get(),
-                    ctr, L, (uint64_t)Kslots.cap);
     C2.ctet = ctet.get();
     DBuf<uint32_t> pos;
+    /* enumeration and first filter stage in one kernel; a rerun (queue
+     * overflow, eager only) recounts the candidates from zero */
+    int runs = 0;
     Kalpha = filter_stage(3, C2, Kslots, nullptr, nullptr, flag.get(),
-                          KP_TET_FILL, pos, RC_KALPHA);
+                          KP_TET_FILL, pos, RC_KALPHA, [&](Queue qq) {
+                            if (runs++ && !rec)
+                              zero_u64(ctr + (CTR_CAND + 2) * CTR_SLOTS,
+                                       CTR_SLOTS, s);
+                            launch_tet_enum_filter(
+                                SG, Talpha, trow.get(), tk.get(), tedge.get(),
+                                teik.get(), tejk.get(), eleft.get(), ecol.get(),
+                                off.get(), ctet.get(), flag.get(), qq, L,
+                                (uint64_t)Kslots.cap);
+                          });
     tets = DBuf<uint4>(P, Kalpha.cap);
     launch_compact_flat(3, Kslots, C2, nullptr, nullptr, flag.get(), pos.get(),
                         nullptr, nullptr, tets.get(), nullptr, nullptr, nullptr,
