@@ -271,10 +271,10 @@ __device__ __forceinline__ int tet_pair_code_T(const TetQ<T>& t, int sD,
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
     Stage S, QRef q, const uint4* tets, uint8_t* flag,
     Queue q2) {
-  int64_t t = gtid();
   StageCnt sc;
-  if (t < q.size() && live_entry(q.at(t), flag)) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    if (!live_entry(q.at((uint32_t)t), flag)) continue;
+    QEntry e = q.at((uint32_t)t);
     uint4 tt = tets[e.item];
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
     TetQ<FE<double>> t64;
@@ -380,10 +380,10 @@ __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     Stage S, QRef q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
-  int64_t t = gtid();
   unsigned cnt = 0, sos = 0;
-  if (t < q.size() && live_entry(q.at(t), flag)) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    if (!live_entry(q.at((uint32_t)t), flag)) continue;
+    QEntry e = q.at((uint32_t)t);
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
@@ -612,10 +612,10 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
     Stage S, QRef q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag, Queue q2) {
-  int64_t t = gtid();
   StageCnt sc;
-  if (t < q.size() && live_entry(q.at(t), flag)) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    if (!live_entry(q.at((uint32_t)t), flag)) continue;
+    QEntry e = q.at((uint32_t)t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -712,10 +712,10 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     Stage S, QRef q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag) {
-  int64_t t = gtid();
   unsigned cnt = 0;
-  if (t < q.size() && live_entry(q.at(t), flag)) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    if (!live_entry(q.at((uint32_t)t), flag)) continue;
+    QEntry e = q.at((uint32_t)t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -817,14 +817,14 @@ void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L) {
   if (!q.launch_items()) return;
-  hipLaunchKernelGGL(k_tet_ac2_esc, dim3((q.launch_items() + kBlock - 1) / kBlock),
+  hipLaunchKernelGGL(k_tet_ac2_esc, qgrid(q, kEscBlocks),
                      dim3(kBlock), 0, L.s, S, q, tets, flag, q2);
 }
 void launch_tet_ac2_exact(const Stage& S, QRef q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L) {
   if (!q.launch_items()) return;
-  hipLaunchKernelGGL(k_tet_ac2_exact, dim3((q.launch_items() + kBlock - 1) / kBlock),
+  hipLaunchKernelGGL(k_tet_ac2_exact, qgrid(q, kExactBlocks),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
 template <class R, int KIND, bool FINE>
@@ -885,7 +885,7 @@ void launch_low_ac2_esc(const Stage& S, int kind, QRef q,
                         const uint32_t* tk, const uint32_t* tedge,
                         uint8_t* flag, Queue q2, PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
+  dim3 gr = qgrid(q, kEscBlocks);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_esc<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag, q2);
@@ -897,7 +897,7 @@ void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
+  dim3 gr = qgrid(q, kExactBlocks);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);

@@ -526,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(
     uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr,
     uint64_t lim) {
   const int64_t ne = c_ne.get();
-  constexpr int CAP = 2048;
+  constexpr int CAP = 2040; /* 5 blocks per CU fit in LDS with block_flush's 16 B */
   __shared__ uint32_t s_out[4 * CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= ne) return; /* whole block past the (device) count */
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
     uint4* ctet, uint8_t* flag, Queue q1, uint64_t lim) {
   const int64_t nt = c_nt.get();
-  constexpr int CAP = 2048;
+  constexpr int CAP = 2040; /* 5 blocks per CU fit in LDS with block_flush's 16 B */
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= nt) return; /* whole block past the (device) count */
@@ -722,10 +722,9 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
                                                        const uint32_t* cval,
                                                        uint8_t* flag,
                                                        Queue q2) {
-  int64_t t = gtid();
   StageCnt sc;
-  if (t < q.size()) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    QEntry e = q.at((uint32_t)t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     sc.c[1]++;
@@ -746,10 +745,9 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
                                                          QRef q,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
-  int64_t t = gtid();
   unsigned cnt = 0;
-  if (t < q.size()) {
-    QEntry e = q.at(t);
+  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
+    QEntry e = q.at((uint32_t)t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
@@ -780,7 +778,7 @@ void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
 void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
+  dim3 gr = qgrid(q, kEscBlocks);
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_esc<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag, q2);
@@ -794,7 +792,7 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr((q.launch_items() + kBlock - 1) / kBlock);
+  dim3 gr = qgrid(q, kExactBlocks);
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);

@@ -279,16 +279,21 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
 /* ------------------------------------------------------------------ */
 /* recorded (fixed-capacity) pipelines                                 */
 /* ------------------------------------------------------------------ */
-/* consumer view of QSLOTS sub-queues from their counters; a counter past
- * the capacity (entries were dropped) sets `bit` in *status */
-__global__ void k_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
+/* Consumer view of QSLOTS sub-queues from their counters; a counter past
+ * the capacity (entries were dropped) sets `bit` in *status and the
+ * largest counter goes to *fill (fresh: both overwritten and fill[1]
+ * cleared, for the ladder's second view; else merged).
+ * The counters it read, and *also (if set), are cleared for the next
+ * producer, so no separate zeroing launch precedes a ladder. */
+__global__ void k_make_view(const QEntry* q, uint32_t cap, unsigned* cnt,
                             QView* v, unsigned* status, unsigned bit,
-                            unsigned* fill) {
+                            unsigned* fill, unsigned* also, int fresh) {
   if (threadIdx.x != 0) return;
   uint32_t acc = 0;
   unsigned mx = 0;
   for (int k = 0; k < QSLOTS; ++k) {
     const unsigned c = cnt[k];
+    cnt[k] = 0;
     v->off[k] = acc;
     acc += c < cap ? c : cap;
     mx = c > mx ? c : mx;
@@ -296,14 +301,22 @@ __global__ void k_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
   v->off[QSLOTS] = acc;
   v->q = q;
   v->cap = cap;
-  atomicMax(fill, mx);
-  if (mx > cap) atomicOr(status, bit);
+  if (also) *also = 0;
+  const unsigned b = mx > cap ? bit : 0u;
+  if (fresh) { /* first view of a ladder: also resets the second's fill */
+    fill[0] = mx;
+    fill[1] = 0;
+    *status = b;
+  } else {
+    atomicMax(fill, mx);
+    if (b) atomicOr(status, b);
+  }
 }
-void launch_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
-                      QView* v, unsigned* status, unsigned bit,
-                      unsigned* fill, hipStream_t s) {
+void launch_make_view(const QEntry* q, uint32_t cap, unsigned* cnt, QView* v,
+                      unsigned* status, unsigned bit, unsigned* fill,
+                      unsigned* also, bool fresh, hipStream_t s) {
   hipLaunchKernelGGL(k_make_view, dim3(1), dim3(64), 0, s, q, cap, cnt, v,
-                     status, bit, fill);
+                     status, bit, fill, also, (int)fresh);
 }
 
 __global__ void k_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,

@@ -106,6 +106,7 @@ int64_t with_headroom(int64_t v, double h) {
 void record_all(pac_graph* G) {
   hipStream_t s = G->s;
   zero_u32(G->status.get(), 4, s);
+  zero_u32(G->qn.get(), 2 * QSLOTS + 1, s); /* then kept clear by the views */
   zero_u64(G->ctr.get(), CTR_COUNT * CTR_SLOTS, s);
   Timer T(s);
   T.on = false;

@@ -138,6 +138,14 @@ struct QRef {
   __device__ QEntry at(uint32_t i) const { return dv ? dv->at(i) : v.at(i); }
   uint32_t launch_items() const { return dv ? bound : v.size(); }
 };
+/* grid of a queue consumer: one item per thread up to maxb blocks (the
+ * consumers stride over the rest), so a queue sized for many items but
+ * holding few costs few idle blocks */
+constexpr unsigned kEscBlocks = 1024, kExactBlocks = 512;
+inline dim3 qgrid(const QRef& q, unsigned maxb) {
+  const unsigned b = (q.launch_items() + kBlock - 1) / kBlock;
+  return dim3(b < maxb ? (b ? b : 1) : maxb);
+}
 
 /* ---- recorded pipelines (k_prims.hip) ----
  * Fixed-capacity runs keep every count on the device; overflows and a
@@ -146,9 +154,11 @@ enum { REC_CAP = 0,    /* bits 0..8: a count exceeded its capacity */
        REC_Q1 = 9,     /* hot queue overflow */
        REC_Q2 = 10,    /* escalation queue overflow */
        REC_STALE = 11  /* the input no longer matches the recorded plan */ };
-void launch_make_view(const QEntry* q, uint32_t cap, const unsigned* cnt,
-                      QView* v, unsigned* status, unsigned bit,
-                      unsigned* fill, hipStream_t s);
+/* view of QSLOTS sub-queues (counters cleared after reading, and *also);
+ * fresh: status / fill overwritten, else merged */
+void launch_make_view(const QEntry* q, uint32_t cap, unsigned* cnt, QView* v,
+                      unsigned* status, unsigned bit, unsigned* fill,
+                      unsigned* also, bool fresh, hipStream_t s);
 void launch_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,
                       unsigned bit, hipStream_t s);
 void launch_plan_check(const Bounds* now, const Bounds* ref, int64_t m,

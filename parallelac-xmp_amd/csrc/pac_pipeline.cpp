@@ -240,8 +240,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* counts left on the device by the scans (RC_*) */
   DBuf<uint64_t> dcnt_own;
   uint64_t* dcnt = rec ? rec->dcnt : nullptr;
-  if (!rec) {
-    dcnt_own = DBuf<uint64_t>(P, RC_COUNT);
+  if (!rec) { /* the counts, then a ladder's status and two fills (u32) */
+    dcnt_own = DBuf<uint64_t>(P, RC_COUNT + 2);
     dcnt = dcnt_own.get();
   }
   if (rec) {
@@ -290,7 +290,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* [0, QSLOTS) hot queue, [QSLOTS, 2 QSLOTS) escalation queue, then the
    * count of simplices left for the second emptiness phase */
   DBuf<unsigned> qn_own;
-  if (!rec) qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 1);
+  if (!rec) {
+    qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 1);
+    zero_u32(qn_own.get(), 2 * QSLOTS + 1, s); /* then kept clear by the views */
+  }
   unsigned* const qn = rec ? rec->qn : qn_own.get();
   unsigned* const nrest = qn + 2 * QSLOTS;
 
@@ -357,94 +360,83 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     return Cnt(rec->cap[q], dcnt + q);
   };
 
-  /* precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
-   * double-double (q1 -> q2)] -> exact wide-integer kernel.  Queue
-   * overflows rerun the producing kernel with a larger queue; only the
-   * emptiness kernels can overflow, and they are idempotent. */
+  /* Precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
+   * double-double (q1 -> q2)] -> exact wide-integer kernel, then `tail`,
+   * the scan that leaves the ladder's count at dcnt[q].  Consumers read
+   * their queue through a view built on the device, so the ladder costs no
+   * read-back of its own: the count and the queues' overflow status come
+   * back together, and an overflow (rare: queues start at a generous
+   * size) reruns the ladder with larger queues; every kernel of a ladder
+   * recomputes its flags from scratch, so a rerun is exact. */
   /* sub-queue capacity for up to `total` entries spread over the slots;
    * a one-item-per-thread producer of n items never exceeds n/QSLOTS + 512
    * per slot (its blocks of 256 are dealt round-robin) */
-  auto slot_cap = [](int64_t total) {
-    return (uint32_t)std::min<int64_t>(total / QSLOTS + 512,
-                                       (int64_t)(UINT32_MAX / QSLOTS) - 1);
+  const int64_t qmax = (int64_t)(UINT32_MAX / QSLOTS) - 1;
+  auto slot_cap = [&](int64_t total) {
+    return (uint32_t)std::min<int64_t>(total / QSLOTS + 512, qmax);
   };
-  /* counts of the sub-queues -> consumer view; false on overflow, with
-   * *need = the per-slot capacity that would have sufficed */
-  auto view_of = [&](const QEntry* q, uint32_t cap, const unsigned* dcq,
-                     QView* v, uint32_t* need) -> bool {
-    unsigned h[QSLOTS];
-    HIPCHK(hipMemcpyAsync(E->pinned, dcq, sizeof(h), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::memcpy(h, E->pinned, sizeof(h));
-    v->q = q;
-    v->cap = cap;
-    uint64_t acc = 0;
-    unsigned mx = 0;
-    for (int k = 0; k < QSLOTS; ++k) {
-      v->off[k] = (uint32_t)acc;
-      acc += std::min(h[k], cap);
-      mx = std::max(mx, h[k]);
-    }
-    v->off[QSLOTS] = (uint32_t)acc;
-    *need = mx;
-    int64_t& fill = R->sizes[(dcq == qn) ? 9 : 10];
-    fill = std::max<int64_t>(fill, mx);
-    return mx <= cap;
+  auto grow = [&](uint32_t need, uint32_t had) {
+    return (uint32_t)std::min<int64_t>(
+        std::max<int64_t>((int64_t)need + need / 4 + 256, (int64_t)had * 2), qmax);
   };
-  auto grow = [](uint32_t need) {
-    return (uint32_t)std::min<int64_t>((int64_t)need + need / 4 + 256,
-                                       (int64_t)(UINT32_MAX / QSLOTS) - 1);
-  };
+  /* eager ladders: views on the device, status word and fills after the
+   * counts (one read-back) */
+  DBuf<QView> lviews;
+  if (!rec) lviews = DBuf<QView>(P, 2);
+  unsigned* const lstat = rec ? nullptr : (unsigned*)(dcnt + RC_COUNT);
   auto ladder = [&](int64_t cap1_total, int kp_hot, int kp_exact, auto&& hot,
-                    auto&& esc, auto&& exact) {
-    if (rec) { /* once, over the recorded queues; views built on the device */
-      zero_u32(qn, 2 * QSLOTS, s);
-      hot(Queue{rec->q1, qn, rec->c1});
-      launch_make_view(rec->q1, rec->c1, qn, rec->views, rec->status,
-                       1u << REC_Q1, rec->fill, s);
-      QRef r{QView{}, rec->views, QSLOTS * rec->c1};
-      if (staged) {
-        esc(r, Queue{rec->q2, qn + QSLOTS, rec->c2});
-        launch_make_view(rec->q2, rec->c2, qn + QSLOTS, rec->views + 1,
-                         rec->status, 1u << REC_Q2, rec->fill + 1, s);
-        r = QRef{QView{}, rec->views + 1, QSLOTS * rec->c2};
-      }
-      exact(r);
-      KCHK();
-      return;
-    }
-    uint32_t c1 = slot_cap(cap1_total);
+                    auto&& esc, auto&& exact, auto&& tail, int q) -> Cnt {
+    const bool rq = rec != nullptr;
+    QEntry* const gq1 = rq ? rec->q1 : nullptr;
+    QEntry* const gq2 = rq ? rec->q2 : nullptr;
+    QView* const views = rq ? rec->views : lviews.get();
+    unsigned* const status = rq ? rec->status : lstat;
+    unsigned* const fill = rq ? rec->fill : lstat + 1;
+    uint32_t c1 = rq ? rec->c1 : slot_cap(cap1_total);
+    uint32_t c2 = rq ? rec->c2 : (uint32_t)std::min<int64_t>((int64_t)c1 + 512, qmax);
     for (;;) {
-      DBuf<QEntry> q1(P, (size_t)c1 * QSLOTS);
-      zero_u32(qn, 2 * QSLOTS, s);
-      if (kp_hot >= 0) KP.begin(kp_hot);
-      hot(Queue{q1.get(), qn, c1});
-      if (kp_hot >= 0) KP.end();
-      KCHK();
-      QView v1;
-      uint32_t need = 0;
-      if (!view_of(q1.get(), c1, qn, &v1, &need)) {
-        c1 = grow(need);
-        continue;
+      DBuf<QEntry> q1o, q2o;
+      if (!rq) {
+        q1o = DBuf<QEntry>(P, (size_t)c1 * QSLOTS);
+        q2o = DBuf<QEntry>(P, staged ? (size_t)c2 * QSLOTS : 1);
       }
-      QView vx = v1;
-      DBuf<QEntry> q2;
-      if (staged && v1.size()) {
-        uint32_t c2 = slot_cap(4 * (int64_t)v1.size() + 4096);
-        for (;;) {
-          q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
-          zero_u32(qn + QSLOTS, QSLOTS, s);
-          esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn + QSLOTS, c2});
-          KCHK();
-          if (view_of(q2.get(), c2, qn + QSLOTS, &vx, &need)) break;
-          c2 = grow(need);
-        }
+      QEntry* const q1 = rq ? gq1 : q1o.get();
+      QEntry* const q2 = rq ? gq2 : q2o.get();
+      if (kp_hot >= 0) KP.begin(kp_hot);
+      hot(Queue{q1, qn, c1});
+      if (kp_hot >= 0) KP.end();
+      /* eager: the first view resets status and fills; the second-phase
+       * counter (nrest) is cleared for the next emptiness kernel */
+      launch_make_view(q1, c1, qn, views, status, 1u << REC_Q1, fill, nrest,
+                       !rq, s);
+      QRef r{QView{}, views, QSLOTS * c1};
+      if (staged) {
+        esc(r, Queue{q2, qn + QSLOTS, c2});
+        launch_make_view(q2, c2, qn + QSLOTS, views + 1, status, 1u << REC_Q2,
+                         fill + 1, nullptr, false, s);
+        r = QRef{QView{}, views + 1, QSLOTS * c2};
       }
       if (kp_exact >= 0) KP.begin(kp_exact);
-      exact(QRef{vx, nullptr, 0});
+      exact(r);
       if (kp_exact >= 0) KP.end();
       KCHK();
-      return;
+      tail();
+      KCHK();
+      if (rq) return settle(q);
+      /* one read-back: the counts, then status and the two largest fills */
+      uint64_t blk[RC_COUNT + 2];
+      HIPCHK(hipMemcpyAsync(E->pinned, dcnt, sizeof(blk), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      std::memcpy(blk, E->pinned, sizeof(blk));
+      const uint64_t cnt = blk[q];
+      unsigned h[3];
+      std::memcpy(h, blk + RC_COUNT, sizeof(h));
+      R->sizes[9] = std::max<int64_t>(R->sizes[9], h[1]);
+      R->sizes[10] = std::max<int64_t>(R->sizes[10], h[2]);
+      if (!(h[0] & ((1u << REC_Q1) | (1u << REC_Q2)))) return Cnt((int64_t)cnt);
+      if (h[0] & (1u << REC_Q1)) c1 = grow(h[1], c1);
+      if (h[0] & (1u << REC_Q2)) c2 = grow(h[2], c2);
+      c2 = std::max(c2, (uint32_t)std::min<int64_t>((int64_t)c1 + 512, qmax));
     }
   };
 
@@ -456,23 +448,21 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                           const uint32_t* cseg, uint8_t* flag, int kp,
                           DBuf<uint32_t>& pos, int q, auto&& hot) -> Cnt {
     if (kind == 1) host_evals[0] += nc.cap; /* tris / tets: slots != candidates */
-    /* few candidates stay undecided in float: start small, a rerun of the
-     * (idempotent) filter sizes the queue exactly if it overflows */
-    ladder(
+    pos = DBuf<uint32_t>(P, nc.cap + 1);
+    auto tmp = scan_tmp(nc.cap);
+    /* few candidates stay undecided in float: start small */
+    return ladder(
         std::min<int64_t>(nc.cap, nc.cap / 32 + 65536), kp, -1, hot,
         [&](QRef qr, Queue q2) {
           launch_filter_esc(SG, kind, C, qr, cval, flag, q2, L);
         },
         [&](QRef qr) {
           launch_filter_exact(SG, kind, C, qr, cval, flag, L);
-        });
-    pos = DBuf<uint32_t>(P, nc.cap + 1);
-    {
-      auto tmp = scan_tmp(nc.cap);
-      scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s, dcnt + q);
-    }
-    KCHK();
-    return settle(q);
+        },
+        [&] {
+          scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s, dcnt + q);
+        },
+        q);
   };
   auto count_scan = [&](const uint32_t* cnt, Cnt nseg, uint64_t* off, int q,
                         const char* what) -> Cnt {
@@ -614,32 +604,35 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
   DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
   DBuf<uint8_t> kflag(P, Kalpha.cap);
-  if (opt.skip_tet_ac2) {
-    fill_u8(kflag.get(), Kalpha.cap, 1, s);
-  } else if (Kalpha.cap) {
-    ladder(
-        ac2_cap(Kalpha.cap), KP_TET_AC2, KP_TET_AC2_EXACT,
-        [&](Queue q) {
-          zero_u32(nrest, 1, s);
-          launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q, rest.get(),
-                             nrest, budget, L);
-        },
-        [&](QRef q, Queue q2) {
-          launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);
-        },
-        [&](QRef q) {
-          launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
-        });
-  }
   Cnt Kf(0);
   DBuf<uint4> ftets;
   {
     DBuf<uint32_t> pos(P, Kalpha.cap + 1);
     auto tmp = scan_tmp(Kalpha.cap);
-    scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s,
-                       dcnt + RC_KF);
-    KCHK();
-    Kf = settle(RC_KF);
+    auto tail = [&] {
+      scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s,
+                         dcnt + RC_KF);
+    };
+    if (opt.skip_tet_ac2 || !Kalpha.cap) {
+      fill_u8(kflag.get(), Kalpha.cap, 1, s);
+      tail();
+      KCHK();
+      Kf = settle(RC_KF);
+    } else {
+      Kf = ladder(
+          ac2_cap(Kalpha.cap), KP_TET_AC2, KP_TET_AC2_EXACT,
+          [&](Queue q) {
+            launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q,
+                               rest.get(), nrest, budget, L);
+          },
+          [&](QRef q, Queue q2) {
+            launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);
+          },
+          [&](QRef q) {
+            launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
+          },
+          tail, RC_KF);
+    }
     ftets = DBuf<uint4>(P, Kf.cap);
     launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(),
                         L, (uint64_t)Kf.cap);
@@ -653,32 +646,37 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   fill_u8(tri_in.get(), Talpha.cap, 0, s);
   launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
                     tk.get(), tri_in.get(), L);
-  if (Talpha.cap)
-    ladder(
-        ac2_cap(Talpha.cap), KP_TRI_AC2, -1,
-        [&](Queue q) {
-          zero_u32(nrest, 1, s);
-          launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
-                             tedge.get(), tri_in.get(), tflag.get(), q,
-                             rest.get(), nrest, budget, L);
-        },
-        [&](QRef q, Queue q2) {
-          launch_low_ac2_esc(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
-                             tedge.get(), tflag.get(), q2, L);
-        },
-        [&](QRef q) {
-          launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
-                               tedge.get(), tflag.get(), L);
-        });
   Cnt Tf(0);
   DBuf<uint32_t> ftri;
   {
     DBuf<uint32_t> pos(P, Talpha.cap + 1);
     auto tmp = scan_tmp(Talpha.cap);
-    scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s,
-                       dcnt + RC_TF);
-    KCHK();
-    Tf = settle(RC_TF);
+    auto tail = [&] {
+      scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s,
+                         dcnt + RC_TF);
+    };
+    if (!Talpha.cap) {
+      tail();
+      KCHK();
+      Tf = settle(RC_TF);
+    } else {
+      Tf = ladder(
+          ac2_cap(Talpha.cap), KP_TRI_AC2, -1,
+          [&](Queue q) {
+            launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
+                               tedge.get(), tri_in.get(), tflag.get(), q,
+                               rest.get(), nrest, budget, L);
+          },
+          [&](QRef q, Queue q2) {
+            launch_low_ac2_esc(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
+                               tedge.get(), tflag.get(), q2, L);
+          },
+          [&](QRef q) {
+            launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
+                                 tedge.get(), tflag.get(), L);
+          },
+          tail, RC_TF);
+    }
     ftri = DBuf<uint32_t>(P, Tf.cap);
     launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L,
                        (uint64_t)Tf.cap);
@@ -691,31 +689,36 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   fill_u8(edge_in.get(), Ealpha.cap, 0, s);
   launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
                     edge_in.get(), L);
-  if (Ealpha.cap)
-    ladder(
-        ac2_cap(Ealpha.cap), KP_EDGE_AC2, -1,
-        [&](Queue q) {
-          zero_u32(nrest, 1, s);
-          launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
-                             nullptr, edge_in.get(), eflag.get(), q,
-                             rest.get(), nrest, budget, L);
-        },
-        [&](QRef q, Queue q2) {
-          launch_low_ac2_esc(SG, 1, q, ecol.get(), eleft.get(), nullptr,
-                             nullptr, eflag.get(), q2, L);
-        },
-        [&](QRef q) {
-          launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
-                               nullptr, eflag.get(), L);
-        });
   Cnt Ef(0);
   DBuf<uint32_t> pos_e(P, Ealpha.cap + 1);
   {
     auto tmp = scan_tmp(Ealpha.cap);
-    scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s,
-                       dcnt + RC_EF);
-    KCHK();
-    Ef = settle(RC_EF);
+    auto tail = [&] {
+      scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s,
+                         dcnt + RC_EF);
+    };
+    if (!Ealpha.cap) {
+      tail();
+      KCHK();
+      Ef = settle(RC_EF);
+    } else {
+      Ef = ladder(
+          ac2_cap(Ealpha.cap), KP_EDGE_AC2, -1,
+          [&](Queue q) {
+            launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
+                               nullptr, edge_in.get(), eflag.get(), q,
+                               rest.get(), nrest, budget, L);
+          },
+          [&](QRef q, Queue q2) {
+            launch_low_ac2_esc(SG, 1, q, ecol.get(), eleft.get(), nullptr,
+                               nullptr, eflag.get(), q2, L);
+          },
+          [&](QRef q) {
+            launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
+                                 nullptr, eflag.get(), L);
+          },
+          tail, RC_EF);
+    }
   }
 
   /* ---- emit final lists (original ids, reference order) ---- */
