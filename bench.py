@@ -351,22 +351,35 @@ def bench_stress(args, torch, pac):
     dev = torch.from_numpy(balls).to("cuda:0")
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
+    # autotuned per path on a 200k-ball sample of the same construction
+    # (10% lattice block); the full input is too large to sweep
+    sample = pac.synth_stress(200000, None, seed=6, lattice_frac=0.1,
+                              spacing=2.0, lattice_r=1.70)
+    tuned = {}
+    for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+        if args.budget and args.grid:
+            tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
+                           "ac2_fine": args.fine}
+        else:
+            tuned[prec], _ = pac.autotune(sample, 1.4, 0.0, precision=prec)
+    log("stress: settings %s" % tuned)
     log("stress: warmup")
     for _ in range(args.warmup):
-        pac.compute_device(ptr, n, 1.4, 0.0)
+        pac.compute_device(ptr, n, 1.4, 0.0, **tuned[pac.PREC_STAGED])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = None
     for _ in range(args.steps):
         r = None
-        r = pac.compute_device(ptr, n, 1.4, 0.0)
+        r = pac.compute_device(ptr, n, 1.4, 0.0, **tuned[pac.PREC_STAGED])
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / args.steps
     st = r.stats()
     log("stress: plain FP64 path")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rp = pac.compute_device(ptr, n, 1.4, 0.0, precision=pac.PREC_PLAIN_FP64)
+    rp = pac.compute_device(ptr, n, 1.4, 0.0, precision=pac.PREC_PLAIN_FP64,
+                            **tuned[pac.PREC_PLAIN_FP64])
     torch.cuda.synchronize()
     t_pl = time.perf_counter() - t0
     L, Lp = r.lists(), rp.lists()
@@ -396,7 +409,8 @@ def bench_stress(args, torch, pac):
                                "cubic lattice spacing 2.0 A r=1.70, FP32, solvent 1.4, "
                                "alpha 0, one pass", "n_balls": n},
         "bit_exact": {"staged_eq_plain": same_paths, "staged_eq_oracle": oracle_ok},
-        "plain_fp64": {"ms_per_step": t_pl * 1e3},
+        "plain_fp64": {"ms_per_step": t_pl * 1e3, "tuned": tuned[pac.PREC_PLAIN_FP64]},
+        "tuned": tuned[pac.PREC_STAGED],
         "counts": {"classified": classified, "edges": int(st.edges_final),
                    "triangles": int(st.tris_final), "tets": int(st.tets_final)},
         "escalations": _esc(pac, st), "sos_calls": int(st.sos_calls),

@@ -272,9 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
     Stage S, QRef q, const uint4* tets, uint8_t* flag,
     Queue q2) {
   StageCnt sc;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    if (!live_entry(q.at((uint32_t)t), flag)) continue;
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    if (!live_entry(q.at(t), flag)) return;
+    QEntry e = q.at(t);
     uint4 tt = tets[e.item];
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
     TetQ<FE<double>> t64;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
         if (dead) flag[e.item] = 0;
       }
     }
-  }
+  });
   sc.flush(S.ctr, FAM_TET_AC2);
 }
 
@@ -381,9 +381,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     Stage S, QRef q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
   unsigned cnt = 0, sos = 0;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    if (!live_entry(q.at((uint32_t)t), flag)) continue;
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    if (!live_entry(q.at(t), flag)) return;
+    QEntry e = q.at(t);
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     tet_DN(pa, wa, L.p(v[1]), L.w(v[1]), L.p(v[2]), L.w(v[2]), L.p(v[3]),
            L.w(v[3]), &D, &N);
     if (e.aux != 0xffffffffu) {
-      cnt = 1;
+      ++cnt;
       if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, e.aux, &sos))
         flag[e.item] = 0;
     } else if (wi_sign(D) != 0) {
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
       }
       flag[e.item] = viol ? 0 : 2;
     }
-  }
+  });
   flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, cnt);
   flush_ctr(S.ctr, CTR_SOS, sos);
 }
@@ -613,9 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag, Queue q2) {
   StageCnt sc;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    if (!live_entry(q.at((uint32_t)t), flag)) continue;
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    if (!live_entry(q.at(t), flag)) return;
+    QEntry e = q.at(t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
         if (dead) flag[e.item] = 0;
       }
     }
-  }
+  });
   sc.flush(S.ctr, KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2);
 }
 
@@ -713,9 +713,9 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag) {
   unsigned cnt = 0;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    if (!live_entry(q.at((uint32_t)t), flag)) continue;
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    if (!live_entry(q.at(t), flag)) return;
+    QEntry e = q.at(t);
     uint32_t vid[3];
     if (KIND == 0) {
       uint32_t ed = tedge[e.item];
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
       num = vscale(A, u);
     }
     if (e.aux != 0xffffffffu) {
-      cnt = 1;
+      ++cnt;
       Ball E = S.ball[e.aux];
       if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0)
         flag[e.item] = 0;
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     } else {
       flag[e.item] = 0; /* degenerate simplex: cannot be an alpha simplex */
     }
-  }
+  });
   flush_ctr(S.ctr, CTR_STAGE + (KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2) * 4 + 3,
             cnt);
 }

@@ -723,8 +723,8 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
                                                        uint8_t* flag,
                                                        Queue q2) {
   StageCnt sc;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     sc.c[1]++;
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
       push_q(q2, e.item, e.aux);
     else
       flag[e.item] = (uint8_t)r;
-  }
+  });
   sc.flush(S.ctr, fam_of(KIND));
 }
 template <int KIND>
@@ -746,13 +746,13 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
   unsigned cnt = 0;
-  for (int64_t t = gtid(); t < (int64_t)q.size(); t += (int64_t)gridDim.x * blockDim.x) {
-    QEntry e = q.at((uint32_t)t);
+  for_queue(q, [&](uint32_t t) {
+    QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
-    cnt = 1;
-  }
+    ++cnt;
+  });
   flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
 

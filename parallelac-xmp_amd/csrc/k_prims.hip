@@ -301,6 +301,7 @@ __global__ void k_make_view(const QEntry* q, uint32_t cap, unsigned* cnt,
   v->off[QSLOTS] = acc;
   v->q = q;
   v->cap = cap;
+  v->work = 0;
   if (also) *also = 0;
   const unsigned b = mx > cap ? bit : 0u;
   if (fresh) { /* first view of a ladder: also resets the second's fill */

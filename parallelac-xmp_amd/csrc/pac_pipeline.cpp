@@ -240,8 +240,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* counts left on the device by the scans (RC_*) */
   DBuf<uint64_t> dcnt_own;
   uint64_t* dcnt = rec ? rec->dcnt : nullptr;
-  if (!rec) { /* the counts, then a ladder's status and two fills (u32) */
-    dcnt_own = DBuf<uint64_t>(P, RC_COUNT + 2);
+  if (!rec) {
+    dcnt_own = DBuf<uint64_t>(P, RC_COUNT);
     dcnt = dcnt_own.get();
   }
   if (rec) {
@@ -362,82 +362,102 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
 
   /* Precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
    * double-double (q1 -> q2)] -> exact wide-integer kernel, then `tail`,
-   * the scan that leaves the ladder's count at dcnt[q].  Consumers read
-   * their queue through a view built on the device, so the ladder costs no
-   * read-back of its own: the count and the queues' overflow status come
-   * back together, and an overflow (rare: queues start at a generous
-   * size) reruns the ladder with larger queues; every kernel of a ladder
-   * recomputes its flags from scratch, so a rerun is exact. */
+   * the scan that leaves the ladder's count at dcnt[q].
+   * Eager: the queue counters are read back to build each consumer's view
+   * on the host (sizing the escalation queue from the entries actually
+   * queued and skipping empty stages); a queue overflow reruns its
+   * producer with a larger queue (the producers are idempotent).
+   * Recorded: views built on the device, queues of fixed size, overflows
+   * reported through the status word. */
+  const int64_t qmax = (int64_t)(UINT32_MAX / QSLOTS) - 1;
   /* sub-queue capacity for up to `total` entries spread over the slots;
    * a one-item-per-thread producer of n items never exceeds n/QSLOTS + 512
    * per slot (its blocks of 256 are dealt round-robin) */
-  const int64_t qmax = (int64_t)(UINT32_MAX / QSLOTS) - 1;
   auto slot_cap = [&](int64_t total) {
     return (uint32_t)std::min<int64_t>(total / QSLOTS + 512, qmax);
   };
-  auto grow = [&](uint32_t need, uint32_t had) {
-    return (uint32_t)std::min<int64_t>(
-        std::max<int64_t>((int64_t)need + need / 4 + 256, (int64_t)had * 2), qmax);
+  auto grow = [&](uint32_t need) {
+    return (uint32_t)std::min<int64_t>((int64_t)need + need / 4 + 256, qmax);
   };
-  /* eager ladders: views on the device, status word and fills after the
-   * counts (one read-back) */
-  DBuf<QView> lviews;
-  if (!rec) lviews = DBuf<QView>(P, 2);
-  unsigned* const lstat = rec ? nullptr : (unsigned*)(dcnt + RC_COUNT);
+  /* counts of the sub-queues -> consumer view; false on overflow, with
+   * *need = the per-slot capacity that would have sufficed; the counters
+   * are left cleared for the next producer */
+  auto view_of = [&](const QEntry* qq, uint32_t cap, unsigned* dcq, QView* v,
+                     uint32_t* need) -> bool {
+    unsigned h[QSLOTS];
+    HIPCHK(hipMemcpyAsync(E->pinned, dcq, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(h, E->pinned, sizeof(h));
+    zero_u32(dcq, QSLOTS, s);
+    v->q = qq;
+    v->cap = cap;
+    v->work = 0;
+    uint64_t acc = 0;
+    unsigned mx = 0;
+    for (int k = 0; k < QSLOTS; ++k) {
+      v->off[k] = (uint32_t)acc;
+      acc += std::min(h[k], cap);
+      mx = std::max(mx, h[k]);
+    }
+    v->off[QSLOTS] = (uint32_t)acc;
+    *need = mx;
+    int64_t& fill = R->sizes[(dcq == qn) ? 9 : 10];
+    fill = std::max<int64_t>(fill, mx);
+    return mx <= cap;
+  };
   auto ladder = [&](int64_t cap1_total, int kp_hot, int kp_exact, auto&& hot,
                     auto&& esc, auto&& exact, auto&& tail, int q) -> Cnt {
-    const bool rq = rec != nullptr;
-    QEntry* const gq1 = rq ? rec->q1 : nullptr;
-    QEntry* const gq2 = rq ? rec->q2 : nullptr;
-    QView* const views = rq ? rec->views : lviews.get();
-    unsigned* const status = rq ? rec->status : lstat;
-    unsigned* const fill = rq ? rec->fill : lstat + 1;
-    uint32_t c1 = rq ? rec->c1 : slot_cap(cap1_total);
-    uint32_t c2 = rq ? rec->c2 : (uint32_t)std::min<int64_t>((int64_t)c1 + 512, qmax);
-    for (;;) {
-      DBuf<QEntry> q1o, q2o;
-      if (!rq) {
-        q1o = DBuf<QEntry>(P, (size_t)c1 * QSLOTS);
-        q2o = DBuf<QEntry>(P, staged ? (size_t)c2 * QSLOTS : 1);
-      }
-      QEntry* const q1 = rq ? gq1 : q1o.get();
-      QEntry* const q2 = rq ? gq2 : q2o.get();
-      if (kp_hot >= 0) KP.begin(kp_hot);
-      hot(Queue{q1, qn, c1});
-      if (kp_hot >= 0) KP.end();
-      /* eager: the first view resets status and fills; the second-phase
-       * counter (nrest) is cleared for the next emptiness kernel */
-      launch_make_view(q1, c1, qn, views, status, 1u << REC_Q1, fill, nrest,
-                       !rq, s);
-      QRef r{QView{}, views, QSLOTS * c1};
+    if (rec) { /* once, over the recorded queues; views built on the device */
+      hot(Queue{rec->q1, qn, rec->c1});
+      launch_make_view(rec->q1, rec->c1, qn, rec->views, rec->status,
+                       1u << REC_Q1, rec->fill, nrest, false, s);
+      QRef r{QView{}, rec->views, QSLOTS * rec->c1};
       if (staged) {
-        esc(r, Queue{q2, qn + QSLOTS, c2});
-        launch_make_view(q2, c2, qn + QSLOTS, views + 1, status, 1u << REC_Q2,
-                         fill + 1, nullptr, false, s);
-        r = QRef{QView{}, views + 1, QSLOTS * c2};
+        esc(r, Queue{rec->q2, qn + QSLOTS, rec->c2});
+        launch_make_view(rec->q2, rec->c2, qn + QSLOTS, rec->views + 1,
+                         rec->status, 1u << REC_Q2, rec->fill + 1, nullptr,
+                         false, s);
+        r = QRef{QView{}, rec->views + 1, QSLOTS * rec->c2};
       }
-      if (kp_exact >= 0) KP.begin(kp_exact);
       exact(r);
-      if (kp_exact >= 0) KP.end();
       KCHK();
       tail();
       KCHK();
-      if (rq) return settle(q);
-      /* one read-back: the counts, then status and the two largest fills */
-      uint64_t blk[RC_COUNT + 2];
-      HIPCHK(hipMemcpyAsync(E->pinned, dcnt, sizeof(blk), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      std::memcpy(blk, E->pinned, sizeof(blk));
-      const uint64_t cnt = blk[q];
-      unsigned h[3];
-      std::memcpy(h, blk + RC_COUNT, sizeof(h));
-      R->sizes[9] = std::max<int64_t>(R->sizes[9], h[1]);
-      R->sizes[10] = std::max<int64_t>(R->sizes[10], h[2]);
-      if (!(h[0] & ((1u << REC_Q1) | (1u << REC_Q2)))) return Cnt((int64_t)cnt);
-      if (h[0] & (1u << REC_Q1)) c1 = grow(h[1], c1);
-      if (h[0] & (1u << REC_Q2)) c2 = grow(h[2], c2);
-      c2 = std::max(c2, (uint32_t)std::min<int64_t>((int64_t)c1 + 512, qmax));
+      return settle(q);
     }
+    uint32_t c1 = slot_cap(cap1_total);
+    DBuf<QEntry> q1, q2;
+    QView v1;
+    for (;;) {
+      q1 = DBuf<QEntry>(P, (size_t)c1 * QSLOTS);
+      zero_u32(nrest, 1, s);
+      if (kp_hot >= 0) KP.begin(kp_hot);
+      hot(Queue{q1.get(), qn, c1});
+      if (kp_hot >= 0) KP.end();
+      KCHK();
+      uint32_t need = 0;
+      if (view_of(q1.get(), c1, qn, &v1, &need)) break;
+      c1 = grow(need);
+    }
+    QView vx = v1;
+    if (staged && v1.size()) {
+      uint32_t c2 = slot_cap(4 * (int64_t)v1.size() + 4096);
+      for (;;) {
+        q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
+        esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn + QSLOTS, c2});
+        KCHK();
+        uint32_t need = 0;
+        if (view_of(q2.get(), c2, qn + QSLOTS, &vx, &need)) break;
+        c2 = grow(need);
+      }
+    }
+    if (kp_exact >= 0) KP.begin(kp_exact);
+    exact(QRef{vx, nullptr, 0});
+    if (kp_exact >= 0) KP.end();
+    KCHK();
+    tail();
+    KCHK();
+    return settle(q);
   };
 
   /* alpha filter of one dimension over its enumerated candidates: the
