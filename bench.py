@@ -26,9 +26,15 @@ sys.path.insert(0, os.path.join(ROOT, "parallelac-xmp_amd"))
 
 METRIC = "classified simplices/s at 100k balls, staged vs plain FP64 filter, bit-exact"
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
-# nominal algorithmic flops per unit (candidate or (simplex, ball) pair)
-WORK = {"edge_fill": 25, "tri_fill": 150, "tet_fill": 130, "tet_ac2": 85,
-        "tri_ac2": 11, "edge_ac2": 11}
+# algorithmic flops per unit (candidate, or (simplex, ball) pair), counted
+# from the kernels' source: the certified pair test of the emptiness scans
+# is 28 flops (gap, its bound, the comparison's operands); a tet's scan
+# setup adds 290 per alpha tet (orthocentre numerators / denominator with
+# their permanents and the pair constants, 163 in FP32; certified centre,
+# reach, box and sphere, 126 in FP64)
+WORK = {"edge_fill": 25, "tri_fill": 150, "tet_fill": 130, "tet_ac2": 28,
+        "tri_ac2": 28, "edge_ac2": 28}
+TET_SETUP_FLOPS = 290
 
 
 def log(msg):
@@ -133,7 +139,8 @@ def main():
              "tet_fill": s_st.tet_candidates, "tet_ac2": s_st.ac2_pairs[0],
              "tri_ac2": s_st.ac2_pairs[1], "edge_ac2": s_st.ac2_pairs[2]}
     dom = max(WORK, key=lambda k: kms[k])
-    flops = WORK[dom] * units[dom] + (23 * s_st.tet_alpha if dom == "tet_ac2" else 0)
+    flops = WORK[dom] * units[dom] + (TET_SETUP_FLOPS * s_st.tet_alpha
+                                      if dom == "tet_ac2" else 0)
     achieved = flops / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_globule100k.json")
@@ -147,6 +154,8 @@ def main():
                 "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom, "kernel_ms": kms[dom],
                 "work_units": int(units[dom]), "flops_per_unit": WORK[dom],
+                "setup_units": int(s_st.tet_alpha) if dom == "tet_ac2" else 0,
+                "flops_per_setup": TET_SETUP_FLOPS if dom == "tet_ac2" else 0,
                 "note": "vector FP32 ALU (no MFMA on this path); peak = MI355X "
                         "FP32 157.3 TFLOP/s"}
 
