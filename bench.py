@@ -57,6 +57,8 @@ def main():
                     help="second emptiness phase grid (blocks)")
     ap.add_argument("--fine", type=int, default=0,
                     help="emptiness scan grid x split (with --budget and --grid)")
+    ap.add_argument("--quick", type=int, default=0,
+                    help="quick rejection pass (with --budget and --grid)")
     ap.add_argument("--workload", choices=("globule", "batch", "stress", "copy"),
                     default="globule",
                     help="globule: the 100k headline (default); batch: the "
@@ -96,7 +98,7 @@ def main():
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
         if args.budget and args.grid:
             tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
-                           "ac2_fine": args.fine}
+                           "ac2_fine": args.fine, "ac2_quick": args.quick}
         else:
             tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
                                           n=n, precision=prec)
@@ -368,7 +370,7 @@ def bench_stress(args, torch, pac):
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
         if args.budget and args.grid:
             tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
-                           "ac2_fine": args.fine}
+                           "ac2_fine": args.fine, "ac2_quick": args.quick}
         else:
             tuned[prec], _ = pac.autotune(sample, 1.4, 0.0, precision=prec)
     log("stress: settings %s" % tuned)

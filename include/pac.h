@@ -55,7 +55,9 @@ typedef struct pac_options {
   int32_t ac2_fine;      /* exact policy: emptiness scans over the grid's
                             cells split in ac2_fine (2..8) along x
                             (0 = 2, < 0 or 1: the grid's own cells)       */
-  int32_t reserved[2];
+  int32_t ac2_quick;     /* fine scan grid: 1 = a quick rejection pass over
+                            the alpha tets first (0 = off)                */
+  int32_t reserved[1];
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -166,7 +168,8 @@ void pac_graph_free(pac_graph* g);
 /* Autotuner: times the pipeline on the given balls for each candidate
  * setting of the tunables and writes the fastest into opt->ac2_fine (the
  * emptiness scan grid, exact cell policy: x subdivision 2, 4 or the
- * reference grid, tried first), then opt->ac2_budget / opt->ac2_grid
+ * reference grid, tried first), opt->ac2_quick (with a fine grid: the
+ * quick rejection pass on or off), then opt->ac2_budget / opt->ac2_grid
  * (emptiness phase-one pair budget x second-phase grid, over the chosen
  * scan grid); the other fields of *opt are used as given.  offsets ==
  * NULL: the n balls are one molecule; otherwise they are a batch

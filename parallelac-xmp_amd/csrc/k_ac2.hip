@@ -138,9 +138,17 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 
 /* per-simplex setup at precision R: D, N with a-priori error bounds, the
  * certified centre box and the semi-static pair constants. */
-template <class R, class Box>
-__device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
-                                          PairSS<R>& P, Box& box) {
+/* FP32/FP64 orthocentre numerators and denominator of a tet with their
+ * certified errors (semi-static), and the pair-test constants */
+template <class R>
+struct TetCert {
+  FE<R> D;
+  V3<FE<R>> N;
+  FE<R> wa;
+};
+template <class R>
+__device__ __forceinline__ bool tet_pairs(const Ball v[4], PairSS<R>& P,
+                                          TetCert<R>& T) {
   const R ax = v[0].x, ay = v[0].y, az = v[0].z;
   R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
   R vx = (R)v[2].x - ax, vy = (R)v[2].y - ay, vz = (R)v[2].z - az;
@@ -167,14 +175,90 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     N[q].p = A.p * vt[q].p + B.p * tu[q].p + C.p * uv[q].p;
-  FE<R> Df = ss_fe(D, 6);
-  V3<FE<R>> Nf{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
+  T.D = ss_fe(D, 6);
+  T.N = V3<FE<R>>{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
-  FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
-  if (!pairss_init(P, pa, waf, Df, Nf)) return false;
+  T.wa = FE<R>{wa, FEc<R>::u() * wa * FEc<R>::slack()};
+  return pairss_init(P, pa, T.wa, T.D, T.N);
+}
+/* the pair constants and the certified box that holds every violator */
+template <class R, class Box>
+__device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
+                                          PairSS<R>& P, Box& box) {
+  TetCert<R> T;
+  if (!tet_pairs(v, P, T)) return false;
   double c[3], e[3], rho_up;
-  if (!centre_err(Df, Nf, v[0], waf, c, e, &rho_up)) return false;
+  if (!centre_err(T.D, T.N, v[0], T.wa, c, e, &rho_up)) return false;
   return place_box(g, simplex_reach(g, rho_up), c, e, box);
+}
+
+/* Quick rejection before the certified box is built: the balls of the
+ * fine-grid row through an FP32 estimate of the centre, within the grid's
+ * reach of it in x, go through the certified pair test, and a certain
+ * violator rejects the simplex.  The region needs no guarantee (nothing is
+ * concluded from its other balls), so the estimate need not be certified;
+ * a vertex of the simplex never tests as a certain violator.  Most alpha
+ * simplices are rejected by a ball near their centre: this spares them the
+ * FP64 centre bounds and box. */
+template <class R>
+__device__ __forceinline__ bool quick_reject(const Stage& S, const GridP& g,
+                                             const PairSS<R>& P,
+                                             unsigned& pairs) {
+  const float h = 0.5f / (float)P.den;
+  const float cx = (float)P.ax + (float)P.nx * h;
+  const float cy = (float)P.ay + (float)P.ny * h;
+  const float cz = (float)P.az + (float)P.nz * h;
+  if (!(fabsf(cx) < 1e30f && fabsf(cy) < 1e30f && fabsf(cz) < 1e30f))
+    return false;
+  const float iyz = (float)g.inv_yz, ix = (float)g.inv_step, rf = (float)g.reach;
+  auto cell = [](float q, int e) {
+    return (int)floorf(fminf(fmaxf(q, 0.f), (float)(e - 1)));
+  };
+  const int y = cell((cy - g.miny) * iyz, g.ey);
+  const int z = cell((cz - g.minz) * iyz, g.ez);
+  const int xl = cell((cx - rf - g.minx) * ix, g.ex);
+  const int xh = cell((cx + rf - g.minx) * ix, g.ex);
+  const int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+  const uint32_t lo = S.fstart[row + xl], hi = S.fstart[row + xh + 1];
+  bool dead = false;
+  auto visit = [&](uint32_t, const Ball& E) {
+    ++pairs;
+    if (pairss_sign(P, E) * P.sden < 0) {
+      dead = true;
+      return true;
+    }
+    return false;
+  };
+  scan_range4(S.fball, lo, hi, visit);
+  return dead;
+}
+
+/* Quick pass over the alpha tets (fine scan grid): FP32 pair constants
+ * and quick_reject; the tets it does not reject (or whose pair constants
+ * are not certified) are listed in `next` for the certified scan.  A
+ * kernel of its own: the certified scan's registers stay off this path. */
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
+    Stage S, Cnt c_nk, const uint4* tets, uint8_t* flag, uint32_t* next,
+    unsigned* nnext) {
+  unsigned pairs = 0;
+  const int64_t nk = c_nk.get();
+  const int64_t id = gtid();
+  uint32_t keep = 0xffffffffu;
+  if (id < nk) {
+    const uint4 t = tets[id];
+    const Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
+    const GridP g = sgrid_of(S, t.x);
+    PairSS<R> P;
+    TetCert<R> T;
+    if (tet_pairs(v, P, T) && quick_reject(S, g, P, pairs))
+      flag[id] = 0;
+    else
+      keep = (uint32_t)id;
+  }
+  const unsigned at = block_append(nnext, keep != 0xffffffffu);
+  if (keep != 0xffffffffu) next[at] = keep;
+  block_flush(S.ctr, CTR_PAIRS + 0, pairs);
 }
 
 /* Phase A (budget > 0): at most `budget` pair tests per tet, in scan order;
@@ -189,14 +273,18 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
  * fine scan looks a ball's id up only then, to skip the vertices. */
 template <class R, bool PHASE_B, bool FINE>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
-    Stage S, Cnt c_nk, const uint4* tets, uint8_t* flag, Queue q1,
-    uint32_t* rest, unsigned* nrest, int budget) {
+    Stage S, Cnt c_nk, const uint32_t* items, const unsigned* nitems,
+    const uint4* tets, uint8_t* flag, Queue q1, uint32_t* rest,
+    unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  int64_t total = PHASE_B ? (int64_t)*nrest : c_nk.get();
+  /* phase A: the alpha tets, or the list `items` of them (those the quick
+   * pass left); phase B: `rest` */
+  int64_t total = PHASE_B ? (int64_t)*nrest
+                          : (items ? (int64_t)*nitems : c_nk.get());
   int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   for (int64_t w = gtid(); w < total; w += stride) {
-    int64_t id = PHASE_B ? (int64_t)rest[w] : w;
+    int64_t id = PHASE_B ? (int64_t)rest[w] : (items ? (int64_t)items[w] : w);
     uint4 t = tets[id];
     Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
     const GridP g = FINE ? sgrid_of(S, t.x) : grid_of(S, t.x);
@@ -786,32 +874,44 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 /* ---------------- launchers ---------------- */
 template <class R, bool FINE>
 static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
-                     Queue q1, uint32_t* rest, unsigned* nrest, int budget,
-                     PipeLaunch L) {
+                     Queue q1, uint32_t* rest, unsigned* nrest, uint32_t* next,
+                     unsigned* nnext, int budget, PipeLaunch L) {
   dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  const uint32_t* items = nullptr;
+  if (FINE && S.quick) { /* quick pass, then the certified scan of the rest */
+    zero_u32(nnext, 1, L.s);
+    hipLaunchKernelGGL((k_tet_ac2_quick<R>), gr, dim3(kBlock), 0, L.s, S, nk,
+                       tets, flag, next, nnext);
+    items = next;
+  }
   hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE>), gr, dim3(kBlock), 0, L.s,
-                     S, nk, tets, flag, q1, rest, nrest, budget);
+                     S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
   if (budget > 0)
     hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE>), gb, dim3(kBlock), 0, L.s,
-                       S, nk, tets, flag, q1, rest, nrest, budget);
+                       S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
 }
 void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
-                        unsigned* nrest, int budget, PipeLaunch L) {
+                        unsigned* nrest, uint32_t* next, unsigned* nnext,
+                        int budget, PipeLaunch L) {
   if (nk.cap <= 0) return;
   if constexpr (!kBallF64) {
     if (S.staged) {
       if (S.fine)
-        tet_hot2<float, true>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+        tet_hot2<float, true>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                              budget, L);
       else
-        tet_hot2<float, false>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+        tet_hot2<float, false>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                               budget, L);
       return;
     }
   }
   if (S.fine)
-    tet_hot2<double, true>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+    tet_hot2<double, true>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                           budget, L);
   else
-    tet_hot2<double, false>(S, nk, tets, flag, q1, rest, nrest, budget, L);
+    tet_hot2<double, false>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                            budget, L);
 }
 void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,

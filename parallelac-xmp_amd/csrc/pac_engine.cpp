@@ -459,22 +459,25 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
     }
     return tmin;
   };
-  /* the scan grid first (at the default budget and grid), then the
-   * phase-one budget x second-phase grid over it */
-  static const int fines[] = {2, 4, -1};
+  /* the scan grid and the quick rejection pass together (at the default
+   * budget and grid), then the phase-one budget x second-phase grid over
+   * them */
+  static const int scans[][2] = {{2, 0}, {4, 0}, {2, 1}, {4, 1}, {-1, 0}};
   static const int budgets[] = {4, 6, 8, 12, 16, 24};
   static const int grids[] = {1024, 2048, 4096};
   double best = 1e300;
   pac_options pick = base;
   pick.profile = 0;
   if (base.cell_policy == PAC_CELL_EXACT)
-    for (int f : fines) {
+    for (const auto& sc : scans) {
       pac_options t = pick;
-      t.ac2_fine = f;
+      t.ac2_fine = sc[0];
+      t.ac2_quick = sc[1];
       const double tm = time_one(t);
       if (tm < best) {
         best = tm;
-        pick.ac2_fine = f;
+        pick.ac2_fine = sc[0];
+        pick.ac2_quick = sc[1];
       }
     }
   const pac_options fixed = pick;
@@ -493,6 +496,7 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
   o->ac2_budget = pick.ac2_budget;
   o->ac2_grid = pick.ac2_grid;
   o->ac2_fine = pick.ac2_fine;
+  o->ac2_quick = pick.ac2_quick;
   if (best_ms) *best_ms = best;
 }
 

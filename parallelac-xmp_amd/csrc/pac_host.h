@@ -298,10 +298,6 @@ inline void stats_from_counters(const unsigned long long* hs, bool staged,
   st.sos_calls += (int64_t)h[CTR_SOS];
 }
 
-/* small fills on a stream (kernels, not hipMemsetAsync) */
-void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
-void zero_u32(unsigned* p, int n, hipStream_t s);
-void zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 
 /* One pipeline pass recorded with fixed capacities (for capture into a
  * graph): nothing is read back, every count stays on the device, and
@@ -318,7 +314,7 @@ struct PassRec {
   unsigned* status;        /* device: REC_* bits */
   unsigned* fill;          /* device [2]: largest sub-queue fills */
   unsigned long long* ctr; /* device counters (shared by the passes) */
-  unsigned* qn;            /* device [2 QSLOTS + 1] queue counters */
+  unsigned* qn;            /* device [2 QSLOTS + 2] queue counters */
   QView* views;            /* device [2] */
   QEntry *q1, *q2;         /* hot / escalation queues */
   uint32_t c1, c2;         /* their sub-queue capacities */

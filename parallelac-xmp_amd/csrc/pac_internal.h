@@ -96,6 +96,12 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                       size_t tmp_bytes, hipStream_t s, bool* result_in_alt);
 size_t radix_tmp_bytes(int64_t n);
 
+/* small fills on a stream (kernels, not hipMemsetAsync: they may be
+ * captured into graphs) */
+void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+void zero_u32(unsigned* p, int n, hipStream_t s);
+void zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
+
 struct PipeLaunch {
   hipStream_t s;
   int block;
@@ -272,6 +278,7 @@ struct Stage {
   /* exact cell policy: the emptiness scans run over a finer grid of their
    * own (reference cells split along x), its balls in fine-cell order */
   int fine;
+  int quick;             /* fine grid: quick rejection pass over the tets */
   GridP sg;              /* fine grid of a single molecule */
   const GridP* sgrids;   /* batch: fine grid of each molecule */
   const Ball* fball;     /* balls in fine-cell order */
@@ -331,9 +338,12 @@ void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =
  * number of items); budget = phase A pair budget (0: single phase) */
+/* next / nnext: scratch list (capacity = number of tets) of the tets a
+ * quick rejection pass leaves for the certified scan (fine scan grid) */
 void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
-                        unsigned* nrest, int budget, PipeLaunch L);
+                        unsigned* nrest, uint32_t* next, unsigned* nnext,
+                        int budget, PipeLaunch L);
 void launch_tet_ac2_esc(const Stage& S, QRef q,
                         const uint4* tets, uint8_t* flag, Queue q2,
                         PipeLaunch L);

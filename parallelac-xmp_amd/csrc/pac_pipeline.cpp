@@ -288,11 +288,12 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   }
   unsigned long long* const ctr = SG.ctr;
   /* [0, QSLOTS) hot queue, [QSLOTS, 2 QSLOTS) escalation queue, then the
-   * count of simplices left for the second emptiness phase */
+   * count of simplices left for the second emptiness phase and the count
+   * of tets a quick rejection pass left */
   DBuf<unsigned> qn_own;
   if (!rec) {
-    qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 1);
-    zero_u32(qn_own.get(), 2 * QSLOTS + 1, s); /* then kept clear by the views */
+    qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 2);
+    zero_u32(qn_own.get(), 2 * QSLOTS + 2, s); /* then kept clear by the views */
   }
   unsigned* const qn = rec ? rec->qn : qn_own.get();
   unsigned* const nrest = qn + 2 * QSLOTS;
@@ -340,6 +341,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                       fball.get(), fid.get(), fstart.get(), L);
     KCHK();
     SG.fine = 1;
+    SG.quick = opt.ac2_quick > 0;
     SG.sg = pl.sgrids[0];
     SG.sgrids = dsg;
     SG.fball = fball.get();
@@ -623,6 +625,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* ---- tet emptiness ---- */
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
   DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
+  DBuf<uint32_t> next(P, pl.kf ? std::max(Kalpha.cap, (int64_t)1) : 1);
   DBuf<uint8_t> kflag(P, Kalpha.cap);
   Cnt Kf(0);
   DBuf<uint4> ftets;
@@ -643,7 +646,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           ac2_cap(Kalpha.cap), KP_TET_AC2, KP_TET_AC2_EXACT,
           [&](Queue q) {
             launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q,
-                               rest.get(), nrest, budget, L);
+                               rest.get(), nrest, next.get(), nrest + 1,
+                               budget, L);
           },
           [&](QRef q, Queue q2) {
             launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);

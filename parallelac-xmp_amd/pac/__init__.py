@@ -43,7 +43,8 @@ class Options(ctypes.Structure):
                 ("stream", ctypes.c_void_p), ("block", ctypes.c_int32),
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
                 ("batch_balls", ctypes.c_int32), ("ac2_grid", ctypes.c_int32),
-                ("ac2_fine", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("ac2_fine", ctypes.c_int32), ("ac2_quick", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 class Stats(ctypes.Structure):
@@ -169,7 +170,8 @@ def _check(rc):
 
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
-            profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0, ac2_fine=0):
+            profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0, ac2_fine=0,
+            ac2_quick=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -184,6 +186,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.batch_balls = int(batch_balls)
     o.ac2_grid = int(ac2_grid)
     o.ac2_fine = int(ac2_fine)
+    o.ac2_quick = int(ac2_quick)
     return o
 
 
@@ -402,7 +405,7 @@ def autotune(xyzr, solvent, alpha, offsets=None, **kw):
         _check(lib().pac_autotune(ptr, int(off[-1]), off.ctypes.data, off.shape[0] - 1,
                                   solvent, alpha, ctypes.byref(o), ctypes.byref(best)))
     return ({"ac2_budget": o.ac2_budget, "ac2_grid": o.ac2_grid,
-             "ac2_fine": o.ac2_fine}, best.value)
+             "ac2_fine": o.ac2_fine, "ac2_quick": o.ac2_quick}, best.value)
 
 
 def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):

@@ -112,8 +112,10 @@ def test_graph_device_inputs_single_pass(ensemble):
 
 def test_graph_raw_policy_and_skip(ensemble):
     xyzr, off, mols = ensemble
-    _say("case: graph with the reference's raw cell policy / skipped tet test")
-    for kw in (dict(cell_policy=pac.CELL_REFERENCE_RAW), dict(skip_tet_ac2=True)):
+    _say("case: graph with the reference's raw cell policy / skipped tet test "
+         "/ quick rejection pass")
+    for kw in (dict(cell_policy=pac.CELL_REFERENCE_RAW), dict(skip_tet_ac2=True),
+               dict(ac2_fine=4, ac2_quick=1)):
         g = pac.Graph(xyzr, off, 1.4, 0.0, batch_balls=7000, **kw)
         g.launch()
         _same_batch(g.result(), pac.compute_batch(xyzr, off, 1.4, 0.0, **kw),

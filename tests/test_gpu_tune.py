@@ -22,6 +22,7 @@ def test_autotune_single_and_batch():
     assert tuned["ac2_budget"] in (4, 6, 8, 12, 16, 24)
     assert tuned["ac2_grid"] in (1024, 2048, 4096)
     assert tuned["ac2_fine"] in (-1, 2, 4)
+    assert tuned["ac2_quick"] in (0, 1)
     assert ms > 0
     ref = pac.compute(b, 1.4, 0.0).lists()
     got = pac.compute(b, 1.4, 0.0, **tuned).lists()
@@ -47,12 +48,14 @@ def test_extreme_settings_same_output(budget, grid):
         assert np.array_equal(ref[d], got[d])
 
 
-@pytest.mark.parametrize("fine", [-1, 1, 2, 3, 4, 8])
-def test_scan_grids_same_output(fine):
-    """the emptiness scans give the same lists over any scan grid: the
-    reference grid, x-subdivided and y/z-subdivided fine grids; on a
-    globule, a degenerate lattice (exact ties, SoS) and a small batch"""
-    _say("case: scan grid %d" % fine)
+@pytest.mark.parametrize("fine,quick", [(-1, 0), (1, 0), (2, 0), (3, 0), (4, 0),
+                                        (8, 0), (2, 1), (4, 1)])
+def test_scan_grids_same_output(fine, quick):
+    """the emptiness scans give the same lists over any scan grid (the
+    reference cells or cells split along x) with or without the quick
+    rejection pass; on a globule, a degenerate lattice (exact ties, SoS)
+    and a small batch"""
+    _say("case: scan grid %d quick %d" % (fine, quick))
     b = pac.synth_globule(5000, None, 1.3, seed=15)
     m = 6
     g = np.stack(np.meshgrid(*[np.arange(m) * 2.0] * 3, indexing="ij"), -1).reshape(-1, 3)
@@ -60,12 +63,14 @@ def test_scan_grids_same_output(fine):
     for what, x in (("globule", b), ("lattice", lat)):
         for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
             ref = pac.compute(x, 1.4, 0.0, precision=prec, ac2_fine=-1).lists()
-            got = pac.compute(x, 1.4, 0.0, precision=prec, ac2_fine=fine).lists()
+            got = pac.compute(x, 1.4, 0.0, precision=prec, ac2_fine=fine,
+                              ac2_quick=quick).lists()
             for d in (1, 2, 3):
                 assert np.array_equal(ref[d], got[d]), (what, prec, d)
     xb, off = pac.synth_batch(9, 200, 3000, seed=16)
     r1 = pac.compute_batch(xb, off, 1.4, 0.0, ac2_fine=-1)
-    r2 = pac.compute_batch(xb, off, 1.4, 0.0, ac2_fine=fine, batch_balls=4000)
+    r2 = pac.compute_batch(xb, off, 1.4, 0.0, ac2_fine=fine, ac2_quick=quick,
+                           batch_balls=4000)
     for k in range(9):
         a, c = r1.lists(k), r2.lists(k)
         for d in (1, 2, 3):
