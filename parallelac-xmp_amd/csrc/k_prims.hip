@@ -124,6 +124,35 @@ size_t scan_tmp_bytes(int64_t n) {
 template <class TOut>
 __global__ void k_store(TOut* out, TOut v) { *out = v; }
 
+/* Exclusive scan of up to kSumsMax tile sums by one block of 1024
+ * threads (out[m] = total): each thread sums a run of consecutive values,
+ * the block scans the 1024 run sums (wave64 shuffles, then the 16 wave
+ * totals), and each thread writes its run's prefixes. */
+constexpr int kSumsBlock = 1024;
+constexpr int64_t kSumsMax = (int64_t)kSumsBlock * 64;
+__global__ __launch_bounds__(kSumsBlock) void k_scan_sums(const uint64_t* in,
+                                                          int64_t m,
+                                                          uint64_t* out) {
+  __shared__ uint64_t wsum[kSumsBlock / 64];
+  const int per = (int)((m + kSumsBlock - 1) / kSumsBlock);
+  const int64_t b = (int64_t)threadIdx.x * per;
+  uint64_t run = 0;
+  for (int k = 0; k < per; ++k)
+    if (b + k < m) run += in[b + k];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t inc = wave_incl_scan_u64(run);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t ex = inc - run;
+  for (int q = 0; q < w; ++q) ex += wsum[q];
+  for (int k = 0; k < per; ++k)
+    if (b + k < m) {
+      out[b + k] = ex;
+      ex += in[b + k];
+    }
+  if (threadIdx.x == kSumsBlock - 1) out[m] = ex;
+}
+
 /* recursive scan over uint64 tile sums */
 static void scan_u64_rec(const uint64_t* in, uint64_t* out, int64_t n,
                          uint64_t* tmp, hipStream_t s);
@@ -148,7 +177,11 @@ static void scan_impl(const TIn* in, TOut* out, Cnt n, uint64_t* tmp,
   uint64_t* offs = tmp + (tiles + 1);
   hipLaunchKernelGGL((k_tile_sums<TIn, kFlag>), dim3((unsigned)tiles),
                      dim3(kBlock), 0, s, in, n, sums);
-  scan_u64_rec(sums, offs, tiles, tmp + 2 * (tiles + 1), s);
+  if (tiles <= kSumsMax) /* the tile sums in one block: three launches */
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSumsBlock), 0, s,
+                       (const uint64_t*)sums, tiles, offs);
+  else
+    scan_u64_rec(sums, offs, tiles, tmp + 2 * (tiles + 1), s);
   hipLaunchKernelGGL((k_tile_scan<TIn, kFlag, TOut>), dim3((unsigned)tiles),
                      dim3(kBlock), 0, s, in, n, (const uint64_t*)offs, out,
                      total64);
