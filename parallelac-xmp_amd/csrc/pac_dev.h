@@ -465,7 +465,7 @@ __device__ __forceinline__ bool make_sph(const GridP& g, double reach,
   s.fy = (float)((c[1] - (double)g.miny) * iyz - s.b.lo[1]);
   s.fz = (float)((c[2] - (double)g.minz) * iyz - s.b.lo[2]);
   s.R2 = (float)(R * R) * 1.0001f;
-  s.kx = (float)(g.inv_step / iyz) * 1.0001f;
+  s.kx = g.xsplit * 1.0001f;
   s.cy = clamp_cell(s.fy, s.b.lo[1], s.b.hi[1]);
   s.cz = clamp_cell(s.fz, s.b.lo[2], s.b.hi[2]);
   return true;

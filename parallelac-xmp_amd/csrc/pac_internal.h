@@ -34,6 +34,7 @@ struct GridP {
   double inv_step; /* 1 / cell size along x */
   double inv_yz;   /* 1 / cell size along y and z (a fine scan grid splits
                       cells along x only; the reference grid is cubic) */
+  float xsplit;    /* x cells per y-z cell width (inv_step / inv_yz) */
   double maxw;    /* max r^2 of the molecule; < 0: keep the global reach */
 };
 

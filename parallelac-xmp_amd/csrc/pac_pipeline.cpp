@@ -106,6 +106,7 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
   g.margin = margin;
   g.inv_step = 1.0 / (double)step;
   g.inv_yz = g.inv_step;
+  g.xsplit = 1.0f;
   g.maxw = policy == PAC_CELL_REFERENCE_RAW ? -1.0 : maxw;
   return P;
 }
@@ -154,6 +155,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
       gp.g.step = 1;
       gp.g.inv_step = 1;
       gp.g.inv_yz = 1;
+      gp.g.xsplit = 1.0f;
       gp.g.maxw = -1;
       gp.g.ex = gp.g.ey = gp.g.ez = gp.g.k_enum = gp.g.k_ac2 = 1;
       gp.G = 1;
@@ -185,6 +187,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
     for (GridP& f : pl.sgrids) {
       f.inv_yz = 1.0 / (double)f.step;
       f.inv_step = kx / (double)f.step;
+      f.xsplit = (float)kx;
       f.ex *= kx;
       f.cbase *= (uint32_t)kx;
     }
