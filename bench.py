@@ -59,6 +59,9 @@ def main():
                     help="emptiness scan grid x split (with --budget and --grid)")
     ap.add_argument("--quick", type=int, default=0,
                     help="quick rejection pass (with --budget and --grid)")
+    ap.add_argument("--chunk", type=int, default=100000,
+                    help="stress workload: also run the chunked (LD) driver "
+                         "with this chunk_balls (0 = skip)")
     ap.add_argument("--workload", choices=("globule", "batch", "stress", "copy"),
                     default="globule",
                     help="globule: the 100k headline (default); batch: the "
@@ -414,6 +417,26 @@ def bench_stress(args, torch, pac):
     t_pl = time.perf_counter() - t0
     L, Lp = r.lists(), rp.lists()
     same_paths = all(np.array_equal(L[d], Lp[d]) for d in (1, 2, 3))
+    chunked = None
+    if args.chunk > 0:
+        # the LD driver (parallelac-largeData <chunk-size>) on the same input
+        log("stress: chunked, chunk_balls=%d" % args.chunk)
+        kw = dict(tuned[pac.PREC_STAGED], chunk_balls=args.chunk)
+        pac.compute_device(ptr, n, 1.4, 0.0, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = pac.compute_device(ptr, n, 1.4, 0.0, **kw)
+        torch.cuda.synchronize()
+        t_ch = time.perf_counter() - t0
+        Lc, stc = rc.lists(), rc.stats()
+        c = 512
+        while c < args.chunk:
+            c *= 2
+        chunked = {"chunk_balls": args.chunk, "chunk": c, "chunks": -(-n // c),
+                   "ms": t_ch * 1e3,
+                   "eq_one_pass": all(np.array_equal(Lc[d], L[d]) for d in (1, 2, 3)),
+                   "alpha_tets_vs_one_pass": stc.tet_alpha / max(1, st.tet_alpha)}
+        del rc, Lc
     oracle_ok = None
     cpu = None
     if not args.no_cpu:
@@ -441,6 +464,7 @@ def bench_stress(args, torch, pac):
         "bit_exact": {"staged_eq_plain": same_paths, "staged_eq_oracle": oracle_ok},
         "plain_fp64": {"ms_per_step": t_pl * 1e3, "tuned": tuned[pac.PREC_PLAIN_FP64]},
         "tuned": tuned[pac.PREC_STAGED],
+        "chunked": chunked,
         "counts": {"classified": classified, "edges": int(st.edges_final),
                    "triangles": int(st.tris_final), "tets": int(st.tets_final)},
         "escalations": _esc(pac, st), "sos_calls": int(st.sos_calls),

@@ -57,7 +57,12 @@ typedef struct pac_options {
                             (0 = 2, < 0 or 1: the grid's own cells)       */
   int32_t ac2_quick;     /* fine scan grid: 1 = a quick rejection pass over
                             the alpha tets first (0 = off)                */
-  int32_t reserved[1];
+  int32_t chunk_balls;   /* > 0: the LD driver (AlphaComplexLargeData.cu:
+                            1227-1233): the cell-sorted balls are processed
+                            in chunks of 512 * 2^k >= chunk_balls, each by
+                            its own pipeline run over the chunk and a halo;
+                            the lists equal one pass.  0 = one pass.
+                            pac_compute / pac_compute_f64 only.           */
 } pac_options;
 
 typedef struct pac_result pac_result;

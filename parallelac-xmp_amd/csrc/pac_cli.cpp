@@ -6,10 +6,12 @@
  *
  * Same positional interface, output file and stdout summary as the
  * reference (AlphaComplexParallel.cu:1053-1057, :1809-1856).  Invoked as
- * `parallelac-largeData ... <chunk>` (AlphaComplexLargeData.cu:1054-1058)
- * the extra chunk argument is accepted and ignored: the whole input is
- * processed in one pass.
+ * `parallelac-largeData ... <chunk-size>` (AlphaComplexLargeData.cu:
+ * 1054-1058) the input is processed in chunks of 512 * 2^k >= chunk-size
+ * cell-sorted balls (pac_options.chunk_balls) and the per-chunk progress
+ * lines of that driver are printed (:1235); the lists are those of one pass.
  */
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -30,6 +32,7 @@ int main(int argc, char** argv) {
   }
   pac_options opt;
   pac_options_init(&opt);
+  if (large) opt.chunk_balls = std::max(1, std::atoi(argv[5]));
   for (int i = npos; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--staged")) opt.precision = PAC_PREC_STAGED;
     else if (!std::strcmp(argv[i], "--plain")) opt.precision = PAC_PREC_PLAIN_FP64;
@@ -59,6 +62,12 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "error %d: %s\n", rc, pac_last_error());
     pac_free(xyzr);
     return 4;
+  }
+  if (large) {
+    int64_t chunk = 512;
+    while (chunk < opt.chunk_balls) chunk *= 2;
+    for (int64_t at = 0; at < n; at += chunk)
+      std::printf("Processed %d atoms ... \n", (int)at);
   }
   int64_t c[4];
   pac_result_counts(res, c);

@@ -113,6 +113,18 @@ static void run_variant(Engine* E, const double4* raw, int64_t n,
                         pac_result* R, hipStream_t s, Timer& T, KProf& KP) {
   vf64::run_pipeline(E, raw, n, solvent, alpha, opt, R, s, T, KP);
 }
+static void run_chunked_variant(Engine* E, const float4* raw, int64_t n,
+                                float solvent, float alpha,
+                                const pac_options& opt, pac_result* R,
+                                hipStream_t s, Timer& T, KProf& KP) {
+  vf32::run_chunked(E, raw, n, solvent, alpha, opt, R, s, T, KP);
+}
+static void run_chunked_variant(Engine* E, const double4* raw, int64_t n,
+                                double solvent, double alpha,
+                                const pac_options& opt, pac_result* R,
+                                hipStream_t s, Timer& T, KProf& KP) {
+  vf64::run_chunked(E, raw, n, solvent, alpha, opt, R, s, T, KP);
+}
 
 /* canonical order: vertices of each row by original id, rows lexicographic */
 static void canonicalize(Engine* E, pac_result* R, int64_t n, hipStream_t s) {
@@ -150,6 +162,7 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
   pac_options opt;
   pac_options_init(&opt);
   if (o) opt = *o;
+  if (opt.chunk_balls < 0) fail(PAC_ERR_INVALID, "chunk_balls < 0");
   Engine* E = engine_for(opt.device);
   std::lock_guard<std::mutex> lk(E->run_mu);
   HIPCHK(hipSetDevice(E->device));
@@ -179,7 +192,10 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
     hipLaunchKernelGGL((k_balls_orig<B, F>), dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, s, raw, n, solvent, R->balls.get());
     KCHK();
-    run_variant(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
+    if (opt.chunk_balls > 0)
+      run_chunked_variant(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
+    else
+      run_variant(E, raw, n, solvent, alpha, opt, R.get(), s, T, KP);
     if (opt.order == PAC_ORDER_CANONICAL) canonicalize(E, R.get(), n, s);
   } else {
     for (int i = 2; i <= 8; ++i) T.mark(i);
@@ -188,6 +204,7 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
   auto t_host1 = std::chrono::steady_clock::now();
   pac_stats& st = R->st;
   fill_phase_ms(T, st);
+  for (int q = 0; q < 8; ++q) st.phase_ms[q] += R->phase_add[q];
   st.total_ms =
       std::chrono::duration<double, std::milli>(t_host1 - t_host0).count();
   KP.collect(st.kernel_ms);
@@ -313,6 +330,8 @@ void pac::compute_batch(const float* xyzr, const int64_t* offsets,
   pac_options opt;
   pac_options_init(&opt);
   if (o) opt = *o;
+  if (opt.chunk_balls != 0)
+    fail(PAC_ERR_INVALID, "chunk_balls applies to single inputs, not batches");
   const int64_t cap = opt.batch_balls > 0 ? opt.batch_balls : (int64_t)4 << 20;
   Engine* E = engine_for(opt.device);
   std::lock_guard<std::mutex> lk(E->run_mu);

@@ -145,6 +145,8 @@ struct pac_result {
   pac::DBuf<double4> balls;        /* original order, r = r_raw + s (as computed) */
   double alpha = 0;
   pac_stats st;
+  /* phase times (ms) measured outside the run's own timer (chunked runs) */
+  double phase_add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   /* sizes seen by the run (sizing a recorded graph): candidate / survivor
    * counts of each stage, final counts, largest sub-queue fills */
   int64_t sizes[12] = {0};
@@ -339,21 +341,32 @@ struct PassPlan {
 };
 void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
                Real alpha, const pac_options& opt, hipStream_t s,
-               const int64_t* moff, int64_t nmol, PassPlan& pl);
+               const int64_t* moff, int64_t nmol, PassPlan& pl,
+               const Bounds* gridb = nullptr);
 }  // namespace PAC_V
 namespace vf32 {
 void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                   float alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP,
                   const int64_t* moff = nullptr, int64_t nmol = 1,
-                  const PassRec* rec = nullptr);
+                  const PassRec* rec = nullptr,
+                  const Bounds* gridb = nullptr);
+/* the chunked (LD) driver over opt.chunk_balls */
+void run_chunked(Engine* E, const float4* raw, int64_t n, float solvent,
+                 float alpha, const pac_options& opt, pac_result* R,
+                 hipStream_t s, Timer& T, KProf& KP);
 }
 namespace vf64 {
 void run_pipeline(Engine* E, const double4* raw, int64_t n, double solvent,
                   double alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP,
                   const int64_t* moff = nullptr, int64_t nmol = 1,
-                  const PassRec* rec = nullptr);
+                  const PassRec* rec = nullptr,
+                  const Bounds* gridb = nullptr);
+/* the chunked (LD) driver over opt.chunk_balls */
+void run_chunked(Engine* E, const double4* raw, int64_t n, double solvent,
+                 double alpha, const pac_options& opt, pac_result* R,
+                 hipStream_t s, Timer& T, KProf& KP);
 }
 
 }  // namespace pac

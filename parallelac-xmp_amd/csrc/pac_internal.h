@@ -103,6 +103,30 @@ void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void zero_u32(unsigned* p, int n, hipStream_t s);
 void zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 
+/* ---- chunked driver (k_chunk.hip) ---- */
+/* rank[sidx[p]] = p */
+void launch_rank_of(const uint32_t* sidx, int64_t n, uint32_t* rank,
+                    hipStream_t s);
+/* flag[i] = lo <= rank[i] < hi */
+void launch_rank_flag(const uint32_t* rank, int64_t n, uint32_t lo,
+                      uint32_t hi, uint8_t* flag, hipStream_t s);
+/* flagged balls in input order (pos: exclusive scan of flag, m = total):
+ * out[pos[i]] = raw[i], sid[pos[i]] = i */
+void launch_select_balls(const float4* raw, const uint8_t* flag,
+                         const uint32_t* pos, int64_t n, int64_t m,
+                         float4* out, uint32_t* sid, hipStream_t s);
+void launch_select_balls(const double4* raw, const uint8_t* flag,
+                         const uint32_t* pos, int64_t n, int64_t m,
+                         double4* out, uint32_t* sid, hipStream_t s);
+/* out[0..1] = the run of rows of a reference-order list (width w, sub-input
+ * ids) whose first vertex has rank[sid[.]] in [lo, hi) */
+void launch_row_bounds(const int32_t* list, int64_t rows, int w,
+                       const uint32_t* sid, const uint32_t* rank, uint32_t lo,
+                       uint32_t hi, uint64_t* out, hipStream_t s);
+/* out[j] = sid[in[j]] */
+void launch_map_ids(const int32_t* in, int64_t cnt, const uint32_t* sid,
+                    int32_t* out, hipStream_t s);
+
 struct PipeLaunch {
   hipStream_t s;
   int block;

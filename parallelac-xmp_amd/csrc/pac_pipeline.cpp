@@ -115,7 +115,8 @@ static GridPlan plan_grid(const Bounds& B, Real alpha, int policy) {
  * exact scale and the device-side molecule tables of a batch pass. */
 void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
                Real alpha, const pac_options& opt, hipStream_t s,
-               const int64_t* moff, int64_t nmol, PassPlan& pl) {
+               const int64_t* moff, int64_t nmol, PassPlan& pl,
+               const Bounds* gridb) {
   Pool* P = &E->pool;
   PipeLaunch L{s, kBlock};
   const bool batch = nmol > 1;
@@ -137,6 +138,21 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
     HIPCHK(hipMemcpyAsync(hb.data(), bout.get(), nmol * sizeof(Bounds),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+  }
+  if (gridb && !batch) {
+    /* a sub-input of the chunked driver: plan on the grid of the whole
+     * input, so that cell keys, stencils and the row order are those of a
+     * single pass (the exact scale stays the sub-input's own) */
+    Bounds& B = hb[0];
+    B.minx = gridb->minx;
+    B.miny = gridb->miny;
+    B.minz = gridb->minz;
+    B.maxx = gridb->maxx;
+    B.maxy = gridb->maxy;
+    B.maxz = gridb->maxz;
+    B.max_raw_r = gridb->max_raw_r;
+    B.max_r = gridb->max_r;
+    B.max_abs = gridb->max_abs;
   }
   pl.bounds = hb;
   pl.grids.assign(hb.size(), GridP());
@@ -228,7 +244,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
 void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                   Real alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
-                  int64_t nmol, const PassRec* rec) {
+                  int64_t nmol, const PassRec* rec, const Bounds* gridb) {
   Pool* P = rec ? rec->pool : &E->pool;
   PipeLaunch L{s, kBlock};
   pac_stats& st = R->st;
@@ -238,7 +254,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
 
   /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
   PassPlan own;
-  if (!rec) plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, own);
+  if (!rec) plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, own, gridb);
   const PassPlan& pl = rec ? *(const PassPlan*)rec->plan : own;
   /* counts left on the device by the scans (RC_*) */
   DBuf<uint64_t> dcnt_own;
@@ -781,6 +797,167 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   HIPCHK(hipMemcpyAsync(hs, ctr, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   stats_from_counters(hs, SG.staged != 0, host_evals[0], st);
+}
+
+/* The LD driver (AlphaComplexLargeData.cu:1227-1992).  The cell-sorted
+ * balls are walked in chunks of C = 512 * 2^k >= opt.chunk_balls sorted
+ * ids (:1228-1233); a chunk owns the simplices whose lowest sorted vertex
+ * lies in it (:1232, :775-778).
+ *
+ * Here each chunk is one ordinary pipeline run over a sub-input: the balls
+ * of the chunk's z-layers of reference cells plus H layers on either side,
+ * in input order, planned on the grid of the whole input.  Its cell-sorted
+ * order is then the whole input's restricted, so the rows it owns are one
+ * contiguous run of each output list, already in reference order; they are
+ * mapped back to input ids and concatenated chunk after chunk.  Every
+ * decision about a simplex with lowest vertex v -- its alpha filter, its
+ * emptiness test, and the cofaces that close it -- involves only balls
+ * within D = 2 sqrt(maxR^2 + alpha) of v (alpha edges are at most D long;
+ * orthocentres lie within D/2 of every vertex; a ball of power <= rho lies
+ * within D/2 of the centre), and with the reference's own 27-cell stencils
+ * only balls within D/2 plus three cells; H covers both with a margin for
+ * the FP32 cell key, so the lists equal those of one pass.  (LD keeps every
+ * ball resident and lets out-of-chunk vertices through unchecked,
+ * :210-216, :383-399; this driver instead bounds each run's memory by its
+ * sub-input.)  Candidate and escalation counts include the halo's repeated
+ * work. */
+void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
+                 Real alpha, const pac_options& opt, pac_result* R,
+                 hipStream_t s, Timer& T, KProf& KP) {
+  int64_t C = 512;
+  while (C < (int64_t)opt.chunk_balls) C *= 2;
+  if (n <= C) {
+    run_pipeline(E, raw, n, solvent, alpha, opt, R, s, T, KP);
+    return;
+  }
+  if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
+  Pool* P = &E->pool;
+  PipeLaunch L{s, kBlock};
+
+  /* the whole input's grid and cell-sorted order */
+  PassPlan gp;
+  plan_pass(E, raw, n, solvent, alpha, opt, s, nullptr, 1, gp);
+  const GridP g = gp.grids[0];
+  const Bounds gb = gp.bounds[0];
+  DBuf<uint32_t> rank(P, n);
+  std::vector<uint32_t> hkey(n);
+  {
+    DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
+    launch_cell_key(raw, n, g, nullptr, nullptr, key.get(), idx.get(), L);
+    KCHK();
+    int kbits = 0;
+    while (kbits < 32 && ((int64_t)1 << kbits) < gp.G) ++kbits;
+    size_t rtb = radix_tmp_bytes(n);
+    DBuf<uint8_t> rtmp(P, rtb);
+    bool alt = false;
+    radix_sort_pairs(key.get(), idx.get(), key2.get(), idx2.get(), n, kbits,
+                     rtmp.get(), rtb, s, &alt);
+    KCHK();
+    launch_rank_of(alt ? idx2.get() : idx.get(), n, rank.get(), s);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(hkey.data(), alt ? key2.get() : key.get(),
+                          (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  for (int q = 2; q <= 8; ++q) T.mark(q);
+
+  const uint64_t exy = (uint64_t)g.ex * (uint64_t)g.ey;
+  const double reach2 = (double)gb.max_r * (double)gb.max_r + (double)alpha;
+  const double D = reach2 > 0 ? 2 * std::sqrt(reach2) : 0.0;
+  const int64_t H =
+      opt.cell_policy == PAC_CELL_REFERENCE_RAW
+          ? (int64_t)std::ceil(0.5 * D / (double)g.step) + 4
+          : (int64_t)std::ceil(D / (double)g.step) + 2;
+  auto first_at = [&](uint64_t k) -> int64_t { /* first sorted id, key >= k */
+    return (int64_t)(std::lower_bound(hkey.begin(), hkey.end(), k,
+                                      [](uint32_t a, uint64_t b) {
+                                        return (uint64_t)a < b;
+                                      }) -
+                     hkey.begin());
+  };
+
+  pac_stats acc;
+  std::memset(&acc, 0, sizeof(acc));
+  std::vector<DBuf<int32_t>> piece[3];
+  std::vector<int64_t> prows[3];
+  DBuf<uint8_t> flag(P, n);
+  DBuf<uint32_t> pos(P, n + 1);
+  DBuf<uint8_t> stmp(P, scan_tmp_bytes(n) + 64);
+  DBuf<uint64_t> rb(P, 6);
+  uint64_t hb6[6];
+  for (int64_t lo = 0; lo < n; lo += C) {
+    const int64_t hi = std::min(n, lo + C);
+    const int64_t czl = (int64_t)(hkey[lo] / exy), czh = (int64_t)(hkey[hi - 1] / exy);
+    const int64_t za = std::max<int64_t>(0, czl - H), zb = czh + H;
+    const int64_t slo = first_at((uint64_t)za * exy);
+    const int64_t shi = first_at((uint64_t)(zb + 1) * exy);
+    const int64_t m = shi - slo;
+    if (slo > lo || shi < hi || m <= 0) fail(PAC_ERR_DEVICE, "internal: chunk halo");
+    Timer Tc(s);
+    Tc.mark(0);
+    launch_rank_flag(rank.get(), n, (uint32_t)slo, (uint32_t)shi, flag.get(), s);
+    scan_u8flag_to_u32(flag.get(), pos.get(), Cnt(n), stmp.get(), 0, s);
+    DBuf<Ball> sraw(P, m);
+    DBuf<uint32_t> sid(P, m);
+    launch_select_balls(raw, flag.get(), pos.get(), n, m, sraw.get(), sid.get(), s);
+    KCHK();
+    Tc.mark(1);
+    pac_result Rs;
+    Rs.eng = E;
+    Rs.alpha = (double)alpha;
+    std::memset(&Rs.st, 0, sizeof(Rs.st));
+    run_pipeline(E, sraw.get(), m, solvent, alpha, opt, &Rs, s, Tc, KP,
+                 nullptr, 1, nullptr, &gb);
+    for (int d = 1; d <= 3; ++d)
+      launch_row_bounds(Rs.lists[d - 1].get(), Rs.counts[d], d + 1, sid.get(),
+                        rank.get(), (uint32_t)lo, (uint32_t)hi,
+                        rb.get() + 2 * (d - 1), s);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(hb6, rb.get(), sizeof(hb6), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int d = 1; d <= 3; ++d) {
+      const int64_t r0 = (int64_t)hb6[2 * (d - 1)], r1 = (int64_t)hb6[2 * d - 1];
+      if (r0 > r1 || r1 > Rs.counts[d]) fail(PAC_ERR_DEVICE, "internal: chunk rows");
+      if (r1 == r0) continue;
+      DBuf<int32_t> out(P, (r1 - r0) * (d + 1));
+      launch_map_ids(Rs.lists[d - 1].get() + r0 * (d + 1), (r1 - r0) * (d + 1),
+                     sid.get(), out.get(), s);
+      piece[d - 1].push_back(std::move(out));
+      prows[d - 1].push_back(r1 - r0);
+    }
+    KCHK();
+    Tc.mark(9);
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 2; q < 8; ++q) Rs.st.phase_ms[q] = Tc.ms(q, q + 1);
+    Rs.st.phase_ms[1] = Tc.ms(0, 2);  /* selection + binning */
+    Rs.st.phase_ms[0] = Tc.ms(8, 9);  /* owned rows out */
+    add_stats(acc, Rs.st);
+  }
+
+  /* concatenate the chunks' rows */
+  for (int d = 1; d <= 3; ++d) {
+    int64_t rows = 0;
+    for (int64_t r : prows[d - 1]) rows += r;
+    R->lists[d - 1] = DBuf<int32_t>(P, rows * (d + 1));
+    int64_t at = 0;
+    for (size_t k = 0; k < piece[d - 1].size(); ++k) {
+      const int64_t len = prows[d - 1][k] * (d + 1);
+      HIPCHK(hipMemcpyAsync(R->lists[d - 1].get() + at, piece[d - 1][k].get(),
+                            (size_t)len * sizeof(int32_t),
+                            hipMemcpyDeviceToDevice, s));
+      at += len;
+    }
+    R->counts[d] = rows;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  for (int q = 0; q < 8; ++q) R->phase_add[q] = acc.phase_ms[q];
+  acc.n = n;
+  acc.edges_final = R->counts[1];
+  acc.tris_final = R->counts[2];
+  acc.tets_final = R->counts[3];
+  acc.grid_cells = (int32_t)gp.G;
+  R->st = acc;
 }
 }  // namespace PAC_V
 }  // namespace pac

@@ -44,7 +44,7 @@ class Options(ctypes.Structure):
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
                 ("batch_balls", ctypes.c_int32), ("ac2_grid", ctypes.c_int32),
                 ("ac2_fine", ctypes.c_int32), ("ac2_quick", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("chunk_balls", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -171,7 +171,7 @@ def _check(rc):
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
             profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0, ac2_fine=0,
-            ac2_quick=0):
+            ac2_quick=0, chunk_balls=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -187,6 +187,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.ac2_grid = int(ac2_grid)
     o.ac2_fine = int(ac2_fine)
     o.ac2_quick = int(ac2_quick)
+    o.chunk_balls = int(chunk_balls)
     return o
 
 
