@@ -233,6 +233,35 @@ __device__ __forceinline__ bool quick_reject(const Stage& S, const GridP& g,
   return dead;
 }
 
+__device__ __forceinline__ bool owned(const Stage& S, uint32_t v) {
+  return v >= S.own_lo && v < S.own_hi;
+}
+/* chunk sub-input: does an owned simplex depend on this tet? (one of its
+ * faces or edges owned: its lowest, second or third vertex owned) */
+__device__ __forceinline__ bool tet_needed(const Stage& S, const uint4& t) {
+  return !S.chunk || owned(S, t.x) || owned(S, t.y) || owned(S, t.z);
+}
+
+/* chunk sub-input without the quick pass: the tets that need a test.
+ * (The decision is carried in a bool: written as a sentinel id assigned in
+ * one branch of an if/else whose other branch stores, hipcc dropped the
+ * assignment.) */
+__global__ __launch_bounds__(kBlock) void k_tet_need(Stage S, Cnt c_nk,
+                                                     const uint4* tets,
+                                                     uint8_t* flag,
+                                                     uint32_t* next,
+                                                     unsigned* nnext) {
+  const int64_t nk = c_nk.get();
+  const int64_t id = gtid();
+  bool need = false;
+  if (id < nk) {
+    need = tet_needed(S, tets[id]);
+    if (!need) flag[id] = 0;
+  }
+  const unsigned at = block_append(nnext, need);
+  if (need) next[at] = (uint32_t)id;
+}
+
 /* Quick pass over the alpha tets (fine scan grid): FP32 pair constants
  * and quick_reject; the tets it does not reject (or whose pair constants
  * are not certified) are listed in `next` for the certified scan.  A
@@ -251,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
     const GridP g = sgrid_of(S, t.x);
     PairSS<R> P;
     TetCert<R> T;
-    if (tet_pairs(v, P, T) && quick_reject(S, g, P, pairs))
+    if (!tet_needed(S, t) || (tet_pairs(v, P, T) && quick_reject(S, g, P, pairs)))
       flag[id] = 0;
     else
       keep = (uint32_t)id;
@@ -883,12 +912,39 @@ static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
     hipLaunchKernelGGL((k_tet_ac2_quick<R>), gr, dim3(kBlock), 0, L.s, S, nk,
                        tets, flag, next, nnext);
     items = next;
+  } else if (S.chunk) {
+    zero_u32(nnext, 1, L.s);
+    hipLaunchKernelGGL(k_tet_need, gr, dim3(kBlock), 0, L.s, S, nk, tets, flag,
+                       next, nnext);
+    items = next;
   }
   hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE>), gr, dim3(kBlock), 0, L.s,
                      S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
   if (budget > 0)
     hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE>), gb, dim3(kBlock), 0, L.s,
                        S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
+}
+__global__ __launch_bounds__(kBlock) void k_own_mark(
+    Stage S, int kind, Cnt c_n, const uint32_t* ecol, const uint32_t* eleft,
+    const uint32_t* tk, const uint32_t* tedge, uint8_t* in_flag) {
+  const int64_t id = gtid();
+  if (id >= c_n.get()) return;
+  bool need;
+  if (kind == 0) {
+    const uint32_t e = tedge[id];
+    need = owned(S, eleft[e]) || owned(S, ecol[e]);
+  } else {
+    need = owned(S, eleft[id]);
+  }
+  if (!need) in_flag[id] = 1;
+}
+void launch_own_mark(const Stage& S, int kind, Cnt n, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint32_t* tk,
+                     const uint32_t* tedge, uint8_t* in_flag, PipeLaunch L) {
+  if (!S.chunk || n.cap <= 0) return;
+  hipLaunchKernelGGL(k_own_mark, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, S, kind, n, ecol, eleft, tk, tedge,
+                     in_flag);
 }
 void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,

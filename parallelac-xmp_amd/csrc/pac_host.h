@@ -322,6 +322,14 @@ struct PassRec {
   uint32_t c1, c2;         /* their sub-queue capacities */
 };
 
+/* A chunk of the chunked driver run as a sub-input: planned on the grid
+ * of the whole input, its output kept only for the simplices whose lowest
+ * vertex has a cell-sorted id (of the sub-input) in [own_lo, own_hi). */
+struct SubInput {
+  Bounds grid;
+  uint32_t own_lo, own_hi;
+};
+
 namespace PAC_V {
 /* grids, exact scale and device molecule tables of one pipeline pass */
 struct PassPlan {
@@ -350,7 +358,7 @@ void run_pipeline(Engine* E, const float4* raw, int64_t n, float solvent,
                   hipStream_t s, Timer& T, KProf& KP,
                   const int64_t* moff = nullptr, int64_t nmol = 1,
                   const PassRec* rec = nullptr,
-                  const Bounds* gridb = nullptr);
+                  const SubInput* sub = nullptr);
 /* the chunked (LD) driver over opt.chunk_balls */
 void run_chunked(Engine* E, const float4* raw, int64_t n, float solvent,
                  float alpha, const pac_options& opt, pac_result* R,
@@ -362,7 +370,7 @@ void run_pipeline(Engine* E, const double4* raw, int64_t n, double solvent,
                   hipStream_t s, Timer& T, KProf& KP,
                   const int64_t* moff = nullptr, int64_t nmol = 1,
                   const PassRec* rec = nullptr,
-                  const Bounds* gridb = nullptr);
+                  const SubInput* sub = nullptr);
 /* the chunked (LD) driver over opt.chunk_balls */
 void run_chunked(Engine* E, const double4* raw, int64_t n, double solvent,
                  double alpha, const pac_options& opt, pac_result* R,

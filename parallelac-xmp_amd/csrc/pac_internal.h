@@ -309,6 +309,14 @@ struct Stage {
   const Ball* fball;     /* balls in fine-cell order */
   const uint32_t* fid;   /* their ball (cell-sorted) ids */
   const uint32_t* fstart;
+  /* chunk != 0: a chunk's sub-input, whose output keeps only the simplices
+   * with lowest vertex in [own_lo, own_hi) (cell-sorted ids).  Emptiness
+   * tests that cannot change one of those are skipped: tets with none of
+   * their three lowest vertices owned are rejected, triangles with neither
+   * of their two lowest owned and edges with the lower one not owned pass
+   * untested (none of them has an owned face or edge). */
+  int chunk;
+  uint32_t own_lo, own_hi;
 };
 
 
@@ -365,6 +373,11 @@ void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
  * number of items); budget = phase A pair budget (0: single phase) */
 /* next / nnext: scratch list (capacity = number of tets) of the tets a
  * quick rejection pass leaves for the certified scan (fine scan grid) */
+/* chunk sub-input: in_flag[id] = 1 (pass untested) for the triangles
+ * (kind 0) / edges (kind 1) no owned simplex depends on */
+void launch_own_mark(const Stage& S, int kind, Cnt n, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint32_t* tk,
+                     const uint32_t* tedge, uint8_t* in_flag, PipeLaunch L);
 void launch_tet_ac2_hot(const Stage& S, Cnt nk, const uint4* tets,
                         uint8_t* flag, Queue q1, uint32_t* rest,
                         unsigned* nrest, uint32_t* next, unsigned* nnext,

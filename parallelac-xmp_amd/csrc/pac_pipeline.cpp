@@ -244,7 +244,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
 void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                   Real alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
-                  int64_t nmol, const PassRec* rec, const Bounds* gridb) {
+                  int64_t nmol, const PassRec* rec, const SubInput* sub) {
   Pool* P = rec ? rec->pool : &E->pool;
   PipeLaunch L{s, kBlock};
   pac_stats& st = R->st;
@@ -254,7 +254,9 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
 
   /* ---- bounds, grids (AlphaComplexParallel.cu:1089-1124) ---- */
   PassPlan own;
-  if (!rec) plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, own, gridb);
+  if (!rec)
+    plan_pass(E, raw, n, solvent, alpha, opt, s, moff, nmol, own,
+              sub ? &sub->grid : nullptr);
   const PassPlan& pl = rec ? *(const PassPlan*)rec->plan : own;
   /* counts left on the device by the scans (RC_*) */
   DBuf<uint64_t> dcnt_own;
@@ -297,6 +299,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
   SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
   SG.bgrid = opt.ac2_grid > 0 ? std::min(opt.ac2_grid, 1 << 16) : 2048;
+  if (sub) {
+    SG.chunk = 1;
+    SG.own_lo = sub->own_lo;
+    SG.own_hi = sub->own_hi;
+  }
   DBuf<unsigned long long> ctr_own;
   if (rec) {
     SG.ctr = rec->ctr;
@@ -644,7 +651,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* ---- tet emptiness ---- */
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
   DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
-  DBuf<uint32_t> next(P, pl.kf ? std::max(Kalpha.cap, (int64_t)1) : 1);
+  DBuf<uint32_t> next(P, pl.kf || sub ? std::max(Kalpha.cap, (int64_t)1) : 1);
   DBuf<uint8_t> kflag(P, Kalpha.cap);
   Cnt Kf(0);
   DBuf<uint4> ftets;
@@ -689,6 +696,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   fill_u8(tri_in.get(), Talpha.cap, 0, s);
   launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
                     tk.get(), tri_in.get(), L);
+  launch_own_mark(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                  tri_in.get(), L);
   Cnt Tf(0);
   DBuf<uint32_t> ftri;
   {
@@ -732,6 +741,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   fill_u8(edge_in.get(), Ealpha.cap, 0, s);
   launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
                     edge_in.get(), L);
+  launch_own_mark(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr, nullptr,
+                  edge_in.get(), L);
   Cnt Ef(0);
   DBuf<uint32_t> pos_e(P, Ealpha.cap + 1);
   {
@@ -819,8 +830,9 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
  * the FP32 cell key, so the lists equal those of one pass.  (LD keeps every
  * ball resident and lets out-of-chunk vertices through unchecked,
  * :210-216, :383-399; this driver instead bounds each run's memory by its
- * sub-input.)  Candidate and escalation counts include the halo's repeated
- * work. */
+ * sub-input.)  A sub-input skips the emptiness tests no owned simplex
+ * depends on (Stage::chunk); candidate and escalation counts include the
+ * halo's repeated enumeration. */
 void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
                  Real alpha, const pac_options& opt, pac_result* R,
                  hipStream_t s, Timer& T, KProf& KP) {
@@ -907,8 +919,12 @@ void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
     Rs.eng = E;
     Rs.alpha = (double)alpha;
     std::memset(&Rs.st, 0, sizeof(Rs.st));
+    SubInput sub;
+    sub.grid = gb;
+    sub.own_lo = (uint32_t)(lo - slo);
+    sub.own_hi = (uint32_t)(hi - slo);
     run_pipeline(E, sraw.get(), m, solvent, alpha, opt, &Rs, s, Tc, KP,
-                 nullptr, 1, nullptr, &gb);
+                 nullptr, 1, nullptr, &sub);
     for (int d = 1; d <= 3; ++d)
       launch_row_bounds(Rs.lists[d - 1].get(), Rs.counts[d], d + 1, sid.get(),
                         rank.get(), (uint32_t)lo, (uint32_t)hi,
