@@ -481,11 +481,42 @@ __global__ __launch_bounds__(kBlock) void k_tail_caps(Cnt c_m,
   if (q >= m) return;
   cap[q] = (uint32_t)(row[owner[q] + 1] - (uint64_t)q - 1);
 }
+/* chunk sub-input: no slots for the triangles (i,j,k) none of whose
+ * vertices is owned -- every tet they start has its three lowest vertices
+ * unowned, and no owned simplex depends on it */
+__global__ __launch_bounds__(kBlock) void k_tail_caps_own(
+    Stage S, Cnt c_m, const uint64_t* row, const uint32_t* owner,
+    const uint32_t* eleft, const uint32_t* ecol, const uint32_t* tk,
+    uint32_t* cap) {
+  const int64_t m = c_m.get();
+  int64_t q = gtid();
+  if (q >= m) return;
+  const uint32_t e = owner[q];
+  const uint32_t i = eleft[e], j = ecol[e], k = tk[q];
+  const bool need = (i >= S.own_lo && i < S.own_hi) ||
+                    (j >= S.own_lo && j < S.own_hi) ||
+                    (k >= S.own_lo && k < S.own_hi);
+  cap[q] = need ? (uint32_t)(row[e + 1] - (uint64_t)q - 1) : 0u;
+}
 void launch_tail_caps(Cnt m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L) {
   if (m.cap <= 0) return;
   hipLaunchKernelGGL(k_tail_caps, dim3((unsigned)((m.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, m, row, owner, cap);
+}
+void launch_tet_caps(const Stage& S, Cnt m, const uint64_t* trow,
+                     const uint32_t* tedge, const uint32_t* eleft,
+                     const uint32_t* ecol, const uint32_t* tk, uint32_t* cap,
+                     PipeLaunch L) {
+  if (m.cap <= 0) return;
+  if (!S.chunk) {
+    launch_tail_caps(m, trow, tedge, cap, L);
+    return;
+  }
+  hipLaunchKernelGGL(k_tail_caps_own,
+                     dim3((unsigned)((m.cap + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, S, m, trow, tedge, eleft, ecol, tk,
+                     cap);
 }
 
 /* number of triangles of alpha edge q = (i,j): the size of row(i) after j
@@ -660,7 +691,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     uint64_t b = trow[eik], be = trow[eik + 1];
     uint64_t c = trow[ejk], ce = trow[ejk + 1];
     uint64_t o = off[q], oe = off[q + 1];
-    while (a < ae && b < be && c < ce) {
+    /* the slots end with T(i,j) (o < oe adds nothing then) unless the
+     * triangle was given none (chunk sub-inputs) */
+    while (a < ae && b < be && c < ce && o < oe) {
       uint32_t x = tk[a], y = tk[b], z = tk[c];
       uint32_t m = max(x, max(y, z));
       if (x == m && y == m && z == m) {

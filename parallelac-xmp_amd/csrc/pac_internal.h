@@ -332,6 +332,12 @@ void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
 /* slot capacity of item q = row[owner[q] + 1] - q - 1 */
 void launch_tail_caps(Cnt m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L);
+/* tet slots of each alpha triangle: launch_tail_caps over its edge's row,
+ * or none for the triangles a chunk sub-input does not need */
+void launch_tet_caps(const Stage& S, Cnt m, const uint64_t* trow,
+                     const uint32_t* tedge, const uint32_t* eleft,
+                     const uint32_t* ecol, const uint32_t* tk, uint32_t* cap,
+                     PipeLaunch L);
 /* exact triangle count per alpha edge */
 void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, uint32_t* cnt, PipeLaunch L);

@@ -614,7 +614,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint64_t> off(P, Talpha.cap + 1);
     {
       DBuf<uint32_t> cap(P, Talpha.cap);
-      launch_tail_caps(Talpha, trow.get(), tedge.get(), cap.get(), L);
+      launch_tet_caps(SG, Talpha, trow.get(), tedge.get(), eleft.get(),
+                      ecol.get(), tk.get(), cap.get(), L);
       Kslots = count_scan(cap.get(), Talpha, off.get(), RC_KSLOTS, "tet");
     }
     DBuf<uint4> ctet(P, Kslots.cap);
