@@ -201,6 +201,25 @@ int pac_read_crd(const char* path, float** xyzr, int64_t* n);
 int pac_write_output(const char* path, const float* xyzr_raw, int64_t n,
                      float solvent, const pac_result* r);
 void pac_free(void* p);
+/* CRD file in the layout of the reference's data files (data/*.crd):
+ * the count, a blank line, "x \ty \tz \tr" rows with 4 decimals, "#TER".
+ * No reference counterpart (the reference only reads CRD). */
+int pac_write_crd(const char* path, const float* xyzr, int64_t n);
+/* PDB -> balls (x, y, z, r) for pac_compute / pac_write_crd.  The ATOM
+ * records of the first MODEL (and the HETATM records with PAC_PDB_HETATM),
+ * in file order; of an atom's alternate locations the first met; hydrogens
+ * and waters (HOH, WAT, DOD, H2O) left out unless PAC_PDB_HYDROGENS /
+ * PAC_PDB_WATER.  Radius by element (columns 77-78, else the atom name's
+ * columns 13-14): C 1.548, N 1.400, O 1.348, S 1.808 -- the radii of the
+ * reference's data files -- and Bondi radii for H, D, P, SE, F, CL, BR, I;
+ * `radii` (may be NULL), e.g. "C=1.7,N=1.55,*=1.8", overrides entries and
+ * "*" gives every other element a radius.  An element without a radius is
+ * PAC_ERR_INVALID.  Returns a malloc'ed n x 4 array; free with pac_free().
+ * No reference counterpart: the reference's CRD files were made from PDB
+ * entries (README.md:44-46) by a tool it does not ship. */
+enum { PAC_PDB_HETATM = 1, PAC_PDB_HYDROGENS = 2, PAC_PDB_WATER = 4 };
+int pac_read_pdb(const char* path, int flags, const char* radii, float** xyzr,
+                 int64_t* n);
 
 /* Synthetic protein-like ball sets (deterministic; SplitMix64 seeds).
  * globule: Poisson-disk centres (min separation min_sep) in a sphere,

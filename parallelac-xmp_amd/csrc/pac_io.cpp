@@ -6,9 +6,16 @@
  *                     separators and a trailing "#TER" line are accepted;
  *                     a short file is an error instead of garbage)
  *   pac_write_output  AlphaComplexParallel.cu:1834-1856 (byte-compatible)
+ *   pac_write_crd     the layout of the reference's data files (data/*.crd)
+ *   pac_read_pdb      PDB -> balls; no reference counterpart (the
+ *                     reference's CRD files were made from PDB entries,
+ *                     README.md:44-46, by a tool it does not ship)
  */
+#include <cctype>
 #include <cerrno>
 #include <cmath>
+#include <map>
+#include <set>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -114,6 +121,57 @@ double round_dec(double v, int decimals) {
   return std::round(v * sc) / sc;
 }
 
+/* Radii by element.  C, N, O, S: the values of the reference's data files
+ * (data/1grm.crd, data/1k4c.crd hold exactly the radii 1.548, 1.400,
+ * 1.348, 1.808 in the proportions of carbon, nitrogen, oxygen and sulphur
+ * of those proteins); the others: Bondi's van der Waals radii. */
+const std::map<std::string, float>& default_radii() {
+  static const std::map<std::string, float> t = {
+      {"C", 1.548f}, {"N", 1.400f}, {"O", 1.348f}, {"S", 1.808f},
+      {"H", 1.200f}, {"D", 1.200f}, {"P", 1.800f}, {"SE", 1.900f},
+      {"F", 1.470f}, {"CL", 1.750f}, {"BR", 1.850f}, {"I", 1.980f}};
+  return t;
+}
+
+std::string trim(const std::string& x) {
+  size_t a = 0, b = x.size();
+  while (a < b && std::isspace((unsigned char)x[a])) ++a;
+  while (b > a && std::isspace((unsigned char)x[b - 1])) --b;
+  return x.substr(a, b - a);
+}
+
+std::string upper(std::string x) {
+  for (char& c : x) c = (char)std::toupper((unsigned char)c);
+  return x;
+}
+
+/* columns [a, b) of a record (1-based a as in the PDB format) */
+std::string field(const std::string& line, size_t a1, size_t b1) {
+  const size_t a = a1 - 1, b = b1;
+  if (line.size() <= a) return std::string();
+  return line.substr(a, std::min(b, line.size()) - a);
+}
+
+bool parse_coord(const std::string& f, double* v) {
+  const std::string t = trim(f);
+  if (t.empty()) return false;
+  char* end = nullptr;
+  errno = 0;
+  *v = std::strtod(t.c_str(), &end);
+  return end && *end == 0 && errno == 0 && std::isfinite(*v);
+}
+
+/* element of an atom record: columns 77-78, else the letters of the atom
+ * name's columns 13-14 (the element is right-justified there) */
+std::string element_of(const std::string& line) {
+  std::string e = upper(trim(field(line, 77, 78)));
+  if (!e.empty()) return e;
+  std::string nm = field(line, 13, 14), r;
+  for (char c : nm)
+    if (std::isalpha((unsigned char)c)) r += (char)std::toupper((unsigned char)c);
+  return r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -161,6 +219,142 @@ int pac_read_crd(const char* path, float** xyzr, int64_t* n) {
   if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(float));
   *xyzr = out;
   *n = cnt;
+  return PAC_OK;
+}
+
+int pac_write_crd(const char* path, const float* xyzr, int64_t n) {
+  if (!path || n < 0 || (n > 0 && !xyzr)) {
+    pac_set_error("pac_write_crd: bad argument");
+    return PAC_ERR_INVALID;
+  }
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    pac_set_error((std::string("cannot write ") + path).c_str());
+    return PAC_ERR_IO;
+  }
+  std::fprintf(f, "%-13lld\n\n", (long long)n);
+  for (int64_t i = 0; i < n; ++i)
+    std::fprintf(f, "%.4f \t%.4f \t%.4f \t%.4f\n", (double)xyzr[4 * i],
+                 (double)xyzr[4 * i + 1], (double)xyzr[4 * i + 2],
+                 (double)xyzr[4 * i + 3]);
+  std::fprintf(f, "#TER\n");
+  if (std::fclose(f) != 0) {
+    pac_set_error("CRD: write failed");
+    return PAC_ERR_IO;
+  }
+  return PAC_OK;
+}
+
+int pac_read_pdb(const char* path, int flags, const char* radii, float** xyzr,
+                 int64_t* n) {
+  if (!path || !xyzr || !n) {
+    pac_set_error("pac_read_pdb: NULL argument");
+    return PAC_ERR_INVALID;
+  }
+  std::map<std::string, float> table = default_radii();
+  float wildcard = -1.0f;
+  if (radii) { /* "C=1.7,N=1.55,*=1.8" */
+    std::string spec = radii;
+    size_t at = 0;
+    while (at <= spec.size()) {
+      size_t comma = spec.find(',', at);
+      if (comma == std::string::npos) comma = spec.size();
+      const std::string item = trim(spec.substr(at, comma - at));
+      at = comma + 1;
+      if (item.empty()) continue;
+      const size_t eq = item.find('=');
+      char* end = nullptr;
+      const double r = eq == std::string::npos
+                           ? -1
+                           : std::strtod(item.c_str() + eq + 1, &end);
+      if (eq == std::string::npos || eq == 0 || !end || *end != 0 || !(r > 0) ||
+          !std::isfinite(r)) {
+        pac_set_error(("PDB: bad radius entry '" + item + "'").c_str());
+        return PAC_ERR_INVALID;
+      }
+      const std::string el = upper(trim(item.substr(0, eq)));
+      if (el == "*")
+        wildcard = (float)r;
+      else
+        table[el] = (float)r;
+    }
+  }
+  FILE* f = std::fopen(path, "rb");
+  if (!f) {
+    pac_set_error((std::string("cannot open ") + path).c_str());
+    return PAC_ERR_IO;
+  }
+  std::vector<float> v;
+  std::set<std::string> seen; /* atoms already taken (alternate locations) */
+  char buf[512];
+  std::string line;
+  long long lineno = 0;
+  bool models = false;
+  int rc = PAC_OK;
+  std::string err;
+  while (std::fgets(buf, sizeof(buf), f)) {
+    line = buf;
+    while (line.size() && (line.back() == '\n' || line.back() == '\r'))
+      line.pop_back();
+    if (line.size() == sizeof(buf) - 1) { /* over-long line: drop the rest */
+      int c;
+      while ((c = std::fgetc(f)) != EOF && c != '\n') {
+      }
+    }
+    ++lineno;
+    const std::string rec = field(line, 1, 6);
+    if (rec == "MODEL ") {
+      models = true;
+      continue;
+    }
+    if (rec == "ENDMDL" && models) break; /* first model only */
+    if (rec == "END   " || rec == "END") break;
+    const bool atom = rec == "ATOM  ", het = rec == "HETATM";
+    if (!atom && !(het && (flags & PAC_PDB_HETATM))) continue;
+    const std::string res = trim(field(line, 18, 20));
+    if (!(flags & PAC_PDB_WATER) &&
+        (res == "HOH" || res == "WAT" || res == "DOD" || res == "H2O"))
+      continue;
+    const std::string el = element_of(line);
+    if ((el == "H" || el == "D") && !(flags & PAC_PDB_HYDROGENS)) continue;
+    /* alternate locations: the first one met for each atom */
+    const std::string key = field(line, 13, 16) + "|" + field(line, 18, 27);
+    if (field(line, 17, 17) != " " && !field(line, 17, 17).empty()) {
+      if (seen.count(key)) continue;
+    }
+    seen.insert(key);
+    double x, y, z;
+    if (!parse_coord(field(line, 31, 38), &x) ||
+        !parse_coord(field(line, 39, 46), &y) ||
+        !parse_coord(field(line, 47, 54), &z)) {
+      err = "PDB line " + std::to_string(lineno) + ": bad coordinates";
+      rc = PAC_ERR_IO;
+      break;
+    }
+    float r = wildcard;
+    auto it = table.find(el);
+    if (it != table.end()) r = it->second;
+    if (!(r > 0)) {
+      err = "PDB line " + std::to_string(lineno) + ": no radius for element '" +
+            el + "' (give one in the radii table)";
+      rc = PAC_ERR_INVALID;
+      break;
+    }
+    v.push_back((float)x);
+    v.push_back((float)y);
+    v.push_back((float)z);
+    v.push_back(r);
+  }
+  std::fclose(f);
+  if (rc) {
+    pac_set_error(err.c_str());
+    return rc;
+  }
+  float* out = (float*)std::malloc(std::max<size_t>(v.size(), 1) * sizeof(float));
+  if (!out) return PAC_ERR_NOMEM;
+  if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(float));
+  *xyzr = out;
+  *n = (int64_t)(v.size() / 4);
   return PAC_OK;
 }
 

@@ -84,7 +84,8 @@ EXPORTS = (
     "pac_write_output", "pac_free", "pac_synth_globule", "pac_synth_globule_f64",
     "pac_synth_batch", "pac_synth_stress", "pac_last_error", "pac_version",
     "pac_autotune", "pac_graph_create", "pac_graph_set_input", "pac_graph_launch",
-    "pac_graph_result", "pac_graph_info", "pac_graph_free",
+    "pac_graph_result", "pac_graph_info", "pac_graph_free", "pac_write_crd",
+    "pac_read_pdb",
 )
 
 _lib = None
@@ -143,6 +144,9 @@ def lib():
                                ctypes.POINTER(i64)]
     L.pac_write_output.argtypes = [ctypes.c_char_p, vp, i64, ctypes.c_float, vp]
     L.pac_free.argtypes = [vp]
+    L.pac_write_crd.argtypes = [ctypes.c_char_p, vp, i64]
+    L.pac_read_pdb.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                               ctypes.POINTER(vp), ctypes.POINTER(i64)]
     L.pac_synth_globule.argtypes = [i64, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint64, ctypes.c_int, vp]
     L.pac_synth_globule_f64.argtypes = [i64, ctypes.c_double, ctypes.c_double,
@@ -416,16 +420,44 @@ def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):
     return L[1], L[2], L[3]
 
 
-def read_crd(path):
-    p = ctypes.c_void_p()
-    n = ctypes.c_int64()
-    _check(lib().pac_read_crd(path.encode(), ctypes.byref(p), ctypes.byref(n)))
+def _take_balls(p, n):
     try:
         buf = (ctypes.c_float * (4 * n.value)).from_address(p.value) if n.value else []
         a = np.array(buf, dtype=np.float32).reshape(-1, 4)
     finally:
         lib().pac_free(p)
     return a
+
+
+def read_crd(path):
+    p = ctypes.c_void_p()
+    n = ctypes.c_int64()
+    _check(lib().pac_read_crd(path.encode(), ctypes.byref(p), ctypes.byref(n)))
+    return _take_balls(p, n)
+
+
+PDB_HETATM, PDB_HYDROGENS, PDB_WATER = 1, 2, 4
+
+
+def read_pdb(path, hetatm=False, hydrogens=False, water=False, radii=None):
+    """Balls (x, y, z, r) of a PDB file (pac_read_pdb).  radii: dict or
+    "E=r,..." string overriding the element radii ("*": any other)."""
+    flags = ((PDB_HETATM if hetatm else 0) | (PDB_HYDROGENS if hydrogens else 0)
+             | (PDB_WATER if water else 0))
+    if isinstance(radii, dict):
+        radii = ",".join("%s=%r" % (k, float(v)) for k, v in radii.items())
+    p = ctypes.c_void_p()
+    n = ctypes.c_int64()
+    _check(lib().pac_read_pdb(path.encode(), flags,
+                              radii.encode() if radii else None,
+                              ctypes.byref(p), ctypes.byref(n)))
+    return _take_balls(p, n)
+
+
+def write_crd(path, xyzr):
+    """CRD file in the layout of the reference's data files."""
+    a = np.ascontiguousarray(xyzr, dtype=np.float32).reshape(-1, 4)
+    _check(lib().pac_write_crd(path.encode(), a.ctypes.data, a.shape[0]))
 
 
 def write_output(path, xyzr, solvent, result):
