@@ -63,6 +63,11 @@ typedef struct pac_options {
                             its own pipeline run over the chunk and a halo;
                             the lists equal one pass.  0 = one pass.
                             pac_compute / pac_compute_f64 only.           */
+  int32_t filtration;    /* 1: also decide, for every output edge and
+                            triangle, whether it enters the filtration at
+                            its own rho (pac_result_copy_birth /
+                            pac_result_copy_filtration).  Reference row
+                            order; pac_compute / pac_compute_f64 only.    */
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -122,6 +127,24 @@ const int32_t* pac_result_device_list(const pac_result* r, int dim);
 int pac_result_copy_alpha(const pac_result* r, int dim, double* rho);
 int pac_result_stats(const pac_result* r, pac_stats* st);
 void pac_result_free(pac_result* r);
+
+/* Filtration of the complex (a result computed with opt.filtration = 1).
+ * The value at which a simplex enters the filtration of weighted alpha
+ * complexes: a ball at -r^2; an edge or triangle whose smallest
+ * orthosphere is empty (no other ball of power <= rho: the reference's
+ * emptiness rule, AlphaComplexLargeData.cu:838, :971) at its own rho; any
+ * other edge or triangle at the smallest value of the triangles / tets of
+ * the complex it is a face of; a tet at its rho.  The complex at any
+ * alpha' <= alpha is exactly the simplices with value <= alpha'.  FP64.
+ * No reference counterpart: the reference computes rho for triangles and
+ * tets (float4 .w, AlphaComplexParallel.cu:370, :603) and discards it.
+ * copy_birth: values of the simplices of dim 0..3 (balls in input order,
+ * the lists in list order).  copy_filtration: all n + E + T + K simplices
+ * sorted by value, ties in the order balls, edges, triangles, tets (so
+ * every simplex follows its faces), as (dim, index in its list, value). */
+int pac_result_copy_birth(const pac_result* r, int dim, double* value);
+int pac_result_copy_filtration(const pac_result* r, int32_t* dim,
+                               int64_t* index, double* value);
 
 /* Batched ensemble: n_mol molecules concatenated, offsets[n_mol+1]
  * (int64, offsets[0] = 0) into the ball array (n x 4 FP32, device memory

@@ -52,6 +52,41 @@ void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
                      dim3(kBlock), 0, L.s, n, flag, pos, out, ocap);
 }
 
+/* out[i] = a[i] | b[i] (flags) */
+__global__ __launch_bounds__(kBlock) void k_or_flags(Cnt c_n, const uint8_t* a,
+                                                     const uint8_t* b,
+                                                     uint8_t* out) {
+  const int64_t n = c_n.get();
+  const int64_t i = gtid();
+  if (i >= n) return;
+  out[i] = (a[i] | b[i]) != 0;
+}
+void launch_or_flags(Cnt n, const uint8_t* a, const uint8_t* b, uint8_t* out,
+                     PipeLaunch L) {
+  if (n.cap <= 0) return;
+  hipLaunchKernelGGL(k_or_flags, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, a, b, out);
+}
+/* out[pos[i]] = val[i] != 0 for the flagged i */
+__global__ __launch_bounds__(kBlock) void k_compact_u8(Cnt c_n,
+                                                       const uint8_t* flag,
+                                                       const uint32_t* pos,
+                                                       const uint8_t* val,
+                                                       uint8_t* out,
+                                                       uint64_t ocap) {
+  const int64_t n = c_n.get();
+  const int64_t i = gtid();
+  if (i >= n || !flag[i] || pos[i] >= ocap) return;
+  out[pos[i]] = val[i] != 0;
+}
+void launch_compact_u8(Cnt n, const uint8_t* flag, const uint32_t* pos,
+                       const uint8_t* val, uint8_t* out, PipeLaunch L,
+                       uint64_t ocap) {
+  if (n.cap <= 0) return;
+  hipLaunchKernelGGL(k_compact_u8, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, flag, pos, val, out, ocap);
+}
+
 /* faces of the final tets -> tri_in */
 __global__ __launch_bounds__(kBlock) void k_mark_faces(
     Cnt c_nk, const uint4* ftets, const uint64_t* erow, const uint32_t* ecol,

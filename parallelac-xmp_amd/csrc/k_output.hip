@@ -97,6 +97,199 @@ void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
     hipLaunchKernelGGL(k_rho<3>, gr, dim3(kBlock), 0, s, m, rows, balls, rho);
 }
 
+/* ---- filtration values ----
+ * The alpha value at which a simplex enters the filtration of weighted
+ * alpha complexes: a ball at -w; a simplex whose smallest orthosphere is
+ * empty (no other ball of power <= rho, the reference's emptiness rule,
+ * AlphaComplexLargeData.cu:838, :971) at its own rho; any other at the
+ * smallest value among the simplices of the complex it is a face of.  Tets
+ * are always their own rho.  Values are handled as order-preserving
+ * 64-bit keys (atomic minimum, radix sort). */
+__device__ __forceinline__ uint64_t ord64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unord64(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+constexpr uint64_t kNoBirth = ~0ull;
+
+/* key[i] = rho of row i if it is its own (gab == nullptr or gab[i]), else
+ * "not yet" (kNoBirth) */
+__global__ __launch_bounds__(kBlock) void k_birth_init(int64_t m,
+                                                       const double* rho,
+                                                       const uint8_t* gab,
+                                                       uint64_t* key) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  key[i] = (!gab || gab[i]) ? ord64(rho[i]) : kNoBirth;
+}
+/* balls: -w = -r^2 */
+__global__ __launch_bounds__(kBlock) void k_birth_vertex(int64_t n,
+                                                         const double4* balls,
+                                                         uint64_t* key) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const double r = balls[i].w;
+  key[i] = ord64(-(r * r));
+}
+
+/* Row of `face` (K columns, rows lexicographic by rank) equal to the K
+ * ranks f[], or -1. */
+template <int K>
+__device__ __forceinline__ int64_t find_row(const int32_t* face, int64_t mf,
+                                            const uint32_t* rank,
+                                            const uint32_t* f) {
+  int64_t lo = 0, hi = mf; /* first row >= f */
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    int c = 0;
+    for (int q = 0; q < K && c == 0; ++q) {
+      const uint32_t v = rank[face[mid * K + q]];
+      c = v < f[q] ? -1 : (v > f[q] ? 1 : 0);
+    }
+    if (c < 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo >= mf) return -1;
+  for (int q = 0; q < K; ++q)
+    if (rank[face[lo * K + q]] != f[q]) return -1;
+  return lo;
+}
+
+/* Every coface row (K+1 columns) lowers the key of each of its faces that
+ * is not its own value (gab == 0) to its key.  Rows of both lists are in
+ * rank order within the row and lexicographic by rank (reference order). */
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_birth_faces(
+    int64_t mc, const int32_t* cof, const uint64_t* ckey, int64_t mf,
+    const int32_t* face, const uint8_t* fgab, uint64_t* fkey,
+    const uint32_t* rank, unsigned* missing) {
+  const int64_t c = gtid();
+  if (c >= mc) return;
+  uint32_t r[K + 1];
+  for (int q = 0; q <= K; ++q) r[q] = rank[cof[c * (K + 1) + q]];
+  const uint64_t k = ckey[c];
+  for (int drop = 0; drop <= K; ++drop) {
+    uint32_t f[K];
+    int w = 0;
+    for (int q = 0; q <= K; ++q)
+      if (q != drop) f[w++] = r[q];
+    const int64_t row = find_row<K>(face, mf, rank, f);
+    if (row < 0) {
+      atomicAdd(missing, 1u);
+      continue;
+    }
+    if (!fgab[row]) atomicMin((unsigned long long*)&fkey[row], (unsigned long long)k);
+  }
+}
+
+/* one key array over all simplices (balls, edges, triangles, tets) to
+ * values; counts keys left without a value */
+__global__ __launch_bounds__(kBlock) void k_birth_values(int64_t m,
+                                                         const uint64_t* key,
+                                                         double* out,
+                                                         unsigned* missing) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  const uint64_t k = key[i];
+  if (k == kNoBirth) atomicAdd(missing, 1u);
+  out[i] = unord64(k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_split_key(int64_t m,
+                                                      const uint64_t* key,
+                                                      uint32_t* lo,
+                                                      uint32_t* idx) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  lo[i] = (uint32_t)key[i];
+  idx[i] = (uint32_t)i;
+}
+__global__ __launch_bounds__(kBlock) void k_high_key(int64_t m,
+                                                     const uint64_t* key,
+                                                     const uint32_t* idx,
+                                                     uint32_t* hi) {
+  const int64_t i = gtid();
+  if (i >= m) return;
+  hi[i] = (uint32_t)(key[idx[i]] >> 32);
+}
+/* sorted position p -> (dim, index in its list, value) */
+__global__ __launch_bounds__(kBlock) void k_filtration_out(
+    int64_t m, const uint32_t* idx, const uint64_t* key, int64_t b1,
+    int64_t b2, int64_t b3, int32_t* dim, int64_t* index, double* value) {
+  const int64_t p = gtid();
+  if (p >= m) return;
+  const int64_t g = idx[p];
+  const int d = g >= b3 ? 3 : (g >= b2 ? 2 : (g >= b1 ? 1 : 0));
+  const int64_t base = d == 3 ? b3 : (d == 2 ? b2 : (d == 1 ? b1 : 0));
+  dim[p] = d;
+  index[p] = g - base;
+  value[p] = unord64(key[g]);
+}
+
+void birth_keys(const int32_t* const lists[3], const int64_t counts[4],
+                const double4* balls, const uint8_t* const gab[2],
+                const uint32_t* rank, uint64_t* key, unsigned* missing,
+                hipStream_t s) {
+  const int64_t n = counts[0];
+  const int64_t off[4] = {0, n, n + counts[1], n + counts[1] + counts[2]};
+  auto grid = [](int64_t m) { return dim3((unsigned)((m + kBlock - 1) / kBlock)); };
+  if (n > 0)
+    hipLaunchKernelGGL(k_birth_vertex, grid(n), dim3(kBlock), 0, s, n, balls, key);
+  for (int d = 1; d <= 3; ++d) {
+    const int64_t m = counts[d];
+    if (m <= 0) continue;
+    launch_rho(d, m, lists[d - 1], balls, (double*)(key + off[d]), s);
+    /* rho in place, then to keys */
+    hipLaunchKernelGGL(k_birth_init, grid(m), dim3(kBlock), 0, s, m,
+                       (const double*)(key + off[d]), d == 3 ? nullptr : gab[d - 1],
+                       key + off[d]);
+  }
+  /* tets -> triangles, then triangles -> edges */
+  if (counts[3] > 0 && counts[2] > 0)
+    hipLaunchKernelGGL(k_birth_faces<3>, grid(counts[3]), dim3(kBlock), 0, s,
+                       counts[3], lists[2], key + off[3], counts[2], lists[1],
+                       gab[1], key + off[2], rank, missing);
+  if (counts[2] > 0 && counts[1] > 0)
+    hipLaunchKernelGGL(k_birth_faces<2>, grid(counts[2]), dim3(kBlock), 0, s,
+                       counts[2], lists[1], key + off[2], counts[1], lists[0],
+                       gab[0], key + off[1], rank, missing);
+}
+
+void birth_values(int64_t m, const uint64_t* key, double* out,
+                  unsigned* missing, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_birth_values, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, m, key, out, missing);
+}
+
+void filtration_order(int64_t m, const uint64_t* key, const int64_t counts[4],
+                      int32_t* dim, int64_t* index, double* value,
+                      uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* e,
+                      void* rtmp, size_t rtb, hipStream_t s) {
+  if (m <= 0) return;
+  const dim3 gr((unsigned)((m + kBlock - 1) / kBlock));
+  /* stable LSD over the two 32-bit halves of the keys: ties keep the input
+   * order (balls, edges, triangles, tets; list order within), so a face
+   * entering with its coface comes first */
+  hipLaunchKernelGGL(k_split_key, gr, dim3(kBlock), 0, s, m, key, a, b);
+  bool alt = false;
+  radix_sort_pairs(a, b, c, e, m, 32, rtmp, rtb, s, &alt);
+  uint32_t *rk = alt ? c : a, *rv = alt ? e : b;
+  uint32_t *fk = alt ? a : c, *fv = alt ? b : e;
+  hipLaunchKernelGGL(k_high_key, gr, dim3(kBlock), 0, s, m, key, rv, rk);
+  bool alt2 = false;
+  radix_sort_pairs(rk, rv, fk, fv, m, 32, rtmp, rtb, s, &alt2);
+  const uint32_t* fin = alt2 ? fv : rv;
+  const int64_t b1 = counts[0], b2 = b1 + counts[1], b3 = b2 + counts[2];
+  hipLaunchKernelGGL(k_filtration_out, gr, dim3(kBlock), 0, s, m, fin, key, b1,
+                     b2, b3, dim, index, value);
+}
+
 /* ---- adjoint of rho with the combinatorics fixed ----
  * Every vertex q of a simplex satisfies rho = |c - p_q|^2 - w_q.  With
  * c = sum lam_q p_q (sum lam_q = 1, c in the affine hull), summing the

@@ -163,6 +163,8 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
   pac_options_init(&opt);
   if (o) opt = *o;
   if (opt.chunk_balls < 0) fail(PAC_ERR_INVALID, "chunk_balls < 0");
+  if (opt.filtration && opt.order != PAC_ORDER_REFERENCE)
+    fail(PAC_ERR_INVALID, "filtration keeps the reference row order");
   Engine* E = engine_for(opt.device);
   std::lock_guard<std::mutex> lk(E->run_mu);
   HIPCHK(hipSetDevice(E->device));
@@ -170,6 +172,7 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
   std::unique_ptr<pac_result> R(new pac_result());
   R->eng = E;
   R->alpha = alpha;
+  R->filt = opt.filtration != 0;
   std::memset(&R->st, 0, sizeof(R->st));
   R->counts[0] = n;
   Timer T(s);
@@ -332,6 +335,8 @@ void pac::compute_batch(const float* xyzr, const int64_t* offsets,
   if (o) opt = *o;
   if (opt.chunk_balls != 0)
     fail(PAC_ERR_INVALID, "chunk_balls applies to single inputs, not batches");
+  if (opt.filtration != 0)
+    fail(PAC_ERR_INVALID, "filtration applies to single inputs, not batches");
   const int64_t cap = opt.batch_balls > 0 ? opt.batch_balls : (int64_t)4 << 20;
   Engine* E = engine_for(opt.device);
   std::lock_guard<std::mutex> lk(E->run_mu);
@@ -584,6 +589,85 @@ int pac_result_copy_alpha(const pac_result* r, int dim, double* rho) {
                E->stream);
     KCHK();
     HIPCHK(hipMemcpyAsync(rho, d.get(), (size_t)m * sizeof(double),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+  });
+}
+/* filtration keys of a result, made once (engine lock held) */
+static const uint64_t* filtration_keys(const pac_result* r) {
+  if (!r->filt) fail(PAC_ERR_INVALID, "result computed without opt.filtration");
+  Engine* E = r->eng;
+  if (!r->bkey_done) {
+    const int64_t m = r->counts[0] + r->counts[1] + r->counts[2] + r->counts[3];
+    r->bkey = DBuf<uint64_t>(&E->pool, std::max<int64_t>(m, 1));
+    DBuf<unsigned> miss(&E->pool, 1);
+    zero_u32(miss.get(), 1, E->stream);
+    const int32_t* lists[3] = {r->lists[0].get(), r->lists[1].get(),
+                               r->lists[2].get()};
+    const uint8_t* gab[2] = {r->gab[0].get(), r->gab[1].get()};
+    birth_keys(lists, r->counts, r->balls.get(), gab, r->rank.get(),
+               r->bkey.get(), miss.get(), E->stream);
+    KCHK();
+    unsigned h = 0;
+    HIPCHK(hipMemcpyAsync(&h, miss.get(), sizeof(h), hipMemcpyDeviceToHost,
+                          E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (h) fail(PAC_ERR_DEVICE, "internal: a face missing from the lists");
+    r->bkey_done = true;
+  }
+  return r->bkey.get();
+}
+int pac_result_copy_birth(const pac_result* r, int dim, double* value) {
+  if (!r || !value || dim < 0 || dim > 3) return PAC_ERR_INVALID;
+  return guarded([&] {
+    Engine* E = r->eng;
+    std::lock_guard<std::mutex> lk(E->run_mu);
+    HIPCHK(hipSetDevice(E->device));
+    const uint64_t* key = filtration_keys(r);
+    const int64_t m = r->counts[dim];
+    if (m <= 0) return;
+    int64_t off = 0;
+    for (int d = 0; d < dim; ++d) off += r->counts[d];
+    DBuf<double> out(&E->pool, m);
+    DBuf<unsigned> miss(&E->pool, 1);
+    zero_u32(miss.get(), 1, E->stream);
+    birth_values(m, key + off, out.get(), miss.get(), E->stream);
+    KCHK();
+    unsigned h = 0;
+    HIPCHK(hipMemcpyAsync(value, out.get(), (size_t)m * sizeof(double),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipMemcpyAsync(&h, miss.get(), sizeof(h), hipMemcpyDeviceToHost,
+                          E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (h) fail(PAC_ERR_DEVICE, "internal: a simplex without a filtration value");
+  });
+}
+int pac_result_copy_filtration(const pac_result* r, int32_t* dim,
+                               int64_t* index, double* value) {
+  if (!r || !dim || !index || !value) return PAC_ERR_INVALID;
+  return guarded([&] {
+    Engine* E = r->eng;
+    std::lock_guard<std::mutex> lk(E->run_mu);
+    HIPCHK(hipSetDevice(E->device));
+    const uint64_t* key = filtration_keys(r);
+    const int64_t m = r->counts[0] + r->counts[1] + r->counts[2] + r->counts[3];
+    if (m <= 0) return;
+    if (m >= ((int64_t)1 << 32)) fail(PAC_ERR_OVERFLOW, "filtration too large");
+    DBuf<uint32_t> a(&E->pool, m), b(&E->pool, m), c(&E->pool, m),
+        e(&E->pool, m);
+    const size_t rtb = radix_tmp_bytes(m);
+    DBuf<uint8_t> rt(&E->pool, rtb);
+    DBuf<int32_t> dd(&E->pool, m);
+    DBuf<int64_t> di(&E->pool, m);
+    DBuf<double> dv(&E->pool, m);
+    filtration_order(m, key, r->counts, dd.get(), di.get(), dv.get(), a.get(),
+                     b.get(), c.get(), e.get(), rt.get(), rtb, E->stream);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(dim, dd.get(), (size_t)m * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipMemcpyAsync(index, di.get(), (size_t)m * sizeof(int64_t),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipMemcpyAsync(value, dv.get(), (size_t)m * sizeof(double),
                           hipMemcpyDeviceToHost, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
   });

@@ -147,6 +147,14 @@ struct pac_result {
   pac_stats st;
   /* phase times (ms) measured outside the run's own timer (chunked runs) */
   double phase_add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  /* filtration (pac_options.filtration): per output edge / triangle, 1 if
+   * its smallest orthosphere is empty (it enters at its own rho); rank of
+   * each ball in cell-sorted order; filtration keys, made on first use */
+  bool filt = false;
+  pac::DBuf<uint8_t> gab[2];
+  pac::DBuf<uint32_t> rank;
+  mutable pac::DBuf<uint64_t> bkey;
+  mutable bool bkey_done = false;
   /* sizes seen by the run (sizing a recorded graph): candidate / survivor
    * counts of each stage, final counts, largest sub-queue fills */
   int64_t sizes[12] = {0};

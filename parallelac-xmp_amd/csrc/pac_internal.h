@@ -211,6 +211,11 @@ void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
                          uint64_t ocap = ~0ull);
 void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
                         uint32_t* out, PipeLaunch L, uint64_t ocap = ~0ull);
+void launch_or_flags(Cnt n, const uint8_t* a, const uint8_t* b, uint8_t* out,
+                     PipeLaunch L);
+void launch_compact_u8(Cnt n, const uint8_t* flag, const uint32_t* pos,
+                       const uint8_t* val, uint8_t* out, PipeLaunch L,
+                       uint64_t ocap = ~0ull);
 void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
@@ -229,6 +234,24 @@ void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
                        uint64_t ocap = ~0ull);
 
 /* ---- outputs (k_output.hip) ---- */
+/* Filtration keys of all simplices, key[0, n + E + T + K) (balls, edges,
+ * triangles, tets; order-preserving 64-bit images of the values): lists in
+ * reference order, gab[0..1] the own-value flags of edges / triangles,
+ * rank[original id] = cell-sorted position; *missing counts faces not
+ * found among the lists */
+void birth_keys(const int32_t* const lists[3], const int64_t counts[4],
+                const double4* balls, const uint8_t* const gab[2],
+                const uint32_t* rank, uint64_t* key, unsigned* missing,
+                hipStream_t s);
+/* values of m keys; *missing counts keys that never got a value */
+void birth_values(int64_t m, const uint64_t* key, double* out,
+                  unsigned* missing, hipStream_t s);
+/* all m keys sorted by value (stable): dim / index in its list / value;
+ * a, b, c, e: m u32 scratch each; rtmp: radix_tmp_bytes(m) */
+void filtration_order(int64_t m, const uint64_t* key, const int64_t counts[4],
+                      int32_t* dim, int64_t* index, double* value,
+                      uint32_t* a, uint32_t* b, uint32_t* c, uint32_t* e,
+                      void* rtmp, size_t rtb, hipStream_t s);
 void launch_rho(int dim, int64_t m, const int32_t* rows, const double4* balls,
                 double* rho, hipStream_t s);
 /* adjoint: per-(simplex, vertex) contributions at slots slot0 + (dim+1)*id + q

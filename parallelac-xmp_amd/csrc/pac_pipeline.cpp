@@ -693,19 +693,33 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   T.mark(6);
 
   /* ---- triangle closure + dangling emptiness ---- */
+  /* filtration: every final triangle and edge gets its emptiness test (an
+   * empty smallest orthosphere: the simplex enters at its own rho), and
+   * the closure marks join the final flags only after the test */
+  const bool filt = opt.filtration != 0 && !rec;
   DBuf<uint8_t> tri_in(P, Talpha.cap), tflag(P, Talpha.cap);
   fill_u8(tri_in.get(), Talpha.cap, 0, s);
   launch_mark_faces(Kf, ftets.get(), erow.get(), ecol.get(), trow.get(),
                     tk.get(), tri_in.get(), L);
+  DBuf<uint8_t> tri_skip_own, tfin;
+  uint8_t* tri_skip = tri_in.get(); /* passes without a test */
+  if (filt) {
+    tri_skip_own = DBuf<uint8_t>(P, Talpha.cap);
+    fill_u8(tri_skip_own.get(), Talpha.cap, 0, s);
+    tri_skip = tri_skip_own.get();
+    tfin = DBuf<uint8_t>(P, Talpha.cap);
+  }
   launch_own_mark(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(), tedge.get(),
-                  tri_in.get(), L);
+                  tri_skip, L);
+  const uint8_t* tfinal = filt ? tfin.get() : tflag.get();
   Cnt Tf(0);
   DBuf<uint32_t> ftri;
   {
     DBuf<uint32_t> pos(P, Talpha.cap + 1);
     auto tmp = scan_tmp(Talpha.cap);
     auto tail = [&] {
-      scan_u8flag_to_u32(tflag.get(), pos.get(), Talpha, tmp.get(), 0, s,
+      if (filt) launch_or_flags(Talpha, tflag.get(), tri_in.get(), tfin.get(), L);
+      scan_u8flag_to_u32(tfinal, pos.get(), Talpha, tmp.get(), 0, s,
                          dcnt + RC_TF);
     };
     if (!Talpha.cap) {
@@ -717,7 +731,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           ac2_cap(Talpha.cap), KP_TRI_AC2, -1,
           [&](Queue q) {
             launch_low_ac2_hot(SG, 0, Talpha, ecol.get(), eleft.get(), tk.get(),
-                               tedge.get(), tri_in.get(), tflag.get(), q,
+                               tedge.get(), tri_skip, tflag.get(), q,
                                rest.get(), nrest, budget, L);
           },
           [&](QRef q, Queue q2) {
@@ -731,8 +745,13 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           tail, RC_TF);
     }
     ftri = DBuf<uint32_t>(P, Tf.cap);
-    launch_compact_ids(Talpha, tflag.get(), pos.get(), ftri.get(), L,
+    launch_compact_ids(Talpha, tfinal, pos.get(), ftri.get(), L,
                        (uint64_t)Tf.cap);
+    if (filt) {
+      R->gab[1] = DBuf<uint8_t>(P, Tf.cap);
+      launch_compact_u8(Talpha, tfinal, pos.get(), tflag.get(), R->gab[1].get(),
+                        L, (uint64_t)Tf.cap);
+    }
     KCHK();
   }
   T.mark(7);
@@ -742,14 +761,24 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   fill_u8(edge_in.get(), Ealpha.cap, 0, s);
   launch_mark_edges(Tf, ftri.get(), tedge.get(), teik.get(), tejk.get(),
                     edge_in.get(), L);
+  DBuf<uint8_t> edge_skip_own, efin;
+  uint8_t* edge_skip = edge_in.get();
+  if (filt) {
+    edge_skip_own = DBuf<uint8_t>(P, Ealpha.cap);
+    fill_u8(edge_skip_own.get(), Ealpha.cap, 0, s);
+    edge_skip = edge_skip_own.get();
+    efin = DBuf<uint8_t>(P, Ealpha.cap);
+  }
   launch_own_mark(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr, nullptr,
-                  edge_in.get(), L);
+                  edge_skip, L);
+  const uint8_t* efinal = filt ? efin.get() : eflag.get();
   Cnt Ef(0);
   DBuf<uint32_t> pos_e(P, Ealpha.cap + 1);
   {
     auto tmp = scan_tmp(Ealpha.cap);
     auto tail = [&] {
-      scan_u8flag_to_u32(eflag.get(), pos_e.get(), Ealpha, tmp.get(), 0, s,
+      if (filt) launch_or_flags(Ealpha, eflag.get(), edge_in.get(), efin.get(), L);
+      scan_u8flag_to_u32(efinal, pos_e.get(), Ealpha, tmp.get(), 0, s,
                          dcnt + RC_EF);
     };
     if (!Ealpha.cap) {
@@ -761,7 +790,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           ac2_cap(Ealpha.cap), KP_EDGE_AC2, -1,
           [&](Queue q) {
             launch_low_ac2_hot(SG, 1, Ealpha, ecol.get(), eleft.get(), nullptr,
-                               nullptr, edge_in.get(), eflag.get(), q,
+                               nullptr, edge_skip, eflag.get(), q,
                                rest.get(), nrest, budget, L);
           },
           [&](QRef q, Queue q2) {
@@ -780,8 +809,15 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   R->lists[0] = DBuf<int32_t>(P, 2 * Ef.cap);
   R->lists[1] = DBuf<int32_t>(P, 3 * Tf.cap);
   R->lists[2] = DBuf<int32_t>(P, 4 * Kf.cap);
-  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), eflag.get(), pos_e.get(),
+  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), efinal, pos_e.get(),
                     perm.get(), R->lists[0].get(), L, (uint64_t)Ef.cap);
+  if (filt) {
+    R->gab[0] = DBuf<uint8_t>(P, Ef.cap);
+    launch_compact_u8(Ealpha, efinal, pos_e.get(), eflag.get(), R->gab[0].get(),
+                      L, (uint64_t)Ef.cap);
+    R->rank = DBuf<uint32_t>(P, n);
+    launch_rank_of(perm.get(), n, R->rank.get(), s);
+  }
   launch_emit_tris(Tf, ftri.get(), ecol.get(), eleft.get(), tk.get(),
                    tedge.get(), perm.get(), R->lists[1].get(), L);
   launch_emit_tets(Kf, ftets.get(), perm.get(), R->lists[2].get(), L);
@@ -893,6 +929,7 @@ void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
   pac_stats acc;
   std::memset(&acc, 0, sizeof(acc));
   std::vector<DBuf<int32_t>> piece[3];
+  std::vector<DBuf<uint8_t>> gpiece[2]; /* filtration flags of the rows */
   std::vector<int64_t> prows[3];
   DBuf<uint8_t> flag(P, n);
   DBuf<uint32_t> pos(P, n + 1);
@@ -942,6 +979,12 @@ void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
                      sid.get(), out.get(), s);
       piece[d - 1].push_back(std::move(out));
       prows[d - 1].push_back(r1 - r0);
+      if (opt.filtration && d <= 2) {
+        DBuf<uint8_t> g(P, r1 - r0);
+        HIPCHK(hipMemcpyAsync(g.get(), Rs.gab[d - 1].get() + r0, (size_t)(r1 - r0),
+                              hipMemcpyDeviceToDevice, s));
+        gpiece[d - 1].push_back(std::move(g));
+      }
     }
     KCHK();
     Tc.mark(9);
@@ -966,7 +1009,18 @@ void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
       at += len;
     }
     R->counts[d] = rows;
+    if (opt.filtration && d <= 2) {
+      R->gab[d - 1] = DBuf<uint8_t>(P, rows);
+      int64_t g = 0;
+      for (size_t k = 0; k < gpiece[d - 1].size(); ++k) {
+        const int64_t len = prows[d - 1][k];
+        HIPCHK(hipMemcpyAsync(R->gab[d - 1].get() + g, gpiece[d - 1][k].get(),
+                              (size_t)len, hipMemcpyDeviceToDevice, s));
+        g += len;
+      }
+    }
   }
+  if (opt.filtration) R->rank = std::move(rank);
   HIPCHK(hipStreamSynchronize(s));
   for (int q = 0; q < 8; ++q) R->phase_add[q] = acc.phase_ms[q];
   acc.n = n;

@@ -44,7 +44,7 @@ class Options(ctypes.Structure):
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
                 ("batch_balls", ctypes.c_int32), ("ac2_grid", ctypes.c_int32),
                 ("ac2_fine", ctypes.c_int32), ("ac2_quick", ctypes.c_int32),
-                ("chunk_balls", ctypes.c_int32)]
+                ("chunk_balls", ctypes.c_int32), ("filtration", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -85,7 +85,7 @@ EXPORTS = (
     "pac_synth_batch", "pac_synth_stress", "pac_last_error", "pac_version",
     "pac_autotune", "pac_graph_create", "pac_graph_set_input", "pac_graph_launch",
     "pac_graph_result", "pac_graph_info", "pac_graph_free", "pac_write_crd",
-    "pac_read_pdb",
+    "pac_read_pdb", "pac_result_copy_birth", "pac_result_copy_filtration",
 )
 
 _lib = None
@@ -122,6 +122,8 @@ def lib():
     L.pac_result_device_list.restype = vp
     L.pac_result_copy_alpha.argtypes = [vp, ctypes.c_int, vp]
     L.pac_result_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.pac_result_copy_birth.argtypes = [vp, ctypes.c_int, vp]
+    L.pac_result_copy_filtration.argtypes = [vp, vp, vp, vp]
     L.pac_result_free.argtypes = [vp]
     L.pac_compute_batch.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float,
                                     ctypes.POINTER(Options), ctypes.POINTER(vp)]
@@ -175,7 +177,7 @@ def _check(rc):
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
             profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0, ac2_fine=0,
-            ac2_quick=0, chunk_balls=0):
+            ac2_quick=0, chunk_balls=0, filtration=False):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -192,6 +194,7 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.ac2_fine = int(ac2_fine)
     o.ac2_quick = int(ac2_quick)
     o.chunk_balls = int(chunk_balls)
+    o.filtration = int(bool(filtration))
     return o
 
 
@@ -260,6 +263,24 @@ class Result:
         rho = np.zeros(n, dtype=np.float64)
         _check(lib().pac_result_copy_alpha(self._h, dim, rho.ctypes.data))
         return rho
+
+    def births(self, dim):
+        """Filtration values of the simplices of dim 0..3 (needs
+        filtration=True at compute time)."""
+        n = int(self.counts()[dim])
+        v = np.zeros(n, dtype=np.float64)
+        _check(lib().pac_result_copy_birth(self._h, dim, v.ctypes.data))
+        return v
+
+    def filtration(self):
+        """All simplices sorted by filtration value: (dim, index, value)."""
+        m = int(sum(self.counts()))
+        d = np.zeros(m, dtype=np.int32)
+        i = np.zeros(m, dtype=np.int64)
+        v = np.zeros(m, dtype=np.float64)
+        _check(lib().pac_result_copy_filtration(self._h, d.ctypes.data,
+                                                i.ctypes.data, v.ctypes.data))
+        return d, i, v
 
 
 def compute(xyzr, solvent, alpha, **kw):
