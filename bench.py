@@ -183,6 +183,25 @@ def main():
     del g
     log("gpu legs done: staged %.3f ms plain %.3f ms graph %.3f ms"
         % (t_st * 1e3, t_pl * 1e3, t_gr * 1e3))
+
+    # the same run with the filtration (entry values of every simplex and
+    # the sorted filtration copied to the host)
+    def filt_run():
+        rf = pac.compute_device(ptr, n, solvent, alpha, precision=pac.PREC_STAGED,
+                                filtration=True, **tuned[pac.PREC_STAGED])
+        return rf, rf.filtration()
+    filt_run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rf, (fd, fi, fv) = filt_run()
+    torch.cuda.synchronize()
+    t_f = time.perf_counter() - t0
+    sf = rf.stats()
+    filt = {"ms": t_f * 1e3, "entries": int(len(fd)),
+            "lists_eq": all(np.array_equal(rf.lists()[d], L_st[d]) for d in (1, 2, 3)),
+            "tri_ac2_pairs_vs_plain_run": sf.ac2_pairs[1] / max(1, s_st.ac2_pairs[1]),
+            "edge_ac2_pairs_vs_plain_run": sf.ac2_pairs[2] / max(1, s_st.ac2_pairs[2])}
+    del rf, fd, fi, fv
     cpu = None
     oracle_exact = None
     if not args.no_cpu:
@@ -221,6 +240,7 @@ def main():
         "tuned": tuned[pac.PREC_STAGED],
         "staged_over_plain": t_pl / t_st,
         "staged_one_graph_launch": {"ms_per_step": t_gr * 1e3, "eq_eager": graph_eq},
+        "filtration": filt,
         "bit_exact": {"staged_eq_plain": same_paths, "staged_eq_oracle": oracle_exact},
         "counts": {"classified": classified,
                    "edge_cand": int(s_st.edge_candidates), "tri_cand": int(s_st.tri_candidates),
