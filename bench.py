@@ -420,6 +420,7 @@ def bench_stress(args, torch, pac):
     for _ in range(args.warmup):
         pac.compute_device(ptr, n, 1.4, 0.0, **tuned[pac.PREC_STAGED])
     torch.cuda.synchronize()
+    pac.memory_stats(reset_peak=True)
     t0 = time.perf_counter()
     r = None
     for _ in range(args.steps):
@@ -427,6 +428,7 @@ def bench_stress(args, torch, pac):
         r = pac.compute_device(ptr, n, 1.4, 0.0, **tuned[pac.PREC_STAGED])
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / args.steps
+    _, peak_one, _ = pac.memory_stats(reset_peak=True)
     st = r.stats()
     log("stress: plain FP64 path")
     torch.cuda.synchronize()
@@ -444,10 +446,12 @@ def bench_stress(args, torch, pac):
         kw = dict(tuned[pac.PREC_STAGED], chunk_balls=args.chunk)
         pac.compute_device(ptr, n, 1.4, 0.0, **kw)
         torch.cuda.synchronize()
+        pac.memory_stats(reset_peak=True)
         t0 = time.perf_counter()
         rc = pac.compute_device(ptr, n, 1.4, 0.0, **kw)
         torch.cuda.synchronize()
         t_ch = time.perf_counter() - t0
+        _, peak_ch, _ = pac.memory_stats(reset_peak=True)
         Lc, stc = rc.lists(), rc.stats()
         c = 512
         while c < args.chunk:
@@ -455,7 +459,9 @@ def bench_stress(args, torch, pac):
         chunked = {"chunk_balls": args.chunk, "chunk": c, "chunks": -(-n // c),
                    "ms": t_ch * 1e3,
                    "eq_one_pass": all(np.array_equal(Lc[d], L[d]) for d in (1, 2, 3)),
-                   "alpha_tets_vs_one_pass": stc.tet_alpha / max(1, st.tet_alpha)}
+                   "alpha_tets_vs_one_pass": stc.tet_alpha / max(1, st.tet_alpha),
+                   "peak_device_mb": peak_ch / 2**20,
+                   "one_pass_peak_device_mb": peak_one / 2**20}
         del rc, Lc
     oracle_ok = None
     cpu = None

@@ -217,6 +217,16 @@ int pac_alpha_adjoint(const pac_result* r, const double* dl_drho_tri,
                       const double* dl_drho_tet, double* dl_dxyz,
                       double* dl_dr);
 
+/* Device memory of the library's buffer pool on a device (0s before its
+ * first use): bytes in live buffers, their high-water mark since the last
+ * reset (reset_peak != 0 resets it to the current use), and bytes held
+ * (live plus cached for reuse); sizes by power-of-two class.
+ * pac_memory_trim returns the cached blocks to the device.  What the
+ * chunked driver (opt.chunk_balls) bounds.  No reference counterpart. */
+int pac_memory_stats(int device, int64_t* in_use, int64_t* peak,
+                     int64_t* held, int reset_peak);
+int pac_memory_trim(int device);
+
 /* CRD input (AlphaComplexParallel.cu:1069-1088): returns a malloc'ed
  * n x 4 FP32 array (x, y, z, r_raw); free with pac_free(). */
 int pac_read_crd(const char* path, float** xyzr, int64_t* n);

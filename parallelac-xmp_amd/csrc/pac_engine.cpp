@@ -725,6 +725,42 @@ int pac_alpha_adjoint(const pac_result* r, const double* dl_drho_tri,
   return guarded([&] { alpha_adjoint(r, dl_drho_tri, dl_drho_tet, dl_dxyz, dl_dr); });
 }
 
+int pac_memory_stats(int device, int64_t* in_use, int64_t* peak,
+                     int64_t* held, int reset_peak) {
+  Engine* E = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_eng_mu);
+    auto it = g_engines.find(device);
+    if (it != g_engines.end()) E = it->second;
+  }
+  int64_t u = 0, p = 0, h = 0;
+  if (E) {
+    std::lock_guard<std::mutex> g(E->pool.m);
+    u = E->pool.in_use;
+    p = E->pool.peak;
+    h = E->pool.held;
+    if (reset_peak) E->pool.peak = E->pool.in_use;
+  }
+  if (in_use) *in_use = u;
+  if (peak) *peak = p;
+  if (held) *held = h;
+  return PAC_OK;
+}
+int pac_memory_trim(int device) {
+  Engine* E = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_eng_mu);
+    auto it = g_engines.find(device);
+    if (it != g_engines.end()) E = it->second;
+  }
+  if (!E) return PAC_OK;
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(E->run_mu);
+    HIPCHK(hipSetDevice(E->device));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    E->pool.trim();
+  });
+}
 const char* pac_last_error(void) { return g_err.c_str(); }
 const char* pac_version(void) { return "parallelac-mi355x 0.1"; }
 

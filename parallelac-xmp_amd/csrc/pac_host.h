@@ -48,18 +48,32 @@ struct Pool {
   std::mutex m;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> cls_;
+  /* bytes in live buffers, their high-water mark, bytes held (live +
+   * cached) -- by size class */
+  int64_t in_use = 0, peak = 0, held = 0;
+  /* size classes: powers of two up to 1 MiB, then eight per octave over
+   * the request plus 4 KiB (at most 12.5% over: the candidate arrays of a
+   * large input run to gigabytes) */
   static size_t klass(size_t b) {
     size_t c = 4096;
-    while (c < b) c <<= 1;
-    return c;
+    while (c < b && c < ((size_t)1 << 20)) c <<= 1;
+    if (c >= b) return c;
+    b += 4096;
+    size_t oct = (size_t)1 << 20;
+    while (oct * 2 <= b) oct <<= 1;
+    const size_t step = oct >> 3;
+    return (b + step - 1) / step * step;
   }
   void* get(size_t bytes) {
     size_t c = klass(bytes ? bytes : 1);
     {
       std::lock_guard<std::mutex> g(m);
-      auto it = free_.find(c);
-      if (it != free_.end()) {
+      /* the smallest cached block that fits, if within a class or two */
+      auto it = free_.lower_bound(c);
+      if (it != free_.end() && it->first <= c + c / 4) {
         void* p = it->second;
+        in_use += (int64_t)it->first;
+        peak = std::max(peak, in_use);
         free_.erase(it);
         return p;
       }
@@ -78,18 +92,24 @@ struct Pool {
     }
     std::lock_guard<std::mutex> g(m);
     cls_[p] = c;
+    held += (int64_t)c;
+    in_use += (int64_t)c;
+    peak = std::max(peak, in_use);
     return p;
   }
   void put(void* p) {
     if (!p) return;
     std::lock_guard<std::mutex> g(m);
-    free_.insert({cls_[p], p});
+    const size_t c = cls_[p];
+    in_use -= (int64_t)c;
+    free_.insert({c, p});
   }
   void trim() {
     std::lock_guard<std::mutex> g(m);
     for (auto& kv : free_) {
       (void)hipFree(kv.second);
       cls_.erase(kv.second);
+      held -= (int64_t)kv.first;
     }
     free_.clear();
   }

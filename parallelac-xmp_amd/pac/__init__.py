@@ -86,6 +86,7 @@ EXPORTS = (
     "pac_autotune", "pac_graph_create", "pac_graph_set_input", "pac_graph_launch",
     "pac_graph_result", "pac_graph_info", "pac_graph_free", "pac_write_crd",
     "pac_read_pdb", "pac_result_copy_birth", "pac_result_copy_filtration",
+    "pac_memory_stats", "pac_memory_trim",
 )
 
 _lib = None
@@ -123,6 +124,10 @@ def lib():
     L.pac_result_copy_alpha.argtypes = [vp, ctypes.c_int, vp]
     L.pac_result_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.pac_result_copy_birth.argtypes = [vp, ctypes.c_int, vp]
+    L.pac_memory_stats.argtypes = [ctypes.c_int, ctypes.POINTER(i64),
+                                   ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                   ctypes.c_int]
+    L.pac_memory_trim.argtypes = [ctypes.c_int]
     L.pac_result_copy_filtration.argtypes = [vp, vp, vp, vp]
     L.pac_result_free.argtypes = [vp]
     L.pac_compute_batch.argtypes = [vp, vp, i64, ctypes.c_float, ctypes.c_float,
@@ -448,6 +453,18 @@ def _take_balls(p, n):
     finally:
         lib().pac_free(p)
     return a
+
+
+def memory_stats(device=0, reset_peak=False):
+    """Device bytes of the library's pool: (in_use, peak, held)."""
+    u, p, h = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().pac_memory_stats(device, ctypes.byref(u), ctypes.byref(p),
+                                  ctypes.byref(h), int(bool(reset_peak))))
+    return u.value, p.value, h.value
+
+
+def memory_trim(device=0):
+    _check(lib().pac_memory_trim(device))
 
 
 def read_crd(path):

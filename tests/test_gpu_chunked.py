@@ -140,6 +140,28 @@ def test_chunked_rejects_batches():
     assert e.value.code == pac.PAC_ERR_INVALID
 
 
+def test_chunked_bounds_device_memory():
+    """What the LD driver is for: a chunked run's peak device memory is
+    that of its largest sub-input, far below one pass over everything."""
+    _say("case: 200k globule, peak pool memory one pass vs chunk 16384")
+    b = pac.synth_globule(200000, None, 1.3, seed=12)
+    pac.memory_trim()
+    pac.memory_stats(reset_peak=True)
+    one = pac.compute(b, 1.4, 0.0)
+    _, peak_one, _ = pac.memory_stats(reset_peak=True)
+    lo = one.lists()
+    del one
+    ch = pac.compute(b, 1.4, 0.0, chunk_balls=16384)
+    _, peak_ch, held = pac.memory_stats(reset_peak=True)
+    _same(ch.lists(), lo, "200k chunk 16384")
+    _say("peak one pass %.1f MB, chunked %.1f MB" % (peak_one / 2**20, peak_ch / 2**20))
+    assert 0 < peak_ch < 0.5 * peak_one
+    assert held >= peak_ch
+    pac.memory_trim()
+    u, _, h = pac.memory_stats()
+    assert h == u  # nothing cached after a trim
+
+
 def test_largedata_cli_equals_one_pass(golden_dir, tmp_path):
     crd = os.path.join(golden_dir, "1k4c.crd")
     n = int(open(crd).read().split()[0])
