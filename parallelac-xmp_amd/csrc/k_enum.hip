@@ -254,20 +254,30 @@ void launch_gather(const Ball* raw, const uint32_t* sidx,
                      perm, cellk);
 }
 
-/* cstart[c] = first sorted ball with key >= c, for c in [0, G] */
+/* cstart[c] = first sorted ball with key >= c, for c in [0, G]: one
+ * thread per cell, binary search over the sorted keys (neighbouring cells
+ * share the first probes, so the search runs from cache; a grid that is
+ * mostly empty -- a chunk's sub-input on the whole input's grid -- costs no
+ * serial fill of its empty runs) */
 __global__ __launch_bounds__(kBlock) void k_cell_starts(const uint32_t* cellk,
                                                         int64_t n, int64_t G,
                                                         uint32_t* cstart) {
-  int64_t s = gtid();
-  if (s > n) return;
-  int64_t lo = (s == 0) ? 0 : (int64_t)cellk[s - 1] + 1;
-  int64_t hi = (s == n) ? G : (int64_t)cellk[s];
-  for (int64_t c = lo; c <= hi; ++c) cstart[c] = (uint32_t)s;
+  const int64_t c = gtid();
+  if (c > G) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)cellk[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  cstart[c] = (uint32_t)lo;
 }
 void launch_cell_starts(const uint32_t* cellk, int64_t n, int64_t G,
                         uint32_t* cstart, PipeLaunch L) {
   hipLaunchKernelGGL(k_cell_starts,
-                     dim3((unsigned)((n + 1 + kBlock - 1) / kBlock)),
+                     dim3((unsigned)((G + 1 + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, cellk, n, G, cstart);
 }
 
