@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_need(Stage S, Cnt c_nk,
  * and quick_reject; the tets it does not reject (or whose pair constants
  * are not certified) are listed in `next` for the certified scan.  A
  * kernel of its own: the certified scan's registers stay off this path. */
-template <class R>
+template <class R, bool MOL>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
     Stage S, Cnt c_nk, const uint4* tets, uint8_t* flag, uint32_t* next,
     unsigned* nnext) {
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
   if (id < nk) {
     const uint4 t = tets[id];
     const Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
-    const GridP g = sgrid_of(S, t.x);
+    const GridP g = sgrid_at<MOL>(S, t.x);
     PairSS<R> P;
     TetCert<R> T;
     if (!tet_needed(S, t) || (tet_pairs(v, P, T) && quick_reject(S, g, P, pairs)))
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
  * (balls in fine-cell order, fid their ids).  A vertex of the simplex lies
  * on its orthosphere, so its certified test is always undecided: the
  * fine scan looks a ball's id up only then, to skip the vertices. */
-template <class R, bool PHASE_B, bool FINE>
+template <class R, bool PHASE_B, bool FINE, bool MOL>
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     Stage S, Cnt c_nk, const uint32_t* items, const unsigned* nitems,
     const uint4* tets, uint8_t* flag, Queue q1, uint32_t* rest,
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     int64_t id = PHASE_B ? (int64_t)rest[w] : (items ? (int64_t)items[w] : w);
     uint4 t = tets[id];
     Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
-    const GridP g = FINE ? sgrid_of(S, t.x) : grid_of(S, t.x);
+    const GridP g = FINE ? sgrid_at<MOL>(S, t.x) : grid_at<MOL>(S, t.x);
     PairSS<R> P;
     typename std::conditional<FINE, SphBox, CellBox>::type box;
     if (!tet_setup(g, v, P, box)) {
@@ -326,7 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       }
       continue;
     }
-    bool pending = false, dead = false, unfinished = false;
+    /* why the scan stopped early: 1 a violator, 2 the budget (one
+     * variable: two flags set in different exits were kept in scratch
+     * memory, their stores merged through a select of addresses) */
+    bool pending = false;
+    int stop = 0;
     int seen = 0;
     auto visit = [&](uint32_t j, const Ball& E) {
       if (!FINE && (j == t.x || j == t.y || j == t.z || j == t.w)) return false;
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
           return false;
         }
       } else if (budget > 0 && seen == budget) {
-        unfinished = true;
+        stop = 2;
         return true;
       } else {
         ++seen;
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         return false;
       }
       if (s * P.sden < 0) { /* pi_e < rho: violator */
-        dead = true;
+        stop = 1;
         return true;
       }
       return false;
@@ -362,13 +366,13 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     else
       scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {
-      if (dead)
+      if (stop == 1)
         flag[id] = 0;
       else if (pending)
         flag[id] = 2;
     } else {
-      flag[id] = dead ? 0 : (pending ? 2 : 1);
-      if (unfinished && !dead) rest_id = (uint32_t)id;
+      flag[id] = stop == 1 ? 0 : (pending ? 2 : 1);
+      if (stop == 2) rest_id = (uint32_t)id;
     }
   }
   if (!PHASE_B) {
@@ -635,7 +639,7 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
  * Simplices already in the closure (in_flag) pass without a test.  Same
  * phase A / phase B split as the tet kernel. */
-template <class R, int KIND, bool PHASE_B, bool FINE>
+template <class R, int KIND, bool PHASE_B, bool FINE, bool MOL>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     Stage S, Cnt c_nitems, const uint32_t* ecol, const uint32_t* eleft,
     const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
@@ -662,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       vid[2] = vid[1];
     }
     Ball v[3] = {S.ball[vid[0]], S.ball[vid[1]], S.ball[vid[2]]};
-    const GridP g = FINE ? sgrid_of(S, vid[0]) : grid_of(S, vid[0]);
+    const GridP g = FINE ? sgrid_at<MOL>(S, vid[0]) : grid_at<MOL>(S, vid[0]);
     PairSS<R> P;
     typename std::conditional<FINE, SphBox, CellBox>::type box;
     if (!low_setup<R, KIND>(g, v, P, box)) {
@@ -672,7 +676,8 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       }
       continue;
     }
-    bool pending = false, dead = false, unfinished = false;
+    bool pending = false;
+    int stop = 0; /* 1 a violator, 2 the budget (as in k_tet_ac2_hot) */
     int seen = 0;
     auto visit = [&](uint32_t j, const Ball& E) {
       if (!FINE && (j == vid[0] || j == vid[1] || j == vid[2])) return false;
@@ -682,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
           return false;
         }
       } else if (budget > 0 && seen == budget) {
-        unfinished = true;
+        stop = 2;
         return true;
       } else {
         ++seen;
@@ -691,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       ++evals;
       int s = pairss_sign(P, E);
       if (s < 0) { /* power <= rho strictly: violator */
-        dead = true;
+        stop = 1;
         return true;
       }
       if (s == 0) {
@@ -707,13 +712,13 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     else
       scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {
-      if (dead)
+      if (stop == 1)
         flag[id] = 0;
       else if (pending)
         flag[id] = 2;
     } else {
-      flag[id] = dead ? 0 : (pending ? 2 : 1);
-      if (unfinished && !dead) rest_id = (uint32_t)id;
+      flag[id] = stop == 1 ? 0 : (pending ? 2 : 1);
+      if (stop == 2) rest_id = (uint32_t)id;
     }
   }
   if (!PHASE_B) {
@@ -901,16 +906,16 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
 
 
 /* ---------------- launchers ---------------- */
-template <class R, bool FINE>
-static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
+template <class R, bool FINE, bool MOL>
+static void tet_hot3(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
                      Queue q1, uint32_t* rest, unsigned* nrest, uint32_t* next,
                      unsigned* nnext, int budget, PipeLaunch L) {
   dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   const uint32_t* items = nullptr;
   if (FINE && S.quick) { /* quick pass, then the certified scan of the rest */
     zero_u32(nnext, 1, L.s);
-    hipLaunchKernelGGL((k_tet_ac2_quick<R>), gr, dim3(kBlock), 0, L.s, S, nk,
-                       tets, flag, next, nnext);
+    hipLaunchKernelGGL((k_tet_ac2_quick<R, MOL>), gr, dim3(kBlock), 0, L.s, S,
+                       nk, tets, flag, next, nnext);
     items = next;
   } else if (S.chunk) {
     zero_u32(nnext, 1, L.s);
@@ -918,11 +923,24 @@ static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
                        next, nnext);
     items = next;
   }
-  hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE>), gr, dim3(kBlock), 0, L.s,
-                     S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
+  hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE, MOL>), gr, dim3(kBlock), 0,
+                     L.s, S, nk, items, nnext, tets, flag, q1, rest, nrest,
+                     budget);
   if (budget > 0)
-    hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE>), gb, dim3(kBlock), 0, L.s,
-                       S, nk, items, nnext, tets, flag, q1, rest, nrest, budget);
+    hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE, MOL>), gb, dim3(kBlock), 0,
+                       L.s, S, nk, items, nnext, tets, flag, q1, rest, nrest,
+                       budget);
+}
+template <class R, bool FINE>
+static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
+                     Queue q1, uint32_t* rest, unsigned* nrest, uint32_t* next,
+                     unsigned* nnext, int budget, PipeLaunch L) {
+  if (S.mol)
+    tet_hot3<R, FINE, true>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                            budget, L);
+  else
+    tet_hot3<R, FINE, false>(S, nk, tets, flag, q1, rest, nrest, next, nnext,
+                             budget, L);
 }
 __global__ __launch_bounds__(kBlock) void k_own_mark(
     Stage S, int kind, Cnt c_n, const uint32_t* ecol, const uint32_t* eleft,
@@ -983,20 +1001,33 @@ void launch_tet_ac2_exact(const Stage& S, QRef q,
   hipLaunchKernelGGL(k_tet_ac2_exact, qgrid(q, kExactBlocks),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
+template <class R, int KIND, bool FINE, bool MOL>
+static void low_hot4(const Stage& S, Cnt nitems, const uint32_t* ecol,
+                     const uint32_t* eleft, const uint32_t* tk,
+                     const uint32_t* tedge, const uint8_t* in_flag,
+                     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
+                     int budget, PipeLaunch L) {
+  dim3 gr((unsigned)((nitems.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
+  hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false, FINE, MOL>), gr,
+                     dim3(kBlock), 0, L.s, S, nitems, ecol, eleft, tk, tedge,
+                     in_flag, flag, q1, rest, nrest, budget);
+  if (budget > 0)
+    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true, FINE, MOL>), gb,
+                       dim3(kBlock), 0, L.s, S, nitems, ecol, eleft, tk, tedge,
+                       in_flag, flag, q1, rest, nrest, budget);
+}
 template <class R, int KIND, bool FINE>
 static void low_hot3(const Stage& S, Cnt nitems, const uint32_t* ecol,
                      const uint32_t* eleft, const uint32_t* tk,
                      const uint32_t* tedge, const uint8_t* in_flag,
                      uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest,
                      int budget, PipeLaunch L) {
-  dim3 gr((unsigned)((nitems.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
-  hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false, FINE>), gr, dim3(kBlock), 0,
-                     L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag, q1,
-                     rest, nrest, budget);
-  if (budget > 0)
-    hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true, FINE>), gb, dim3(kBlock), 0,
-                       L.s, S, nitems, ecol, eleft, tk, tedge, in_flag, flag,
-                       q1, rest, nrest, budget);
+  if (S.mol)
+    low_hot4<R, KIND, FINE, true>(S, nitems, ecol, eleft, tk, tedge, in_flag,
+                                  flag, q1, rest, nrest, budget, L);
+  else
+    low_hot4<R, KIND, FINE, false>(S, nitems, ecol, eleft, tk, tedge, in_flag,
+                                   flag, q1, rest, nrest, budget, L);
 }
 template <class R, int KIND>
 static void low_hot2(const Stage& S, Cnt nitems, const uint32_t* ecol,

@@ -539,6 +539,21 @@ __device__ __forceinline__ bool place_box(const GridP& g, double reach,
 __device__ __forceinline__ GridP sgrid_of(const Stage& S, uint32_t b) {
   return S.sgrids ? S.sgrids[S.mol[b]] : S.sg;
 }
+/* The same with the choice made at compile time: MOL (a batch) reads the
+ * molecule's grid from its table; otherwise the single grid is read from
+ * the kernel's arguments (scalar loads, once per wave).  The run-time
+ * choice above becomes one generic (flat) pointer whose 88-byte grid is
+ * reloaded per item through the vector memory path. */
+template <bool MOL>
+__device__ __forceinline__ GridP grid_at(const Stage& S, uint32_t b) {
+  if constexpr (MOL) return S.grids[S.mol[b]];
+  else return S.g;
+}
+template <bool MOL>
+__device__ __forceinline__ GridP sgrid_at(const Stage& S, uint32_t b) {
+  if constexpr (MOL) return S.sgrids[S.mol[b]];
+  else return S.sg;
+}
 
 /* Centre c = p0 + num / (2 den) and per-axis error bounds from certified
  * num, den, with two divisions.  With D = |den|, eD its bound (D > eD):
