@@ -67,7 +67,8 @@ typedef struct pac_options {
                             triangle, whether it enters the filtration at
                             its own rho (pac_result_copy_birth /
                             pac_result_copy_filtration).  Reference row
-                            order; pac_compute / pac_compute_f64 only.    */
+                            order, with the tet emptiness test;
+                            pac_compute / pac_compute_f64 only.           */
 } pac_options;
 
 typedef struct pac_result pac_result;

@@ -165,6 +165,8 @@ static void compute_one(const F* xyzr, int64_t n, F solvent, F alpha,
   if (opt.chunk_balls < 0) fail(PAC_ERR_INVALID, "chunk_balls < 0");
   if (opt.filtration && opt.order != PAC_ORDER_REFERENCE)
     fail(PAC_ERR_INVALID, "filtration keeps the reference row order");
+  if (opt.filtration && opt.skip_tet_ac2)
+    fail(PAC_ERR_INVALID, "filtration needs the tet emptiness test");
   Engine* E = engine_for(opt.device);
   std::lock_guard<std::mutex> lk(E->run_mu);
   HIPCHK(hipSetDevice(E->device));

@@ -179,6 +179,9 @@ def test_filtration_argument_errors():
     with pytest.raises(pac.PacError) as e:
         pac.compute_batch(b, np.array([0, 150, 300]), 1.4, 0.0, filtration=True)
     assert e.value.code == pac.PAC_ERR_INVALID
+    with pytest.raises(pac.PacError) as e:
+        pac.compute(b, 1.4, 0.0, filtration=True, skip_tet_ac2=True)
+    assert e.value.code == pac.PAC_ERR_INVALID
     r0 = pac.compute(np.zeros((0, 4), np.float32), 1.4, 0.0, filtration=True)
     d, i, v = r0.filtration()
     assert len(d) == 0
