@@ -61,6 +61,24 @@ def test_1grm_full(crd_1grm, prec, policy):
     assert [len(g[d]) for d in (1, 2, 3)] == [812, 650, 136]
 
 
+@pytest.fixture(scope="module")
+def crd_1k4c(golden_dir):
+    return pac.read_crd(os.path.join(golden_dir, "1k4c.crd"))
+
+
+@pytest.mark.parametrize("s,a", [(1.4, 0.0), (0.0, 1.0)])
+def test_1k4c_full(crd_1k4c, s, a):
+    """The reference's own medium protein (data/1k4c.crd, 16,068 atoms with
+    the four CRD radii), full pipeline, both precisions and cell policies."""
+    _say("case: 1k4c full s=%g a=%g" % (s, a))
+    for policy in (pac.CELL_EXACT, pac.CELL_REFERENCE_RAW):
+        o, ost, _ = oracle.run(crd_1k4c, s, a, cell_policy=policy)
+        for prec in (pac.PREC_PLAIN_FP64, pac.PREC_STAGED):
+            g, st = _gpu_lists(crd_1k4c, s, a, precision=prec, cell_policy=policy)
+            _assert_same(g, o, "1k4c s=%g a=%g policy=%d prec=%d" % (s, a, policy, prec))
+            assert st.edge_candidates == ost["E_cand"]
+
+
 @pytest.mark.parametrize("prec", [pac.PREC_PLAIN_FP64, pac.PREC_STAGED])
 def test_c1_1000(prec):
     _say("case: C1 1000 balls prec=%d" % prec)
