@@ -567,7 +567,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(
     uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr,
     uint64_t lim) {
   const int64_t ne = c_ne.get();
-  constexpr int CAP = 1360; /* 7 blocks per CU fit in LDS with block_flush's 16 B */
+  /* 5 blocks per CU fit in LDS with block_flush's 16 B; a block of 256
+   * alpha edges averages ~2000 candidates, which a smaller stage would
+   * send to global memory */
+  constexpr int CAP = 2040;
   __shared__ uint32_t s_out[4 * CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= ne) return; /* whole block past the (device) count */
@@ -686,7 +689,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
     uint4* ctet, uint8_t* flag, Queue q1, uint64_t lim) {
   const int64_t nt = c_nt.get();
-  constexpr int CAP = 1360; /* 7 blocks per CU fit in LDS with block_flush's 16 B */
+  /* 7 blocks per CU fit in LDS with block_flush's 16 B; a block of 256
+   * alpha triangles averages ~600 slots */
+  constexpr int CAP = 1360;
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= nt) return; /* whole block past the (device) count */
