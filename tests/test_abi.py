@@ -90,3 +90,14 @@ def test_graph_argument_errors():
     with pytest.raises(pac.PacError) as e:
         pac.Graph(xyzr, np.array([0, 3, 2, 4]), 1.4, 0.0)
     assert e.value.code == pac.PAC_ERR_INVALID
+
+
+def test_memory_stats_without_a_device():
+    """A device the library never used (no engine): the pool reads zero
+    and trim is a no-op."""
+    dev = 7
+    assert pac.memory_stats(dev) == (0, 0, 0)
+    assert pac.memory_stats(dev, reset_peak=True) == (0, 0, 0)
+    pac.memory_trim(dev)
+    u, p, h = pac.memory_stats(0)
+    assert 0 <= u <= p and u <= h
