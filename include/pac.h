@@ -235,7 +235,7 @@ int pac_read_crd(const char* path, float** xyzr, int64_t* n);
 int pac_write_output(const char* path, const float* xyzr_raw, int64_t n,
                      float solvent, const pac_result* r);
 void pac_free(void* p);
-/* CRD file in the layout of the reference's data files (data/*.crd):
+/* CRD file in the layout of the reference's data files (the .crd files of data/):
  * the count, a blank line, "x \ty \tz \tr" rows with 4 decimals, "#TER".
  * No reference counterpart (the reference only reads CRD). */
 int pac_write_crd(const char* path, const float* xyzr, int64_t n);

@@ -6,7 +6,7 @@
  *                     separators and a trailing "#TER" line are accepted;
  *                     a short file is an error instead of garbage)
  *   pac_write_output  AlphaComplexParallel.cu:1834-1856 (byte-compatible)
- *   pac_write_crd     the layout of the reference's data files (data/*.crd)
+ *   pac_write_crd     the layout of the reference's data files (the .crd files of data/)
  *   pac_read_pdb      PDB -> balls; no reference counterpart (the
  *                     reference's CRD files were made from PDB entries,
  *                     README.md:44-46, by a tool it does not ship)
