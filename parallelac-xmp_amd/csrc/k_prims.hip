@@ -196,9 +196,95 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, Cnt n, void* tmp,
                      size_t, hipStream_t s, uint64_t* total64) {
   scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s, total64);
 }
+/* Flag scans (u8 flags, nonzero = 1 -> u32 positions), the scan every
+ * compaction runs: a thread owns kScanItems (8) consecutive flags, read as
+ * one 8-byte word, and writes their 8 positions as two 16-byte stores;
+ * 32-bit arithmetic, no LDS transpose.  Same tiles, tile sums and totals
+ * as the generic scan (flag arrays and positions are pool buffers, aligned
+ * far beyond 16 bytes, and a tile starts at a multiple of kTile items). */
+static_assert(kScanItems == 8, "a thread reads its flags as one 8-byte word");
+__device__ __forceinline__ unsigned nz_bytes(uint32_t x) {
+  /* high bit of each byte set iff the byte is nonzero */
+  const uint32_t y = ((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x;
+  return (unsigned)__popc(y & 0x80808080u);
+}
+__device__ __forceinline__ uint2 load_flags8(const uint8_t* in, int64_t i0,
+                                             int64_t n) {
+  if (i0 + 8 <= n) return *reinterpret_cast<const uint2*>(in + i0);
+  uint32_t w[2] = {0u, 0u};
+  for (int q = 0; q < 8; ++q)
+    if (i0 + q < n) w[q >> 2] |= (uint32_t)in[i0 + q] << (8 * (q & 3));
+  return make_uint2(w[0], w[1]);
+}
+__global__ __launch_bounds__(kBlock) void k_flag_sums(const uint8_t* in,
+                                                      Cnt cn, uint64_t* sums) {
+  const int64_t n = cn.get();
+  const int64_t i0 = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * 8;
+  const uint2 f = load_flags8(in, i0, n);
+  uint64_t tot;
+  block_excl_scan_u64(nz_bytes(f.x) + nz_bytes(f.y), &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(kBlock) void k_flag_scan(const uint8_t* in, Cnt cn,
+                                                      const uint64_t* tile_off,
+                                                      uint32_t* out,
+                                                      uint64_t* total64) {
+  const int64_t n = cn.get();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t i0 = base + (int64_t)threadIdx.x * 8;
+  const uint2 f = load_flags8(in, i0, n);
+  uint64_t tot;
+  const uint64_t ex = block_excl_scan_u64(nz_bytes(f.x) + nz_bytes(f.y), &tot) +
+                      (tile_off ? tile_off[blockIdx.x] : 0);
+  uint32_t p[8];
+  uint32_t e = (uint32_t)ex;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    p[q] = e;
+    e += ((q < 4 ? f.x : f.y) >> (8 * (q & 3)) & 0xffu) != 0;
+  }
+  if (i0 + 8 <= n) {
+    uint4* o = reinterpret_cast<uint4*>(out + i0);
+    o[0] = make_uint4(p[0], p[1], p[2], p[3]);
+    o[1] = make_uint4(p[4], p[5], p[6], p[7]);
+  } else {
+    for (int q = 0; q < 8; ++q)
+      if (i0 + q < n) out[i0 + q] = p[q];
+  }
+  /* the tile holding position n (the last tile when n ends a tile) writes
+   * the grand total at out[n] */
+  const bool holds = (n >= base && n < base + kTile) ||
+                     (blockIdx.x == gridDim.x - 1 && n == base + kTile);
+  if (holds && threadIdx.x == 0) {
+    const uint64_t t = tot + (tile_off ? tile_off[blockIdx.x] : 0);
+    out[n] = (uint32_t)t;
+    if (total64) *total64 = t;
+  }
+}
 void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, Cnt n, void* tmp,
                         size_t, hipStream_t s, uint64_t* total64) {
-  scan_impl<uint8_t, true, uint32_t>(in, out, n, (uint64_t*)tmp, s, total64);
+  if (n.cap <= 0) {
+    scan_impl<uint8_t, true, uint32_t>(in, out, n, (uint64_t*)tmp, s, total64);
+    return;
+  }
+  const int64_t tiles = (n.cap + kTile - 1) / kTile;
+  uint64_t* t = (uint64_t*)tmp;
+  if (tiles == 1) {
+    hipLaunchKernelGGL(k_flag_scan, dim3(1), dim3(kBlock), 0, s, in, n,
+                       (const uint64_t*)nullptr, out, total64);
+    return;
+  }
+  uint64_t* sums = t;
+  uint64_t* offs = t + (tiles + 1);
+  hipLaunchKernelGGL(k_flag_sums, dim3((unsigned)tiles), dim3(kBlock), 0, s, in,
+                     n, sums);
+  if (tiles <= kSumsMax)
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kSumsBlock), 0, s,
+                       (const uint64_t*)sums, tiles, offs);
+  else
+    scan_u64_rec(sums, offs, tiles, t + 2 * (tiles + 1), s);
+  hipLaunchKernelGGL(k_flag_scan, dim3((unsigned)tiles), dim3(kBlock), 0, s, in,
+                     n, (const uint64_t*)offs, out, total64);
 }
 
 /* ------------------------------------------------------------------ */
