@@ -134,22 +134,39 @@ __global__ __launch_bounds__(kSumsBlock) void k_scan_sums(const uint64_t* in,
                                                           int64_t m,
                                                           uint64_t* out) {
   __shared__ uint64_t wsum[kSumsBlock / 64];
-  const int per = (int)((m + kSumsBlock - 1) / kSumsBlock);
+  const int per = (int)((m + kSumsBlock - 1) / kSumsBlock); /* <= 64 */
   const int64_t b = (int64_t)threadIdx.x * per;
+  /* the run in batches of 16 values held in registers: a batch's loads
+   * are all issued before any is used (one dependent round trip per
+   * batch, not per value) */
+  constexpr int kB = 16;
   uint64_t run = 0;
-  for (int k = 0; k < per; ++k)
-    if (b + k < m) run += in[b + k];
+  for (int k0 = 0; k0 < per; k0 += kB) {
+    uint64_t v[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k)
+      v[k] = (k0 + k < per && b + k0 + k < m) ? in[b + k0 + k] : 0;
+#pragma unroll
+    for (int k = 0; k < kB; ++k) run += v[k];
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t inc = wave_incl_scan_u64(run);
   if (lane == 63) wsum[w] = inc;
   __syncthreads();
   uint64_t ex = inc - run;
   for (int q = 0; q < w; ++q) ex += wsum[q];
-  for (int k = 0; k < per; ++k)
-    if (b + k < m) {
-      out[b + k] = ex;
-      ex += in[b + k];
-    }
+  for (int k0 = 0; k0 < per; k0 += kB) {
+    uint64_t v[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k)
+      v[k] = (k0 + k < per && b + k0 + k < m) ? in[b + k0 + k] : 0;
+#pragma unroll
+    for (int k = 0; k < kB; ++k)
+      if (k0 + k < per && b + k0 + k < m) {
+        out[b + k0 + k] = ex;
+        ex += v[k];
+      }
+  }
   if (threadIdx.x == kSumsBlock - 1) out[m] = ex;
 }
 
