@@ -445,15 +445,23 @@ void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
 /* ------------------------------------------------------------------ */
 /* enumeration: candidates as (segment, new vertex) pairs              */
 /* ------------------------------------------------------------------ */
+/* One wave per ball: the wave walks the ball's stencil rows together (the
+ * ball, its grid and each row's bounds are the same for all lanes) and
+ * consecutive lanes write a row's candidates to consecutive addresses.  (A
+ * thread per ball wrote its ~130 candidates one after another, every lane
+ * of a store in a cache line of its own.) */
+constexpr int kEdgeBallsPerBlock = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
                                                       const uint64_t* off,
                                                       uint32_t* cval,
                                                       uint32_t* cseg,
                                                       uint64_t lim) {
-  int64_t i = gtid();
-  if (i >= S.n) return;
-  const GridP g = grid_of(S, (uint32_t)i);
-  uint32_t c = S.cellk[i];
+  const int64_t iw = (int64_t)blockIdx.x * kEdgeBallsPerBlock + (threadIdx.x >> 6);
+  if (iw >= S.n) return; /* a whole wave */
+  const uint32_t i = __builtin_amdgcn_readfirstlane((uint32_t)iw);
+  const uint32_t lane = threadIdx.x & 63;
+  const GridP g = grid_of(S, i);
+  const uint32_t c = S.cellk[i];
   int gx, gy, gz;
   cell_xyz(c - g.cbase, g, gx, gy, gz);
   const int k = g.k_enum;
@@ -465,18 +473,24 @@ __global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
       int64_t c0 = row + x0, c1 = row + x1;
       if (c1 < (int64_t)c) continue;
       uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
-      if (b < (uint32_t)i + 1) b = (uint32_t)i + 1;
-      for (uint32_t j = b; j < e && o < lim; ++j, ++o) {
-        cval[o] = j;
-        cseg[o] = (uint32_t)i;
+      if (b < i + 1) b = i + 1;
+      for (uint32_t j = b + lane; j < e; j += 64) {
+        const uint64_t w = o + (j - b);
+        if (w < lim) {
+          cval[w] = j;
+          cseg[w] = i;
+        }
       }
+      if (e > b) o += e - b;
     }
 }
 void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
                       uint32_t* cseg, PipeLaunch L, uint64_t lim) {
   if (S.n <= 0) return;
-  hipLaunchKernelGGL(k_edge_enum, dim3((unsigned)((S.n + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, S, off, cval, cseg, lim);
+  hipLaunchKernelGGL(
+      k_edge_enum,
+      dim3((unsigned)((S.n + kEdgeBallsPerBlock - 1) / kEdgeBallsPerBlock)),
+      dim3(kBlock), 0, L.s, S, off, cval, cseg, lim);
 }
 
 /* Slot capacities that need no merge: a triangle from alpha edge (i,j) has
