@@ -147,6 +147,11 @@ def main():
     flops = WORK[dom] * units[dom] + (TET_SETUP_FLOPS * s_st.tet_alpha
                                       if dom == "tet_ac2" else 0)
     achieved = flops / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
+    # the same model for every filter kernel (vector FP32/FP64 rate)
+    rates = {}
+    for k in WORK:
+        fk = WORK[k] * units[k] + (TET_SETUP_FLOPS * s_st.tet_alpha if k == "tet_ac2" else 0)
+        rates[k] = round(fk / (kms[k] * 1e-3) / 1e12, 3) if kms[k] > 0 else None
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_globule100k.json")
     if os.path.exists(tpath) and dom in ("tet_ac2",):
@@ -250,6 +255,7 @@ def main():
         "phase_ms": {pac.PHASE_LABELS[i]: round(s_st.phase_ms[i], 3) for i in range(8)},
         "kernel_ms_staged": {k: round(v, 3) for k, v in kms.items()},
         "kernel_ms_plain": {pac.KERNELS[i]: round(p_pl.kernel_ms[i], 3) for i in range(8)},
+        "kernel_tflops_staged": rates,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
