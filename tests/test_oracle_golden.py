@@ -46,8 +46,9 @@ def test_1grm_exact_complex(golden_dir, policy):
 
 
 def test_rows_sorted_in_reference_order(golden_dir):
-    """Rows are lexicographic in cell-sorted index space (F7 of the survey:
-    AlphaComplexParallel.cu:1157 sort, :1842-1856 id mapping)."""
+    """Rows are lexicographic in cell-sorted index space and written as
+    original ids (AlphaComplexParallel.cu:1157 sort, :1842-1856 id
+    mapping)."""
     b = _crd(golden_dir, "1grm.crd")
     out, _, _ = oracle.run(b, 0.0, 1.0)
     import ctypes
