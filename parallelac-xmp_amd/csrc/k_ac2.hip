@@ -136,49 +136,15 @@ __device__ __forceinline__ int tet_pair_code(const TetQ<T>& t, int sD,
 }
 
 
-/* per-simplex setup at precision R: D, N with a-priori error bounds, the
- * certified centre box and the semi-static pair constants. */
-/* FP32/FP64 orthocentre numerators and denominator of a tet with their
- * certified errors (semi-static), and the pair-test constants */
-template <class R>
-struct TetCert {
-  FE<R> D;
-  V3<FE<R>> N;
-  FE<R> wa;
-};
+/* per-simplex setup at precision R: D, N with a-priori error bounds
+ * (tet_dn_ss), the certified centre box and the semi-static pair
+ * constants */
 template <class R>
 __device__ __forceinline__ bool tet_pairs(const Ball v[4], PairSS<R>& P,
                                           TetCert<R>& T) {
-  const R ax = v[0].x, ay = v[0].y, az = v[0].z;
-  R ux = (R)v[1].x - ax, uy = (R)v[1].y - ay, uz = (R)v[1].z - az;
-  R vx = (R)v[2].x - ax, vy = (R)v[2].y - ay, vz = (R)v[2].z - az;
-  R tx = (R)v[3].x - ax, ty = (R)v[3].y - ay, tz = (R)v[3].z - az;
-  R wa = (R)v[0].w * (R)v[0].w;
-  R w1 = (R)v[1].w * (R)v[1].w, w2 = (R)v[2].w * (R)v[2].w,
-    w3 = (R)v[3].w * (R)v[3].w;
-  SS<R> vt[3], tu[3], uv[3];
-  cross_ss(vx, vy, vz, tx, ty, tz, vt);
-  cross_ss(tx, ty, tz, ux, uy, uz, tu);
-  cross_ss(ux, uy, uz, vx, vy, vz, uv);
-  SS<R> D{ux * vt[0].v + uy * vt[1].v + uz * vt[2].v,
-          fabs(ux) * vt[0].p + fabs(uy) * vt[1].p + fabs(uz) * vt[2].p};
-  R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz,
-    tt = tx * tx + ty * ty + tz * tz;
-  SS<R> A{(uu + wa) - w1, (uu + wa) + w1}, B{(vv + wa) - w2, (vv + wa) + w2},
-      C{(tt + wa) - w3, (tt + wa) + w3};
-  SS<R> N[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-    N[q] = SS<R>{A.v * vt[q].v + B.v * tu[q].v + C.v * uv[q].v,
-                 fabs(A.v) * vt[q].p + fabs(B.v) * tu[q].p + fabs(C.v) * uv[q].p};
-  /* A, B, C permanents exceed |A| etc.; use them for the N bound */
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-    N[q].p = A.p * vt[q].p + B.p * tu[q].p + C.p * uv[q].p;
-  T.D = ss_fe(D, 6);
-  T.N = V3<FE<R>>{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
-  V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
-  T.wa = FE<R>{wa, FEc<R>::u() * wa * FEc<R>::slack()};
+  tet_dn_ss(v, T);
+  V3<FE<R>> pa{FE<R>{(R)v[0].x, (R)0}, FE<R>{(R)v[0].y, (R)0},
+               FE<R>{(R)v[0].z, (R)0}};
   return pairss_init(P, pa, T.wa, T.D, T.N);
 }
 /* the pair constants and the certified box that holds every violator */

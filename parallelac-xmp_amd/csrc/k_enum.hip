@@ -387,6 +387,16 @@ __device__ int tet_code(const Ball& a, const Ball& b, const Ball& c,
   if (fe_sign(D) == 0) return UNDEC;
   return code_of(fe_sign(tet_val(D, N, wa, Al<T>::c(alpha))));
 }
+/* the first stage of the tet filter: D and N with a-priori error bounds
+ * (tet_dn_ss: one extra operation per operation, on absolute values),
+ * the final 4 D^2 (w_a + alpha) - |N|^2 with running bounds */
+template <class R>
+__device__ __forceinline__ int tet_code_ss(const Ball* v, Real alpha) {
+  TetCert<R> T;
+  tet_dn_ss(v, T);
+  if (fe_sign(T.D) == 0) return UNDEC;
+  return code_of(fe_sign(tet_val(T.D, T.N, T.wa, Al<FE<R>>::c(alpha))));
+}
 __device__ int tet_exact(const Ball& a, const Ball& b, const Ball& c,
                          const Ball& d, const ExactScale& X) {
   LdWI L{X.S};
@@ -749,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     int r = REJECT;
     if (v.x != 0xffffffffu) {
       const Ball bv[4] = {S.ball[v.x], S.ball[v.y], S.ball[v.z], S.ball[v.w]};
-      r = filter_code<3, FE<R>>(bv, S.alpha);
+      r = tet_code_ss<R>(bv, S.alpha);
     }
     flag[base + t] = (uint8_t)r;
     if (r != REJECT && staged) ctet[base + t] = v;
