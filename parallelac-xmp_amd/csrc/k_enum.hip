@@ -713,9 +713,10 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
     uint4* ctet, uint8_t* flag, Queue q1, uint64_t lim) {
   const int64_t nt = c_nt.get();
-  /* 7 blocks per CU fit in LDS with block_flush's 16 B; a block of 256
-   * alpha triangles averages ~600 slots */
-  constexpr int CAP = 1360;
+  /* 8 blocks per CU fit in LDS with block_flush's 16 B (the kernel's 60
+   * VGPRs allow 8 waves per SIMD); a block of 256 alpha triangles averages
+   * ~600 slots */
+  constexpr int CAP = 1272;
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= nt) return; /* whole block past the (device) count */
