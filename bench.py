@@ -25,16 +25,70 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parallelac-xmp_amd"))
 
 METRIC = "classified simplices/s at 100k balls, staged vs plain FP64 filter, bit-exact"
-PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
-# algorithmic flops per unit (candidate, or (simplex, ball) pair), counted
-# from the kernels' source: the certified pair test of the emptiness scans
-# is 28 flops (gap, its bound, the comparison's operands); a tet's scan
-# setup adds 290 per alpha tet (orthocentre numerators / denominator with
-# their permanents and the pair constants, 163 in FP32; certified centre,
-# reach, box and sphere, 126 in FP64)
-WORK = {"edge_fill": 25, "tri_fill": 150, "tet_fill": 130, "tet_ac2": 28,
-        "tri_ac2": 28, "edge_ac2": 28}
-TET_SETUP_FLOPS = 290
+# algorithmic work per unit (flops counted from the kernels' source) and
+# MI355X peaks: bench/work_model.json
+WORK_MODEL = json.load(open(os.path.join(ROOT, "bench", "work_model.json")))
+PEAK_TFLOPS = {"f32": WORK_MODEL["peaks"]["valu_f32_TFLOPs"],
+               "f64": WORK_MODEL["peaks"]["valu_f64_TFLOPs"]}
+KERNEL_KEYS = ("edge_fill", "tri_fill", "tet_fill", "tet_ac2", "tri_ac2", "edge_ac2")
+
+
+def kernel_flops(key, units, alpha_tets, fp64):
+    """(FP32 flops, FP64 flops) of one launch of kernel `key` over `units`
+    work units; the tet emptiness scan adds its per-alpha-tet setup.  On the
+    plain FP64 path every FP32 flop runs in FP64."""
+    f = WORK_MODEL["flops"][key]
+    f32, f64 = f["f32"] * units, f["f64"] * units
+    if key == "tet_ac2":
+        su = WORK_MODEL["flops"]["tet_ac2_setup"]
+        f32 += su["f32"] * alpha_tets
+        f64 += su["f64"] * alpha_tets
+    return (0.0, f32 + f64) if fp64 else (float(f32), float(f64))
+
+
+def valu_roofline(f32, f64, ms):
+    """achieved TFLOP/s and the peak of this FP32/FP64 mix: the time the
+    mix takes at the published peaks, t* = f32/P32 + f64/P64, gives the
+    mix's peak (f32 + f64)/t*; frac = t*/t."""
+    total = f32 + f64
+    t_star = f32 / (PEAK_TFLOPS["f32"] * 1e12) + f64 / (PEAK_TFLOPS["f64"] * 1e12)
+    achieved = total / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    peak = total / t_star / 1e12 if t_star > 0 else PEAK_TFLOPS["f32"]
+    return achieved, peak
+
+
+def cpu_info(threads):
+    """the host CPU the baseline ran on: model string, logical CPUs seen by
+    the OS and by this process, threads used and why."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "threads": threads,
+            "threads_note": "OpenMP threads in one process = OMP_NUM_THREADS if set "
+                            "(the host's CPU share per GPU), else every CPU this "
+                            "process may run on"}
+
+
+def cpu_threads(args):
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
 
 
 def log(msg):
@@ -143,15 +197,22 @@ def main():
     units = {"edge_fill": s_st.edge_candidates, "tri_fill": s_st.tri_candidates,
              "tet_fill": s_st.tet_candidates, "tet_ac2": s_st.ac2_pairs[0],
              "tri_ac2": s_st.ac2_pairs[1], "edge_ac2": s_st.ac2_pairs[2]}
-    dom = max(WORK, key=lambda k: kms[k])
-    flops = WORK[dom] * units[dom] + (TET_SETUP_FLOPS * s_st.tet_alpha
-                                      if dom == "tet_ac2" else 0)
-    achieved = flops / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
-    # the same model for every filter kernel (vector FP32/FP64 rate)
-    rates = {}
-    for k in WORK:
-        fk = WORK[k] * units[k] + (TET_SETUP_FLOPS * s_st.tet_alpha if k == "tet_ac2" else 0)
-        rates[k] = round(fk / (kms[k] * 1e-3) / 1e12, 3) if kms[k] > 0 else None
+    dom = max(KERNEL_KEYS, key=lambda k: kms[k])
+    f32, f64 = kernel_flops(dom, units[dom], s_st.tet_alpha, fp64=False)
+    achieved, peak = valu_roofline(f32, f64, kms[dom])
+    # the same model for every filter kernel (vector FP32/FP64 rate), both paths
+    kms_pl = {pac.KERNELS[i]: p_pl.kernel_ms[i] for i in range(8)}
+    units_pl = {"edge_fill": s_pl.edge_candidates, "tri_fill": s_pl.tri_candidates,
+                "tet_fill": s_pl.tet_candidates, "tet_ac2": s_pl.ac2_pairs[0],
+                "tri_ac2": s_pl.ac2_pairs[1], "edge_ac2": s_pl.ac2_pairs[2]}
+    rates, fracs = {}, {}
+    for k in KERNEL_KEYS:
+        for tag, km, un, fp64 in (("staged", kms, units, False),
+                                  ("plain", kms_pl, units_pl, True)):
+            a32, a64 = kernel_flops(k, un[k], (s_st if not fp64 else s_pl).tet_alpha, fp64)
+            ach, pk = valu_roofline(a32, a64, km[k])
+            rates.setdefault(k, {})[tag] = round(ach, 3) if km[k] > 0 else None
+            fracs.setdefault(k, {})[tag] = round(ach / pk, 4) if km[k] > 0 else None
     traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_globule100k.json")
     if os.path.exists(tpath) and dom in ("tet_ac2",):
@@ -159,15 +220,22 @@ def main():
         traffic = tj.get("bytes_per_run", {}).get(dom)
         traffic_src = "profiles/hbm_traffic_globule100k.json (rocprofv3 --pmc " \
                       "FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy)"
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f32"],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["f32"],
+    roofline = {"bound": "valu", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak if peak else None,
                 "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": dom, "kernel_ms": kms[dom],
-                "work_units": int(units[dom]), "flops_per_unit": WORK[dom],
+                "work_units": int(units[dom]),
+                "flops_per_unit": WORK_MODEL["flops"][dom],
                 "setup_units": int(s_st.tet_alpha) if dom == "tet_ac2" else 0,
-                "flops_per_setup": TET_SETUP_FLOPS if dom == "tet_ac2" else 0,
-                "note": "vector FP32 ALU (no MFMA on this path); peak = MI355X "
-                        "FP32 157.3 TFLOP/s"}
+                "flops_per_setup": WORK_MODEL["flops"]["tet_ac2_setup"]
+                if dom == "tet_ac2" else None,
+                "flops_f32": f32, "flops_f64": f64,
+                "note": "vector ALU (no MFMA: nothing on this path is a dense "
+                        "contraction); peak = the published MI355X vector peaks "
+                        "for this kernel's FP32/FP64 mix (FP32 157.3, FP64 78.6 "
+                        "TFLOP/s), t* = f32/157.3 + f64/78.6; without packed FP32 "
+                        "(not used in this build) the attainable FP32 rate is 78.6; "
+                        "work model: bench/work_model.json"}
 
     # the same run recorded as one graph (one molecule): device time of the
     # pipeline without its host read-backs; buffer capacities come from an
@@ -213,7 +281,7 @@ def main():
         log("cpu oracle leg")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         t0 = time.perf_counter()
         o, ost, _ = oracle.run(balls, solvent, alpha, nthreads=threads)
         t1 = time.perf_counter()
@@ -221,7 +289,8 @@ def main():
         cpu = {"value": ocl / (t1 - t0), "unit": "classified simplices/s",
                "cores": threads, "kind": "port",
                "sample": "full 100k-ball workload, one run of the C++ oracle "
-                         "(FP64 filter + exact integer fallback), %.2f s" % (t1 - t0)}
+                         "(FP64 filter + exact integer fallback), %.2f s" % (t1 - t0),
+               "host": cpu_info(threads)}
         oracle_exact = all(np.array_equal(L_st[d], o[d]) for d in (1, 2, 3))
 
     esc = {pac.FAMILIES[f]: {pac.STAGES[s]: int(s_st.stage_evals[f][s]) for s in range(4)}
@@ -255,7 +324,8 @@ def main():
         "phase_ms": {pac.PHASE_LABELS[i]: round(s_st.phase_ms[i], 3) for i in range(8)},
         "kernel_ms_staged": {k: round(v, 3) for k, v in kms.items()},
         "kernel_ms_plain": {pac.KERNELS[i]: round(p_pl.kernel_ms[i], 3) for i in range(8)},
-        "kernel_tflops_staged": rates,
+        "kernel_tflops": rates,
+        "kernel_roofline_frac": fracs,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -325,7 +395,7 @@ def bench_batch(args, torch, pac):
         oracle_ok = True
         for m in sample[:2]:
             o, _, _ = oracle.run(xyzr[off[m]:off[m + 1]], 1.4, 0.0,
-                                 nthreads=min(16, os.cpu_count() or 1))
+                                 nthreads=cpu_threads(args))
             got = r_st.lists(m)
             oracle_ok &= all(np.array_equal(got[d], o[d]) for d in (1, 2, 3))
     # the same batch recorded once as one HIP graph: each step is one
@@ -436,6 +506,9 @@ def bench_stress(args, torch, pac):
     t = (time.perf_counter() - t0) / args.steps
     _, peak_one, _ = pac.memory_stats(reset_peak=True)
     st = r.stats()
+    # per-kernel device times of one more run (hipEvents on the library's stream)
+    kst = pac.compute_device(ptr, n, 1.4, 0.0, profile=True,
+                             **tuned[pac.PREC_STAGED]).stats()
     log("stress: plain FP64 path")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -480,7 +553,7 @@ def bench_stress(args, torch, pac):
         log("stress: oracle on an 8000-ball sample")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         xyz = balls[:, :3].astype(np.float64)
         ctr = xyz.mean(0)
         half = 0.5 * (0.1 * n * 2.0 ** 3) ** (1.0 / 3.0)
@@ -496,7 +569,7 @@ def bench_stress(args, torch, pac):
                "unit": "classified simplices/s", "cores": threads, "kind": "port",
                "sample": "the %d balls of the stress input nearest the lattice "
                          "block's +x face (GPU lists of the same sample compared), "
-                         "%.1f s" % (k, tc)}
+                         "%.1f s" % (k, tc), "host": cpu_info(threads)}
     classified = _classified(st)
     line = {
         "metric": "classified simplices/s, 2M-ball degenerate stress, bit-exact",
@@ -516,6 +589,7 @@ def bench_stress(args, torch, pac):
                    "triangles": int(st.tris_final), "tets": int(st.tets_final)},
         "escalations": _esc(pac, st), "sos_calls": int(st.sos_calls),
         "phase_ms": {pac.PHASE_LABELS[i]: round(st.phase_ms[i], 3) for i in range(8)},
+        "kernel_ms_staged": {pac.KERNELS[i]: round(kst.kernel_ms[i], 3) for i in range(8)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
