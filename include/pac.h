@@ -112,6 +112,33 @@ int pac_compute(const float* xyzr, int64_t n, float solvent, float alpha,
 int pac_compute_f64(const double* xyzr, int64_t n, double solvent,
                     double alpha, const pac_options* opt, pac_result** out);
 
+/* Allocation-free pac_compute.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid hipMalloc).  Every intermediate buffer comes from a device
+ * scratch array of PAC_NOALLOC_SCRATCH_BYTES compiled into the library
+ * (one per GPU, held until the process ends); the call makes no hipMalloc,
+ * hipFree, pinned-memory or event allocation.  xyzr: n x (x, y, z, r_raw)
+ * FP32, device memory with opt->device_inputs, else host memory; n <=
+ * PAC_NOALLOC_MAX_BALLS, and an input whose intermediates outgrow the
+ * scratch fails with PAC_ERR_NOMEM (8192 balls of protein density fit).
+ * The lists are written to edges / triangles / tets (device or host memory
+ * of cap_* rows of 2 / 3 / 4 int32, the same rows pac_compute returns in
+ * opt->order) and counts[4] (host) receives vertices, edges, triangles,
+ * tets; if a list exceeds its capacity nothing is written to the lists,
+ * counts[] holds the sizes needed and PAC_ERR_OVERFLOW is returned.
+ * opt->chunk_balls, opt->filtration and opt->profile must be 0.  Calls on
+ * one GPU share its scratch: this library serializes them within a process
+ * and drains the stream before returning; calls that overlap on one GPU
+ * through any other route race.  Not capturable in a HIP graph (it reads
+ * counts back to size its stages). */
+#define PAC_NOALLOC_SCRATCH_BYTES (256u << 20)
+#define PAC_NOALLOC_MAX_BALLS 8192
+int pac_compute_noalloc(const float* xyzr, int64_t n, float solvent,
+                        float alpha, const pac_options* opt, int32_t* edges,
+                        int64_t cap_edges, int32_t* triangles,
+                        int64_t cap_triangles, int32_t* tets, int64_t cap_tets,
+                        int64_t counts[4]);
+
 /* counts[0..3] = vertices, edges, triangles, tetrahedra
  * (AlphaComplexParallel.cu:1809-1816). */
 int pac_result_counts(const pac_result* r, int64_t counts[4]);

@@ -133,11 +133,11 @@ static void canonicalize(Engine* E, pac_result* R, int64_t n, hipStream_t s) {
   for (int d = 1; d <= 3; ++d) {
     int64_t m = R->counts[d];
     if (m <= 0) continue;
-    DBuf<uint32_t> a(&E->pool, m), b(&E->pool, m), c(&E->pool, m),
-        e(&E->pool, m);
-    DBuf<int32_t> copy(&E->pool, m * (d + 1));
+    Pool* P = pool_of(E);
+    DBuf<uint32_t> a(P, m), b(P, m), c(P, m), e(P, m);
+    DBuf<int32_t> copy(P, m * (d + 1));
     size_t rtb = radix_tmp_bytes(m);
-    DBuf<uint8_t> rt(&E->pool, rtb);
+    DBuf<uint8_t> rt(P, rtb);
     canonical_rows(d + 1, m, R->lists[d - 1].get(), idbits, a.get(), b.get(),
                    c.get(), e.get(), rt.get(), rtb, copy.get(), s);
     KCHK();
@@ -527,6 +527,104 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
 }
 
 /* ------------------------------------------------------------------ */
+/* allocation-free entry point                                         */
+/* ------------------------------------------------------------------ */
+/* Device scratch of pac_compute_noalloc: every intermediate buffer of its
+ * runs is carved from this array (Pool arena mode).  Each GPU the library
+ * runs on holds its own copy until the process ends.  In host code the
+ * name is a host-side stand-in, not a device address: host code uses only
+ * Engine::scratch, the address hipGetSymbolAddress returned with that
+ * device current. */
+static __device__ __align__(256) unsigned char scratch[PAC_NOALLOC_SCRATCH_BYTES];
+
+/* the device's scratch address, looked up once per device (engine lock
+ * held, device current) */
+static void* scratch_of(Engine* E) {
+  if (!E->scratch) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(PAC_ERR_DEVICE, "scratch symbol has no device address");
+    E->scratch = p;
+  }
+  return E->scratch;
+}
+
+/* pac_compute over the device scratch into the caller's lists: no device
+ * or pinned allocation, no hipEvent.  The engine lock serializes calls in
+ * this process; the stream is drained before returning, also on errors,
+ * so no kernel still uses the scratch when the next call reuses it. */
+static void compute_noalloc(const float* xyzr, int64_t n, float solvent,
+                            float alpha, const pac_options* o, int32_t* lists[3],
+                            const int64_t caps[3], int64_t* counts) {
+  if (!counts) fail(PAC_ERR_INVALID, "counts is NULL");
+  if (n < 0 || (n > 0 && !xyzr)) fail(PAC_ERR_INVALID, "bad input array");
+  if (n > PAC_NOALLOC_MAX_BALLS)
+    fail(PAC_ERR_INVALID, "n = " + std::to_string(n) + " exceeds the "
+         "allocation-free entry point's maximum of " +
+         std::to_string(PAC_NOALLOC_MAX_BALLS) + " balls (use pac_compute)");
+  for (int d = 0; d < 3; ++d)
+    if (caps[d] < 0 || (caps[d] > 0 && !lists[d]))
+      fail(PAC_ERR_INVALID, "bad output list or capacity");
+  if (!std::isfinite(solvent) || !std::isfinite(alpha))
+    fail(PAC_ERR_INVALID, "non-finite solvent or alpha");
+  pac_options opt;
+  pac_options_init(&opt);
+  if (o) opt = *o;
+  if (opt.chunk_balls != 0 || opt.filtration != 0 || opt.profile != 0)
+    fail(PAC_ERR_INVALID, "chunk_balls, filtration and profile are not "
+                          "available without allocation");
+  Engine* E = engine_for(opt.device);
+  std::lock_guard<std::mutex> lk(E->run_mu);
+  HIPCHK(hipSetDevice(E->device));
+  void* base = scratch_of(E);
+  hipStream_t s = opt.stream ? (hipStream_t)opt.stream : E->stream;
+  E->arena.set_arena(base, PAC_NOALLOC_SCRATCH_BYTES);
+  E->run_pool = &E->arena;
+  struct Restore {
+    Engine* E;
+    hipStream_t s;
+    ~Restore() {
+      (void)hipStreamSynchronize(s); /* no kernel may still use the scratch */
+      E->run_pool = nullptr;
+    }
+  } restore{E, s};
+  counts[0] = n;
+  counts[1] = counts[2] = counts[3] = 0;
+  if (n == 0) return;
+  pac_result R;
+  R.eng = E;
+  R.alpha = alpha;
+  std::memset(&R.st, 0, sizeof(R.st));
+  R.counts[0] = n;
+  Timer T(s, false);
+  KProf KP;
+  DBuf<float4> raw_own;
+  const float4* raw = (const float4*)xyzr;
+  if (!opt.device_inputs) {
+    raw_own = DBuf<float4>(&E->arena, n);
+    HIPCHK(hipMemcpyAsync(raw_own.get(), xyzr, (size_t)n * sizeof(float4),
+                          hipMemcpyHostToDevice, s));
+    raw = raw_own.get();
+  }
+  run_variant(E, raw, n, solvent, alpha, opt, &R, s, T, KP);
+  if (opt.order == PAC_ORDER_CANONICAL) canonicalize(E, &R, n, s);
+  bool over = false;
+  for (int d = 1; d <= 3; ++d) {
+    counts[d] = R.counts[d];
+    if (R.counts[d] > caps[d - 1]) over = true;
+  }
+  if (over)
+    fail(PAC_ERR_OVERFLOW, "an output list exceeds its capacity (counts[] "
+                           "holds the sizes needed)");
+  for (int d = 1; d <= 3; ++d)
+    if (R.counts[d] > 0)
+      HIPCHK(hipMemcpyAsync(lists[d - 1], R.lists[d - 1].get(),
+                            (size_t)R.counts[d] * (d + 1) * sizeof(int32_t),
+                            hipMemcpyDefault, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+/* ------------------------------------------------------------------ */
 /* C ABI                                                               */
 /* ------------------------------------------------------------------ */
 extern "C" {
@@ -547,6 +645,18 @@ int pac_compute(const float* xyzr, int64_t n, float solvent, float alpha,
 int pac_compute_f64(const double* xyzr, int64_t n, double solvent,
                     double alpha, const pac_options* opt, pac_result** out) {
   return guarded([&] { compute_one<double>(xyzr, n, solvent, alpha, opt, out); });
+}
+
+int pac_compute_noalloc(const float* xyzr, int64_t n, float solvent,
+                        float alpha, const pac_options* opt, int32_t* edges,
+                        int64_t cap_edges, int32_t* triangles,
+                        int64_t cap_triangles, int32_t* tets, int64_t cap_tets,
+                        int64_t counts[4]) {
+  int32_t* lists[3] = {edges, triangles, tets};
+  const int64_t caps[3] = {cap_edges, cap_triangles, cap_tets};
+  return guarded([&] {
+    compute_noalloc(xyzr, n, solvent, alpha, opt, lists, caps, counts);
+  });
 }
 
 int pac_result_counts(const pac_result* r, int64_t counts[4]) {

@@ -64,7 +64,39 @@ struct Pool {
     const size_t step = oct >> 3;
     return (b + step - 1) / step * step;
   }
+  /* Arena mode (set_arena): blocks are carved from one fixed device range
+   * instead of hipMalloc, 256-byte aligned, top of stack first; a block
+   * freed while it is the topmost is popped with any freed blocks below
+   * it.  Exhausting the range fails with PAC_ERR_NOMEM. */
+  char* abase = nullptr;
+  size_t acap = 0, atop = 0, ahigh = 0;
+  std::vector<std::pair<size_t, size_t>> ablk; /* (offset, 1 = freed) */
+  void set_arena(void* base, size_t cap) {
+    std::lock_guard<std::mutex> g(m);
+    abase = (char*)base;
+    acap = cap;
+    atop = ahigh = 0;
+    ablk.clear();
+  }
+  bool in_arena(const void* p) const {
+    return abase && (const char*)p >= abase && (const char*)p < abase + acap;
+  }
   void* get(size_t bytes) {
+    if (abase) {
+      std::lock_guard<std::mutex> g(m);
+      const size_t b = ((bytes ? bytes : 1) + 255) & ~(size_t)255;
+      if (b > acap - atop)
+        fail(PAC_ERR_NOMEM, "device scratch of " + std::to_string(acap) +
+                                " bytes exhausted (" + std::to_string(atop) +
+                                " in use, " + std::to_string(b) +
+                                " more requested): input too large for the "
+                                "allocation-free entry point");
+      ablk.push_back({atop, 0});
+      void* p = abase + atop;
+      atop += b;
+      ahigh = std::max(ahigh, atop);
+      return p;
+    }
     size_t c = klass(bytes ? bytes : 1);
     {
       std::lock_guard<std::mutex> g(m);
@@ -100,6 +132,19 @@ struct Pool {
   void put(void* p) {
     if (!p) return;
     std::lock_guard<std::mutex> g(m);
+    if (in_arena(p)) {
+      const size_t off = (size_t)((char*)p - abase);
+      for (size_t k = ablk.size(); k-- > 0;)
+        if (ablk[k].first == off && !ablk[k].second) {
+          ablk[k].second = 1;
+          break;
+        }
+      while (!ablk.empty() && ablk.back().second) {
+        atop = ablk.back().first;
+        ablk.pop_back();
+      }
+      return;
+    }
     const size_t c = cls_[p];
     in_use -= (int64_t)c;
     free_.insert({c, p});
@@ -121,7 +166,15 @@ struct Engine {
   Pool pool;
   std::mutex run_mu;
   void* pinned = nullptr; /* small pinned staging for scalar read-backs */
+  /* the allocation-free entry point: the device's scratch array (its
+   * address on this device) as an arena, and the pool a run takes its
+   * buffers from while it holds run_mu (NULL: `pool`) */
+  void* scratch = nullptr;
+  Pool arena;
+  Pool* run_pool = nullptr;
 };
+/* the pool the current run allocates from */
+inline Pool* pool_of(Engine* E) { return E->run_pool ? E->run_pool : &E->pool; }
 /* the per-device engine (created on first use) */
 Engine* engine_for(int dev);
 
@@ -238,16 +291,23 @@ struct Timer {
   int used = 0;
   hipStream_t s;
   bool on = true; /* off: marks are skipped (recorded pipelines) */
-  explicit Timer(hipStream_t st) : s(st) {
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  bool made = true;
+  /* events = false: no events are created, marks are skipped and every
+   * phase reads 0 (the allocation-free entry point) */
+  explicit Timer(hipStream_t st, bool events = true)
+      : s(st), on(events), made(events) {
+    if (on)
+      for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   }
   ~Timer() {
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (made)
+      for (auto& e : ev) (void)hipEventDestroy(e);
   }
   void mark(int i) {
     if (on) HIPCHK(hipEventRecord(ev[i], s));
   }
   double ms(int a, int b) {
+    if (!made) return 0.0;
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, ev[a], ev[b]));
     return t;

@@ -117,7 +117,7 @@ void plan_pass(Engine* E, const Ball* raw, int64_t n, Real solvent,
                Real alpha, const pac_options& opt, hipStream_t s,
                const int64_t* moff, int64_t nmol, PassPlan& pl,
                const Bounds* gridb) {
-  Pool* P = &E->pool;
+  Pool* P = pool_of(E);
   PipeLaunch L{s, kBlock};
   const bool batch = nmol > 1;
   std::vector<Bounds> hb(std::max<int64_t>(nmol, 1));
@@ -245,7 +245,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                   Real alpha, const pac_options& opt, pac_result* R,
                   hipStream_t s, Timer& T, KProf& KP, const int64_t* moff,
                   int64_t nmol, const PassRec* rec, const SubInput* sub) {
-  Pool* P = rec ? rec->pool : &E->pool;
+  Pool* P = rec ? rec->pool : pool_of(E);
   PipeLaunch L{s, kBlock};
   pac_stats& st = R->st;
   st.n = n;
@@ -880,7 +880,7 @@ void run_chunked(Engine* E, const Ball* raw, int64_t n, Real solvent,
     return;
   }
   if (n > (int64_t)INT32_MAX) fail(PAC_ERR_OVERFLOW, "too many balls");
-  Pool* P = &E->pool;
+  Pool* P = pool_of(E);
   PipeLaunch L{s, kBlock};
 
   /* the whole input's grid and cell-sorted order */

@@ -86,8 +86,11 @@ EXPORTS = (
     "pac_autotune", "pac_graph_create", "pac_graph_set_input", "pac_graph_launch",
     "pac_graph_result", "pac_graph_info", "pac_graph_free", "pac_write_crd",
     "pac_read_pdb", "pac_result_copy_birth", "pac_result_copy_filtration",
-    "pac_memory_stats", "pac_memory_trim",
+    "pac_memory_stats", "pac_memory_trim", "pac_compute_noalloc",
 )
+# the allocation-free entry point's limits (include/pac.h)
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_BALLS = 8192
 
 _lib = None
 
@@ -115,6 +118,9 @@ def lib():
                               ctypes.POINTER(Options), ctypes.POINTER(vp)]
     L.pac_compute_f64.argtypes = [vp, i64, ctypes.c_double, ctypes.c_double,
                                   ctypes.POINTER(Options), ctypes.POINTER(vp)]
+    L.pac_compute_noalloc.argtypes = [vp, i64, ctypes.c_float, ctypes.c_float,
+                                      ctypes.POINTER(Options), vp, i64, vp, i64,
+                                      vp, i64, vp]
     L.pac_result_counts.argtypes = [vp, vp]
     for f in ("pac_result_copy_edges", "pac_result_copy_triangles",
               "pac_result_copy_tets"):
