@@ -17,99 +17,104 @@
  * :4669), checkDanglingTris :807-881, checkDanglingEdges :928-1015.
  */
 #include "pac_dev.h"
+#include "pac_wide.h"
 #include "pac_sos_table.h"
 
 namespace pac {
 namespace PAC_V {
 
-/* exact lifted row for SoS */
-__device__ void lifted_row(const Ball& b, const LdWI& L, WI out[4]) {
-  V3<WI> p = L.p(b);
-  out[0] = p.x;
-  out[1] = p.y;
-  out[2] = p.z;
-  out[3] = p.x * p.x + p.y * p.y + p.z * p.z - L.w(b);
-}
-__device__ __forceinline__ WI ent(const WI M[5][4], int r, int c) {
-  return c == 4 ? wi_from_i64(1) : M[r][c];
-}
-__device__ WI det2(const WI M[5][4], const int* r, const int* c) {
-  return ent(M, r[0], c[0]) * ent(M, r[1], c[1]) -
-         ent(M, r[0], c[1]) * ent(M, r[1], c[0]);
-}
-__device__ WI det3(const WI M[5][4], const int* r, const int* c) {
-  WI acc = wi_from_i64(0);
-  for (int q = 0; q < 3; ++q) {
-    int cc[2], m = 0;
-    for (int t = 0; t < 3; ++t)
-      if (t != q) cc[m++] = c[t];
-    WI term = ent(M, r[0], c[q]) * det2(M, r + 1, cc);
-    acc = (q & 1) ? acc - term : acc + term;
-  }
-  return acc;
-}
-__device__ WI det4(const WI M[5][4], const int* r, const int* c) {
-  WI acc = wi_from_i64(0);
-  for (int q = 0; q < 4; ++q) {
-    int cc[3], m = 0;
-    for (int t = 0; t < 4; ++t)
-      if (t != q) cc[m++] = c[t];
-    WI term = ent(M, r[0], c[q]) * det3(M, r + 1, cc);
-    acc = (q & 1) ? acc - term : acc + term;
-  }
-  return acc;
-}
+/* Exact stage: every lane of a group of kTPI lanes evaluates the same
+ * item over the wavefront-cooperative integers (pac_wide.h). */
+using XI = CW<kTPI>;
 
-/* Sign of the SoS-perturbed det[x y z l 1] of the five balls, rows in
- * argument order, perturbation order given by ids (original input ids,
- * as the reference's Atom.index, AlphaComplexParallel.cu:1087). */
+/* Sign of the SoS-perturbed det[x y z l 1] of the five balls (l = |p|^2 -
+ * w), rows in argument order, perturbation order given by ids (original
+ * input ids, as the reference's Atom.index, AlphaComplexParallel.cu:1087).
+ * Every term of the expansion (pac_sos_table) is a minor that keeps the
+ * column of ones, and such an m x m minor equals the (m-1) x (m-1)
+ * determinant of its other columns with the last row subtracted from the
+ * rest (subtracting a row changes no determinant; expanding along the
+ * ones column, now (0, ..., 0, 1), leaves that block with sign +1).  The
+ * lifted matrix lives in LDS (lm: this wave's 20 x 64 words, entry (r, c)
+ * at (4 r + c) * 64 + lane), so that the table's run-time rows and columns
+ * index it without private-memory arrays. */
 __device__ int sos_det5(const Ball b[5], const uint32_t ids[5],
-                        const LdWI& L) {
+                        const LdCW<kTPI>& L, uint32_t* lm) {
+  /* rows in increasing id order by a 9-comparator sorting network (all
+   * indices known at compile time: no private-memory array), the
+   * permutation's parity counted by its swaps */
+  uint32_t k[5] = {ids[0], ids[1], ids[2], ids[3], ids[4]};
   int ord[5] = {0, 1, 2, 3, 4};
   int swaps = 0;
-  for (int a = 0; a < 5; ++a)
-    for (int q = 0; q + 1 < 5 - a; ++q)
-      if (ids[ord[q + 1]] < ids[ord[q]]) {
-        int t = ord[q];
-        ord[q] = ord[q + 1];
-        ord[q + 1] = t;
-        ++swaps;
-      }
-  WI M[5][4];
-  for (int r = 0; r < 5; ++r) lifted_row(b[ord[r]], L, M[r]);
-  int parity = (swaps & 1) ? -1 : 1;
+  constexpr int net[9][2] = {{0, 1}, {3, 4}, {2, 4}, {2, 3}, {0, 3},
+                             {0, 2}, {1, 4}, {1, 3}, {1, 2}};
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const int i = net[c][0], j = net[c][1];
+    const bool sw = k[j] < k[i];
+    const uint32_t ki = k[i], kj = k[j];
+    const int oi = ord[i], oj = ord[j];
+    k[i] = sw ? kj : ki;
+    k[j] = sw ? ki : kj;
+    ord[i] = sw ? oj : oi;
+    ord[j] = sw ? oi : oj;
+    swaps += sw;
+  }
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    Ball br = b[0];
+#pragma unroll
+    for (int q = 1; q < 5; ++q)
+      if (ord[r] == q) br = b[q];
+    const V3<XI> p = L.p(br);
+    const XI l = p.x * p.x + p.y * p.y + p.z * p.z - L.w(br);
+    lm[(4 * r + 0) * 64 + lane] = p.x.v;
+    lm[(4 * r + 1) * 64 + lane] = p.y.v;
+    lm[(4 * r + 2) * 64 + lane] = p.z.v;
+    lm[(4 * r + 3) * 64 + lane] = l.v;
+  }
+  auto at = [&](int r, int c) { return XI{lm[(4 * r + c) * 64 + lane]}; };
+  const int parity = (swaps & 1) ? -1 : 1;
+#pragma unroll 1
   for (int t = 0; t < PAC_SOS_TERMS; ++t) {
-    int rows[5], cols[5], m = 0, mc = 0;
-    int rm = pac_sos_table[t][0], cm = pac_sos_table[t][1];
-    for (int r = 0; r < 5; ++r)
-      if (rm & (1 << r)) rows[m++] = r;
-    for (int c = 0; c < 5; ++c)
-      if (cm & (1 << c)) cols[mc++] = c;
+    unsigned rm = (unsigned)pac_sos_table[t][0];
+    /* the ones column (bit 4) is in every term */
+    unsigned cm = (unsigned)pac_sos_table[t][1] & 0xfu;
+    const int m = __builtin_popcount(rm);
     int s;
-    if (m == 1)
-      s = wi_sign(ent(M, rows[0], cols[0]));
-    else if (m == 2)
-      s = wi_sign(det2(M, rows, cols));
-    else if (m == 3)
-      s = wi_sign(det3(M, rows, cols));
-    else
-      s = wi_sign(det4(M, rows, cols));
+    if (m == 1) {
+      s = 1; /* the ones column alone */
+    } else {
+      /* rows r0 < r1 < ... < z (z the last), columns c0 < c1 < c2 */
+      const int z = 31 - __builtin_clz(rm);
+      const int r0 = __builtin_ctz(rm);
+      rm &= rm - 1;
+      const int r1 = __builtin_ctz(rm);
+      rm &= rm - 1;
+      const int r2 = __builtin_ctz(rm);
+      const int c0 = __builtin_ctz(cm);
+      cm &= cm - 1;
+      const int c1 = cm ? __builtin_ctz(cm) : 0;
+      cm &= cm - 1;
+      const int c2 = cm ? __builtin_ctz(cm) : 0;
+      auto d = [&](int r, int c) { return at(r, c) - at(z, c); };
+      if (m == 2) {
+        s = fe_sign(d(r0, c0));
+      } else if (m == 3) {
+        s = fe_sign(d(r0, c0) * d(r1, c1) - d(r0, c1) * d(r1, c0));
+      } else {
+        const XI a00 = d(r0, c0), a01 = d(r0, c1), a02 = d(r0, c2);
+        const XI a10 = d(r1, c0), a11 = d(r1, c1), a12 = d(r1, c2);
+        const XI a20 = d(r2, c0), a21 = d(r2, c1), a22 = d(r2, c2);
+        s = fe_sign(a00 * (a11 * a22 - a12 * a21) -
+                    a01 * (a10 * a22 - a12 * a20) +
+                    a02 * (a10 * a21 - a11 * a20));
+      }
+    }
     if (s != 0) return parity * pac_sos_table[t][2] * s;
   }
   return parity;
-}
-
-/* WI -> double (relative error < 2^-50), scaled back by 2^-S */
-__device__ double wi_to_double(const WI& a, int S) {
-  bool neg = (a.l[WL - 1] >> 31) != 0;
-  WI m = neg ? wi_neg(a) : a;
-  int top = WL - 1;
-  while (top > 0 && m.l[top] == 0) --top;
-  double v = 0;
-  for (int q = top; q >= 0 && q >= top - 2; --q)
-    v += __builtin_ldexp((double)m.l[q], 32 * q);
-  v = __builtin_ldexp(v, -S);
-  return neg ? -v : v;
 }
 
 /* ---------------- tet emptiness ---------------- */
@@ -443,15 +448,17 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_esc(
 }
 
 /* exact resolution of one (tet, ball) pair, or a full exact scan when
- * aux == ~0 (tet whose orientation no float stage could certify) */
+ * aux == ~0 (tet whose orientation no float stage could certify); one
+ * item per group of kTPI lanes */
 __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
                               const uint32_t vid[4], const uint32_t* perm,
-                              const WI& D, const V3<WI>& N, const V3<WI>& pa,
-                              const WI& wa, uint32_t p, unsigned* sos) {
-  LdWI L{S.ex.S};
+                              const XI& D, const V3<XI>& N, const V3<XI>& pa,
+                              const XI& wa, uint32_t p, unsigned* sos,
+                              uint32_t* lm) {
+  const LdCW<kTPI> L{S.ex.S};
   Ball E = S.ball[p];
-  WI g = power_gap(pa, wa, L.p(E), L.w(E), D, N);
-  int s = wi_sign(g), sD = wi_sign(D);
+  XI g = power_gap(pa, wa, L.p(E), L.w(E), D, N);
+  int s = fe_sign(g), sD = fe_sign(D);
   if (s != 0) return (s * sD) < 0;
   /* exact tie: insphere_w = det[x y z l 1](a,b,c,d,e) with SoS; the tet
    * is violated iff SoS * orient3d(a,b,c,d) > 0, orient3d = -sign(D)
@@ -460,7 +467,7 @@ __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
   Ball bb[5] = {v[0], v[1], v[2], v[3], E};
   uint32_t ids[5] = {perm[vid[0]], perm[vid[1]], perm[vid[2]], perm[vid[3]],
                      perm[p]};
-  int ss = sos_det5(bb, ids, L);
+  int ss = sos_det5(bb, ids, L, lm);
   return (ss * -sD) > 0;
 }
 
@@ -468,27 +475,31 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     Stage S, QRef q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
   unsigned cnt = 0, sos = 0;
-  for_queue(q, [&](uint32_t t) {
-    if (!live_entry(q.at(t), flag)) return;
+  const bool lead = Grp<kTPI>::idx() == 0;
+  __shared__ uint32_t s_lift[kBlock / 64][20 * 64]; /* SoS lifted matrices */
+  uint32_t* const lm = s_lift[threadIdx.x >> 6];
+  for_queue_grp<kTPI>(q, [&](uint32_t t) {
+    /* the group's first lane decides, so the group stays together */
+    if (!Grp<kTPI>::from(live_entry(q.at(t), flag) ? 1u : 0u, 0)) return;
     QEntry e = q.at(t);
     uint4 tt = tets[e.item];
     uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
     Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
-    LdWI L{S.ex.S};
-    V3<WI> pa = L.p(v[0]);
-    WI wa = L.w(v[0]), D;
-    V3<WI> N;
+    const LdCW<kTPI> L{S.ex.S};
+    V3<XI> pa = L.p(v[0]);
+    XI wa = L.w(v[0]), D;
+    V3<XI> N;
     tet_DN(pa, wa, L.p(v[1]), L.w(v[1]), L.p(v[2]), L.w(v[2]), L.p(v[3]),
            L.w(v[3]), &D, &N);
     if (e.aux != 0xffffffffu) {
       ++cnt;
-      if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, e.aux, &sos))
+      if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, e.aux, &sos, lm) && lead)
         flag[e.item] = 0;
-    } else if (wi_sign(D) != 0) {
-      double D2 = 2.0 * wi_to_double(D, 3 * S.ex.S);
-      double cx = (double)v[0].x + wi_to_double(N.x, 4 * S.ex.S) / D2;
-      double cy = (double)v[0].y + wi_to_double(N.y, 4 * S.ex.S) / D2;
-      double cz = (double)v[0].z + wi_to_double(N.z, 4 * S.ex.S) / D2;
+    } else if (fe_sign(D) != 0) {
+      double D2 = 2.0 * cw_to_double(D, 3 * S.ex.S);
+      double cx = (double)v[0].x + cw_to_double(N.x, 4 * S.ex.S) / D2;
+      double cy = (double)v[0].y + cw_to_double(N.y, 4 * S.ex.S) / D2;
+      double cz = (double)v[0].z + cw_to_double(N.z, 4 * S.ex.S) / D2;
       double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
       CellBox box;
       bool viol = false;
@@ -502,18 +513,19 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
             for (uint32_t p = b; p < en; ++p) {
               if (p == tt.x || p == tt.y || p == tt.z || p == tt.w) continue;
               ++cnt;
-              if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, p, &sos)) {
+              if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, p, &sos, lm)) {
                 viol = true;
                 break;
               }
             }
           }
       }
-      flag[e.item] = viol ? 0 : 2;
+      if (lead) flag[e.item] = viol ? 0 : 2;
     }
   });
-  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, cnt);
-  flush_ctr(S.ctr, CTR_SOS, sos);
+  /* every lane of a group counted the group's evaluations */
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, lead ? cnt : 0u);
+  flush_ctr(S.ctr, CTR_SOS, lead ? sos : 0u);
 }
 
 
@@ -794,15 +806,18 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
   sc.flush(S.ctr, KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2);
 }
 
-/* exact triangle / edge pair (or full scan when aux == ~0) */
+/* exact triangle / edge pair (or full scan when aux == ~0); one item per
+ * group of kTPI lanes */
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
     Stage S, QRef q, const uint32_t* ecol,
     const uint32_t* eleft, const uint32_t* tk, const uint32_t* tedge,
     uint8_t* flag) {
   unsigned cnt = 0;
-  for_queue(q, [&](uint32_t t) {
-    if (!live_entry(q.at(t), flag)) return;
+  const bool lead = Grp<kTPI>::idx() == 0;
+  for_queue_grp<kTPI>(q, [&](uint32_t t) {
+    /* the group's first lane decides, so the group stays together */
+    if (!Grp<kTPI>::from(live_entry(q.at(t), flag) ? 1u : 0u, 0)) return;
     QEntry e = q.at(t);
     uint32_t vid[3];
     if (KIND == 0) {
@@ -815,30 +830,30 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
       vid[1] = ecol[e.item];
       vid[2] = 0xffffffffu;
     }
-    LdWI L{S.ex.S};
+    const LdCW<kTPI> L{S.ex.S};
     Ball a = S.ball[vid[0]], b = S.ball[vid[1]];
-    V3<WI> pa = L.p(a), num;
-    WI wa = L.w(a), den;
+    V3<XI> pa = L.p(a), num;
+    XI wa = L.w(a), den;
     if (KIND == 0) {
       tri_X(pa, wa, L.p(b), L.w(b), L.p(S.ball[vid[2]]), L.w(S.ball[vid[2]]),
             &den, &num);
     } else {
-      V3<WI> u = vsub(L.p(b), pa);
+      V3<XI> u = vsub(L.p(b), pa);
       den = vdot(u, u);
-      WI A = den + wa - L.w(b);
+      XI A = den + wa - L.w(b);
       num = vscale(A, u);
     }
     if (e.aux != 0xffffffffu) {
       ++cnt;
       Ball E = S.ball[e.aux];
-      if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0)
+      if (fe_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0 && lead)
         flag[e.item] = 0;
-    } else if (wi_sign(den) > 0) {
-      double d2 = 2.0 * wi_to_double(den, (KIND == 0 ? 4 : 2) * S.ex.S);
+    } else if (fe_sign(den) > 0) {
+      double d2 = 2.0 * cw_to_double(den, (KIND == 0 ? 4 : 2) * S.ex.S);
       int sh = (KIND == 0 ? 5 : 3) * S.ex.S;
-      double cx = (double)a.x + wi_to_double(num.x, sh) / d2;
-      double cy = (double)a.y + wi_to_double(num.y, sh) / d2;
-      double cz = (double)a.z + wi_to_double(num.z, sh) / d2;
+      double cx = (double)a.x + cw_to_double(num.x, sh) / d2;
+      double cy = (double)a.y + cw_to_double(num.y, sh) / d2;
+      double cz = (double)a.z + cw_to_double(num.z, sh) / d2;
       double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
       CellBox box;
       bool viol = false;
@@ -854,20 +869,20 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
                 continue;
               ++cnt;
               Ball E = S.ball[p];
-              if (wi_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0) {
+              if (fe_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0) {
                 viol = true;
                 break;
               }
             }
           }
       }
-      flag[e.item] = viol ? 0 : 2;
-    } else {
+      if (lead) flag[e.item] = viol ? 0 : 2;
+    } else if (lead) {
       flag[e.item] = 0; /* degenerate simplex: cannot be an alpha simplex */
     }
   });
   flush_ctr(S.ctr, CTR_STAGE + (KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2) * 4 + 3,
-            cnt);
+            lead ? cnt : 0u);
 }
 
 
@@ -964,7 +979,7 @@ void launch_tet_ac2_exact(const Stage& S, QRef q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L) {
   if (!q.launch_items()) return;
-  hipLaunchKernelGGL(k_tet_ac2_exact, qgrid(q, kExactBlocks),
+  hipLaunchKernelGGL(k_tet_ac2_exact, qgrid_grp(q, kExactBlocks, kTPI),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
 }
 template <class R, int KIND, bool FINE, bool MOL>
@@ -1050,7 +1065,7 @@ void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr = qgrid(q, kExactBlocks);
+  dim3 gr = qgrid_grp(q, kExactBlocks, kTPI);
   if (kind == 0)
     hipLaunchKernelGGL(k_low_ac2_exact<0>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);

@@ -18,6 +18,7 @@
  *                        + markRealTets :531-591 (fused)
  */
 #include "pac_dev.h"
+#include "pac_wide.h"
 
 namespace pac {
 namespace PAC_V {
@@ -350,12 +351,15 @@ __device__ int edge_code(const Ball& a, const Ball& b, Real alpha) {
                  Al<T>::c(alpha));
   return code_of(fe_sign(v));
 }
+/* exact decisions over the wavefront-cooperative integers (every lane of
+ * a group of kTPI lanes calls them with the same balls) */
+using XI = CW<kTPI>;
 __device__ int edge_exact(const Ball& a, const Ball& b,
                           const ExactScale& X) {
   if (same_pos(a, b)) return REJECT;
-  LdWI L{X.S};
-  WI v = edge_val(L.p(a), L.w(a), L.p(b), L.w(b), X.alpha);
-  return wi_sign(v) >= 0 ? KEEP : REJECT;
+  const LdCW<kTPI> L{X.S};
+  XI v = edge_val(L.p(a), L.w(a), L.p(b), L.w(b), cw_of<kTPI>(X.alpha));
+  return fe_sign(v) >= 0 ? KEEP : REJECT;
 }
 
 template <class T>
@@ -369,11 +373,12 @@ __device__ int tri_code(const Ball& a, const Ball& b, const Ball& c,
 }
 __device__ int tri_exact(const Ball& a, const Ball& b, const Ball& c,
                          const ExactScale& X) {
-  LdWI L{X.S};
-  WI nn;
-  WI v = tri_val(L.p(a), L.w(a), L.p(b), L.w(b), L.p(c), L.w(c), X.alpha, &nn);
-  if (wi_sign(nn) == 0) return REJECT;
-  return wi_sign(v) >= 0 ? KEEP : REJECT;
+  const LdCW<kTPI> L{X.S};
+  XI nn;
+  XI v = tri_val(L.p(a), L.w(a), L.p(b), L.w(b), L.p(c), L.w(c),
+                 cw_of<kTPI>(X.alpha), &nn);
+  if (fe_sign(nn) == 0) return REJECT;
+  return fe_sign(v) >= 0 ? KEEP : REJECT;
 }
 
 template <class T>
@@ -399,13 +404,13 @@ __device__ __forceinline__ int tet_code_ss(const Ball* v, Real alpha) {
 }
 __device__ int tet_exact(const Ball& a, const Ball& b, const Ball& c,
                          const Ball& d, const ExactScale& X) {
-  LdWI L{X.S};
-  WI D;
-  V3<WI> N;
-  WI wa = L.w(a);
+  const LdCW<kTPI> L{X.S};
+  XI D;
+  V3<XI> N;
+  XI wa = L.w(a);
   tet_DN(L.p(a), wa, L.p(b), L.w(b), L.p(c), L.w(c), L.p(d), L.w(d), &D, &N);
-  if (wi_sign(D) == 0) return REJECT;
-  return wi_sign(tet_val(D, N, wa, X.alpha)) >= 0 ? KEEP : REJECT;
+  if (fe_sign(D) == 0) return REJECT;
+  return fe_sign(tet_val(D, N, wa, cw_of<kTPI>(X.alpha))) >= 0 ? KEEP : REJECT;
 }
 
 /* ------------------------------------------------------------------ */
@@ -818,13 +823,17 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
                                                          QRef q,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
+  /* one queued candidate per group of kTPI lanes */
   unsigned cnt = 0;
-  for_queue(q, [&](uint32_t t) {
+  for_queue_grp<kTPI>(q, [&](uint32_t t) {
     QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
-    flag[e.item] = (uint8_t)filter_exact<KIND>(v, S.ex);
-    ++cnt;
+    const int r = filter_exact<KIND>(v, S.ex);
+    if (Grp<kTPI>::idx() == 0) {
+      flag[e.item] = (uint8_t)r;
+      ++cnt;
+    }
   });
   flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
@@ -865,7 +874,7 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr = qgrid(q, kExactBlocks);
+  dim3 gr = qgrid_grp(q, kExactBlocks, kTPI);
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);

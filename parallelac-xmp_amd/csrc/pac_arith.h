@@ -11,9 +11,10 @@
  *                          AccurateDWPlusDW / DWTimesDW3, relative error
  *                          <= 4u^2) with a double error bound charged at
  *                          2^-100 per op.
- *   WI                     two's-complement integer in WL u32 limbs (512
- *                          bits for FP32 balls, 768 for FP64; exact stage
- *                          on inputs scaled by 2^S).
+ *   WI                     storage of a WL-limb integer (alpha scaled by
+ *                          2^(2S)); the exact stage itself computes with
+ *                          the wavefront-cooperative integers of
+ *                          pac_wide.h on inputs scaled by 2^S.
  *
  * The predicate polynomials are written once over any of these rings; they
  * restate the reference filters (AlphaComplexLargeData.cu:167-190, :298-348,
@@ -169,79 +170,13 @@ PAC_DEV int fe_sign(const FEDD& a) {
 }
 
 /* ------------------------------------------------------------------ */
-/* WI: WL-limb two's-complement integer                                */
+/* WI: storage of a WL-limb two's-complement integer (alpha * 2^(2S),   */
+/* built on the host); the exact stage computes with the wavefront-     */
+/* cooperative integers of pac_wide.h                                   */
 /* ------------------------------------------------------------------ */
 struct WI {
   uint32_t l[WL];
 };
-PAC_DEV WI wi_from_i64(int64_t v) {
-  WI r;
-  r.l[0] = (uint32_t)(uint64_t)v;
-  r.l[1] = (uint32_t)((uint64_t)v >> 32);
-  uint32_t ext = v < 0 ? 0xffffffffu : 0u;
-#pragma unroll
-  for (int i = 2; i < WL; ++i) r.l[i] = ext;
-  return r;
-}
-PAC_DEV WI operator+(const WI& a, const WI& b) {
-  WI r;
-  uint64_t c = 0;
-#pragma unroll
-  for (int i = 0; i < WL; ++i) {
-    c += (uint64_t)a.l[i] + b.l[i];
-    r.l[i] = (uint32_t)c;
-    c >>= 32;
-  }
-  return r;
-}
-PAC_DEV WI operator-(const WI& a, const WI& b) {
-  WI r;
-  uint64_t borrow = 0;
-#pragma unroll
-  for (int i = 0; i < WL; ++i) {
-    uint64_t t = (uint64_t)a.l[i] - (uint64_t)b.l[i] - borrow;
-    r.l[i] = (uint32_t)t;
-    borrow = (t >> 63) & 1u;
-  }
-  return r;
-}
-PAC_DEV WI wi_neg(const WI& a) {
-  WI z;
-#pragma unroll
-  for (int i = 0; i < WL; ++i) z.l[i] = 0;
-  return z - a;
-}
-PAC_DEV int wi_sign(const WI& a) {
-  if (a.l[WL - 1] >> 31) return -1;
-  uint32_t o = 0;
-#pragma unroll
-  for (int i = 0; i < WL; ++i) o |= a.l[i];
-  return o ? 1 : 0;
-}
-PAC_DEV WI operator*(const WI& a, const WI& b) {
-  bool na = (a.l[WL - 1] >> 31) != 0, nb = (b.l[WL - 1] >> 31) != 0;
-  WI ma = na ? wi_neg(a) : a;
-  WI mb = nb ? wi_neg(b) : b;
-  WI r;
-#pragma unroll
-  for (int i = 0; i < WL; ++i) r.l[i] = 0;
-  int la = WL, lb = WL;
-  while (la > 0 && ma.l[la - 1] == 0) --la;
-  while (lb > 0 && mb.l[lb - 1] == 0) --lb;
-  for (int i = 0; i < la; ++i) {
-    uint64_t carry = 0;
-    uint64_t ai = ma.l[i];
-    int j = 0;
-    for (; j < lb && i + j < WL; ++j) {
-      uint64_t t = ai * mb.l[j] + r.l[i + j] + carry;
-      r.l[i + j] = (uint32_t)t;
-      carry = t >> 32;
-    }
-    if (i + j < WL) r.l[i + j] = (uint32_t)carry;
-  }
-  return (na != nb) ? wi_neg(r) : r;
-}
-PAC_DEV int fe_sign(const WI& a) { return wi_sign(a); }
 
 /* ------------------------------------------------------------------ */
 /* Ring constants                                                      */
@@ -255,10 +190,6 @@ struct K<FE<R>> {
 template <>
 struct K<FEDD> {
   static PAC_DEV FEDD c(int v) { return FEDD{dd{(double)v, 0.0}, 0.0}; }
-};
-template <>
-struct K<WI> {
-  static PAC_DEV WI c(int v) { return wi_from_i64(v); }
 };
 
 /* ------------------------------------------------------------------ */
@@ -296,56 +227,6 @@ struct Ld<FEDD> {
     return r * r;
   }
 };
-/* v * 2^S as an integer; S is at least minus the exponent of v's lowest
- * set bit, so the product is exact.  FP32: |v| 2^S < 2^63 (checked on the
- * host), one int64 conversion. */
-PAC_DEV WI wi_scaled(float v, int S) {
-  return wi_from_i64((int64_t)__builtin_ldexp((double)v, S));
-}
-/* FP64: the 53-bit significand shifted into place limb by limb. */
-PAC_DEV WI wi_scaled(double v, int S) {
-  WI r;
-  uint64_t bits = (uint64_t)__double_as_longlong(v);
-  int E = (int)((bits >> 52) & 0x7ff);
-  uint64_t m = bits & ((1ull << 52) - 1);
-  int e;
-  if (E == 0) {
-    e = -1074;
-  } else {
-    m |= 1ull << 52;
-    e = E - 1075;
-  }
-  int sh = e + S;
-  if (sh < 0) { /* the shifted-out bits are zero by the choice of S */
-    m >>= -sh;
-    sh = 0;
-  }
-  const int q = sh >> 5, b = sh & 31;
-  const uint64_t lo = m << b;
-  const uint32_t hi = b ? (uint32_t)(m >> (64 - b)) : 0u;
-#pragma unroll
-  for (int i = 0; i < WL; ++i)
-    r.l[i] = i == q ? (uint32_t)lo
-                    : (i == q + 1 ? (uint32_t)(lo >> 32) : (i == q + 2 ? hi : 0u));
-  if (bits >> 63) {
-    WI z;
-#pragma unroll
-    for (int i = 0; i < WL; ++i) z.l[i] = 0;
-    return z - r;
-  }
-  return r;
-}
-struct LdWI {
-  int S;
-  PAC_DEV V3<WI> p(const Ball& b) const {
-    return V3<WI>{wi_scaled(b.x, S), wi_scaled(b.y, S), wi_scaled(b.z, S)};
-  }
-  PAC_DEV WI w(const Ball& b) const {
-    WI r = wi_scaled(b.w, S);
-    return r * r;
-  }
-};
-
 /* ------------------------------------------------------------------ */
 /* Predicate polynomials (ring-generic).                               */
 /* ------------------------------------------------------------------ */

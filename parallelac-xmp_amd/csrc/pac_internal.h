@@ -180,6 +180,12 @@ inline dim3 qgrid(const QRef& q, unsigned maxb) {
   if (!q.dv) return dim3(b ? b : 1); /* host view: one entry per thread */
   return dim3(b < maxb ? (b ? b : 1) : maxb);
 }
+/* the same for consumers that take one entry per group of tpi lanes */
+inline dim3 qgrid_grp(const QRef& q, unsigned maxb, unsigned tpi) {
+  const uint64_t b = ((uint64_t)q.launch_items() * tpi + kBlock - 1) / kBlock;
+  if (!q.dv) return dim3(b ? (unsigned)b : 1u);
+  return dim3(b < maxb ? (b ? (unsigned)b : 1u) : maxb);
+}
 
 /* ---- recorded pipelines (k_prims.hip) ----
  * Fixed-capacity runs keep every count on the device; overflows and a

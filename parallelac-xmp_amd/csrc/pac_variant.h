@@ -4,8 +4,10 @@
  * The enumeration / filter / emptiness kernels and the pipeline driver are
  * compiled once per input type from the same source:
  *   vf32  FP32 balls (float4), as the reference reads them
- *         (AlphaComplexParallel.cu:63-67); exact stage 512-bit;
- *   vf64  FP64 balls (double4), continuous coordinates; exact stage 768-bit
+ *         (AlphaComplexParallel.cu:63-67); exact stage in 512-bit integers
+ *         (16 cooperating lanes), values bounded by WL = 16 limbs;
+ *   vf64  FP64 balls (double4), continuous coordinates; exact stage in
+ *         1024-bit integers (32 lanes), values bounded by WL = 24 limbs
  *         (a coordinate's lowest set bit can lie ~2^-70 below its units,
  *         and the degree-8 tet filter needs 8 times that many bits).
  * Each compilation lives in its own namespace pac::vf32 / pac::vf64, so the

@@ -21,6 +21,7 @@ import pac
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "parallelac-xmp_amd", "csrc"))
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 import gen_sos_table  # noqa: E402
 
 
@@ -38,6 +39,12 @@ def test_tables_agree():
     assert np.array_equal(py, dev)
     # first term: orient3d of points 1..4 (non-lifted), negative coefficient
     assert tuple(py[0]) == (0x1E, 0x17, -1)
+    # every term keeps the column of ones (the perturbation never touches
+    # it) and is square: the device evaluates each m x m minor as the
+    # (m-1) x (m-1) determinant of row differences
+    for rm, cm, _ in py:
+        assert cm & 0x10
+        assert bin(rm).count("1") == bin(cm).count("1")
     # flip set of the reference's table (predicates.h:5420-5447)
     flips = {i + 1 for i, t in enumerate(py) if t[2] < 0}
     assert flips == {1, 3, 7, 9, 11, 14, 15, 16, 18, 20, 22, 23, 26, 30, 31, 32,
