@@ -711,12 +711,15 @@ __global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
  * and the four vertex ids only of the slots that are kept or undecided
  * (the later stages read no others).  Slots past the LDS stage were
  * written to global memory during the merges and are filtered from there. */
+/* FP32 first stage: 8 waves per SIMD (at most 64 VGPRs, no spills; the
+ * LDS stage admits 8 blocks per CU); FP64: the compiler's choice */
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_filter(
     Stage S, Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
-    uint4* ctet, uint8_t* flag, Queue q1, uint64_t lim) {
+    uint4* ctet, uint8_t* flag, uint32_t* bcnt, Queue q1, uint64_t lim) {
   const int64_t nt = c_nt.get();
   /* 8 blocks per CU fit in LDS with block_flush's 16 B (the kernel's 60
    * VGPRs allow 8 waves per SIMD); a block of 256 alpha triangles averages
@@ -724,7 +727,10 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
   constexpr int CAP = 1272;
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
-  if (q0 >= nt) return; /* whole block past the (device) count */
+  if (q0 >= nt) { /* whole block past the (device) count */
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    return;
+  }
   const int64_t q1e = q0 + kBlock < nt ? q0 + kBlock : nt;
   BlockOut4<CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
   int64_t q = gtid();
@@ -757,20 +763,49 @@ __global__ __launch_bounds__(kBlock) void k_tet_enum_filter(
     for (; o < oe; ++o) B.put(ctet, o, make_uint4(0xffffffffu, 0u, 0u, 0u));
   }
   __syncthreads(); /* LDS stage and the overflow slots written */
+  /* First precision stage over the block's slots, 256 per round; the kept
+   * and undecided ones are written compacted, in slot order, from the
+   * block's first slot on: position base + (kept before them) <= their
+   * own slot, and every slot of this round and the earlier ones was read
+   * before the round's first barrier, so no unread overflow slot is
+   * overwritten. */
+  __shared__ unsigned s_wsum[kBlock / 64];
   const uint64_t base = B.base;
   const uint64_t m = base >= lim ? 0 : (B.total < lim - base ? B.total : lim - base);
-  for (uint64_t t = threadIdx.x; t < m; t += kBlock) {
-    const bool staged = t < (uint64_t)CAP;
-    const uint4 v = staged ? s_out[t] : ctet[base + t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t run = 0; /* kept + undecided written so far (block-uniform) */
+  for (uint64_t r0 = 0; r0 < m; r0 += kBlock) {
+    const uint64_t t = r0 + threadIdx.x;
+    uint4 v = make_uint4(0xffffffffu, 0u, 0u, 0u);
     int r = REJECT;
-    if (v.x != 0xffffffffu) {
-      const Ball bv[4] = {S.ball[v.x], S.ball[v.y], S.ball[v.z], S.ball[v.w]};
-      r = tet_code_ss<R>(bv, S.alpha);
+    if (t < m) {
+      v = t < (uint64_t)CAP ? s_out[t] : ctet[base + t];
+      if (v.x != 0xffffffffu) {
+        const Ball bv[4] = {S.ball[v.x], S.ball[v.y], S.ball[v.z], S.ball[v.w]};
+        r = tet_code_ss<R>(bv, S.alpha);
+      }
     }
-    flag[base + t] = (uint8_t)r;
-    if (r != REJECT && staged) ctet[base + t] = v;
-    if (r == UNDEC) push_q(q1, (uint32_t)(base + t), v.x);
+    const uint64_t bm = __ballot(r != REJECT);
+    if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+    __syncthreads();
+    uint32_t pre = run + (uint32_t)__popcll(bm & lt), tot = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      const unsigned c = s_wsum[q];
+      if (q < w) pre += c;
+      tot += c;
+    }
+    if (r != REJECT) {
+      const uint64_t at = base + pre;
+      ctet[at] = v;
+      flag[at] = (uint8_t)r;
+      if (r == UNDEC) push_q(q1, (uint32_t)at, v.x);
+    }
+    run += tot;
+    __syncthreads(); /* s_wsum is rewritten next round */
   }
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = run;
   block_flush(S.ctr, CTR_CAND + 2, found);
 }
 void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
@@ -778,20 +813,93 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
                             const uint64_t* off, uint4* ctet, uint8_t* flag,
-                            Queue q1, PipeLaunch L, uint64_t lim) {
+                            uint32_t* bcnt, Queue q1, PipeLaunch L,
+                            uint64_t lim) {
   if (nt.cap <= 0) return;
   dim3 gr((unsigned)((nt.cap + kBlock - 1) / kBlock));
   if constexpr (!kBallF64) {
     if (S.staged) {
       hipLaunchKernelGGL(k_tet_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
                          nt, trow, tk, tedge, teik, tejk, eleft, ecol, off,
-                         ctet, flag, q1, lim);
+                         ctet, flag, bcnt, q1, lim);
       return;
     }
   }
   hipLaunchKernelGGL(k_tet_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S, nt,
                      trow, tk, tedge, teik, tejk, eleft, ecol, off, ctet, flag,
-                     q1, lim);
+                     bcnt, q1, lim);
+}
+
+/* After the ladder: the kept tets of each enumeration block (its first
+ * bcnt[b] positions from off[256 b] hold its kept and formerly undecided
+ * tets), one wave per block. */
+__global__ __launch_bounds__(kBlock) void k_tet_block_keep(
+    Cnt c_nt, const uint64_t* off, const uint32_t* bcnt, const uint8_t* flag,
+    uint32_t* bkeep) {
+  const int64_t nt = c_nt.get();
+  const int64_t nblk = (c_nt.cap + kBlock - 1) / kBlock;
+  const int64_t b = gtid() >> 6;
+  const int lane = threadIdx.x & 63;
+  if (b >= nblk) return;
+  unsigned c = 0;
+  if (b * kBlock < nt) {
+    const uint64_t base = off[b * kBlock];
+    const uint32_t n = bcnt[b];
+    for (uint32_t t = lane; t < n; t += 64) c += flag[base + t] == KEEP;
+  }
+  c = wave_sum(c);
+  if (lane == 0) bkeep[b] = c;
+}
+/* the kept tets of block b to tets[bout[b], ...), in order */
+__global__ __launch_bounds__(kBlock) void k_tet_block_copy(
+    Cnt c_nt, const uint64_t* off, const uint32_t* bcnt, const uint4* ctet,
+    const uint8_t* flag, const uint64_t* bout, uint4* tets, uint64_t ocap) {
+  const int64_t nt = c_nt.get();
+  const int64_t b = blockIdx.x;
+  if (b * kBlock >= nt) return;
+  __shared__ unsigned s_wsum[kBlock / 64];
+  const uint64_t base = off[b * kBlock];
+  const uint32_t n = bcnt[b];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint64_t o = bout[b];
+  for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+    const uint32_t t = r0 + threadIdx.x;
+    const bool keep = t < n && flag[base + t] == KEEP;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (keep) v = ctet[base + t];
+    const uint64_t bm = __ballot(keep);
+    if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+    __syncthreads();
+    uint64_t pre = o + (uint64_t)__popcll(bm & lt), tot = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      const unsigned c = s_wsum[q];
+      if (q < w) pre += c;
+      tot += c;
+    }
+    if (keep && pre < ocap) tets[pre] = v;
+    o += tot;
+    __syncthreads();
+  }
+}
+void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
+                           const uint8_t* flag, uint32_t* bkeep, PipeLaunch L) {
+  if (nt.cap <= 0) return;
+  const int64_t nblk = (nt.cap + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_tet_block_keep,
+                     dim3((unsigned)((nblk * 64 + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, off, bcnt, flag, bkeep);
+}
+void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
+                           const uint4* ctet, const uint8_t* flag,
+                           const uint64_t* bout, uint4* tets, PipeLaunch L,
+                           uint64_t ocap) {
+  if (nt.cap <= 0) return;
+  hipLaunchKernelGGL(k_tet_block_copy,
+                     dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, nt, off, bcnt, ctet, flag, bout,
+                     tets, ocap);
 }
 
 template <int KIND>

@@ -375,14 +375,25 @@ void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
                      unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
 /* tet candidates of the alpha triangles, each filtered by the first
- * precision stage in the same kernel: flag per slot, (i, j, k, l) written
- * for kept / undecided slots, undecided ones queued on q1 */
+ * precision stage in the same kernel; enumeration block b (256 alpha
+ * triangles) writes its kept and undecided tets (i, j, k, l) with their
+ * flags, in order, compacted from its first slot off[256 b] on and their
+ * number to bcnt[b]; undecided ones are queued on q1 by position */
 void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* tk, const uint32_t* tedge,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
                             const uint64_t* off, uint4* ctet, uint8_t* flag,
-                            Queue q1, PipeLaunch L, uint64_t lim = ~0ull);
+                            uint32_t* bcnt, Queue q1, PipeLaunch L,
+                            uint64_t lim = ~0ull);
+/* after the ladder: kept tets per enumeration block (ceil(nt.cap / 256)
+ * entries), then, with bout their exclusive scan, the kept tets in order */
+void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
+                           const uint8_t* flag, uint32_t* bkeep, PipeLaunch L);
+void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
+                           const uint4* ctet, const uint8_t* flag,
+                           const uint64_t* bout, uint4* tets, PipeLaunch L,
+                           uint64_t ocap = ~0ull);
 /* first stage: kind 1 edges, 2 triangles (tets: launch_tet_enum_filter);
  * escalation and exact stages: kind 1, 2, 3 */
 void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,

@@ -621,25 +621,41 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint4> ctet(P, Kslots.cap);
     DBuf<uint8_t> flag(P, Kslots.cap);
     C2.ctet = ctet.get();
-    DBuf<uint32_t> pos;
-    /* enumeration and first filter stage in one kernel; a rerun (queue
-     * overflow, eager only) recounts the candidates from zero */
+    /* enumeration and first filter stage in one kernel, each block's kept
+     * and undecided tets compacted in place (its count in bcnt); the
+     * ladder settles the undecided ones; then the kept tets of each block
+     * are counted, scanned and copied out in order */
+    const int64_t nblk = (Talpha.cap + kBlock - 1) / kBlock;
+    DBuf<uint32_t> bcnt(P, nblk + 1), bkeep(P, nblk + 1);
+    DBuf<uint64_t> bout(P, nblk + 1);
+    auto tmp = scan_tmp(nblk);
+    /* a rerun (queue overflow, eager only) recounts the candidates from
+     * zero */
     int runs = 0;
-    Kalpha = filter_stage(3, C2, Kslots, nullptr, nullptr, flag.get(),
-                          KP_TET_FILL, pos, RC_KALPHA, [&](Queue qq) {
-                            if (runs++ && !rec)
-                              zero_u64(ctr + (CTR_CAND + 2) * CTR_SLOTS,
-                                       CTR_SLOTS, s);
-                            launch_tet_enum_filter(
-                                SG, Talpha, trow.get(), tk.get(), tedge.get(),
-                                teik.get(), tejk.get(), eleft.get(), ecol.get(),
-                                off.get(), ctet.get(), flag.get(), qq, L,
-                                (uint64_t)Kslots.cap);
-                          });
+    Kalpha = ladder(
+        std::min<int64_t>(Kslots.cap, Kslots.cap / 32 + 65536), KP_TET_FILL, -1,
+        [&](Queue qq) {
+          if (runs++ && !rec)
+            zero_u64(ctr + (CTR_CAND + 2) * CTR_SLOTS, CTR_SLOTS, s);
+          launch_tet_enum_filter(SG, Talpha, trow.get(), tk.get(), tedge.get(),
+                                 teik.get(), tejk.get(), eleft.get(), ecol.get(),
+                                 off.get(), ctet.get(), flag.get(), bcnt.get(),
+                                 qq, L, (uint64_t)Kslots.cap);
+        },
+        [&](QRef qr, Queue q2) {
+          launch_filter_esc(SG, 3, C2, qr, nullptr, flag.get(), q2, L);
+        },
+        [&](QRef qr) { launch_filter_exact(SG, 3, C2, qr, nullptr, flag.get(), L); },
+        [&] {
+          launch_tet_block_keep(Talpha, off.get(), bcnt.get(), flag.get(),
+                                bkeep.get(), L);
+          scan_u32_to_u64(bkeep.get(), bout.get(), Cnt(nblk), tmp.get(), 0, s,
+                          dcnt + RC_KALPHA);
+        },
+        RC_KALPHA);
     tets = DBuf<uint4>(P, Kalpha.cap);
-    launch_compact_flat(3, Kslots, C2, nullptr, nullptr, flag.get(), pos.get(),
-                        nullptr, nullptr, tets.get(), nullptr, nullptr, nullptr,
-                        nullptr, L, (uint64_t)Kalpha.cap);
+    launch_tet_block_copy(Talpha, off.get(), bcnt.get(), ctet.get(), flag.get(),
+                          bout.get(), tets.get(), L, (uint64_t)Kalpha.cap);
     KCHK();
   }
   T.mark(5);
