@@ -142,15 +142,16 @@ int pac_compute_noalloc(const float* xyzr, int64_t n, float solvent,
 /* counts[0..3] = vertices, edges, triangles, tetrahedra
  * (AlphaComplexParallel.cu:1809-1816). */
 int pac_result_counts(const pac_result* r, int64_t counts[4]);
-/* Copy simplex lists (row-major, 2/3/4 int32 per row) to host memory
- * (AlphaComplexParallel.cu:1794-1806 + the id mapping at :1842-1856). */
+/* Copy simplex lists (row-major, 2/3/4 int32 per row) to host or device
+ * memory (AlphaComplexParallel.cu:1794-1806 + the id mapping at
+ * :1842-1856). */
 int pac_result_copy_edges(const pac_result* r, int32_t* dst);
 int pac_result_copy_triangles(const pac_result* r, int32_t* dst);
 int pac_result_copy_tets(const pac_result* r, int32_t* dst);
 /* Device pointer of the list for dim 1/2/3 (valid until pac_result_free). */
 const int32_t* pac_result_device_list(const pac_result* r, int dim);
 /* FP64 alpha values (squared orthoradius rho) of the output simplices of
- * dim 1/2/3, in list order.  The reference computes them for triangles and
+ * dim 1/2/3, in list order, to host or device memory.  The reference computes them for triangles and
  * tets (float4 .w, AlphaComplexParallel.cu:370, :603) and discards them. */
 int pac_result_copy_alpha(const pac_result* r, int dim, double* rho);
 int pac_result_stats(const pac_result* r, pac_stats* st);

@@ -671,8 +671,8 @@ static int copy_list(const pac_result* r, int dim, int32_t* dst) {
     size_t bytes = (size_t)r->counts[dim] * (dim + 1) * sizeof(int32_t);
     if (!bytes) return;
     HIPCHK(hipSetDevice(r->eng->device));
-    HIPCHK(hipMemcpy(dst, r->lists[dim - 1].get(), bytes,
-                     hipMemcpyDeviceToHost));
+    /* host or device destination (unified addressing picks the copy) */
+    HIPCHK(hipMemcpy(dst, r->lists[dim - 1].get(), bytes, hipMemcpyDefault));
   });
 }
 int pac_result_copy_edges(const pac_result* r, int32_t* dst) {
@@ -696,12 +696,18 @@ int pac_result_copy_alpha(const pac_result* r, int dim, double* rho) {
     Engine* E = r->eng;
     std::lock_guard<std::mutex> lk(E->run_mu);
     HIPCHK(hipSetDevice(E->device));
-    DBuf<double> d(&E->pool, m);
-    launch_rho(dim, m, r->lists[dim - 1].get(), r->balls.get(), d.get(),
-               E->stream);
-    KCHK();
-    HIPCHK(hipMemcpyAsync(rho, d.get(), (size_t)m * sizeof(double),
-                          hipMemcpyDeviceToHost, E->stream));
+    if (is_device_ptr(rho)) { /* device destination: written in place */
+      launch_rho(dim, m, r->lists[dim - 1].get(), r->balls.get(), rho,
+                 E->stream);
+      KCHK();
+    } else {
+      DBuf<double> d(&E->pool, m);
+      launch_rho(dim, m, r->lists[dim - 1].get(), r->balls.get(), d.get(),
+                 E->stream);
+      KCHK();
+      HIPCHK(hipMemcpyAsync(rho, d.get(), (size_t)m * sizeof(double),
+                            hipMemcpyDeviceToHost, E->stream));
+    }
     HIPCHK(hipStreamSynchronize(E->stream));
   });
 }

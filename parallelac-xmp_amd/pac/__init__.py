@@ -275,6 +275,27 @@ class Result:
         _check(lib().pac_result_copy_alpha(self._h, dim, rho.ctypes.data))
         return rho
 
+    def copy_list_to(self, dim, ptr):
+        """copy the dim-1/2/3 list into device or host memory at ptr
+        (counts()[dim] x (dim+1) int32)"""
+        fns = (lib().pac_result_copy_edges, lib().pac_result_copy_triangles,
+               lib().pac_result_copy_tets)
+        if int(self.counts()[dim]):
+            _check(fns[dim - 1](self._h, ctypes.c_void_p(ptr)))
+
+    def copy_alpha_to(self, dim, ptr):
+        """rho of the dim-1/2/3 simplices into device or host memory at ptr
+        (counts()[dim] float64)"""
+        if int(self.counts()[dim]):
+            _check(lib().pac_result_copy_alpha(self._h, dim, ctypes.c_void_p(ptr)))
+
+    def adjoint_to(self, g_tri_ptr, g_tet_ptr, dxyz_ptr, dr_ptr):
+        """pac_alpha_adjoint on raw (host or device) pointers; a gradient
+        pointer may be None (zero)"""
+        vp = ctypes.c_void_p
+        _check(lib().pac_alpha_adjoint(self._h, vp(g_tri_ptr), vp(g_tet_ptr),
+                                       vp(dxyz_ptr), vp(dr_ptr)))
+
     def births(self, dim):
         """Filtration values of the simplices of dim 0..3 (needs
         filtration=True at compute time)."""
@@ -320,6 +341,15 @@ def compute_device(ptr, n, solvent, alpha, **kw):
     o = options(device_inputs=True, **kw)
     _check(lib().pac_compute(ctypes.c_void_p(ptr), n, solvent, alpha,
                              ctypes.byref(o), ctypes.byref(h)))
+    return Result(h.value)
+
+
+def compute_device_f64(ptr, n, solvent, alpha, **kw):
+    """Run on a device-resident FP64 n x 4 array."""
+    h = ctypes.c_void_p()
+    o = options(device_inputs=True, **kw)
+    _check(lib().pac_compute_f64(ctypes.c_void_p(ptr), n, solvent, alpha,
+                                 ctypes.byref(o), ctypes.byref(h)))
     return Result(h.value)
 
 

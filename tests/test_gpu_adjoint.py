@@ -114,6 +114,42 @@ def test_torch_autograd():
     assert np.abs(gx[:, 3] - dr).max() <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_torch_autograd_device_tensors(dtype):
+    """a GPU tensor stays on the device: the lists, rho and the gradient
+    come back as device tensors equal to the host path's (same kernels,
+    same order: bit-exact)"""
+    import torch
+    from pac.torch_ops import alpha_values
+    _say("case: torch autograd on device tensors (%s)" % dtype)
+    if dtype == "float64":
+        b = pac.synth_globule_f64(2500, None, 1.3, seed=8, sigma=0.01)
+        res = pac.compute_f64(b, 1.4, 0.0)
+    else:
+        b = pac.synth_globule(2500, None, 1.3, seed=8)
+        res = pac.compute(b, 1.4, 0.0)
+    x = torch.tensor(b, device="cuda:0", requires_grad=True)
+    rho_tri, rho_tet, tris, tets = alpha_values(x, 1.4, 0.0)
+    for t in (rho_tri, rho_tet, tris, tets):
+        assert t.is_cuda
+    L = res.lists()
+    assert np.array_equal(tris.cpu().numpy(), L[2])
+    assert np.array_equal(tets.cpu().numpy(), L[3])
+    assert np.array_equal(rho_tri.detach().cpu().numpy(), res.alpha_values(2))
+    assert np.array_equal(rho_tet.detach().cpu().numpy(), res.alpha_values(3))
+    loss = rho_tri.sum() + (rho_tet ** 2).sum()
+    loss.backward()
+    assert x.grad.is_cuda
+    dxyz, dr = res.adjoint(np.ones(len(L[2])), 2.0 * rho_tet.detach().cpu().numpy())
+    gx = x.grad.double().cpu().numpy()
+    if dtype == "float64":
+        assert np.array_equal(gx[:, :3], dxyz) and np.array_equal(gx[:, 3], dr)
+    else:
+        scale = max(np.abs(dxyz).max(), 1.0)
+        assert np.abs(gx[:, :3] - dxyz).max() <= 1e-5 * scale  # grad stored in FP32
+        assert np.abs(gx[:, 3] - dr).max() <= 1e-5 * scale
+
+
 def test_adjoint_c4_fp64():
     """the differentiation workload: 20,000 balls, continuous FP64 centres
     with N(0, 0.01 A) jitter, solvent 1.4, alpha 0; FD on 256 coordinates.
