@@ -113,6 +113,12 @@ def main():
                     help="emptiness scan grid x split (with --budget and --grid)")
     ap.add_argument("--quick", type=int, default=0,
                     help="quick rejection pass (with --budget and --grid)")
+    ap.add_argument("--ac2-block", type=int, default=0,
+                    help="threads per block of the persistent second "
+                         "emptiness phase (with --budget and --grid)")
+    ap.add_argument("--ac2-chunk", type=int, default=0,
+                    help="queue granularity of the persistent second "
+                         "emptiness phase (with --budget and --grid)")
     ap.add_argument("--chunk", type=int, default=100000,
                     help="stress workload: also run the chunked (LD) driver "
                          "with this chunk_balls (0 = skip)")
@@ -155,7 +161,8 @@ def main():
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
         if args.budget and args.grid:
             tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
-                           "ac2_fine": args.fine, "ac2_quick": args.quick}
+                           "ac2_fine": args.fine, "ac2_quick": args.quick,
+                           "block": args.ac2_block, "ac2_chunk": args.ac2_chunk}
         else:
             tuned[prec], _ = pac.autotune(ptr, solvent, alpha, device_inputs=True,
                                           n=n, precision=prec)
@@ -488,7 +495,8 @@ def bench_stress(args, torch, pac):
     for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
         if args.budget and args.grid:
             tuned[prec] = {"ac2_budget": args.budget, "ac2_grid": args.grid,
-                           "ac2_fine": args.fine, "ac2_quick": args.quick}
+                           "ac2_fine": args.fine, "ac2_quick": args.quick,
+                           "block": args.ac2_block, "ac2_chunk": args.ac2_chunk}
         else:
             tuned[prec], _ = pac.autotune(sample, 1.4, 0.0, precision=prec)
     log("stress: settings %s" % tuned)

@@ -47,7 +47,8 @@ typedef struct pac_options {
   int32_t device_inputs; /* 1: xyzr is a device pointer (e.g. torch)      */
   int32_t device;        /* HIP device ordinal                            */
   void* stream;          /* hipStream_t to run on, NULL = library stream  */
-  int32_t block;         /* threads per block override (0 = tuned)        */
+  int32_t block;         /* threads per block of the persistent second
+                            emptiness phase: 64, 128 or 256 (0 = 256)     */
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
   int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
   int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
@@ -69,6 +70,10 @@ typedef struct pac_options {
                             pac_result_copy_filtration).  Reference row
                             order, with the tet emptiness test;
                             pac_compute / pac_compute_f64 only.           */
+  int32_t ac2_chunk;     /* entries of the left-over list each wave of the
+                            persistent second emptiness phase takes from
+                            its work counter at a time (rounded up to a
+                            multiple of 64; 0 = 64)                       */
 } pac_options;
 
 typedef struct pac_result pac_result;
@@ -228,7 +233,9 @@ void pac_graph_free(pac_graph* g);
  * reference grid, tried first), opt->ac2_quick (with a fine grid: the
  * quick rejection pass on or off), then opt->ac2_budget / opt->ac2_grid
  * (emptiness phase-one pair budget x second-phase grid, over the chosen
- * scan grid); the other fields of *opt are used as given.  offsets ==
+ * scan grid), then opt->block / opt->ac2_chunk (block shape and queue
+ * granularity of the persistent second phase); the other fields of *opt
+ * are used as given.  offsets ==
  * NULL: the n balls are one molecule; otherwise they are a batch
  * (offsets[n_mol+1]) and the leading molecules, up to about 2^20 balls,
  * are the sample.  best_ms (may be NULL) receives the fastest wall time

@@ -158,9 +158,10 @@ __device__ __forceinline__ bool tet_setup(const GridP& g, const Ball v[4],
                                           PairSS<R>& P, Box& box) {
   TetCert<R> T;
   if (!tet_pairs(v, P, T)) return false;
-  double c[3], e[3], rho_up;
-  if (!centre_err(T.D, T.N, v[0], T.wa, c, e, &rho_up)) return false;
-  return place_box(g, simplex_reach(g, rho_up), c, e, box);
+  /* centre, reach and box in the stage's own precision */
+  R c[3], e[3], rho_up;
+  if (!centre_err<R>(T.D, T.N, v[0], T.wa, c, e, &rho_up)) return false;
+  return place_box<R>(g, simplex_reach<R>(g, rho_up), c, e, box);
 }
 
 /* Quick rejection before the certified box is built: the balls of the
@@ -281,9 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
    * pass left); phase B: `rest` */
   int64_t total = PHASE_B ? (int64_t)*nrest
                           : (items ? (int64_t)*nitems : c_nk.get());
-  int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
-  for (int64_t w = gtid(); w < total; w += stride) {
+  auto item = [&](int64_t w) {
     int64_t id = PHASE_B ? (int64_t)rest[w] : (items ? (int64_t)items[w] : w);
     uint4 t = tets[id];
     Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         push_q(q1, (uint32_t)id, 0xffffffffu);
         flag[id] = 2;
       }
-      continue;
+      return;
     }
     /* why the scan stopped early: 1 a violator, 2 the budget (one
      * variable: two flags set in different exits were kept in scratch
@@ -345,6 +345,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       flag[id] = stop == 1 ? 0 : (pending ? 2 : 1);
       if (stop == 2) rest_id = (uint32_t)id;
     }
+  };
+  if constexpr (PHASE_B) {
+    for_work(S.bwork, S.bchunk, total, item); /* persistent work queue */
+  } else {
+    if (gtid() < total) item(gtid());
   }
   if (!PHASE_B) {
     unsigned at = block_append(nrest, rest_id != 0xffffffffu);
@@ -609,9 +614,9 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
   V3<FE<R>> pa{FE<R>{ax, (R)0}, FE<R>{ay, (R)0}, FE<R>{az, (R)0}};
   FE<R> waf{wa, FEc<R>::u() * wa * FEc<R>::slack()};
   if (!pairss_init(P, pa, waf, den, num) || P.sden < 0) return false;
-  double c[3], e[3], rho_up;
-  if (!centre_err(den, num, v[0], waf, c, e, &rho_up)) return false;
-  return place_box(g, simplex_reach(g, rho_up), c, e, box);
+  R c[3], e[3], rho_up;
+  if (!centre_err<R>(den, num, v[0], waf, c, e, &rho_up)) return false;
+  return place_box<R>(g, simplex_reach<R>(g, rho_up), c, e, box);
 }
 
 /* KIND 0: triangles (items = alpha triangle ids), 1: edges (alpha edge ids).
@@ -624,13 +629,12 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
   int64_t total = PHASE_B ? (int64_t)*nrest : c_nitems.get();
-  int64_t stride = PHASE_B ? (int64_t)gridDim.x * blockDim.x : total + 1;
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
-  for (int64_t w = gtid(); w < total; w += stride) {
+  auto item = [&](int64_t w) {
     int64_t id = PHASE_B ? (int64_t)rest[w] : w;
     if (!PHASE_B && in_flag[id]) {
       flag[id] = 1;
-      continue;
+      return;
     }
     uint32_t vid[3];
     if (KIND == 0) {
@@ -652,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
         push_q(q1, (uint32_t)id, 0xffffffffu);
         flag[id] = 2;
       }
-      continue;
+      return;
     }
     bool pending = false;
     int stop = 0; /* 1 a violator, 2 the budget (as in k_tet_ac2_hot) */
@@ -698,6 +702,11 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       flag[id] = stop == 1 ? 0 : (pending ? 2 : 1);
       if (stop == 2) rest_id = (uint32_t)id;
     }
+  };
+  if constexpr (PHASE_B) {
+    for_work(S.bwork, S.bchunk, total, item); /* persistent work queue */
+  } else {
+    if (gtid() < total) item(gtid());
   }
   if (!PHASE_B) {
     unsigned at = block_append(nrest, rest_id != 0xffffffffu);
@@ -907,10 +916,12 @@ static void tet_hot3(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
   hipLaunchKernelGGL((k_tet_ac2_hot<R, false, FINE, MOL>), gr, dim3(kBlock), 0,
                      L.s, S, nk, items, nnext, tets, flag, q1, rest, nrest,
                      budget);
-  if (budget > 0)
-    hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE, MOL>), gb, dim3(kBlock), 0,
-                       L.s, S, nk, items, nnext, tets, flag, q1, rest, nrest,
-                       budget);
+  if (budget > 0) {
+    zero_u32(S.bwork, 1, L.s);
+    hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE, MOL>), gb,
+                       dim3((unsigned)S.bblock), 0, L.s, S, nk, items, nnext,
+                       tets, flag, q1, rest, nrest, budget);
+  }
 }
 template <class R, bool FINE>
 static void tet_hot2(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
@@ -992,10 +1003,12 @@ static void low_hot4(const Stage& S, Cnt nitems, const uint32_t* ecol,
   hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, false, FINE, MOL>), gr,
                      dim3(kBlock), 0, L.s, S, nitems, ecol, eleft, tk, tedge,
                      in_flag, flag, q1, rest, nrest, budget);
-  if (budget > 0)
+  if (budget > 0) {
+    zero_u32(S.bwork, 1, L.s);
     hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true, FINE, MOL>), gb,
-                       dim3(kBlock), 0, L.s, S, nitems, ecol, eleft, tk, tedge,
-                       in_flag, flag, q1, rest, nrest, budget);
+                       dim3((unsigned)S.bblock), 0, L.s, S, nitems, ecol, eleft,
+                       tk, tedge, in_flag, flag, q1, rest, nrest, budget);
+  }
 }
 template <class R, int KIND, bool FINE>
 static void low_hot3(const Stage& S, Cnt nitems, const uint32_t* ecol,

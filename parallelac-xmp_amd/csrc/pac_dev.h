@@ -29,7 +29,8 @@ __device__ __forceinline__ void flush_ctr(unsigned long long* ctr, int idx,
               (unsigned long long)v);
 }
 /* Block-level versions (every thread of the block must call them): one
- * global atomic per block instead of one per wave. */
+ * global atomic per block instead of one per wave.  block_flush takes any
+ * block of up to kBlock threads in whole waves. */
 __device__ __forceinline__ void block_flush(unsigned long long* ctr, int idx,
                                             unsigned v) {
   __shared__ unsigned s_part[kBlock / 64];
@@ -38,7 +39,8 @@ __device__ __forceinline__ void block_flush(unsigned long long* ctr, int idx,
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned t = 0;
-    for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+    const int waves = (int)(blockDim.x >> 6);
+    for (int w = 0; w < waves; ++w) t += s_part[w];
     if (t)
       atomicAdd(&ctr[idx * CTR_SLOTS + (blockIdx.x & (CTR_SLOTS - 1))],
                 (unsigned long long)t);
@@ -145,6 +147,24 @@ struct StageCnt {
 
 __device__ __forceinline__ int64_t gtid() {
   return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+/* Persistent work queue over [0, total): each wave of the grid takes
+ * `chunk` consecutive entries (a multiple of 64) at a time from *work
+ * (cleared before the launch), each lane every 64th entry of its chunk,
+ * until none are left.  The chunk loop is wave-uniform. */
+template <class F>
+__device__ __forceinline__ void for_work(unsigned* work, unsigned chunk,
+                                         int64_t total, F&& body) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(work, chunk);
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if ((int64_t)base >= total) break;
+    const int64_t end =
+        (int64_t)base + chunk < total ? (int64_t)base + chunk : total;
+    for (int64_t w = (int64_t)base + lane; w < end; w += 64) body(w);
+  }
 }
 __device__ __forceinline__ bool same_pos(const Ball& a, const Ball& b) {
   return a.x == b.x && a.y == b.y && a.z == b.z;
@@ -271,44 +291,61 @@ struct CellBox {
 __device__ __forceinline__ GridP grid_of(const Stage& S, uint32_t b) {
   return S.mol ? S.grids[S.mol[b]] : S.g;
 }
-__device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
-                                           double cx, double cy, double cz,
-                                           double ex, double ey, double ez,
-                                           CellBox& b);
+/* Rounding allowances of the box computations in the computing type C:
+ * FP64 for the plain path and the escalation / exact kernels, FP32 for the
+ * staged path's hot kernels.  `cell`: cells added on either side of a
+ * box edge (relative to |index| + 1); `quot`: relative rounding of a
+ * computed quotient; `add`: relative rounding of the centre's final
+ * addition; `rel`: relative allowance on a certified upper bound. */
+template <class C>
+struct BoxEps;
+template <>
+struct BoxEps<double> {
+  static constexpr double cell = 1e-9, quot = 1e-14, add = 0.0, rel = 1e-12;
+};
+template <>
+struct BoxEps<float> {
+  static constexpr float cell = 1e-4f, quot = 1e-6f, add = 2.4e-7f, rel = 1e-5f;
+};
+template <class C>
+__device__ __forceinline__ bool make_box_r(const GridP& g, C reach, C cx, C cy,
+                                           C cz, C ex, C ey, C ez, CellBox& b);
 __device__ __forceinline__ bool make_box(const GridP& g, double cx, double cy,
                                          double cz, double ex, double ey,
                                          double ez, CellBox& b) {
-  return make_box_r(g, g.reach, cx, cy, cz, ex, ey, ez, b);
+  return make_box_r<double>(g, g.reach, cx, cy, cz, ex, ey, ez, b);
 }
 /* A simplex with power-centre c and alpha value rho is violated only by
  * balls with |p_e - c|^2 < rho + w_e <= rho + max w, so its own reach
  * sqrt(rho + max w) (rho certified from above) can replace the global
  * one (exact cell policy; the reference's stencil keeps its 27 cells). */
-__device__ __forceinline__ double simplex_reach(const GridP& g, double rho_up) {
-  if (!(g.maxw >= 0)) return g.reach;
-  const double r2 = rho_up + g.maxw;
-  const double r = r2 > 0 ? sqrt(r2) * (1 + 1e-12) + 1e-12 : 1e-12;
-  return r < g.reach ? r : g.reach;
+template <class C>
+__device__ __forceinline__ C simplex_reach(const GridP& g, C rho_up) {
+  const C full = (C)g.reach * (1 + BoxEps<C>::rel);
+  if (!(g.maxw >= 0)) return full;
+  const C e = BoxEps<C>::rel;
+  const C r2 = rho_up + (C)g.maxw * (1 + e);
+  const C r = r2 > 0 ? sqrt(r2) * (1 + e) + e : e;
+  return r < full ? r : full;
 }
-__device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
-                                           double cx, double cy, double cz,
-                                           double ex, double ey, double ez,
-                                           CellBox& b) {
-  const double margin = g.margin;
-  double c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
-  double mn[3] = {(double)g.minx, (double)g.miny, (double)g.minz};
+template <class C>
+__device__ __forceinline__ bool make_box_r(const GridP& g, C reach, C cx, C cy,
+                                           C cz, C ex, C ey, C ez, CellBox& b) {
+  const C margin = (C)g.margin, eps = BoxEps<C>::cell;
+  C c[3] = {cx, cy, cz}, e[3] = {ex, ey, ez};
+  C mn[3] = {(C)g.minx, (C)g.miny, (C)g.minz};
   int ext[3] = {g.ex, g.ey, g.ez};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double inv = a == 0 ? g.inv_step : g.inv_yz;
-    double r = reach + margin + e[a];
-    double ql = (c[a] - r - mn[a]) * inv, qh = (c[a] + r - mn[a]) * inv;
-    double qc = (c[a] - mn[a]) * inv;
-    if (!(ql == ql) || !(qh == qh) || !(r < 1e30)) return false;
-    double lo = floor(ql - 1e-9 * (fabs(ql) + 1.0));
-    double hi = floor(qh + 1e-9 * (fabs(qh) + 1.0));
+    const C inv = (C)(a == 0 ? g.inv_step : g.inv_yz);
+    C r = reach + margin + e[a];
+    C ql = (c[a] - r - mn[a]) * inv, qh = (c[a] + r - mn[a]) * inv;
+    C qc = (c[a] - mn[a]) * inv;
+    if (!(ql == ql) || !(qh == qh) || !(r < (C)1e30)) return false;
+    C lo = floor(ql - eps * (fabs(ql) + 1));
+    C hi = floor(qh + eps * (fabs(qh) + 1));
     if (lo < 0) lo = 0;
-    if (hi > ext[a] - 1) hi = ext[a] - 1;
+    if (hi > (C)(ext[a] - 1)) hi = (C)(ext[a] - 1);
     if (lo > hi) {
       b.lo[a] = 1;
       b.hi[a] = 0;
@@ -317,7 +354,7 @@ __device__ __forceinline__ bool make_box_r(const GridP& g, double reach,
     }
     b.lo[a] = (int)lo;
     b.hi[a] = (int)hi;
-    double f = floor(qc);
+    C f = floor(qc);
     b.cc[a] = f < lo ? (int)lo : (f > hi ? (int)hi : (int)f);
   }
   return true;
@@ -453,17 +490,17 @@ __device__ __forceinline__ int clamp_cell(float f, int lo, int hi) {
   const int c = lo + (int)q;
   return c < lo ? lo : (c > hi ? hi : c);
 }
-__device__ __forceinline__ bool make_sph(const GridP& g, double reach,
-                                         const double c[3], const double e[3],
-                                         SphBox& s) {
-  if (!make_box_r(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], s.b))
+template <class C>
+__device__ __forceinline__ bool make_sph(const GridP& g, C reach, const C c[3],
+                                         const C e[3], SphBox& s) {
+  if (!make_box_r<C>(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], s.b))
     return false;
-  const double iyz = g.inv_yz;
-  const double el = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-  const double R = (reach + g.margin + el) * iyz * (1 + 1e-6) + 1e-3;
-  s.fx = (float)((c[0] - (double)g.minx) * g.inv_step - s.b.lo[0]);
-  s.fy = (float)((c[1] - (double)g.miny) * iyz - s.b.lo[1]);
-  s.fz = (float)((c[2] - (double)g.minz) * iyz - s.b.lo[2]);
+  const C iyz = (C)g.inv_yz;
+  const C el = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  const C R = (reach + (C)g.margin + el) * iyz * (C)(1 + 1e-5) + (C)1e-3;
+  s.fx = (float)((c[0] - (C)g.minx) * (C)g.inv_step - (C)s.b.lo[0]);
+  s.fy = (float)((c[1] - (C)g.miny) * iyz - (C)s.b.lo[1]);
+  s.fz = (float)((c[2] - (C)g.minz) * iyz - (C)s.b.lo[2]);
   s.R2 = (float)(R * R) * 1.0001f;
   s.kx = g.xsplit * 1.0001f;
   s.cy = clamp_cell(s.fy, s.b.lo[1], s.b.hi[1]);
@@ -526,15 +563,15 @@ __device__ __forceinline__ void scan_sph4(const GridP& g,
   while (sph_next(g, fstart, s, it, lo, hi))
     if (scan_range4(fball, lo, hi, visit)) return;
 }
-__device__ __forceinline__ bool place_box(const GridP& g, double reach,
-                                          const double c[3], const double e[3],
-                                          CellBox& b) {
-  return make_box_r(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], b);
+template <class C>
+__device__ __forceinline__ bool place_box(const GridP& g, C reach, const C c[3],
+                                          const C e[3], CellBox& b) {
+  return make_box_r<C>(g, reach, c[0], c[1], c[2], e[0], e[1], e[2], b);
 }
-__device__ __forceinline__ bool place_box(const GridP& g, double reach,
-                                          const double c[3], const double e[3],
-                                          SphBox& b) {
-  return make_sph(g, reach, c, e, b);
+template <class C>
+__device__ __forceinline__ bool place_box(const GridP& g, C reach, const C c[3],
+                                          const C e[3], SphBox& b) {
+  return make_sph<C>(g, reach, c, e, b);
 }
 __device__ __forceinline__ GridP sgrid_of(const Stage& S, uint32_t b) {
   return S.sgrids ? S.sgrids[S.mol[b]] : S.sg;
@@ -560,28 +597,29 @@ __device__ __forceinline__ GridP sgrid_at(const Stage& S, uint32_t b) {
  * |num/den - num'/den'| <= (eN + |N| eD / (D - eD)) / (D - eD), which is
  * at least quot_err's bound; 1e-14 |N| / (D - eD) covers the rounding of
  * the computed quotient.  Returns false when den is not certified. */
-template <class R>
+template <class C, class R>
 __device__ __forceinline__ bool centre_err(const FE<R>& den,
                                            const V3<FE<R>>& num, const Ball& p0,
-                                           const FE<R>& wa, double c[3],
-                                           double e[3], double* rho_up) {
-  const double D = fabs((double)den.v), eD = (double)den.e;
-  if (!(D > eD * 1.01)) return false;
-  const double inv2 = 0.5 / (double)den.v, im = 0.5 / (D - eD);
-  const double n[3] = {(double)num.x.v, (double)num.y.v, (double)num.z.v};
-  const double en[3] = {(double)num.x.e, (double)num.y.e, (double)num.z.e};
-  const double p[3] = {(double)p0.x, (double)p0.y, (double)p0.z};
-  double nn = 0;
+                                           const FE<R>& wa, C c[3], C e[3],
+                                           C* rho_up) {
+  const C rel = BoxEps<C>::rel;
+  const C D = fabs((C)den.v), eD = (C)den.e;
+  if (!(D > eD * (C)1.01)) return false;
+  const C inv2 = (C)0.5 / (C)den.v, im = (C)0.5 / (D - eD) * (1 + rel);
+  const C n[3] = {(C)num.x.v, (C)num.y.v, (C)num.z.v};
+  const C en[3] = {(C)num.x.e, (C)num.y.e, (C)num.z.e};
+  const C p[3] = {(C)p0.x, (C)p0.y, (C)p0.z};
+  C nn = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     c[a] = p[a] + n[a] * inv2;
-    e[a] = (en[a] + fabs(n[a]) * eD * (2.0 * im)) * im * 1.001 +
-           fabs(n[a]) * im * 1e-14;
-    const double up = (fabs(n[a]) + en[a]) * im; /* >= |x_a| */
+    e[a] = (en[a] + fabs(n[a]) * eD * (2 * im)) * im * (C)1.001 +
+           fabs(n[a]) * im * BoxEps<C>::quot + fabs(c[a]) * BoxEps<C>::add;
+    const C up = (fabs(n[a]) + en[a]) * im; /* >= |x_a| */
     nn += up * up;
   }
   /* rho = |x|^2 - w_a with x = num / (2 den) */
-  *rho_up = nn * (1 + 1e-12) - ((double)wa.v - (double)wa.e) * (1 - 1e-12);
+  *rho_up = nn * (1 + rel) - ((C)wa.v - (C)wa.e) * (1 - rel);
   return true;
 }
 

@@ -519,10 +519,30 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
         pick = t;
       }
     }
+  /* block shape and queue granularity of the persistent second phase
+   * (with a phase-one budget: no second phase otherwise) */
+  if (pick.ac2_budget >= 0) {
+    static const int blocks[] = {64, 128, 256};
+    static const int chunks[] = {64, 256, 1024};
+    const pac_options tuned = pick;
+    for (int bl : blocks)
+      for (int ch : chunks) {
+        pac_options t = tuned;
+        t.block = bl;
+        t.ac2_chunk = ch;
+        const double tm = time_one(t);
+        if (tm < best) {
+          best = tm;
+          pick = t;
+        }
+      }
+  }
   o->ac2_budget = pick.ac2_budget;
   o->ac2_grid = pick.ac2_grid;
   o->ac2_fine = pick.ac2_fine;
   o->ac2_quick = pick.ac2_quick;
+  o->block = pick.block;
+  o->ac2_chunk = pick.ac2_chunk;
   if (best_ms) *best_ms = best;
 }
 

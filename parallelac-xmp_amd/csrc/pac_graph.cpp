@@ -106,7 +106,7 @@ int64_t with_headroom(int64_t v, double h) {
 void record_all(pac_graph* G) {
   hipStream_t s = G->s;
   zero_u32(G->status.get(), 4, s);
-  zero_u32(G->qn.get(), 2 * QSLOTS + 2, s); /* then kept clear by the views */
+  zero_u32(G->qn.get(), QN_WORDS, s); /* then kept clear by the views */
   zero_u64(G->ctr.get(), CTR_COUNT * CTR_SLOTS, s);
   Timer T(s);
   T.on = false;
@@ -267,7 +267,7 @@ void graph_create(const float* xyzr, const int64_t* offsets, int64_t nmol,
       std::max(with_headroom(fill2, headroom) + 256, (int64_t)G->c1 + 512), qmax);
   G->status = DBuf<unsigned>(&G->pool, 4);
   G->ctr = DBuf<unsigned long long>(&G->pool, CTR_COUNT * CTR_SLOTS);
-  G->qn = DBuf<unsigned>(&G->pool, 2 * QSLOTS + 2);
+  G->qn = DBuf<unsigned>(&G->pool, QN_WORDS);
   G->views = DBuf<QView>(&G->pool, 2);
   G->q1 = DBuf<QEntry>(&G->pool, (size_t)G->c1 * QSLOTS);
   G->q2 = DBuf<QEntry>(&G->pool, (size_t)G->c2 * QSLOTS);

@@ -404,7 +404,7 @@ struct PassRec {
   unsigned* status;        /* device: REC_* bits */
   unsigned* fill;          /* device [2]: largest sub-queue fills */
   unsigned long long* ctr; /* device counters (shared by the passes) */
-  unsigned* qn;            /* device [2 QSLOTS + 2] queue counters */
+  unsigned* qn;            /* device [QN_WORDS] queue counters */
   QView* views;            /* device [2] */
   QEntry *q1, *q2;         /* hot / escalation queues */
   uint32_t c1, c2;         /* their sub-queue capacities */

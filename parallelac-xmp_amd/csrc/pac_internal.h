@@ -137,6 +137,11 @@ struct PipeLaunch {
  * that many simultaneous appends do not serialize on one counter; sub-queue
  * k holds entries [k*cap, k*cap + n[k]). */
 constexpr int QSLOTS = 32;
+/* queue counter words of a run: the hot and escalation sub-queue counters
+ * (2 QSLOTS), then the count of simplices left for the second emptiness
+ * phase, the count of tets a quick rejection pass left, and the work
+ * counter of the persistent second phase */
+constexpr int QN_WORDS = 2 * QSLOTS + 3;
 struct Queue {
   QEntry* q;
   unsigned* n;   /* QSLOTS counters */
@@ -328,7 +333,13 @@ struct Stage {
   ExactScale ex;
   unsigned long long* ctr;
   int staged;
-  int bgrid;     /* blocks of the second emptiness phase */
+  /* second emptiness phase: a persistent work-queue launch of bgrid blocks
+   * of bblock threads whose waves take bchunk entries of the left-over
+   * list at a time from the counter *bwork (cleared before each launch) */
+  int bgrid;
+  int bblock;
+  unsigned bchunk;
+  unsigned* bwork;
   /* exact cell policy: the emptiness scans run over a finer grid of their
    * own (reference cells split along x), its balls in fine-cell order */
   int fine;

@@ -299,6 +299,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* FP64 balls are not exact in FP32: their ladder starts at FP64 */
   SG.staged = opt.precision == PAC_PREC_STAGED && !kBallF64;
   SG.bgrid = opt.ac2_grid > 0 ? std::min(opt.ac2_grid, 1 << 16) : 2048;
+  /* persistent second emptiness phase: block shape and hand-out size */
+  SG.bblock = opt.block == 64 || opt.block == 128 ? opt.block : kBlock;
+  SG.bchunk = opt.ac2_chunk > 0
+                  ? (unsigned)std::min((opt.ac2_chunk + 63) / 64 * 64, 1 << 16)
+                  : 64u;
   if (sub) {
     SG.chunk = 1;
     SG.own_lo = sub->own_lo;
@@ -318,11 +323,12 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
    * of tets a quick rejection pass left */
   DBuf<unsigned> qn_own;
   if (!rec) {
-    qn_own = DBuf<unsigned>(P, 2 * QSLOTS + 2);
-    zero_u32(qn_own.get(), 2 * QSLOTS + 2, s); /* then kept clear by the views */
+    qn_own = DBuf<unsigned>(P, QN_WORDS);
+    zero_u32(qn_own.get(), QN_WORDS, s); /* then kept clear by the views */
   }
   unsigned* const qn = rec ? rec->qn : qn_own.get();
   unsigned* const nrest = qn + 2 * QSLOTS;
+  SG.bwork = qn + 2 * QSLOTS + 2;
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);

@@ -44,7 +44,8 @@ class Options(ctypes.Structure):
                 ("profile", ctypes.c_int32), ("ac2_budget", ctypes.c_int32),
                 ("batch_balls", ctypes.c_int32), ("ac2_grid", ctypes.c_int32),
                 ("ac2_fine", ctypes.c_int32), ("ac2_quick", ctypes.c_int32),
-                ("chunk_balls", ctypes.c_int32), ("filtration", ctypes.c_int32)]
+                ("chunk_balls", ctypes.c_int32), ("filtration", ctypes.c_int32),
+                ("ac2_chunk", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -188,7 +189,7 @@ def _check(rc):
 def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
             order=ORDER_REFERENCE, device_inputs=False, device=0, stream=None,
             profile=False, ac2_budget=0, batch_balls=0, ac2_grid=0, ac2_fine=0,
-            ac2_quick=0, chunk_balls=0, filtration=False):
+            ac2_quick=0, chunk_balls=0, filtration=False, block=0, ac2_chunk=0):
     o = Options()
     lib().pac_options_init(ctypes.byref(o))
     o.cell_policy = cell_policy
@@ -206,6 +207,8 @@ def options(cell_policy=CELL_EXACT, precision=PREC_STAGED, skip_tet_ac2=False,
     o.ac2_quick = int(ac2_quick)
     o.chunk_balls = int(chunk_balls)
     o.filtration = int(bool(filtration))
+    o.block = int(block)
+    o.ac2_chunk = int(ac2_chunk)
     return o
 
 
@@ -472,7 +475,8 @@ def autotune(xyzr, solvent, alpha, offsets=None, **kw):
         _check(lib().pac_autotune(ptr, int(off[-1]), off.ctypes.data, off.shape[0] - 1,
                                   solvent, alpha, ctypes.byref(o), ctypes.byref(best)))
     return ({"ac2_budget": o.ac2_budget, "ac2_grid": o.ac2_grid,
-             "ac2_fine": o.ac2_fine, "ac2_quick": o.ac2_quick}, best.value)
+             "ac2_fine": o.ac2_fine, "ac2_quick": o.ac2_quick,
+             "block": o.block, "ac2_chunk": o.ac2_chunk}, best.value)
 
 
 def alpha_complex(xyzr, solvent=0.0, alpha=0.0, **kw):

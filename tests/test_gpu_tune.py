@@ -23,6 +23,8 @@ def test_autotune_single_and_batch():
     assert tuned["ac2_grid"] in (1024, 2048, 4096)
     assert tuned["ac2_fine"] in (-1, 2, 4)
     assert tuned["ac2_quick"] in (0, 1)
+    assert tuned["block"] in (64, 128, 256)
+    assert tuned["ac2_chunk"] in (64, 256, 1024)
     assert ms > 0
     ref = pac.compute(b, 1.4, 0.0).lists()
     got = pac.compute(b, 1.4, 0.0, **tuned).lists()
@@ -75,3 +77,36 @@ def test_scan_grids_same_output(fine, quick):
         a, c = r1.lists(k), r2.lists(k)
         for d in (1, 2, 3):
             assert np.array_equal(a[d], c[d]), ("batch", k, d)
+
+
+@pytest.mark.parametrize("block,chunk", [(64, 64), (128, 1000), (256, 64),
+                                         (256, 4096)])
+def test_persistent_phase_shapes_same_output(block, chunk):
+    """the persistent second emptiness phase gives the same lists for any
+    block shape and queue granularity (a chunk not a multiple of 64 is
+    rounded up), on a globule with a small budget (much work left for the
+    second phase), the degenerate lattice, and a batch recorded as one
+    graph"""
+    _say("case: persistent phase block %d chunk %d" % (block, chunk))
+    b = pac.synth_globule(6000, None, 1.3, seed=17)
+    m = 6
+    g = np.stack(np.meshgrid(*[np.arange(m) * 2.0] * 3, indexing="ij"), -1).reshape(-1, 3)
+    lat = np.concatenate([g, np.full((len(g), 1), 1.7)], 1).astype(np.float32)
+    for what, x in (("globule", b), ("lattice", lat)):
+        for prec in (pac.PREC_STAGED, pac.PREC_PLAIN_FP64):
+            ref = pac.compute(x, 1.4, 0.0, precision=prec).lists()
+            got = pac.compute(x, 1.4, 0.0, precision=prec, ac2_budget=2,
+                              ac2_grid=96, block=block, ac2_chunk=chunk).lists()
+            for d in (1, 2, 3):
+                assert np.array_equal(ref[d], got[d]), (what, prec, d)
+    xb, off = pac.synth_batch(7, 200, 3000, seed=18)
+    r1 = pac.compute_batch(xb, off, 1.4, 0.0)
+    gr = pac.Graph(xb, off, 1.4, 0.0, ac2_budget=2, ac2_grid=96, block=block,
+                   ac2_chunk=chunk)
+    for _ in range(2):
+        gr.launch()
+        r2 = gr.result()
+        for k in range(7):
+            a, c = r1.lists(k), r2.lists(k)
+            for d in (1, 2, 3):
+                assert np.array_equal(a[d], c[d]), ("graph", k, d)
