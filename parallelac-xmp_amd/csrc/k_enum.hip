@@ -668,6 +668,48 @@ __device__ __forceinline__ int filter_code(const Ball* v, Real alpha) {
   if (KIND == 2) return tri_code<T>(v[0], v[1], v[2], alpha);
   return tet_code<T>(v[0], v[1], v[2], v[3], alpha);
 }
+/* Degenerate candidates (collinear triangle centres: n = u x v = 0;
+ * coplanar tet centres: D = u . (v x t) = 0), which the filter rejects,
+ * decided in 128-bit integers on the coordinates scaled by 2^S when their
+ * differences fit 40 bits (n's components stay below 2^81, D below 2^123):
+ * 1 degenerate, 0 not, -1 out of that range.  Lattice inputs send millions
+ * of such candidates past the float stages (D or |n|^2 exactly 0). */
+template <int KIND>
+__device__ __forceinline__ int degenerate_i128(const Ball* v, int S) {
+  constexpr int NB = KIND == 3 ? 4 : 3;
+  int64_t d[NB - 1][3];
+  double o[3] = {(double)v[0].x, (double)v[0].y, (double)v[0].z};
+  int64_t q0[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double sv = __builtin_ldexp(o[a], S);
+    if (!(fabs(sv) < 0x1p62)) return -1;
+    q0[a] = (int64_t)sv;
+  }
+#pragma unroll
+  for (int i = 1; i < NB; ++i) {
+    const double c[3] = {(double)v[i].x, (double)v[i].y, (double)v[i].z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double sv = __builtin_ldexp(c[a], S);
+      if (!(fabs(sv) < 0x1p62)) return -1;
+      d[i - 1][a] = (int64_t)sv - q0[a];
+      if (d[i - 1][a] >= ((int64_t)1 << 40) || d[i - 1][a] <= -((int64_t)1 << 40))
+        return -1;
+    }
+  }
+  const int64_t *u = d[0], *w = d[1];
+  const __int128 nx = (__int128)u[1] * w[2] - (__int128)u[2] * w[1];
+  const __int128 ny = (__int128)u[2] * w[0] - (__int128)u[0] * w[2];
+  const __int128 nz = (__int128)u[0] * w[1] - (__int128)u[1] * w[0];
+  if (KIND == 2) return nx == 0 && ny == 0 && nz == 0;
+  /* D = t . (u x w), the same determinant as u . (w x t) */
+  const int64_t* t = d[NB - 2];
+  const __int128 D = (__int128)t[0] * nx + (__int128)t[1] * ny +
+                     (__int128)t[2] * nz;
+  return D == 0;
+}
+
 template <int KIND>
 __device__ __forceinline__ int filter_exact(const Ball* v,
                                             const ExactScale& X) {
@@ -915,6 +957,10 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
     sc.c[1]++;
     int r = filter_code<KIND, FE<double>>(v, S.alpha);
+    if (KIND > 1 && r == UNDEC && degenerate_i128<KIND>(v, S.ex.S) == 1) {
+      sc.c[3]++; /* decided exactly, in 128-bit integers */
+      r = REJECT;
+    }
     if (r == UNDEC) {
       sc.c[2]++;
       r = filter_code<KIND, FEDD>(v, S.alpha);
@@ -937,7 +983,11 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
     QEntry e = q.at(t);
     Ball v[4];
     cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
-    const int r = filter_exact<KIND>(v, S.ex);
+    /* degenerate candidates need no wide integers (the same balls on
+     * every lane of the group: a group-uniform branch) */
+    const int r = KIND > 1 && degenerate_i128<KIND>(v, S.ex.S) == 1
+                      ? REJECT
+                      : filter_exact<KIND>(v, S.ex);
     if (Grp<kTPI>::idx() == 0) {
       flag[e.item] = (uint8_t)r;
       ++cnt;
