@@ -972,27 +972,61 @@ __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
   });
   sc.flush(S.ctr, fam_of(KIND));
 }
+/* Exact filter decisions in two steps per wave: every lane takes one
+ * queued candidate and settles degenerate ones (collinear / coplanar
+ * centres) in 128-bit integers; the wave's other candidates are listed in
+ * LDS and taken 4 at a time by its groups of kTPI lanes for the
+ * wavefront-cooperative wide-integer evaluation. */
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
                                                          QRef q,
                                                          const uint32_t* cval,
                                                          uint8_t* flag) {
-  /* one queued candidate per group of kTPI lanes */
+  __shared__ uint32_t s_pend[kBlock / 64][64];
   unsigned cnt = 0;
-  for_queue_grp<kTPI>(q, [&](uint32_t t) {
-    QEntry e = q.at(t);
-    Ball v[4];
-    cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
-    /* degenerate candidates need no wide integers (the same balls on
-     * every lane of the group: a group-uniform branch) */
-    const int r = KIND > 1 && degenerate_i128<KIND>(v, S.ex.S) == 1
-                      ? REJECT
-                      : filter_exact<KIND>(v, S.ex);
-    if (Grp<kTPI>::idx() == 0) {
-      flag[e.item] = (uint8_t)r;
-      ++cnt;
+  const uint32_t n = q.size();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  uint32_t* const pend_list = s_pend[w];
+  /* wave-uniform loop over 64 entries at a time */
+  for (uint64_t base = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + w) * 64;
+       base < n; base += waves * 64) {
+    const uint32_t t = (uint32_t)(base + lane);
+    bool pend = false;
+    if (t < n) {
+      pend = true;
+      if (KIND > 1) {
+        const QEntry e = q.at(t);
+        Ball v[4];
+        cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
+        if (degenerate_i128<KIND>(v, S.ex.S) == 1) {
+          flag[e.item] = REJECT;
+          ++cnt;
+          pend = false;
+        }
+      }
     }
-  });
+    const uint64_t m = __ballot(pend);
+    if (pend) pend_list[__popcll(m & lt)] = t;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int np = __popcll(m);
+    /* group-uniform loop: group g takes the pending entries g, g + 4, ... */
+    for (int k = lane / kTPI; k < np; k += 64 / kTPI) {
+      const QEntry e = q.at(pend_list[k]);
+      Ball v[4];
+      cand_balls<KIND>(S, C, e.item, e.aux, KIND == 3 ? 0u : cval[e.item], v);
+      const int r = filter_exact<KIND>(v, S.ex);
+      if (Grp<kTPI>::idx() == 0) {
+        flag[e.item] = (uint8_t)r;
+        ++cnt;
+      }
+    }
+    /* the list is rewritten next round: every group is done with it */
+    __builtin_amdgcn_wave_barrier();
+  }
   flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
 
@@ -1032,7 +1066,7 @@ void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L) {
   if (!q.launch_items()) return;
-  dim3 gr = qgrid_grp(q, kExactBlocks, kTPI);
+  dim3 gr = qgrid(q, kExactBlocks); /* one entry per lane in the first step */
   if (kind == 1)
     hipLaunchKernelGGL(k_filter_exact<1>, gr, dim3(kBlock), 0, L.s, S, C, q,
                        cval, flag);
