@@ -15,7 +15,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpac.so")
+# the in-tree library, unless PAC_LIB names another build of it
+LIB_PATH = os.environ.get("PAC_LIB") or os.path.join(os.path.dirname(_HERE),
+                                                      "lib", "libpac.so")
 
 PAC_OK = 0
 PAC_ERR_IO, PAC_ERR_INVALID, PAC_ERR_DEGENERATE = -1, -2, -3
