@@ -476,6 +476,80 @@ __device__ int tet_pair_exact(const Stage& S, const Ball v[4],
   return (ss * -sD) > 0;
 }
 
+/* The exact tet test's per-tet part: its scaled vertices' D and N, the
+ * first vertex's position and weight (what every pair test of the tet
+ * reuses). */
+struct TetExact {
+  uint32_t item = 0xffffffffu;
+  uint4 tt;
+  uint32_t vid[4];
+  Ball v[4];
+  V3<XI> pa, N;
+  XI wa, D;
+};
+__device__ __forceinline__ void tet_exact_setup(const Stage& S,
+                                                const uint4* tets,
+                                                uint32_t item, TetExact& T) {
+  T.item = item;
+  T.tt = tets[item];
+  T.vid[0] = T.tt.x;
+  T.vid[1] = T.tt.y;
+  T.vid[2] = T.tt.z;
+  T.vid[3] = T.tt.w;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) T.v[i] = S.ball[T.vid[i]];
+  const LdCW<kTPI> L{S.ex.S};
+  T.pa = L.p(T.v[0]);
+  T.wa = L.w(T.v[0]);
+  tet_DN(T.pa, T.wa, L.p(T.v[1]), L.w(T.v[1]), L.p(T.v[2]), L.w(T.v[2]),
+         L.p(T.v[3]), L.w(T.v[3]), &T.D, &T.N);
+}
+/* One queued entry of tet T: a (tet, ball) pair, or a full exact scan of
+ * the tet's box (aux == ~0, a tet whose orientation no float stage could
+ * certify).  Group-uniform; the group's first lane writes the flag. */
+__device__ __forceinline__ void tet_exact_entry(const Stage& S,
+                                                const TetExact& T, uint32_t aux,
+                                                const uint32_t* perm,
+                                                uint8_t* flag, bool lead,
+                                                unsigned& cnt, unsigned& sos,
+                                                uint32_t* lm) {
+  if (aux != 0xffffffffu) {
+    ++cnt;
+    if (tet_pair_exact(S, T.v, T.vid, perm, T.D, T.N, T.pa, T.wa, aux, &sos,
+                       lm) &&
+        lead)
+      flag[T.item] = 0;
+  } else if (fe_sign(T.D) != 0) {
+    double D2 = 2.0 * cw_to_double(T.D, 3 * S.ex.S);
+    double cx = (double)T.v[0].x + cw_to_double(T.N.x, 4 * S.ex.S) / D2;
+    double cy = (double)T.v[0].y + cw_to_double(T.N.y, 4 * S.ex.S) / D2;
+    double cz = (double)T.v[0].z + cw_to_double(T.N.z, 4 * S.ex.S) / D2;
+    double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
+    CellBox box;
+    bool viol = false;
+    const GridP g = grid_of(S, T.tt.x);
+    if (make_box(g, cx, cy, cz, err, err, err, box)) {
+      for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
+        for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
+          int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+          uint32_t b = S.cstart[row + box.lo[0]],
+                   en = S.cstart[row + box.hi[0] + 1];
+          for (uint32_t p = b; p < en; ++p) {
+            if (p == T.tt.x || p == T.tt.y || p == T.tt.z || p == T.tt.w)
+              continue;
+            ++cnt;
+            if (tet_pair_exact(S, T.v, T.vid, perm, T.D, T.N, T.pa, T.wa, p,
+                               &sos, lm)) {
+              viol = true;
+              break;
+            }
+          }
+        }
+    }
+    if (lead) flag[T.item] = viol ? 0 : 2;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     Stage S, QRef q, const uint4* tets,
     const uint32_t* perm, uint8_t* flag) {
@@ -487,50 +561,52 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_exact(
     /* the group's first lane decides, so the group stays together */
     if (!Grp<kTPI>::from(live_entry(q.at(t), flag) ? 1u : 0u, 0)) return;
     QEntry e = q.at(t);
-    uint4 tt = tets[e.item];
-    uint32_t vid[4] = {tt.x, tt.y, tt.z, tt.w};
-    Ball v[4] = {S.ball[tt.x], S.ball[tt.y], S.ball[tt.z], S.ball[tt.w]};
-    const LdCW<kTPI> L{S.ex.S};
-    V3<XI> pa = L.p(v[0]);
-    XI wa = L.w(v[0]), D;
-    V3<XI> N;
-    tet_DN(pa, wa, L.p(v[1]), L.w(v[1]), L.p(v[2]), L.w(v[2]), L.p(v[3]),
-           L.w(v[3]), &D, &N);
-    if (e.aux != 0xffffffffu) {
-      ++cnt;
-      if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, e.aux, &sos, lm) && lead)
-        flag[e.item] = 0;
-    } else if (fe_sign(D) != 0) {
-      double D2 = 2.0 * cw_to_double(D, 3 * S.ex.S);
-      double cx = (double)v[0].x + cw_to_double(N.x, 4 * S.ex.S) / D2;
-      double cy = (double)v[0].y + cw_to_double(N.y, 4 * S.ex.S) / D2;
-      double cz = (double)v[0].z + cw_to_double(N.z, 4 * S.ex.S) / D2;
-      double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
-      CellBox box;
-      bool viol = false;
-      const GridP g = grid_of(S, tt.x);
-      if (make_box(g, cx, cy, cz, err, err, err, box)) {
-        for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
-          for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
-            int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
-            uint32_t b = S.cstart[row + box.lo[0]],
-                     en = S.cstart[row + box.hi[0] + 1];
-            for (uint32_t p = b; p < en; ++p) {
-              if (p == tt.x || p == tt.y || p == tt.z || p == tt.w) continue;
-              ++cnt;
-              if (tet_pair_exact(S, v, vid, perm, D, N, pa, wa, p, &sos, lm)) {
-                viol = true;
-                break;
-              }
-            }
-          }
-      }
-      if (lead) flag[e.item] = viol ? 0 : 2;
-    }
+    TetExact T;
+    tet_exact_setup(S, tets, e.item, T);
+    tet_exact_entry(S, T, e.aux, perm, flag, lead, cnt, sos, lm);
   });
   /* every lane of a group counted the group's evaluations */
   flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, lead ? cnt : 0u);
   flush_ctr(S.ctr, CTR_SOS, lead ? sos : 0u);
+}
+
+/* The same over the entries sorted by tet (item[], aux[]): each group
+ * takes kSortedRun consecutive entries at a time, so the pairs of one tet
+ * (an exact tie usually has several co-spherical balls) share its D and N
+ * instead of recomputing them per pair. */
+constexpr uint32_t kSortedRun = 8;
+__global__ __launch_bounds__(kBlock) void k_tet_ac2_exact_sorted(
+    Stage S, const uint32_t* item, const uint32_t* aux, uint32_t n,
+    const uint4* tets, const uint32_t* perm, uint8_t* flag) {
+  unsigned cnt = 0, sos = 0;
+  const bool lead = Grp<kTPI>::idx() == 0;
+  __shared__ uint32_t s_lift[kBlock / 64][20 * 64];
+  uint32_t* const lm = s_lift[threadIdx.x >> 6];
+  const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kTPI;
+  TetExact T;
+  for (uint64_t c0 = (uint64_t)(gtid() / kTPI) * kSortedRun; c0 < n;
+       c0 += groups * kSortedRun) {
+    const uint32_t c1 = (uint32_t)min((uint64_t)n, c0 + kSortedRun);
+    for (uint32_t k = (uint32_t)c0; k < c1; ++k) {
+      const uint32_t it = item[k], ax = aux[k];
+      const bool live = ax == 0xffffffffu || flag[it] != 0;
+      if (!Grp<kTPI>::from(live ? 1u : 0u, 0)) continue;
+      if (it != T.item) tet_exact_setup(S, tets, it, T);
+      tet_exact_entry(S, T, ax, perm, flag, lead, cnt, sos, lm);
+    }
+  }
+  flush_ctr(S.ctr, CTR_STAGE + FAM_TET_AC2 * 4 + 3, lead ? cnt : 0u);
+  flush_ctr(S.ctr, CTR_SOS, lead ? sos : 0u);
+}
+
+/* entries of a host-built view, as (item, aux) arrays */
+__global__ __launch_bounds__(kBlock) void k_view_split(QView v, uint32_t* item,
+                                                       uint32_t* aux) {
+  const int64_t i = gtid();
+  if (i >= (int64_t)v.size()) return;
+  const QEntry e = v.at((uint32_t)i);
+  item[i] = e.item;
+  aux[i] = e.aux;
 }
 
 
@@ -992,6 +1068,24 @@ void launch_tet_ac2_exact(const Stage& S, QRef q,
   if (!q.launch_items()) return;
   hipLaunchKernelGGL(k_tet_ac2_exact, qgrid_grp(q, kExactBlocks, kTPI),
                      dim3(kBlock), 0, L.s, S, q, tets, perm, flag);
+}
+void launch_view_split(const QView& v, uint32_t* item, uint32_t* aux,
+                       PipeLaunch L) {
+  const uint32_t n = v.size();
+  if (!n) return;
+  hipLaunchKernelGGL(k_view_split, dim3((n + kBlock - 1) / kBlock),
+                     dim3(kBlock), 0, L.s, v, item, aux);
+}
+void launch_tet_ac2_exact_sorted(const Stage& S, const uint32_t* item,
+                                 const uint32_t* aux, uint32_t n,
+                                 const uint4* tets, const uint32_t* perm,
+                                 uint8_t* flag, PipeLaunch L) {
+  if (!n) return;
+  const uint64_t groups = (n + kSortedRun - 1) / kSortedRun;
+  const uint64_t b = (groups * kTPI + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_tet_ac2_exact_sorted,
+                     dim3((unsigned)(b < 65536 ? b : 65536)), dim3(kBlock), 0,
+                     L.s, S, item, aux, n, tets, perm, flag);
 }
 template <class R, int KIND, bool FINE, bool MOL>
 static void low_hot4(const Stage& S, Cnt nitems, const uint32_t* ecol,

@@ -445,6 +445,14 @@ void launch_tet_ac2_esc(const Stage& S, QRef q,
 void launch_tet_ac2_exact(const Stage& S, QRef q,
                           const uint4* tets, const uint32_t* perm,
                           uint8_t* flag, PipeLaunch L);
+/* the entries of a host-built view as (item, aux) arrays, and the exact
+ * tet test over entries sorted by tet (pairs of one tet share its D, N) */
+void launch_view_split(const QView& v, uint32_t* item, uint32_t* aux,
+                       PipeLaunch L);
+void launch_tet_ac2_exact_sorted(const Stage& S, const uint32_t* item,
+                                 const uint32_t* aux, uint32_t n,
+                                 const uint4* tets, const uint32_t* perm,
+                                 uint8_t* flag, PipeLaunch L);
 /* kind 0 triangles, 1 edges */
 void launch_low_ac2_hot(const Stage& S, int kind, Cnt nitems,
                         const uint32_t* ecol, const uint32_t* eleft,

@@ -702,7 +702,25 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
             launch_tet_ac2_esc(SG, q, tets.get(), kflag.get(), q2, L);
           },
           [&](QRef q) {
-            launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
+            /* many exact pairs (degenerate inputs): sorted by tet first, so
+             * that the pairs of a tet share its exact D and N */
+            const uint32_t ne = q.dv ? 0u : q.v.size();
+            if (ne < (1u << 16)) {
+              launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
+              return;
+            }
+            DBuf<uint32_t> k1(P, ne), v1(P, ne), k2(P, ne), v2(P, ne);
+            launch_view_split(q.v, k1.get(), v1.get(), L);
+            int bits = 1;
+            while (bits < 32 && ((int64_t)1 << bits) < Kalpha.cap) ++bits;
+            const size_t rtb = radix_tmp_bytes(ne);
+            DBuf<uint8_t> rt(P, rtb);
+            bool alt = false;
+            radix_sort_pairs(k1.get(), v1.get(), k2.get(), v2.get(), ne, bits,
+                             rt.get(), rtb, s, &alt);
+            launch_tet_ac2_exact_sorted(SG, alt ? k2.get() : k1.get(),
+                                        alt ? v2.get() : v1.get(), ne,
+                                        tets.get(), perm.get(), kflag.get(), L);
           },
           tail, RC_KF);
     }
