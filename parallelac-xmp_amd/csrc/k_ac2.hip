@@ -332,8 +332,50 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       }
       return false;
     };
+    /* fine grid: up to four balls at a time.  Phase A stops before a
+     * group once `budget` balls are seen, phase B skips exactly the groups
+     * phase A tested (the same traversal, the same counts).  The tests of
+     * a group all run; a certain violator among them ends the scan (its
+     * ties need no queueing: the tet is rejected). */
+    auto group = [&](uint32_t p, uint32_t n) -> bool {
+      if (PHASE_B) {
+        if (seen < budget) {
+          seen += (int)n;
+          return false;
+        }
+      } else if (budget > 0 && seen >= budget) {
+        stop = 2;
+        return true;
+      }
+      seen += (int)n;
+      pairs += n;
+      Ball E[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
+      bool viol = false;
+      unsigned ties = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        const int s = pairss_sign(P, E[k]);
+        viol |= k < n && s * P.sden < 0;
+        ties |= (k < n && s == 0) ? 1u << k : 0u;
+      }
+      if (viol) {
+        stop = 1;
+        return true;
+      }
+      while (ties) { /* own vertices (always ties) and exact ties */
+        const uint32_t k = __builtin_ctz(ties);
+        ties &= ties - 1;
+        const uint32_t q = S.fid[p + k];
+        if (q == t.x || q == t.y || q == t.z || q == t.w) continue;
+        pending = true;
+        push_q(q1, (uint32_t)id, q);
+      }
+      return false;
+    };
     if constexpr (FINE)
-      scan_sph4(g, S.fstart, S.fball, box, visit);
+      scan_sph_flat(g, S.fstart, box, group);
     else
       scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {
@@ -765,8 +807,46 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       }
       return false;
     };
+    /* fine grid: four balls at a time, as in k_tet_ac2_hot */
+    auto group = [&](uint32_t p, uint32_t n) -> bool {
+      if (PHASE_B) {
+        if (seen < budget) {
+          seen += (int)n;
+          return false;
+        }
+      } else if (budget > 0 && seen >= budget) {
+        stop = 2;
+        return true;
+      }
+      seen += (int)n;
+      pairs += n;
+      Ball E[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
+      bool viol = false;
+      unsigned ties = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        const int s = pairss_sign(P, E[k]);
+        viol |= k < n && s < 0;
+        ties |= (k < n && s == 0) ? 1u << k : 0u;
+      }
+      if (viol) { /* power <= rho strictly: violator */
+        stop = 1;
+        return true;
+      }
+      while (ties) {
+        const uint32_t k = __builtin_ctz(ties);
+        ties &= ties - 1;
+        const uint32_t q = S.fid[p + k];
+        if (q == vid[0] || q == vid[1] || q == vid[2]) continue;
+        pending = true;
+        push_q(q1, (uint32_t)id, q);
+      }
+      return false;
+    };
     if constexpr (FINE)
-      scan_sph4(g, S.fstart, S.fball, box, visit);
+      scan_sph_flat(g, S.fstart, box, group);
     else
       scan_box4(g, S.cstart, S.ball, box, visit);
     if (PHASE_B) {

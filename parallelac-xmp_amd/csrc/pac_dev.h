@@ -563,6 +563,61 @@ __device__ __forceinline__ void scan_sph4(const GridP& g,
   while (sph_next(g, fstart, s, it, lo, hi))
     if (scan_range4(fball, lo, hi, visit)) return;
 }
+/* The same cells as scan_sph4 in one flat loop: each iteration fetches the
+ * next row's range when the current one is used up and then hands the
+ * next (up to) four balls of the range to group(p, n) (fine-grid
+ * positions p .. p+n-1; true stops the scan).  Rows: the centre row first,
+ * then the box's other rows in (z, y) order.  Nested row and ball loops
+ * make a wave step through every lane's rows in lock step, each row as
+ * long as its longest lane's; here lanes at different rows share
+ * iterations, and the per-ball code is emitted once. */
+template <class F>
+__device__ __forceinline__ void scan_sph_flat(const GridP& g,
+                                              const uint32_t* fstart,
+                                              const SphBox& s, F&& group) {
+  const CellBox& b = s.b;
+  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
+  int ry = s.cy, rz = s.cz;        /* the row fetched next */
+  int ty = b.lo[1], tz = b.lo[2];  /* (z, y) order cursor, at the first row */
+  bool more = true;
+  uint32_t p = 0, hi = 0;
+  for (;;) {
+    if (p >= hi) {
+      if (!more) break;
+      const float d2 = axis_d2(s.fz, rz - b.lo[2]) + axis_d2(s.fy, ry - b.lo[1]);
+      if (d2 <= s.R2) {
+        const float rx = sqrtf(s.R2 - d2) * s.kx + 1e-3f;
+        const int xl = max(0, (int)floorf(fmaxf(s.fx - rx, -1.0e6f)));
+        const int xh =
+            min(b.hi[0] - b.lo[0], (int)floorf(fminf(s.fx + rx, 1.0e6f)));
+        if (xl <= xh) {
+          const int64_t row = (int64_t)g.cbase + b.lo[0] +
+                              (int64_t)g.ex * (ry + (int64_t)g.ey * rz);
+          p = fstart[row + xl];
+          hi = fstart[row + xh + 1];
+        }
+      }
+      /* next in (z, y) order; the centre row's turn goes to the first row,
+       * whose own turn (the cursor's start) is skipped */
+      if (++ty > b.hi[1]) {
+        ty = b.lo[1];
+        ++tz;
+      }
+      more = tz <= b.hi[2];
+      ry = ty;
+      rz = tz;
+      if (ry == s.cy && rz == s.cz) {
+        ry = b.lo[1];
+        rz = b.lo[2];
+      }
+    }
+    if (p < hi) {
+      const uint32_t n = min(hi - p, 4u);
+      if (group(p, n)) break;
+      p += n;
+    }
+  }
+}
 template <class C>
 __device__ __forceinline__ bool place_box(const GridP& g, C reach, const C c[3],
                                           const C e[3], CellBox& b) {
