@@ -50,7 +50,10 @@ typedef struct pac_options {
   int32_t block;         /* threads per block of the persistent second
                             emptiness phase: 64, 128 or 256 (0 = 256)     */
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
-  int32_t ac2_budget;    /* emptiness phase-A pair budget (0 tuned, <0 off)*/
+  int32_t ac2_budget;    /* emptiness phase-one pair budget: > 0 splits
+                            each scan after that many balls, the rest
+                            finished by the persistent second phase;
+                            <= 0: one phase, every scan to its end        */
   int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
   int32_t ac2_grid;      /* blocks of the second emptiness phase (0=2048) */
   int32_t ac2_fine;      /* exact policy: emptiness scans over the grid's
@@ -232,9 +235,10 @@ void pac_graph_free(pac_graph* g);
  * emptiness scan grid, exact cell policy: x subdivision 2, 4 or the
  * reference grid, tried first), opt->ac2_quick (with a fine grid: the
  * quick rejection pass on or off), then opt->ac2_budget / opt->ac2_grid
- * (emptiness phase-one pair budget x second-phase grid, over the chosen
- * scan grid), then opt->block / opt->ac2_chunk (block shape and queue
- * granularity of the persistent second phase); the other fields of *opt
+ * (one phase, or a phase-one pair budget of 8, 16 or 32 x second-phase
+ * grid, over the chosen scan grid), then, with a second phase,
+ * opt->block / opt->ac2_chunk (its block shape and queue granularity);
+ * the other fields of *opt
  * are used as given.  offsets ==
  * NULL: the n balls are one molecule; otherwise they are a batch
  * (offsets[n_mol+1]) and the leading molecules, up to about 2^20 balls,

@@ -151,11 +151,15 @@ __device__ __forceinline__ int64_t gtid() {
 /* Persistent work queue over [0, total): each wave of the grid takes
  * `chunk` consecutive entries (a multiple of 64) at a time from *work
  * (cleared before the launch), each lane every 64th entry of its chunk,
- * until none are left.  The chunk loop is wave-uniform. */
+ * until none are left.  The chunk loop is wave-uniform.  Waves whose index
+ * times chunk reaches total leave at once: the waves below keep taking
+ * chunks until the work is gone, and a short list costs a few atomics on
+ * the counter instead of one per wave of the grid. */
 template <class F>
 __device__ __forceinline__ void for_work(unsigned* work, unsigned chunk,
                                          int64_t total, F&& body) {
   const int lane = threadIdx.x & 63;
+  if ((gtid() >> 6) * (int64_t)chunk >= total) return;
   for (;;) {
     unsigned base = 0;
     if (lane == 0) base = atomicAdd(work, chunk);

@@ -489,7 +489,7 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
    * budget and grid), then the phase-one budget x second-phase grid over
    * them */
   static const int scans[][2] = {{2, 0}, {4, 0}, {2, 1}, {4, 1}, {-1, 0}};
-  static const int budgets[] = {4, 6, 8, 12, 16, 24};
+  static const int budgets[] = {-1, 8, 16, 32};
   static const int grids[] = {1024, 2048, 4096};
   double best = 1e300;
   pac_options pick = base;
@@ -510,6 +510,7 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
   best = 1e300;
   for (int b : budgets)
     for (int g : grids) {
+      if (b < 0 && g != grids[0]) continue; /* no second phase: no grid */
       pac_options t = fixed;
       t.ac2_budget = b;
       t.ac2_grid = g;

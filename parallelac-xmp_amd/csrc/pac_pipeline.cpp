@@ -672,7 +672,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   };
 
   /* ---- tet emptiness ---- */
-  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 8);
+  /* phase-one budget: 0 = no second phase (the default) */
+  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : 0;
   DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
   DBuf<uint32_t> next(P, pl.kf || sub ? std::max(Kalpha.cap, (int64_t)1) : 1);
   DBuf<uint8_t> kflag(P, Kalpha.cap);
