@@ -944,6 +944,197 @@ void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
                      tets, ocap);
 }
 
+/* Triangles of alpha edge q = (i, j) (as k_tri_enum) fused with the
+ * triangle filter's first stage (markRealTriangles, Parallel:1294-1340):
+ * the block's candidate slots are staged in LDS during the row merges,
+ * filtered round robin (uniform work per lane) and the kept and undecided
+ * ones written compacted, in slot order, from the block's first slot on
+ * (as in k_tet_enum_filter), four arrays and the flag; the filter never
+ * reads an array the enumeration wrote for every slot. */
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
+    Stage S, Cnt c_ne, const uint64_t* erow, const uint32_t* ecol,
+    const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
+    uint32_t* cseg, uint32_t* cea, uint32_t* ceb, uint8_t* flag,
+    uint32_t* bcnt, Queue q1, uint64_t lim) {
+  const int64_t ne = c_ne.get();
+  /* 5 blocks per CU fit in LDS with the edge table and block_flush's 16 B */
+  constexpr int CAP = 1912;
+  __shared__ uint32_t s_out[4 * CAP];
+  __shared__ uint2 s_ij[kBlock]; /* the block's edges (i, j) */
+  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
+  if (q0 >= ne) { /* whole block past the (device) count */
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    return;
+  }
+  const int64_t q1e = q0 + kBlock < ne ? q0 + kBlock : ne;
+  BlockOut<4, CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
+  uint32_t* const g[4] = {cval, cseg, cea, ceb};
+  const int64_t q = gtid();
+  unsigned found = 0;
+  if (q < ne) {
+    const uint32_t i = eleft[q], j = ecol[q];
+    s_ij[threadIdx.x] = make_uint2(i, j);
+    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
+    uint64_t b = erow[j], be = erow[j + 1];
+    uint64_t o = off[q];
+    while (a < ae && b < be) {
+      const uint32_t x = ecol[a], y = ecol[b];
+      if (x == y) {
+        const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
+        B.put(g, o, v);
+        ++o;
+        ++found;
+      }
+      a += (x <= y);
+      b += (y <= x);
+    }
+  }
+  __syncthreads(); /* LDS stage, edge table and overflow slots written */
+  __shared__ unsigned s_wsum[kBlock / 64];
+  const uint64_t base = B.base;
+  const uint64_t m = base >= lim ? 0 : (B.total < lim - base ? B.total : lim - base);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t run = 0; /* kept + undecided written so far (block-uniform) */
+  for (uint64_t r0 = 0; r0 < m; r0 += kBlock) {
+    const uint64_t t = r0 + threadIdx.x;
+    uint32_t v[4] = {0u, 0u, 0u, 0u};
+    int r = REJECT;
+    if (t < m) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v[k] = t < (uint64_t)CAP ? s_out[k * CAP + t] : g[k][base + t];
+      const uint2 ij = s_ij[v[1] - (uint32_t)q0];
+      r = tri_code<FE<R>>(S.ball[ij.x], S.ball[ij.y], S.ball[v[0]], S.alpha);
+    }
+    const uint64_t bm = __ballot(r != REJECT);
+    if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+    __syncthreads();
+    uint32_t pre = run + (uint32_t)__popcll(bm & lt), tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+      const unsigned c = s_wsum[k];
+      if (k < w) pre += c;
+      tot += c;
+    }
+    if (r != REJECT) {
+      const uint64_t at = base + pre;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) g[k][at] = v[k];
+      flag[at] = (uint8_t)r;
+      if (r == UNDEC) push_q(q1, (uint32_t)at, v[1]);
+    }
+    run += tot;
+    __syncthreads(); /* s_wsum is rewritten next round */
+  }
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = run;
+  block_flush(S.ctr, CTR_CAND + 1, found);
+}
+void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
+                            const uint32_t* ecol, const uint32_t* eleft,
+                            const uint64_t* off, uint32_t* cval,
+                            uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
+                            uint8_t* flag, uint32_t* bcnt, Queue q1,
+                            PipeLaunch L, uint64_t lim) {
+  if (ne.cap <= 0) return;
+  dim3 gr((unsigned)((ne.cap + kBlock - 1) / kBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_tri_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
+                       ne, erow, ecol, eleft, off, cval, cseg, cea, ceb, flag,
+                       bcnt, q1, lim);
+  else
+    hipLaunchKernelGGL(k_tri_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S,
+                       ne, erow, ecol, eleft, off, cval, cseg, cea, ceb, flag,
+                       bcnt, q1, lim);
+}
+/* the kept triangles of block b to (tk, tedge, teik, tejk)[bout[b], ...)
+ * in order; trow[e] for the block's edges e (and trow[ne], by the block
+ * holding index ne) = bout[b] + the block's kept triangles of edges
+ * before e (counted per edge in LDS, then scanned).  Grid: nblk + 1
+ * blocks, nblk = ceil(ne.cap / 256); bout has nblk + 1 entries. */
+__global__ __launch_bounds__(kBlock) void k_tri_block_copy(
+    Cnt c_ne, const uint64_t* off, const uint32_t* bcnt, const uint32_t* cval,
+    const uint32_t* cseg, const uint32_t* cea, const uint32_t* ceb,
+    const uint8_t* flag, const uint64_t* bout, uint32_t* tk, uint32_t* tedge,
+    uint32_t* teik, uint32_t* tejk, uint64_t* trow, uint64_t ocap) {
+  const int64_t ne = c_ne.get();
+  const int64_t b = blockIdx.x;
+  const int64_t q0 = b * kBlock;
+  if (q0 > ne) return;
+  __shared__ unsigned s_wsum[kBlock / 64];
+  __shared__ uint32_t s_cnt[kBlock];
+  s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint64_t o0 = bout[b];
+  if (q0 < ne) {
+    const uint64_t base = off[q0];
+    const uint32_t n = bcnt[b];
+    uint64_t o = o0;
+    for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+      const uint32_t t = r0 + threadIdx.x;
+      const bool keep = t < n && flag[base + t] == KEEP;
+      uint32_t seg = 0;
+      if (keep) {
+        seg = cseg[base + t];
+        atomicAdd(&s_cnt[seg - (uint32_t)q0], 1u);
+      }
+      const uint64_t bm = __ballot(keep);
+      if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+      __syncthreads();
+      uint64_t pre = o + (uint64_t)__popcll(bm & lt), tot = 0;
+#pragma unroll
+      for (int k = 0; k < kBlock / 64; ++k) {
+        const unsigned c = s_wsum[k];
+        if (k < w) pre += c;
+        tot += c;
+      }
+      if (keep && pre < ocap) {
+        tk[pre] = cval[base + t];
+        tedge[pre] = seg;
+        teik[pre] = cea[base + t];
+        tejk[pre] = ceb[base + t];
+      }
+      o += tot;
+      __syncthreads();
+    }
+  }
+  /* exclusive scan of the per-edge counts (one wave per 64 edges, then
+   * the wave totals) */
+  const uint32_t c = s_cnt[threadIdx.x];
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  uint64_t before = o0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k)
+    if (k < w) before += s_wsum[k];
+  const int64_t e = q0 + threadIdx.x;
+  if (e <= ne) {
+    const uint64_t v = before + incl - c;
+    trow[e] = v < ocap ? v : ocap;
+  }
+}
+void launch_tri_block_copy(Cnt ne, const uint64_t* off, const uint32_t* bcnt,
+                           const uint32_t* cval, const uint32_t* cseg,
+                           const uint32_t* cea, const uint32_t* ceb,
+                           const uint8_t* flag, const uint64_t* bout,
+                           uint32_t* tk, uint32_t* tedge, uint32_t* teik,
+                           uint32_t* tejk, uint64_t* trow, PipeLaunch L,
+                           uint64_t ocap) {
+  const int64_t nblk = (std::max<int64_t>(ne.cap, 0) + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_tri_block_copy, dim3((unsigned)(nblk + 1)), dim3(kBlock),
+                     0, L.s, ne, off, bcnt, cval, cseg, cea, ceb, flag, bout,
+                     tk, tedge, teik, tejk, trow, ocap);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_filter_esc(Stage S, CandCtx C,
                                                        QRef q,

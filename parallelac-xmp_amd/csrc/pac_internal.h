@@ -385,6 +385,29 @@ void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                      const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
                      uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
                      unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
+/* triangle candidates of the alpha edges, each filtered by the first
+ * precision stage in the same kernel: enumeration block b (256 alpha
+ * edges) writes its kept and undecided candidates (k, edge, edge(i,k),
+ * edge(j,k)) with their flags, in order, compacted from its first slot
+ * off[256 b] on and their number to bcnt[b]; undecided ones are queued on
+ * q1 by position (aux = their edge) */
+void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
+                            const uint32_t* ecol, const uint32_t* eleft,
+                            const uint64_t* off, uint32_t* cval,
+                            uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
+                            uint8_t* flag, uint32_t* bcnt, Queue q1,
+                            PipeLaunch L, uint64_t lim = ~0ull);
+/* after the ladder (bkeep by launch_tet_block_keep over the ne alpha
+ * edges, bout its exclusive scan): the kept triangles of each block to
+ * (tk, tedge, teik, tejk) in order, and the alpha-triangle row start of
+ * every alpha edge, trow[0 .. ne] */
+void launch_tri_block_copy(Cnt ne, const uint64_t* off, const uint32_t* bcnt,
+                           const uint32_t* cval, const uint32_t* cseg,
+                           const uint32_t* cea, const uint32_t* ceb,
+                           const uint8_t* flag, const uint64_t* bout,
+                           uint32_t* tk, uint32_t* tedge, uint32_t* teik,
+                           uint32_t* tejk, uint64_t* trow, PipeLaunch L,
+                           uint64_t ocap = ~0ull);
 /* tet candidates of the alpha triangles, each filtered by the first
  * precision stage in the same kernel; enumeration block b (256 alpha
  * triangles) writes its kept and undecided tets (i, j, k, l) with their

@@ -589,25 +589,49 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     DBuf<uint32_t> cval(P, Tslots.cap), cseg(P, Tslots.cap),
         cea(P, Tslots.cap), ceb(P, Tslots.cap);
     DBuf<uint8_t> flag(P, Tslots.cap);
-    launch_tri_enum(Ealpha, erow.get(), ecol.get(), eleft.get(), off.get(),
-                    cval.get(), cseg.get(), cea.get(), ceb.get(), ctr, L,
-                    (uint64_t)Tslots.cap);
-    DBuf<uint32_t> pos;
-    Talpha = filter_stage(2, C1, Tslots, cval.get(), cseg.get(), flag.get(),
-                          KP_TRI_FILL, pos, RC_TALPHA, [&](Queue qq) {
-                            launch_filter(SG, 2, C1, Tslots, cval.get(),
-                                          cseg.get(), flag.get(), qq, L);
-                          });
-    launch_seg_starts(Ealpha, off.get(), pos.get(), trow.get(), L,
-                      (uint64_t)Tslots.cap, (uint64_t)Talpha.cap);
+    /* enumeration and first filter stage in one kernel, each block's kept
+     * and undecided candidates compacted in place (their count in bcnt);
+     * the ladder settles the undecided ones; then the kept triangles of
+     * each block are counted, scanned and copied out in order with the
+     * row starts (as for tets) */
+    const int64_t nblk = (Ealpha.cap + kBlock - 1) / kBlock;
+    DBuf<uint32_t> bcnt(P, nblk + 1), bkeep(P, nblk + 1);
+    DBuf<uint64_t> bout(P, nblk + 1);
+    auto tmp = scan_tmp(nblk);
+    /* a rerun (queue overflow, eager only) recounts the candidates from
+     * zero */
+    int runs = 0;
+    Talpha = ladder(
+        std::min<int64_t>(Tslots.cap, Tslots.cap / 32 + 65536), KP_TRI_FILL, -1,
+        [&](Queue qq) {
+          if (runs++ && !rec)
+            zero_u64(ctr + (CTR_CAND + 1) * CTR_SLOTS, CTR_SLOTS, s);
+          launch_tri_enum_filter(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
+                                 off.get(), cval.get(), cseg.get(), cea.get(),
+                                 ceb.get(), flag.get(), bcnt.get(), qq, L,
+                                 (uint64_t)Tslots.cap);
+        },
+        [&](QRef qr, Queue q2) {
+          launch_filter_esc(SG, 2, C1, qr, cval.get(), flag.get(), q2, L);
+        },
+        [&](QRef qr) {
+          launch_filter_exact(SG, 2, C1, qr, cval.get(), flag.get(), L);
+        },
+        [&] {
+          launch_tet_block_keep(Ealpha, off.get(), bcnt.get(), flag.get(),
+                                bkeep.get(), L);
+          scan_u32_to_u64(bkeep.get(), bout.get(), Cnt(nblk), tmp.get(), 0, s,
+                          dcnt + RC_TALPHA);
+        },
+        RC_TALPHA);
     tk = DBuf<uint32_t>(P, Talpha.cap);
     tedge = DBuf<uint32_t>(P, Talpha.cap);
     teik = DBuf<uint32_t>(P, Talpha.cap);
     tejk = DBuf<uint32_t>(P, Talpha.cap);
-    launch_compact_flat(2, Tslots, C1, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), tk.get(), tedge.get(), nullptr, cea.get(),
-                        ceb.get(), teik.get(), tejk.get(), L,
-                        (uint64_t)Talpha.cap);
+    launch_tri_block_copy(Ealpha, off.get(), bcnt.get(), cval.get(), cseg.get(),
+                          cea.get(), ceb.get(), flag.get(), bout.get(), tk.get(),
+                          tedge.get(), teik.get(), tejk.get(), trow.get(), L,
+                          (uint64_t)Talpha.cap);
     KCHK();
   }
   T.mark(4);
