@@ -74,9 +74,9 @@ typedef struct pac_options {
                             order, with the tet emptiness test;
                             pac_compute / pac_compute_f64 only.           */
   int32_t ac2_chunk;     /* entries of the left-over list each wave of the
-                            persistent second emptiness phase takes from
-                            its work counter at a time (rounded up to a
-                            multiple of 64; 0 = 64)                       */
+                            persistent second emptiness phase takes at a
+                            time, wave w chunks w, w + W, ... (rounded up
+                            to a multiple of 64; 0 = 64)                  */
 } pac_options;
 
 typedef struct pac_result pac_result;

@@ -230,8 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_need(Stage S, Cnt c_nk,
     need = tet_needed(S, tets[id]);
     if (!need) flag[id] = 0;
   }
-  const unsigned at = block_append(nnext, need);
-  if (need) next[at] = (uint32_t)id;
+  list_append(next, nnext, S.lcap, need, (uint32_t)id);
 }
 
 /* Quick pass over the alpha tets (fine scan grid): FP32 pair constants
@@ -257,8 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_quick(
     else
       keep = (uint32_t)id;
   }
-  const unsigned at = block_append(nnext, keep != 0xffffffffu);
-  if (keep != 0xffffffffu) next[at] = keep;
+  list_append(next, nnext, S.lcap, keep != 0xffffffffu, keep);
   block_flush(S.ctr, CTR_PAIRS + 0, pairs);
 }
 
@@ -280,11 +278,17 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
   unsigned evals = 0, pairs = 0;
   /* phase A: the alpha tets, or the list `items` of them (those the quick
    * pass left); phase B: `rest` */
-  int64_t total = PHASE_B ? (int64_t)*nrest
-                          : (items ? (int64_t)*nitems : c_nk.get());
+  __shared__ uint32_t s_pre[LSLOTS + 1];
+  if (PHASE_B) list_table(nrest, S.lcap, s_pre);
+  else if (items) list_table(nitems, S.lcap, s_pre);
+  const int64_t total =
+      PHASE_B || items ? (int64_t)s_pre[LSLOTS] : c_nk.get();
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   auto item = [&](int64_t w) {
-    int64_t id = PHASE_B ? (int64_t)rest[w] : (items ? (int64_t)items[w] : w);
+    const int64_t id =
+        PHASE_B ? (int64_t)list_at(rest, S.lcap, s_pre, (uint32_t)w)
+                : (items ? (int64_t)list_at(items, S.lcap, s_pre, (uint32_t)w)
+                         : w);
     uint4 t = tets[id];
     Ball v[4] = {S.ball[t.x], S.ball[t.y], S.ball[t.z], S.ball[t.w]};
     const GridP g = FINE ? sgrid_at<MOL>(S, t.x) : grid_at<MOL>(S, t.x);
@@ -389,13 +393,12 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
     }
   };
   if constexpr (PHASE_B) {
-    for_work(S.bwork, S.bchunk, total, item); /* persistent work queue */
+    for_work(S.bchunk, total, item); /* persistent launch */
   } else {
     if (gtid() < total) item(gtid());
   }
   if (!PHASE_B) {
-    unsigned at = block_append(nrest, rest_id != 0xffffffffu);
-    if (rest_id != 0xffffffffu) rest[at] = rest_id;
+    list_append(rest, nrest, S.lcap, rest_id != 0xffffffffu, rest_id);
   }
   (void)evals;
   block_flush(S.ctr, CTR_PAIRS + 0, pairs);
@@ -746,10 +749,13 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     const uint32_t* tk, const uint32_t* tedge, const uint8_t* in_flag,
     uint8_t* flag, Queue q1, uint32_t* rest, unsigned* nrest, int budget) {
   unsigned evals = 0, pairs = 0;
-  int64_t total = PHASE_B ? (int64_t)*nrest : c_nitems.get();
+  __shared__ uint32_t s_pre[LSLOTS + 1];
+  if (PHASE_B) list_table(nrest, S.lcap, s_pre);
+  const int64_t total = PHASE_B ? (int64_t)s_pre[LSLOTS] : c_nitems.get();
   uint32_t rest_id = 0xffffffffu; /* phase A: at most one item per thread */
   auto item = [&](int64_t w) {
-    int64_t id = PHASE_B ? (int64_t)rest[w] : w;
+    const int64_t id =
+        PHASE_B ? (int64_t)list_at(rest, S.lcap, s_pre, (uint32_t)w) : w;
     if (!PHASE_B && in_flag[id]) {
       flag[id] = 1;
       return;
@@ -860,13 +866,12 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
     }
   };
   if constexpr (PHASE_B) {
-    for_work(S.bwork, S.bchunk, total, item); /* persistent work queue */
+    for_work(S.bchunk, total, item); /* persistent launch */
   } else {
     if (gtid() < total) item(gtid());
   }
   if (!PHASE_B) {
-    unsigned at = block_append(nrest, rest_id != 0xffffffffu);
-    if (rest_id != 0xffffffffu) rest[at] = rest_id;
+    list_append(rest, nrest, S.lcap, rest_id != 0xffffffffu, rest_id);
   }
   (void)evals;
   block_flush(S.ctr, CTR_PAIRS + 1 + KIND, pairs);
@@ -1059,12 +1064,12 @@ static void tet_hot3(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
   dim3 gr((unsigned)((nk.cap + kBlock - 1) / kBlock)), gb((unsigned)S.bgrid);
   const uint32_t* items = nullptr;
   if (FINE && S.quick) { /* quick pass, then the certified scan of the rest */
-    zero_u32(nnext, 1, L.s);
+    zero_u32(nnext, LSLOTS, L.s);
     hipLaunchKernelGGL((k_tet_ac2_quick<R, MOL>), gr, dim3(kBlock), 0, L.s, S,
                        nk, tets, flag, next, nnext);
     items = next;
   } else if (S.chunk) {
-    zero_u32(nnext, 1, L.s);
+    zero_u32(nnext, LSLOTS, L.s);
     hipLaunchKernelGGL(k_tet_need, gr, dim3(kBlock), 0, L.s, S, nk, tets, flag,
                        next, nnext);
     items = next;
@@ -1073,7 +1078,6 @@ static void tet_hot3(const Stage& S, Cnt nk, const uint4* tets, uint8_t* flag,
                      L.s, S, nk, items, nnext, tets, flag, q1, rest, nrest,
                      budget);
   if (budget > 0) {
-    zero_u32(S.bwork, 1, L.s);
     hipLaunchKernelGGL((k_tet_ac2_hot<R, true, FINE, MOL>), gb,
                        dim3((unsigned)S.bblock), 0, L.s, S, nk, items, nnext,
                        tets, flag, q1, rest, nrest, budget);
@@ -1178,7 +1182,6 @@ static void low_hot4(const Stage& S, Cnt nitems, const uint32_t* ecol,
                      dim3(kBlock), 0, L.s, S, nitems, ecol, eleft, tk, tedge,
                      in_flag, flag, q1, rest, nrest, budget);
   if (budget > 0) {
-    zero_u32(S.bwork, 1, L.s);
     hipLaunchKernelGGL((k_low_ac2_hot<R, KIND, true, FINE, MOL>), gb,
                        dim3((unsigned)S.bblock), 0, L.s, S, nitems, ecol, eleft,
                        tk, tedge, in_flag, flag, q1, rest, nrest, budget);

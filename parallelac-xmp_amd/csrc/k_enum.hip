@@ -460,54 +460,6 @@ void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L) {
 /* ------------------------------------------------------------------ */
 /* enumeration: candidates as (segment, new vertex) pairs              */
 /* ------------------------------------------------------------------ */
-/* One wave per ball: the wave walks the ball's stencil rows together (the
- * ball, its grid and each row's bounds are the same for all lanes) and
- * consecutive lanes write a row's candidates to consecutive addresses.  (A
- * thread per ball wrote its ~130 candidates one after another, every lane
- * of a store in a cache line of its own.) */
-constexpr int kEdgeBallsPerBlock = kBlock / 64;
-__global__ __launch_bounds__(kBlock) void k_edge_enum(Stage S,
-                                                      const uint64_t* off,
-                                                      uint32_t* cval,
-                                                      uint32_t* cseg,
-                                                      uint64_t lim) {
-  const int64_t iw = (int64_t)blockIdx.x * kEdgeBallsPerBlock + (threadIdx.x >> 6);
-  if (iw >= S.n) return; /* a whole wave */
-  const uint32_t i = __builtin_amdgcn_readfirstlane((uint32_t)iw);
-  const uint32_t lane = threadIdx.x & 63;
-  const GridP g = grid_of(S, i);
-  const uint32_t c = S.cellk[i];
-  int gx, gy, gz;
-  cell_xyz(c - g.cbase, g, gx, gy, gz);
-  const int k = g.k_enum;
-  uint64_t o = off[i];
-  for (int z = max(gz - k, 0); z <= min(gz + k, g.ez - 1); ++z)
-    for (int y = max(gy - k, 0); y <= min(gy + k, g.ey - 1); ++y) {
-      int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
-      int x0 = max(gx - k, 0), x1 = min(gx + k, g.ex - 1);
-      int64_t c0 = row + x0, c1 = row + x1;
-      if (c1 < (int64_t)c) continue;
-      uint32_t b = S.cstart[c0], e = S.cstart[c1 + 1];
-      if (b < i + 1) b = i + 1;
-      for (uint32_t j = b + lane; j < e; j += 64) {
-        const uint64_t w = o + (j - b);
-        if (w < lim) {
-          cval[w] = j;
-          cseg[w] = i;
-        }
-      }
-      if (e > b) o += e - b;
-    }
-}
-void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
-                      uint32_t* cseg, PipeLaunch L, uint64_t lim) {
-  if (S.n <= 0) return;
-  hipLaunchKernelGGL(
-      k_edge_enum,
-      dim3((unsigned)((S.n + kEdgeBallsPerBlock - 1) / kEdgeBallsPerBlock)),
-      dim3(kBlock), 0, L.s, S, off, cval, cseg, lim);
-}
-
 /* Slot capacities that need no merge: a triangle from alpha edge (i,j) has
  * its third vertex in row(i) after j, a tet from alpha triangle (i,j,k) has
  * its fourth vertex in T(i,j) after k.  Unused slots carry seg = ~0. */
@@ -585,57 +537,6 @@ void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
   if (ne.cap <= 0) return;
   hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
-}
-
-/* triangles of alpha edge q = (i,j): k in row(i) after j and in row(j); the
- * positions of k in row(i) and row(j) are the alpha-edge ids of (i,k) and
- * (j,k), recorded for the tet stage and the closure. */
-__global__ __launch_bounds__(kBlock) void k_tri_enum(
-    Cnt c_ne, const uint64_t* erow, const uint32_t* ecol,
-    const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-    uint32_t* cseg, uint32_t* cea, uint32_t* ceb, unsigned long long* ctr,
-    uint64_t lim) {
-  const int64_t ne = c_ne.get();
-  /* 5 blocks per CU fit in LDS with block_flush's 16 B; a block of 256
-   * alpha edges averages ~2000 candidates, which a smaller stage would
-   * send to global memory */
-  constexpr int CAP = 2040;
-  __shared__ uint32_t s_out[4 * CAP];
-  const int64_t q0 = (int64_t)blockIdx.x * kBlock;
-  if (q0 >= ne) return; /* whole block past the (device) count */
-  const int64_t q1 = q0 + kBlock < ne ? q0 + kBlock : ne;
-  BlockOut<4, CAP> B{s_out, off[q0], off[q1] - off[q0], lim};
-  uint32_t* const g[4] = {cval, cseg, cea, ceb};
-  int64_t q = gtid();
-  unsigned found = 0;
-  if (q < ne) {
-    uint32_t i = eleft[q], j = ecol[q];
-    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-    uint64_t b = erow[j], be = erow[j + 1];
-    uint64_t o = off[q];
-    while (a < ae && b < be) {
-      uint32_t x = ecol[a], y = ecol[b];
-      if (x == y) {
-        const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
-        B.put(g, o, v);
-        ++o;
-        ++found;
-      }
-      a += (x <= y);
-      b += (y <= x);
-    }
-  }
-  B.flush(g);
-  block_flush(ctr, CTR_CAND + 1, found);
-}
-void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
-                     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                     unsigned long long* ctr, PipeLaunch L, uint64_t lim) {
-  if (ne.cap <= 0) return;
-  hipLaunchKernelGGL(k_tri_enum, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, off, cval,
-                     cseg, cea, ceb, ctr, lim);
 }
 
 /* ------------------------------------------------------------------ */
@@ -721,29 +622,6 @@ constexpr int fam_of(int KIND) {
   return KIND == 1 ? FAM_EDGE : (KIND == 2 ? FAM_TRI : FAM_TET);
 }
 
-template <int KIND, class R>
-__global__ __launch_bounds__(kBlock) void k_filter(Stage S, CandCtx C,
-                                                   Cnt c_nc,
-                                                   const uint32_t* cval,
-                                                   const uint32_t* cseg,
-                                                   uint8_t* flag, Queue q1) {
-  /* one evaluation per candidate: the host books nc evaluations at this
-   * stage, no counter traffic here */
-  const int64_t nc = c_nc.get();
-  int64_t o = gtid();
-  if (o < nc) {
-    const uint32_t seg = KIND == 3 ? C.ctet[o].x : cseg[o];
-    if (seg == 0xffffffffu) {
-      flag[o] = REJECT; /* unused slot */
-    } else {
-      Ball v[4];
-      cand_balls<KIND>(S, C, o, seg, KIND == 3 ? 0u : cval[o], v);
-      int r = filter_code<KIND, FE<R>>(v, S.alpha);
-      flag[o] = (uint8_t)r;
-      if (r == UNDEC) push_q(q1, (uint32_t)o, seg);
-    }
-  }
-}
 /* Tets of alpha triangle q = (i,j,k): l in T(i,j) after k, T(i,k), T(j,k)
  * (computePossibleTetCount / fillPossibleTets, Parallel:418-543), fused
  * with the tet filter's first stage (markRealTets, :531-591).  A block's
@@ -944,7 +822,170 @@ void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
                      tets, ocap);
 }
 
-/* Triangles of alpha edge q = (i, j) (as k_tri_enum) fused with the
+/* Edge candidates of ball i (j > i in the stencil rows, Parallel:122-201)
+ * fused with the edge filter's first stage (markRealEdges,
+ * Parallel:1229-1250).  One wave per ball: lane r finds stencil row r's
+ * candidate range (64 rows per pass), a wave prefix sum lays the ranges
+ * end to end, and the lanes test the concatenation 64 candidates at a
+ * time (a row of ~15 candidates would leave most lanes idle); the kept
+ * and undecided ones are written compacted, in order, from the ball's
+ * first slot on (a ballot prefix per 64), with their flags; nothing is
+ * written for the others. */
+constexpr int kEdgeBallsPerBlock = kBlock / 64;
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_edge_enum_filter(
+    Stage S, const uint64_t* off, uint32_t* cval, uint32_t* cseg,
+    uint8_t* flag, uint32_t* bcnt, Queue q1, uint64_t lim) {
+  const int64_t iw = (int64_t)blockIdx.x * kEdgeBallsPerBlock + (threadIdx.x >> 6);
+  if (iw >= S.n) return; /* a whole wave */
+  const uint32_t i = __builtin_amdgcn_readfirstlane((uint32_t)iw);
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const GridP g = grid_of(S, i);
+  const uint32_t c = S.cellk[i];
+  int gx, gy, gz;
+  cell_xyz(c - g.cbase, g, gx, gy, gz);
+  const int k = g.k_enum;
+  const int z0 = max(gz - k, 0), y0 = max(gy - k, 0);
+  const int nz = min(gz + k, g.ez - 1) - z0 + 1, ny = min(gy + k, g.ey - 1) - y0 + 1;
+  const int x0 = max(gx - k, 0), x1 = min(gx + k, g.ex - 1);
+  const Ball bi = S.ball[i];
+  const uint64_t o0 = off[i];
+  uint64_t o = o0; /* next compacted position */
+  __shared__ uint32_t s_end[kEdgeBallsPerBlock][64], s_beg[kEdgeBallsPerBlock][64];
+  const int wv = threadIdx.x >> 6;
+  for (int r0 = 0; r0 < ny * nz; r0 += 64) {
+    /* this lane's row: candidates [b, b + len) */
+    uint32_t b = 0, len = 0;
+    const int r = r0 + lane;
+    if (r < ny * nz) {
+      const int y = y0 + r % ny, z = z0 + r / ny;
+      const int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+      const int64_t c0 = row + x0, c1 = row + x1;
+      if (c1 >= (int64_t)c) {
+        b = S.cstart[c0];
+        const uint32_t e = S.cstart[c1 + 1];
+        if (b < i + 1) b = i + 1;
+        len = e > b ? e - b : 0u;
+      }
+    }
+    /* inclusive prefix of the lengths: row r's candidates are positions
+     * [incl - len, incl) of the pass; the rows' ends and starts go to LDS
+     * for the lanes' searches */
+    uint32_t incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    s_end[wv][lane] = incl;
+    s_beg[wv][lane] = b - (incl - len); /* j = s_beg[row] + f */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int rr = 0; /* the row of this lane's candidate: first with end > f */
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t f = f0 + (uint32_t)lane;
+      int code = REJECT;
+      uint32_t j = 0;
+      if (f < total) {
+        while (s_end[wv][rr] <= f) ++rr; /* f < total = s_end[63] */
+        j = s_beg[wv][rr] + f;
+        const Ball bj = S.ball[j];
+        code = same_pos(bi, bj) ? REJECT : edge_code<FE<R>>(bi, bj, S.alpha);
+      }
+      const uint64_t m = __ballot(code != REJECT);
+      if (code != REJECT) {
+        const uint64_t w = o + (uint64_t)__popcll(m & lt);
+        if (w < lim) {
+          cval[w] = j;
+          cseg[w] = i;
+          flag[w] = (uint8_t)code;
+          if (code == UNDEC) push_q(q1, (uint32_t)w, i);
+        }
+      }
+      o += (uint64_t)__popcll(m);
+    }
+    /* the next pass rewrites the tables */
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) bcnt[i] = (uint32_t)(o - o0);
+}
+void launch_edge_enum_filter(const Stage& S, const uint64_t* off,
+                             uint32_t* cval, uint32_t* cseg, uint8_t* flag,
+                             uint32_t* bcnt, Queue q1, PipeLaunch L,
+                             uint64_t lim) {
+  if (S.n <= 0) return;
+  dim3 gr((unsigned)((S.n + kEdgeBallsPerBlock - 1) / kEdgeBallsPerBlock));
+  if (S.staged)
+    hipLaunchKernelGGL(k_edge_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
+                       off, cval, cseg, flag, bcnt, q1, lim);
+  else
+    hipLaunchKernelGGL(k_edge_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S,
+                       off, cval, cseg, flag, bcnt, q1, lim);
+}
+/* kept candidates of segment i among its bcnt[i] compacted ones */
+__global__ __launch_bounds__(kBlock) void k_seg_keep(int64_t n,
+                                                     const uint64_t* off,
+                                                     const uint32_t* bcnt,
+                                                     const uint8_t* flag,
+                                                     uint32_t* keep) {
+  const int64_t i = gtid();
+  if (i >= n) return;
+  const uint64_t base = off[i];
+  const uint32_t m = bcnt[i];
+  uint32_t c = 0;
+  for (uint32_t t = 0; t < m; ++t) c += flag[base + t] == KEEP;
+  keep[i] = c;
+}
+void launch_seg_keep(int64_t n, const uint64_t* off, const uint32_t* bcnt,
+                     const uint8_t* flag, uint32_t* keep, PipeLaunch L) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_seg_keep, dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, n, off, bcnt, flag, keep);
+}
+/* ball i's kept edges (one wave per ball) to (ecol, eleft)[erow[i], ...) */
+__global__ __launch_bounds__(kBlock) void k_edge_copy(
+    int64_t n, const uint64_t* off, const uint32_t* bcnt, const uint32_t* cval,
+    const uint8_t* flag, const uint64_t* erow, uint32_t* ecol, uint32_t* eleft,
+    uint64_t ocap) {
+  const int64_t iw = (int64_t)blockIdx.x * kEdgeBallsPerBlock + (threadIdx.x >> 6);
+  if (iw >= n) return;
+  const uint32_t i = __builtin_amdgcn_readfirstlane((uint32_t)iw);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint64_t base = off[i];
+  const uint32_t m = bcnt[i];
+  uint64_t o = erow[i];
+  for (uint32_t t0 = 0; t0 < m; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const bool keep = t < m && flag[base + t] == KEEP;
+    const uint64_t bm = __ballot(keep);
+    if (keep) {
+      const uint64_t w = o + (uint64_t)__popcll(bm & lt);
+      if (w < ocap) {
+        ecol[w] = cval[base + t];
+        eleft[w] = i;
+      }
+    }
+    o += (uint64_t)__popcll(bm);
+  }
+}
+void launch_edge_copy(int64_t n, const uint64_t* off, const uint32_t* bcnt,
+                      const uint32_t* cval, const uint8_t* flag,
+                      const uint64_t* erow, uint32_t* ecol, uint32_t* eleft,
+                      PipeLaunch L, uint64_t ocap) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(
+      k_edge_copy,
+      dim3((unsigned)((n + kEdgeBallsPerBlock - 1) / kEdgeBallsPerBlock)),
+      dim3(kBlock), 0, L.s, n, off, bcnt, cval, flag, erow, ecol, eleft, ocap);
+}
+
+/* Triangles of alpha edge q = (i, j) -- k in row(i) after j and in row(j),
+ * the positions of k in the two rows being the alpha-edge ids of (i, k)
+ * and (j, k), recorded for the tet stage and the closure -- fused with the
  * triangle filter's first stage (markRealTriangles, Parallel:1294-1340):
  * the block's candidate slots are staged in LDS during the row merges,
  * filtered round robin (uniform work per lane) and the kept and undecided
@@ -1221,25 +1262,6 @@ __global__ __launch_bounds__(kBlock) void k_filter_exact(Stage S, CandCtx C,
   flush_ctr(S.ctr, CTR_STAGE + fam_of(KIND) * 4 + 3, cnt);
 }
 
-void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
-                   const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
-                   Queue q1, PipeLaunch L) {
-  if (nc.cap <= 0) return;
-  dim3 gr((unsigned)((nc.cap + kBlock - 1) / kBlock));
-#define PAC_FILTER(K)                                                        \
-  if (S.staged)                                                              \
-    hipLaunchKernelGGL((k_filter<K, float>), gr, dim3(kBlock), 0, L.s, S, C, \
-                       nc, cval, cseg, flag, q1);                            \
-  else                                                                       \
-    hipLaunchKernelGGL((k_filter<K, double>), gr, dim3(kBlock), 0, L.s, S, C, \
-                       nc, cval, cseg, flag, q1);
-  if (kind == 1) {
-    PAC_FILTER(1)
-  } else {
-    PAC_FILTER(2)
-  }
-#undef PAC_FILTER
-}
 void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L) {
   if (!q.launch_items()) return;
@@ -1273,71 +1295,5 @@ void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint
 /* survivors: segment starts and flat stable compaction                */
 /* ------------------------------------------------------------------ */
 /* orow[s] = number of survivors before segment s = pos[off[s]] */
-__global__ __launch_bounds__(kBlock) void k_seg_starts(Cnt c_nseg,
-                                                       const uint64_t* off,
-                                                       const uint32_t* pos,
-                                                       uint64_t* orow,
-                                                       uint64_t poslim,
-                                                       uint64_t vlim) {
-  const int64_t nseg = c_nseg.get();
-  int64_t s = gtid();
-  if (s > nseg) return;
-  const uint64_t o = off[s];
-  const uint64_t v = pos[o < poslim ? o : poslim];
-  orow[s] = v < vlim ? v : vlim;
-}
-void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L, uint64_t poslim,
-                       uint64_t vlim) {
-  hipLaunchKernelGGL(k_seg_starts,
-                     dim3((unsigned)((nseg.cap + 1 + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nseg, off, pos, orow, poslim, vlim);
-}
-/* KIND 1: (ecol, eleft) = (val, seg); 2: (tk, tedge) = (val, seg);
- * 3: tets = (i, j, k, l) */
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_compact_flat(
-    Cnt c_nc, CandCtx C, const uint32_t* cval, const uint32_t* cseg,
-    const uint8_t* flag, const uint32_t* pos, uint32_t* out_val,
-    uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
-    const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b, uint64_t ocap) {
-  const int64_t nc = c_nc.get();
-  int64_t o = gtid();
-  if (o >= nc || flag[o] != KEEP) return;
-  uint64_t w = pos[o];
-  if (w >= ocap) return;
-  if (KIND < 3) {
-    out_val[w] = cval[o];
-    out_seg[w] = cseg[o];
-    if (KIND == 2) {
-      out_a[w] = cea[o];
-      out_b[w] = ceb[o];
-    }
-  } else {
-    out_tet[w] = C.ctet[o];
-  }
-}
-void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
-                         const uint32_t* cseg, const uint8_t* flag,
-                         const uint32_t* pos, uint32_t* out_val,
-                         uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
-                         const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
-                         PipeLaunch L, uint64_t ocap) {
-  if (nc.cap <= 0) return;
-  dim3 gr((unsigned)((nc.cap + kBlock - 1) / kBlock));
-  if (kind == 1)
-    hipLaunchKernelGGL(k_compact_flat<1>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b, ocap);
-  else if (kind == 2)
-    hipLaunchKernelGGL(k_compact_flat<2>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b, ocap);
-  else
-    hipLaunchKernelGGL(k_compact_flat<3>, gr, dim3(kBlock), 0, L.s, nc, C, cval,
-                       cseg, flag, pos, out_val, out_seg, out_tet, cea, ceb,
-                       out_a, out_b, ocap);
-}
-
 }  // namespace PAC_V
 }  // namespace pac

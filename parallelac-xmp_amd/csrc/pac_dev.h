@@ -67,6 +67,46 @@ __device__ __forceinline__ unsigned block_append(unsigned* counter, bool want) {
   __syncthreads();
   return r;
 }
+/* Append v (where want) to the sub-list of this block (LSLOTS sub-lists
+ * of lcap entries, counters cnt[LSLOTS]); every thread of the block must
+ * call it. */
+__device__ __forceinline__ void list_append(uint32_t* data, unsigned* cnt,
+                                            uint32_t lcap, bool want,
+                                            uint32_t v) {
+  const unsigned k = blockIdx.x & (LSLOTS - 1);
+  const unsigned at = block_append(cnt + k, want);
+  if (want && at < lcap) data[(uint64_t)k * lcap + at] = v;
+}
+/* A consumer block's table of the sub-lists' starts in the concatenation:
+ * s_pre[k] = entries of sub-lists before k, s_pre[LSLOTS] = the total.
+ * Every thread of the block must call it. */
+__device__ __forceinline__ void list_table(const unsigned* cnt, uint32_t lcap,
+                                           uint32_t* s_pre) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    uint32_t c = lane < LSLOTS ? min(cnt[lane], lcap) : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane < LSLOTS) s_pre[lane + 1] = incl;
+    if (lane == 0) s_pre[0] = 0;
+  }
+  __syncthreads();
+}
+/* entry w of the concatenation (w < s_pre[LSLOTS]) */
+__device__ __forceinline__ uint32_t list_at(const uint32_t* data, uint32_t lcap,
+                                            const uint32_t* s_pre, uint32_t w) {
+  int lo = 0, hi = LSLOTS - 1; /* last k with s_pre[k] <= w */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_pre[mid] <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return data[(uint64_t)lo * lcap + (w - s_pre[lo])];
+}
 /* Block-staged output of a one-thread-per-segment enumeration.  The
  * segments of a block own the contiguous output range [base, base+total)
  * (their exclusive-scan offsets); each thread's entries go to LDS at their
@@ -148,26 +188,22 @@ struct StageCnt {
 __device__ __forceinline__ int64_t gtid() {
   return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 }
-/* Persistent work queue over [0, total): each wave of the grid takes
- * `chunk` consecutive entries (a multiple of 64) at a time from *work
- * (cleared before the launch), each lane every 64th entry of its chunk,
- * until none are left.  The chunk loop is wave-uniform.  Waves whose index
- * times chunk reaches total leave at once: the waves below keep taking
- * chunks until the work is gone, and a short list costs a few atomics on
- * the counter instead of one per wave of the grid. */
+/* Persistent launch over [0, total): the grid's waves take `chunk`
+ * consecutive entries (a multiple of 64) at a time, wave w chunks w,
+ * w + W, w + 2W, ... (W waves in the grid), each lane every 64th entry of
+ * its chunk.  The chunk loop is wave-uniform.  The assignment is static: a
+ * shared work counter cost one device-scope atomic per chunk, and with
+ * ~10^4 chunks those serialized (second phase 1.57 -> 1.17 ms on the 100k
+ * globule at budget 4 without it). */
 template <class F>
-__device__ __forceinline__ void for_work(unsigned* work, unsigned chunk,
-                                         int64_t total, F&& body) {
+__device__ __forceinline__ void for_work(unsigned chunk, int64_t total,
+                                         F&& body) {
   const int lane = threadIdx.x & 63;
-  if ((gtid() >> 6) * (int64_t)chunk >= total) return;
-  for (;;) {
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(work, chunk);
-    base = (unsigned)__shfl((int)base, 0, 64);
-    if ((int64_t)base >= total) break;
-    const int64_t end =
-        (int64_t)base + chunk < total ? (int64_t)base + chunk : total;
-    for (int64_t w = (int64_t)base + lane; w < end; w += 64) body(w);
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t base = (gtid() >> 6) * (int64_t)chunk; base < total;
+       base += nw * (int64_t)chunk) {
+    const int64_t end = base + chunk < total ? base + chunk : total;
+    for (int64_t w = base + lane; w < end; w += 64) body(w);
   }
 }
 __device__ __forceinline__ bool same_pos(const Ball& a, const Ball& b) {

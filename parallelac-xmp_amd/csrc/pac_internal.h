@@ -137,11 +137,23 @@ struct PipeLaunch {
  * that many simultaneous appends do not serialize on one counter; sub-queue
  * k holds entries [k*cap, k*cap + n[k]). */
 constexpr int QSLOTS = 32;
+/* Item lists appended to a block at a time (block_append) by
+ * one-item-per-thread kernels -- the simplices an emptiness scan's first
+ * phase leaves, the tets a quick rejection pass leaves -- are spread the
+ * same way over LSLOTS sub-lists: block b appends to sub-list b % LSLOTS,
+ * entries [k * lcap, k * lcap + n[k]) with n[k] <= lcap, lcap =
+ * list_slot_cap(the producer's blocks).  One counter for the ~10^5 blocks
+ * of a launch serialized their device-scope atomics (on the 100k globule,
+ * 0.45 ms of a 0.88 ms tet emptiness first phase). */
+constexpr int LSLOTS = 32;
+inline uint32_t list_slot_cap(int64_t blocks) {
+  return (uint32_t)((blocks + LSLOTS - 1) / LSLOTS * kBlock);
+}
 /* queue counter words of a run: the hot and escalation sub-queue counters
- * (2 QSLOTS), then the count of simplices left for the second emptiness
- * phase, the count of tets a quick rejection pass left, and the work
- * counter of the persistent second phase */
-constexpr int QN_WORDS = 2 * QSLOTS + 3;
+ * (2 QSLOTS), then the counters of the list of simplices left for the
+ * second emptiness phase and of the list of tets a quick rejection pass
+ * left (2 LSLOTS) */
+constexpr int QN_WORDS = 2 * QSLOTS + 2 * LSLOTS;
 struct Queue {
   QEntry* q;
   unsigned* n;   /* QSLOTS counters */
@@ -199,7 +211,8 @@ enum { REC_CAP = 0,    /* bits 0..8: a count exceeded its capacity */
        REC_Q1 = 9,     /* hot queue overflow */
        REC_Q2 = 10,    /* escalation queue overflow */
        REC_STALE = 11  /* the input no longer matches the recorded plan */ };
-/* view of QSLOTS sub-queues (counters cleared after reading, and *also);
+/* view of QSLOTS sub-queues (counters cleared after reading, and the
+ * LSLOTS list counters at also);
  * fresh: status / fill overwritten, else merged */
 void launch_make_view(const QEntry* q, uint32_t cap, unsigned* cnt, QView* v,
                       unsigned* status, unsigned bit, unsigned* fill,
@@ -333,13 +346,14 @@ struct Stage {
   ExactScale ex;
   unsigned long long* ctr;
   int staged;
-  /* second emptiness phase: a persistent work-queue launch of bgrid blocks
-   * of bblock threads whose waves take bchunk entries of the left-over
-   * list at a time from the counter *bwork (cleared before each launch) */
+  /* second emptiness phase: a persistent launch of bgrid blocks of bblock
+   * threads whose waves take bchunk entries of the left-over list at a
+   * time (for_work) */
   int bgrid;
   int bblock;
   unsigned bchunk;
-  unsigned* bwork;
+  /* entries per sub-list of the left-over and quick-pass lists */
+  uint32_t lcap;
   /* exact cell policy: the emptiness scans run over a finer grid of their
    * own (reference cells split along x), its balls in fine-cell order */
   int fine;
@@ -366,9 +380,24 @@ struct Stage {
  * ball (edges), alpha edge (triangles), alpha triangle (tets). */
 
 void launch_edge_count(const Stage& S, uint32_t* cnt, PipeLaunch L);
-/* lim: capacity of the candidate arrays (entries past it are dropped) */
-void launch_edge_enum(const Stage& S, const uint64_t* off, uint32_t* cval,
-                      uint32_t* cseg, PipeLaunch L, uint64_t lim = ~0ull);
+/* edge candidates of every ball, each filtered by the first precision
+ * stage in the same kernel: ball i's kept and undecided candidates (j, i)
+ * with their flags, in order, compacted from its first slot off[i] on and
+ * their number to bcnt[i]; undecided ones queued on q1 by position (aux =
+ * i) */
+void launch_edge_enum_filter(const Stage& S, const uint64_t* off,
+                             uint32_t* cval, uint32_t* cseg, uint8_t* flag,
+                             uint32_t* bcnt, Queue q1, PipeLaunch L,
+                             uint64_t lim = ~0ull);
+/* after the ladder: keep[i] = kept candidates of ball i; then, with erow
+ * their exclusive scan (the alpha-edge CSR rows), the kept edges to
+ * (ecol, eleft) in order */
+void launch_seg_keep(int64_t n, const uint64_t* off, const uint32_t* bcnt,
+                     const uint8_t* flag, uint32_t* keep, PipeLaunch L);
+void launch_edge_copy(int64_t n, const uint64_t* off, const uint32_t* bcnt,
+                      const uint32_t* cval, const uint8_t* flag,
+                      const uint64_t* erow, uint32_t* ecol, uint32_t* eleft,
+                      PipeLaunch L, uint64_t ocap = ~0ull);
 /* slot capacity of item q = row[owner[q] + 1] - q - 1 */
 void launch_tail_caps(Cnt m, const uint64_t* row, const uint32_t* owner,
                       uint32_t* cap, PipeLaunch L);
@@ -381,10 +410,6 @@ void launch_tet_caps(const Stage& S, Cnt m, const uint64_t* trow,
 /* exact triangle count per alpha edge */
 void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, uint32_t* cnt, PipeLaunch L);
-void launch_tri_enum(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
-                     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
-                     uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                     unsigned long long* ctr, PipeLaunch L, uint64_t lim = ~0ull);
 /* triangle candidates of the alpha edges, each filtered by the first
  * precision stage in the same kernel: enumeration block b (256 alpha
  * edges) writes its kept and undecided candidates (k, edge, edge(i,k),
@@ -428,25 +453,12 @@ void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
                            const uint4* ctet, const uint8_t* flag,
                            const uint64_t* bout, uint4* tets, PipeLaunch L,
                            uint64_t ocap = ~0ull);
-/* first stage: kind 1 edges, 2 triangles (tets: launch_tet_enum_filter);
- * escalation and exact stages: kind 1, 2, 3 */
-void launch_filter(const Stage& S, int kind, CandCtx C, Cnt nc,
-                   const uint32_t* cval, const uint32_t* cseg, uint8_t* flag,
-                   Queue q1, PipeLaunch L);
+/* escalation and exact stages of the filters (the first stages are fused
+ * with the enumerations): kind 1 edges, 2 triangles, 3 tets */
 void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                        Queue q2, PipeLaunch L);
 void launch_filter_exact(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,
                          PipeLaunch L);
-/* orow[s] = pos[min(off[s], poslim)], clamped to vlim */
-void launch_seg_starts(Cnt nseg, const uint64_t* off, const uint32_t* pos,
-                       uint64_t* orow, PipeLaunch L, uint64_t poslim = ~0ull,
-                       uint64_t vlim = ~0ull);
-void launch_compact_flat(int kind, Cnt nc, CandCtx C, const uint32_t* cval,
-                         const uint32_t* cseg, const uint8_t* flag,
-                         const uint32_t* pos, uint32_t* out_val,
-                         uint32_t* out_seg, uint4* out_tet, const uint32_t* cea,
-                         const uint32_t* ceb, uint32_t* out_a, uint32_t* out_b,
-                         PipeLaunch L, uint64_t ocap = ~0ull);
 
 /* ---- emptiness (k_ac2.hip) ---- */
 /* rest / nrest: scratch list of simplices left for phase B (capacity =

@@ -319,8 +319,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   }
   unsigned long long* const ctr = SG.ctr;
   /* [0, QSLOTS) hot queue, [QSLOTS, 2 QSLOTS) escalation queue, then the
-   * count of simplices left for the second emptiness phase and the count
-   * of tets a quick rejection pass left */
+   * LSLOTS counters of the list of simplices left for the second emptiness
+   * phase and those of the list of tets a quick rejection pass left */
   DBuf<unsigned> qn_own;
   if (!rec) {
     qn_own = DBuf<unsigned>(P, QN_WORDS);
@@ -328,7 +328,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   }
   unsigned* const qn = rec ? rec->qn : qn_own.get();
   unsigned* const nrest = qn + 2 * QSLOTS;
-  SG.bwork = qn + 2 * QSLOTS + 2;
+  unsigned* const nnext = nrest + LSLOTS;
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
@@ -464,7 +464,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     QView v1;
     for (;;) {
       q1 = DBuf<QEntry>(P, (size_t)c1 * QSLOTS);
-      zero_u32(nrest, 1, s);
+      zero_u32(nrest, LSLOTS, s);
       if (kp_hot >= 0) KP.begin(kp_hot);
       hot(Queue{q1.get(), qn, c1});
       if (kp_hot >= 0) KP.end();
@@ -494,30 +494,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     return settle(q);
   };
 
-  /* alpha filter of one dimension over its enumerated candidates: the
-   * ladder settles every flag, a flat scan of the flags gives both the
-   * stable compaction positions and the survivors' CSR row starts. */
-  int64_t host_evals[6] = {0, 0, 0, 0, 0, 0}; /* hot-stage evaluations booked on the host */
-  auto filter_stage = [&](int kind, CandCtx C, Cnt nc, const uint32_t* cval,
-                          const uint32_t* cseg, uint8_t* flag, int kp,
-                          DBuf<uint32_t>& pos, int q, auto&& hot) -> Cnt {
-    if (kind == 1) host_evals[0] += nc.cap; /* tris / tets: slots != candidates */
-    pos = DBuf<uint32_t>(P, nc.cap + 1);
-    auto tmp = scan_tmp(nc.cap);
-    /* few candidates stay undecided in float: start small */
-    return ladder(
-        std::min<int64_t>(nc.cap, nc.cap / 32 + 65536), kp, -1, hot,
-        [&](QRef qr, Queue q2) {
-          launch_filter_esc(SG, kind, C, qr, cval, flag, q2, L);
-        },
-        [&](QRef qr) {
-          launch_filter_exact(SG, kind, C, qr, cval, flag, L);
-        },
-        [&] {
-          scan_u8flag_to_u32(flag, pos.get(), nc, tmp.get(), 0, s, dcnt + q);
-        },
-        q);
-  };
+  /* hot-stage evaluations booked on the host: one per edge candidate
+   * (triangle and tet candidates are counted by their kernels); the
+   * filters' first stages run fused with the enumerations, each a ladder
+   * with the escalation and exact kernels of its kind */
+  int64_t host_evals[6] = {0, 0, 0, 0, 0, 0};
   auto count_scan = [&](const uint32_t* cnt, Cnt nseg, uint64_t* off, int q,
                         const char* what) -> Cnt {
     {
@@ -549,21 +530,38 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     }
     DBuf<uint32_t> cval(P, Ecand.cap), cseg(P, Ecand.cap);
     DBuf<uint8_t> flag(P, Ecand.cap);
-    launch_edge_enum(SG, off.get(), cval.get(), cseg.get(), L,
-                     (uint64_t)Ecand.cap);
-    DBuf<uint32_t> pos;
-    Ealpha = filter_stage(1, C0, Ecand, cval.get(), cseg.get(), flag.get(),
-                          KP_EDGE_FILL, pos, RC_EALPHA, [&](Queue qq) {
-                            launch_filter(SG, 1, C0, Ecand, cval.get(),
-                                          cseg.get(), flag.get(), qq, L);
-                          });
-    launch_seg_starts(n, off.get(), pos.get(), erow.get(), L,
-                      (uint64_t)Ecand.cap, (uint64_t)Ealpha.cap);
+    /* enumeration and first filter stage in one kernel (each ball's kept
+     * and undecided candidates compacted from its first slot, their count
+     * in bcnt); the ladder settles the undecided ones; the kept counts'
+     * scan is the alpha-edge CSR (erow) and the kept edges are copied out
+     * in order */
+    DBuf<uint32_t> bcnt(P, n), keep(P, n);
+    auto tmp = scan_tmp(n);
+    host_evals[0] += Ecand.cap;
+    Ealpha = ladder(
+        std::min<int64_t>(Ecand.cap, Ecand.cap / 32 + 65536), KP_EDGE_FILL, -1,
+        [&](Queue qq) {
+          launch_edge_enum_filter(SG, off.get(), cval.get(), cseg.get(),
+                                  flag.get(), bcnt.get(), qq, L,
+                                  (uint64_t)Ecand.cap);
+        },
+        [&](QRef qr, Queue q2) {
+          launch_filter_esc(SG, 1, C0, qr, cval.get(), flag.get(), q2, L);
+        },
+        [&](QRef qr) {
+          launch_filter_exact(SG, 1, C0, qr, cval.get(), flag.get(), L);
+        },
+        [&] {
+          launch_seg_keep(n, off.get(), bcnt.get(), flag.get(), keep.get(), L);
+          scan_u32_to_u64(keep.get(), erow.get(), Cnt(n), tmp.get(), 0, s,
+                          dcnt + RC_EALPHA);
+        },
+        RC_EALPHA);
     ecol = DBuf<uint32_t>(P, Ealpha.cap);
     eleft = DBuf<uint32_t>(P, Ealpha.cap);
-    launch_compact_flat(1, Ecand, C0, cval.get(), cseg.get(), flag.get(),
-                        pos.get(), ecol.get(), eleft.get(), nullptr, nullptr,
-                        nullptr, nullptr, nullptr, L, (uint64_t)Ealpha.cap);
+    launch_edge_copy(n, off.get(), bcnt.get(), cval.get(), flag.get(),
+                     erow.get(), ecol.get(), eleft.get(), L,
+                     (uint64_t)Ealpha.cap);
     KCHK();
   } else {
     zero_u64((unsigned long long*)erow.get(), n + 1, s);
@@ -698,8 +696,13 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* ---- tet emptiness ---- */
   /* phase-one budget: 0 = no second phase (the default) */
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : 0;
-  DBuf<uint32_t> rest(P, std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}));
-  DBuf<uint32_t> next(P, pl.kf || sub ? std::max(Kalpha.cap, (int64_t)1) : 1);
+  /* the left-over and quick-pass lists: LSLOTS sub-lists sized for the
+   * largest first-phase grid */
+  SG.lcap = list_slot_cap(
+      (std::max({Kalpha.cap, Talpha.cap, Ealpha.cap, (int64_t)1}) + kBlock - 1) /
+      kBlock);
+  DBuf<uint32_t> rest(P, (size_t)LSLOTS * SG.lcap);
+  DBuf<uint32_t> next(P, pl.kf || sub ? (size_t)LSLOTS * SG.lcap : 1);
   DBuf<uint8_t> kflag(P, Kalpha.cap);
   Cnt Kf(0);
   DBuf<uint4> ftets;
@@ -720,7 +723,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           ac2_cap(Kalpha.cap), KP_TET_AC2, KP_TET_AC2_EXACT,
           [&](Queue q) {
             launch_tet_ac2_hot(SG, Kalpha, tets.get(), kflag.get(), q,
-                               rest.get(), nrest, next.get(), nrest + 1,
+                               rest.get(), nrest, next.get(), nnext,
                                budget, L);
           },
           [&](QRef q, Queue q2) {
