@@ -52,8 +52,8 @@ typedef struct pac_options {
   int32_t profile;       /* 1: time the main kernels with hipEvents       */
   int32_t ac2_budget;    /* emptiness phase-one pair budget: > 0 splits
                             each scan after that many balls, the rest
-                            finished by the persistent second phase;
-                            <= 0: one phase, every scan to its end        */
+                            finished by the persistent second phase
+                            (0 = 4); < 0: one phase, every scan to its end */
   int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
   int32_t ac2_grid;      /* blocks of the second emptiness phase (0=2048) */
   int32_t ac2_fine;      /* exact policy: emptiness scans over the grid's
@@ -235,7 +235,7 @@ void pac_graph_free(pac_graph* g);
  * emptiness scan grid, exact cell policy: x subdivision 2, 4 or the
  * reference grid, tried first), opt->ac2_quick (with a fine grid: the
  * quick rejection pass on or off), then opt->ac2_budget / opt->ac2_grid
- * (one phase, or a phase-one pair budget of 8, 16 or 32 x second-phase
+ * (one phase, or a phase-one pair budget of 2, 4, 8 or 16 x second-phase
  * grid, over the chosen scan grid), then, with a second phase,
  * opt->block / opt->ac2_chunk (its block shape and queue granularity);
  * the other fields of *opt

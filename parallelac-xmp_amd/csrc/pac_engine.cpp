@@ -489,7 +489,7 @@ static void autotune(const float* xyzr, int64_t n, const int64_t* offsets,
    * budget and grid), then the phase-one budget x second-phase grid over
    * them */
   static const int scans[][2] = {{2, 0}, {4, 0}, {2, 1}, {4, 1}, {-1, 0}};
-  static const int budgets[] = {-1, 8, 16, 32};
+  static const int budgets[] = {-1, 2, 4, 8, 16};
   static const int grids[] = {1024, 2048, 4096};
   double best = 1e300;
   pac_options pick = base;

@@ -694,8 +694,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   };
 
   /* ---- tet emptiness ---- */
-  /* phase-one budget: 0 = no second phase (the default) */
-  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : 0;
+  /* phase-one budget: 0 = the default (4), < 0 = no second phase */
+  int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 4);
   /* the left-over and quick-pass lists: LSLOTS sub-lists sized for the
    * largest first-phase grid */
   SG.lcap = list_slot_cap(
