@@ -352,18 +352,19 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         return true;
       }
       seen += (int)n;
-      pairs += n;
       Ball E[4];
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
-      bool viol = false;
-      unsigned ties = 0;
+      unsigned viol = 0, ties = 0;
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) {
         const int s = pairss_sign(P, E[k]);
-        viol |= k < n && s * P.sden < 0;
+        viol |= (k < n && s * P.sden < 0) ? 1u << k : 0u;
         ties |= (k < n && s == 0) ? 1u << k : 0u;
       }
+      /* pairs counted as a scan stopping at its first violator tests them
+       * (the group's tests past it are not work the scan needs) */
+      pairs += viol ? (unsigned)__builtin_ctz(viol) + 1u : n;
       if (viol) {
         stop = 1;
         return true;
@@ -825,18 +826,19 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
         return true;
       }
       seen += (int)n;
-      pairs += n;
       Ball E[4];
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
-      bool viol = false;
-      unsigned ties = 0;
+      unsigned viol = 0, ties = 0;
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) {
         const int s = pairss_sign(P, E[k]);
-        viol |= k < n && s < 0;
+        viol |= (k < n && s < 0) ? 1u << k : 0u;
         ties |= (k < n && s == 0) ? 1u << k : 0u;
       }
+      /* pairs counted as a scan stopping at its first violator tests them
+       * (the group's tests past it are not work the scan needs) */
+      pairs += viol ? (unsigned)__builtin_ctz(viol) + 1u : n;
       if (viol) { /* power <= rho strictly: violator */
         stop = 1;
         return true;
