@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--ac2-chunk", type=int, default=0,
                     help="queue granularity of the persistent second "
                          "emptiness phase (with --budget and --grid)")
+    ap.add_argument("--graph-pass-balls", type=int, default=0,
+                    help="batch workload: balls per pass of the recorded "
+                         "graph (0 = the library's default, 2 Mi)")
     ap.add_argument("--chunk", type=int, default=100000,
                     help="stress workload: also run the chunked (LD) driver "
                          "with this chunk_balls (0 = skip)")
@@ -410,7 +413,8 @@ def bench_batch(args, torch, pac):
     log("graph: recording the batch")
     t0 = time.perf_counter()
     g = pac.Graph(ptr, off, 1.4, 0.0, precision=pac.PREC_STAGED,
-                  device_inputs=True, **tuned[pac.PREC_STAGED])
+                  device_inputs=True, batch_balls=args.graph_pass_balls,
+                  **tuned[pac.PREC_STAGED])
     t_rec = time.perf_counter() - t0
     nodes, gbytes = g.info()
     for _ in range(args.warmup):

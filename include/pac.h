@@ -54,7 +54,8 @@ typedef struct pac_options {
                             each scan after that many balls, the rest
                             finished by the persistent second phase
                             (0 = 4); < 0: one phase, every scan to its end */
-  int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi)  */
+  int32_t batch_balls;   /* batch: max balls per pipeline pass (0 = 4Mi;
+                            a recorded batch, pac_graph_create: 2Mi)      */
   int32_t ac2_grid;      /* blocks of the second emptiness phase (0=2048) */
   int32_t ac2_fine;      /* exact policy: emptiness scans over the grid's
                             cells split in ac2_fine (2..8) along x

@@ -185,10 +185,10 @@ void graph_create(const float* xyzr, const int64_t* offsets, int64_t nmol,
   if (!(headroom >= 0)) headroom = 0.125;
   if (headroom > 64) fail(PAC_ERR_INVALID, "headroom above 64");
   /* A recorded batch keeps every pass's outputs and shares one set of
-   * scratch buffers sized for its largest pass: passes of 1 Mi balls
-   * (unless opt.batch_balls says otherwise) hold that scratch to a quarter
-   * of the eager default's at the cost of more graph nodes. */
-  if (opt.batch_balls <= 0) opt.batch_balls = 1 << 20;
+   * scratch buffers sized for its largest pass: passes of 2 Mi balls
+   * (unless opt.batch_balls says otherwise) hold that scratch to half of
+   * the eager default's at the cost of more graph nodes. */
+  if (opt.batch_balls <= 0) opt.batch_balls = 1 << 21;
   /* the eager run: validates the arguments, fixes the passes, measures */
   pac_options popt = opt;
   popt.profile = 0;
