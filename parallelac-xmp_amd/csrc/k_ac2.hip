@@ -978,6 +978,99 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_esc(
   sc.flush(S.ctr, KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2);
 }
 
+/* The exact triangle / edge test's per-simplex part: its vertices, the
+ * first vertex's scaled position and weight, den and num (what every pair
+ * test of the simplex reuses). */
+struct LowExact {
+  uint32_t item = 0xffffffffu;
+  uint32_t vid[3];
+  Ball a;
+  V3<XI> pa, num;
+  XI wa, den;
+};
+template <int KIND>
+__device__ __forceinline__ void low_exact_setup(const Stage& S,
+                                                const uint32_t* ecol,
+                                                const uint32_t* eleft,
+                                                const uint32_t* tk,
+                                                const uint32_t* tedge,
+                                                uint32_t item, LowExact& T) {
+  T.item = item;
+  if (KIND == 0) {
+    const uint32_t ed = tedge[item];
+    T.vid[0] = eleft[ed];
+    T.vid[1] = ecol[ed];
+    T.vid[2] = tk[item];
+  } else {
+    T.vid[0] = eleft[item];
+    T.vid[1] = ecol[item];
+    T.vid[2] = 0xffffffffu;
+  }
+  const LdCW<kTPI> L{S.ex.S};
+  T.a = S.ball[T.vid[0]];
+  const Ball b = S.ball[T.vid[1]];
+  T.pa = L.p(T.a);
+  T.wa = L.w(T.a);
+  if (KIND == 0) {
+    tri_X(T.pa, T.wa, L.p(b), L.w(b), L.p(S.ball[T.vid[2]]),
+          L.w(S.ball[T.vid[2]]), &T.den, &T.num);
+  } else {
+    V3<XI> u = vsub(L.p(b), T.pa);
+    T.den = vdot(u, u);
+    XI A = T.den + T.wa - L.w(b);
+    T.num = vscale(A, u);
+  }
+}
+/* One queued entry of simplex T: a (simplex, ball) pair, or a full exact
+ * scan of its box (aux == ~0, a simplex whose centre no float stage could
+ * certify).  Group-uniform; the group's first lane writes the flag. */
+template <int KIND>
+__device__ __forceinline__ void low_exact_entry(const Stage& S,
+                                                const LowExact& T,
+                                                uint32_t aux, uint8_t* flag,
+                                                bool lead, unsigned& cnt) {
+  const LdCW<kTPI> L{S.ex.S};
+  if (aux != 0xffffffffu) {
+    ++cnt;
+    const Ball E = S.ball[aux];
+    if (fe_sign(power_gap(T.pa, T.wa, L.p(E), L.w(E), T.den, T.num)) <= 0 &&
+        lead)
+      flag[T.item] = 0;
+  } else if (fe_sign(T.den) > 0) {
+    double d2 = 2.0 * cw_to_double(T.den, (KIND == 0 ? 4 : 2) * S.ex.S);
+    int sh = (KIND == 0 ? 5 : 3) * S.ex.S;
+    double cx = (double)T.a.x + cw_to_double(T.num.x, sh) / d2;
+    double cy = (double)T.a.y + cw_to_double(T.num.y, sh) / d2;
+    double cz = (double)T.a.z + cw_to_double(T.num.z, sh) / d2;
+    double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
+    CellBox box;
+    bool viol = false;
+    const GridP g = grid_of(S, T.vid[0]);
+    if (make_box(g, cx, cy, cz, err, err, err, box)) {
+      for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
+        for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
+          int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
+          uint32_t bb = S.cstart[row + box.lo[0]],
+                   en = S.cstart[row + box.hi[0] + 1];
+          for (uint32_t p = bb; p < en; ++p) {
+            if (p == T.vid[0] || p == T.vid[1] || (KIND == 0 && p == T.vid[2]))
+              continue;
+            ++cnt;
+            const Ball E = S.ball[p];
+            if (fe_sign(power_gap(T.pa, T.wa, L.p(E), L.w(E), T.den, T.num)) <=
+                0) {
+              viol = true;
+              break;
+            }
+          }
+        }
+    }
+    if (lead) flag[T.item] = viol ? 0 : 2;
+  } else if (lead) {
+    flag[T.item] = 0; /* degenerate simplex: cannot be an alpha simplex */
+  }
+}
+
 /* exact triangle / edge pair (or full scan when aux == ~0); one item per
  * group of kTPI lanes */
 template <int KIND>
@@ -990,73 +1083,42 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_exact(
   for_queue_grp<kTPI>(q, [&](uint32_t t) {
     /* the group's first lane decides, so the group stays together */
     if (!Grp<kTPI>::from(live_entry(q.at(t), flag) ? 1u : 0u, 0)) return;
-    QEntry e = q.at(t);
-    uint32_t vid[3];
-    if (KIND == 0) {
-      uint32_t ed = tedge[e.item];
-      vid[0] = eleft[ed];
-      vid[1] = ecol[ed];
-      vid[2] = tk[e.item];
-    } else {
-      vid[0] = eleft[e.item];
-      vid[1] = ecol[e.item];
-      vid[2] = 0xffffffffu;
-    }
-    const LdCW<kTPI> L{S.ex.S};
-    Ball a = S.ball[vid[0]], b = S.ball[vid[1]];
-    V3<XI> pa = L.p(a), num;
-    XI wa = L.w(a), den;
-    if (KIND == 0) {
-      tri_X(pa, wa, L.p(b), L.w(b), L.p(S.ball[vid[2]]), L.w(S.ball[vid[2]]),
-            &den, &num);
-    } else {
-      V3<XI> u = vsub(L.p(b), pa);
-      den = vdot(u, u);
-      XI A = den + wa - L.w(b);
-      num = vscale(A, u);
-    }
-    if (e.aux != 0xffffffffu) {
-      ++cnt;
-      Ball E = S.ball[e.aux];
-      if (fe_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0 && lead)
-        flag[e.item] = 0;
-    } else if (fe_sign(den) > 0) {
-      double d2 = 2.0 * cw_to_double(den, (KIND == 0 ? 4 : 2) * S.ex.S);
-      int sh = (KIND == 0 ? 5 : 3) * S.ex.S;
-      double cx = (double)a.x + cw_to_double(num.x, sh) / d2;
-      double cy = (double)a.y + cw_to_double(num.y, sh) / d2;
-      double cz = (double)a.z + cw_to_double(num.z, sh) / d2;
-      double err = 1e-9 * (fabs(cx) + fabs(cy) + fabs(cz) + 1.0);
-      CellBox box;
-      bool viol = false;
-      const GridP g = grid_of(S, vid[0]);
-      if (make_box(g, cx, cy, cz, err, err, err, box)) {
-        for (int z = box.lo[2]; z <= box.hi[2] && !viol; ++z)
-          for (int y = box.lo[1]; y <= box.hi[1] && !viol; ++y) {
-            int64_t row = (int64_t)g.cbase + (int64_t)g.ex * (y + (int64_t)g.ey * z);
-            uint32_t bb = S.cstart[row + box.lo[0]],
-                     en = S.cstart[row + box.hi[0] + 1];
-            for (uint32_t p = bb; p < en; ++p) {
-              if (p == vid[0] || p == vid[1] || (KIND == 0 && p == vid[2]))
-                continue;
-              ++cnt;
-              Ball E = S.ball[p];
-              if (fe_sign(power_gap(pa, wa, L.p(E), L.w(E), den, num)) <= 0) {
-                viol = true;
-                break;
-              }
-            }
-          }
-      }
-      if (lead) flag[e.item] = viol ? 0 : 2;
-    } else if (lead) {
-      flag[e.item] = 0; /* degenerate simplex: cannot be an alpha simplex */
-    }
+    const QEntry e = q.at(t);
+    LowExact T;
+    low_exact_setup<KIND>(S, ecol, eleft, tk, tedge, e.item, T);
+    low_exact_entry<KIND>(S, T, e.aux, flag, lead, cnt);
   });
   flush_ctr(S.ctr, CTR_STAGE + (KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2) * 4 + 3,
             lead ? cnt : 0u);
 }
 
+/* The same over the entries sorted by simplex (item[], aux[]), kSortedRun
+ * consecutive entries per group at a time: the pairs of one simplex (an
+ * exact tie usually has several co-circular balls) share its den and num. */
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_low_ac2_exact_sorted(
+    Stage S, const uint32_t* item, const uint32_t* aux, uint32_t n,
+    const uint32_t* ecol, const uint32_t* eleft, const uint32_t* tk,
+    const uint32_t* tedge, uint8_t* flag) {
+  unsigned cnt = 0;
+  const bool lead = Grp<kTPI>::idx() == 0;
+  const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / kTPI;
+  LowExact T;
+  for (uint64_t c0 = (uint64_t)(gtid() / kTPI) * kSortedRun; c0 < n;
+       c0 += groups * kSortedRun) {
+    const uint32_t c1 = (uint32_t)min((uint64_t)n, c0 + kSortedRun);
+    for (uint32_t k = (uint32_t)c0; k < c1; ++k) {
+      const uint32_t it = item[k], ax = aux[k];
+      const bool live = ax == 0xffffffffu || flag[it] != 0;
+      if (!Grp<kTPI>::from(live ? 1u : 0u, 0)) continue;
+      if (it != T.item)
+        low_exact_setup<KIND>(S, ecol, eleft, tk, tedge, it, T);
+      low_exact_entry<KIND>(S, T, ax, flag, lead, cnt);
+    }
+  }
+  flush_ctr(S.ctr, CTR_STAGE + (KIND == 0 ? FAM_TRI_AC2 : FAM_EDGE_AC2) * 4 + 3,
+            lead ? cnt : 0u);
+}
 
 /* ---------------- launchers ---------------- */
 template <class R, bool FINE, bool MOL>
@@ -1264,6 +1326,22 @@ void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol
   else
     hipLaunchKernelGGL(k_low_ac2_exact<1>, gr, dim3(kBlock), 0, L.s, S, q,
                        ecol, eleft, tk, tedge, flag);
+}
+void launch_low_ac2_exact_sorted(const Stage& S, int kind, const uint32_t* item,
+                                 const uint32_t* aux, uint32_t n,
+                                 const uint32_t* ecol, const uint32_t* eleft,
+                                 const uint32_t* tk, const uint32_t* tedge,
+                                 uint8_t* flag, PipeLaunch L) {
+  if (!n) return;
+  const uint64_t groups = (n + kSortedRun - 1) / kSortedRun;
+  const uint64_t b = (groups * kTPI + kBlock - 1) / kBlock;
+  const dim3 gr((unsigned)(b < 65536 ? b : 65536));
+  if (kind == 0)
+    hipLaunchKernelGGL(k_low_ac2_exact_sorted<0>, gr, dim3(kBlock), 0, L.s, S,
+                       item, aux, n, ecol, eleft, tk, tedge, flag);
+  else
+    hipLaunchKernelGGL(k_low_ac2_exact_sorted<1>, gr, dim3(kBlock), 0, L.s, S,
+                       item, aux, n, ecol, eleft, tk, tedge, flag);
 }
 
 int sos_table_host(int32_t* out3, int cap) {

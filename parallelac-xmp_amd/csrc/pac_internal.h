@@ -502,6 +502,13 @@ void launch_low_ac2_esc(const Stage& S, int kind, QRef q,
 void launch_low_ac2_exact(const Stage& S, int kind, QRef q, const uint32_t* ecol,
                           const uint32_t* eleft, const uint32_t* tk,
                           const uint32_t* tedge, uint8_t* flag, PipeLaunch L);
+/* the same over entries sorted by simplex (the pairs of a simplex share
+ * its exact den and num) */
+void launch_low_ac2_exact_sorted(const Stage& S, int kind, const uint32_t* item,
+                                 const uint32_t* aux, uint32_t n,
+                                 const uint32_t* ecol, const uint32_t* eleft,
+                                 const uint32_t* tk, const uint32_t* tedge,
+                                 uint8_t* flag, PipeLaunch L);
 
 
 /* SoS table as compiled into the device code (for tests) */

@@ -693,6 +693,30 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                                        (int64_t)UINT32_MAX - 1);
   };
 
+  /* A long host view of exact pairs (degenerate inputs) as (item, aux)
+   * arrays sorted by item, so that an item's pairs share its exact
+   * per-simplex terms; false (nothing done) below 65536 entries or for a
+   * device-built view.  items < nitems. */
+  auto sorted_view = [&](const QRef& q, int64_t nitems, DBuf<uint32_t>& k1,
+                         DBuf<uint32_t>& v1, DBuf<uint32_t>& k2,
+                         DBuf<uint32_t>& v2, bool& alt) -> uint32_t {
+    const uint32_t ne = q.dv ? 0u : q.v.size();
+    if (ne < (1u << 16)) return 0;
+    k1 = DBuf<uint32_t>(P, ne);
+    v1 = DBuf<uint32_t>(P, ne);
+    k2 = DBuf<uint32_t>(P, ne);
+    v2 = DBuf<uint32_t>(P, ne);
+    launch_view_split(q.v, k1.get(), v1.get(), L);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < nitems) ++bits;
+    const size_t rtb = radix_tmp_bytes(ne);
+    DBuf<uint8_t> rt(P, rtb);
+    alt = false;
+    radix_sort_pairs(k1.get(), v1.get(), k2.get(), v2.get(), ne, bits, rt.get(),
+                     rtb, s, &alt);
+    return ne;
+  };
+
   /* ---- tet emptiness ---- */
   /* phase-one budget: 0 = the default (4), < 0 = no second phase */
   int budget = opt.ac2_budget > 0 ? opt.ac2_budget : (opt.ac2_budget < 0 ? 0 : 4);
@@ -732,20 +756,13 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           [&](QRef q) {
             /* many exact pairs (degenerate inputs): sorted by tet first, so
              * that the pairs of a tet share its exact D and N */
-            const uint32_t ne = q.dv ? 0u : q.v.size();
-            if (ne < (1u << 16)) {
+            DBuf<uint32_t> k1, v1, k2, v2;
+            bool alt = false;
+            const uint32_t ne = sorted_view(q, Kalpha.cap, k1, v1, k2, v2, alt);
+            if (!ne) {
               launch_tet_ac2_exact(SG, q, tets.get(), perm.get(), kflag.get(), L);
               return;
             }
-            DBuf<uint32_t> k1(P, ne), v1(P, ne), k2(P, ne), v2(P, ne);
-            launch_view_split(q.v, k1.get(), v1.get(), L);
-            int bits = 1;
-            while (bits < 32 && ((int64_t)1 << bits) < Kalpha.cap) ++bits;
-            const size_t rtb = radix_tmp_bytes(ne);
-            DBuf<uint8_t> rt(P, rtb);
-            bool alt = false;
-            radix_sort_pairs(k1.get(), v1.get(), k2.get(), v2.get(), ne, bits,
-                             rt.get(), rtb, s, &alt);
             launch_tet_ac2_exact_sorted(SG, alt ? k2.get() : k1.get(),
                                         alt ? v2.get() : v1.get(), ne,
                                         tets.get(), perm.get(), kflag.get(), L);
@@ -807,8 +824,17 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                                tedge.get(), tflag.get(), q2, L);
           },
           [&](QRef q) {
-            launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
-                                 tedge.get(), tflag.get(), L);
+            DBuf<uint32_t> k1, v1, k2, v2;
+            bool alt = false;
+            const uint32_t ne = sorted_view(q, Talpha.cap, k1, v1, k2, v2, alt);
+            if (!ne)
+              launch_low_ac2_exact(SG, 0, q, ecol.get(), eleft.get(), tk.get(),
+                                   tedge.get(), tflag.get(), L);
+            else
+              launch_low_ac2_exact_sorted(
+                  SG, 0, alt ? k2.get() : k1.get(), alt ? v2.get() : v1.get(),
+                  ne, ecol.get(), eleft.get(), tk.get(), tedge.get(),
+                  tflag.get(), L);
           },
           tail, RC_TF);
     }
@@ -866,8 +892,17 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                                nullptr, eflag.get(), q2, L);
           },
           [&](QRef q) {
-            launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
-                                 nullptr, eflag.get(), L);
+            DBuf<uint32_t> k1, v1, k2, v2;
+            bool alt = false;
+            const uint32_t ne = sorted_view(q, Ealpha.cap, k1, v1, k2, v2, alt);
+            if (!ne)
+              launch_low_ac2_exact(SG, 1, q, ecol.get(), eleft.get(), nullptr,
+                                   nullptr, eflag.get(), L);
+            else
+              launch_low_ac2_exact_sorted(
+                  SG, 1, alt ? k2.get() : k1.get(), alt ? v2.get() : v1.get(),
+                  ne, ecol.get(), eleft.get(), nullptr, nullptr, eflag.get(),
+                  L);
           },
           tail, RC_EF);
     }
