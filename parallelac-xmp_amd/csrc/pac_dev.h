@@ -164,18 +164,6 @@ struct BlockOut4 {
   }
 };
 
-/* wave-aggregated append: one atomic per wave for all lanes with want */
-__device__ __forceinline__ unsigned wave_append(unsigned* counter, bool want) {
-  uint64_t m = __ballot(want);
-  if (!m) return 0;
-  const int lane = threadIdx.x & 63;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
-  base = __shfl(base, leader, 64);
-  uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return base + (unsigned)__popcll(m & lt);
-}
 struct StageCnt {
   unsigned c[4];
   __device__ StageCnt() { c[0] = c[1] = c[2] = c[3] = 0; }
@@ -515,7 +503,7 @@ __device__ __forceinline__ void scan_box4(const GridP& g,
  * (a violator lies within reach of the exact centre, which lies within the
  * error of the computed one): the box rows (y, z) whose cell square comes
  * within R, each cut to the sphere's x-extent over the row's nearest
- * point, rows nearest the centre first.  Worked in cells relative to the
+ * point, the centre's row first.  Worked in cells relative to the
  * box's low corner, in FP32, with R inflated far past the rounding (every
  * ball lies inside its own fine cell: keys are computed in double). */
 struct SphBox {
@@ -547,70 +535,19 @@ __device__ __forceinline__ bool make_sph(const GridP& g, C reach, const C c[3],
   s.cz = clamp_cell(s.fz, s.b.lo[2], s.b.hi[2]);
   return true;
 }
-/* c, c-1, c+1, c-2, c+2, ... */
-__device__ __forceinline__ int outward(int c, int k) {
-  return c + ((k & 1) ? -((k + 1) >> 1) : (k >> 1));
-}
 /* squared distance from f to the cell [q, q+1) */
 __device__ __forceinline__ float axis_d2(float f, int q) {
   const float d = fmaxf(0.f, fmaxf((float)q - f, f - (float)(q + 1)));
   return d * d;
 }
-struct SphIter {
-  int kz, ky, z;
-  float dz2;
-};
-__device__ __forceinline__ bool sph_next(const GridP& g, const uint32_t* cstart,
-                                         const SphBox& s, SphIter& it,
-                                         uint32_t& lo, uint32_t& hi) {
-  const CellBox& b = s.b;
-  const int nzk = 2 * max(s.cz - b.lo[2], b.hi[2] - s.cz);
-  const int nyk = 2 * max(s.cy - b.lo[1], b.hi[1] - s.cy);
-  for (;;) {
-    if (it.ky > nyk) { /* next plane */
-      if (++it.kz > nzk) return false;
-      it.z = outward(s.cz, it.kz);
-      it.ky = nyk + 1;
-      if (it.z < b.lo[2] || it.z > b.hi[2]) continue;
-      it.dz2 = axis_d2(s.fz, it.z - b.lo[2]);
-      if (it.dz2 > s.R2) continue;
-      it.ky = 0;
-    }
-    const int y = outward(s.cy, it.ky++);
-    if (y < b.lo[1] || y > b.hi[1]) continue;
-    const float d2 = it.dz2 + axis_d2(s.fy, y - b.lo[1]);
-    if (d2 > s.R2) continue;
-    const float rx = sqrtf(s.R2 - d2) * s.kx + 1e-3f;
-    const int xl = max(0, (int)floorf(fmaxf(s.fx - rx, -1.0e6f)));
-    const int xh = min(b.hi[0] - b.lo[0], (int)floorf(fminf(s.fx + rx, 1.0e6f)));
-    if (xl > xh) continue;
-    const int64_t row = (int64_t)g.cbase + b.lo[0] +
-                        (int64_t)g.ex * (y + (int64_t)g.ey * it.z);
-    lo = cstart[row + xl];
-    hi = cstart[row + xh + 1];
-    if (hi > lo) return true;
-  }
-}
-template <class F>
-__device__ __forceinline__ void scan_sph4(const GridP& g,
-                                          const uint32_t* fstart,
-                                          const Ball* fball, const SphBox& s,
-                                          F&& visit) {
-  const CellBox& b = s.b;
-  if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) return;
-  SphIter it{-1, 1 << 30, 0, 0.f};
-  uint32_t lo, hi;
-  while (sph_next(g, fstart, s, it, lo, hi))
-    if (scan_range4(fball, lo, hi, visit)) return;
-}
-/* The same cells as scan_sph4 in one flat loop: each iteration fetches the
- * next row's range when the current one is used up and then hands the
- * next (up to) four balls of the range to group(p, n) (fine-grid
- * positions p .. p+n-1; true stops the scan).  Rows: the centre row first,
- * then the box's other rows in (z, y) order.  Nested row and ball loops
- * make a wave step through every lane's rows in lock step, each row as
- * long as its longest lane's; here lanes at different rows share
- * iterations, and the per-ball code is emitted once. */
+/* The fine-grid cells of a sphere box (above) in one flat loop: each
+ * iteration fetches the next row's range when the current one is used up
+ * and then hands the next (up to) four balls of the range to group(p, n)
+ * (fine-grid positions p .. p+n-1; true stops the scan).  Rows: the
+ * centre row first, then the box's other rows in (z, y) order.  Nested
+ * row and ball loops make a wave step through every lane's rows in lock
+ * step, each row as long as its longest lane's; here lanes at different
+ * rows share iterations, and the per-ball code is emitted once. */
 template <class F>
 __device__ __forceinline__ void scan_sph_flat(const GridP& g,
                                               const uint32_t* fstart,
