@@ -521,8 +521,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(Cnt c_ne,
   int64_t q = gtid();
   if (q >= ne) return;
   uint32_t i = eleft[q], j = ecol[q];
-  uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-  uint64_t b = erow[j], be = erow[j + 1];
+  /* alpha-edge ids fit 32 bits (the candidate count is checked below
+   * 2^32), and 32-bit cursors halve the merge's index arithmetic */
+  uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
+  uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
   uint32_t c = 0;
   while (a < ae && b < be) {
     uint32_t x = ecol[a], y = ecol[b];
@@ -658,9 +660,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
   if (q < nt) {
     uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
     const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
-    uint64_t a = (uint64_t)q + 1, ae = trow[eij + 1];
-    uint64_t b = trow[eik], be = trow[eik + 1];
-    uint64_t c = trow[ejk], ce = trow[ejk + 1];
+    /* alpha-triangle ids fit 32 bits (the slot count is checked below
+     * 2^32): 32-bit cursors halve the merge's index arithmetic */
+    uint32_t a = (uint32_t)q + 1, ae = (uint32_t)trow[eij + 1];
+    uint32_t b = (uint32_t)trow[eik], be = (uint32_t)trow[eik + 1];
+    uint32_t c = (uint32_t)trow[ejk], ce = (uint32_t)trow[ejk + 1];
     uint64_t o = off[q], oe = off[q + 1];
     /* the slots end with T(i,j) (o < oe adds nothing then) unless the
      * triangle was given none (chunk sub-inputs) */
@@ -1016,8 +1020,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
   if (q < ne) {
     const uint32_t i = eleft[q], j = ecol[q];
     s_ij[threadIdx.x] = make_uint2(i, j);
-    uint64_t a = (uint64_t)q + 1, ae = erow[i + 1];
-    uint64_t b = erow[j], be = erow[j + 1];
+    uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
+    uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
     uint64_t o = off[q];
     while (a < ae && b < be) {
       const uint32_t x = ecol[a], y = ecol[b];

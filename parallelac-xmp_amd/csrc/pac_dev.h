@@ -568,8 +568,9 @@ __device__ __forceinline__ void scan_sph_flat(const GridP& g,
         const int xh =
             min(b.hi[0] - b.lo[0], (int)floorf(fminf(s.fx + rx, 1.0e6f)));
         if (xl <= xh) {
-          const int64_t row = (int64_t)g.cbase + b.lo[0] +
-                              (int64_t)g.ex * (ry + (int64_t)g.ey * rz);
+          /* fine-grid cell ids stay below 2^31 (the host uses the fine
+           * grid only then): 32-bit index arithmetic */
+          const int row = (int)g.cbase + b.lo[0] + g.ex * (ry + g.ey * rz);
           p = fstart[row + xl];
           hi = fstart[row + xh + 1];
         }
