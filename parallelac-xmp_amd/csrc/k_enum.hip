@@ -707,6 +707,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
       v = t < (uint64_t)CAP ? s_out[t] : ctet[base + t];
       if (v.x != 0xffffffffu) {
         const Ball bv[4] = {S.ball[v.x], S.ball[v.y], S.ball[v.z], S.ball[v.w]};
+        /* the weights are first used after the test of D's sign; left
+         * alone, the compiler loads them there, a second round trip per
+         * round; used here (an empty asm), each ball is one 16-byte load */
+#pragma unroll
+        for (int k = 0; k < 4; ++k) asm volatile("" ::"v"(bv[k].w));
         r = tet_code_ss<R>(bv, S.alpha);
       }
     }
@@ -1051,7 +1056,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
       for (int k = 0; k < 4; ++k)
         v[k] = t < (uint64_t)CAP ? s_out[k * CAP + t] : g[k][base + t];
       const uint2 ij = s_ij[v[1] - (uint32_t)q0];
-      r = tri_code<FE<R>>(S.ball[ij.x], S.ball[ij.y], S.ball[v[0]], S.alpha);
+      const Ball b0 = S.ball[ij.x], b1 = S.ball[ij.y], b2 = S.ball[v[0]];
+      /* weights used here: one 16-byte load per ball (as for tets) */
+      asm volatile("" ::"v"(b0.w), "v"(b1.w), "v"(b2.w));
+      r = tri_code<FE<R>>(b0, b1, b2, S.alpha);
     }
     const uint64_t bm = __ballot(r != REJECT);
     if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
