@@ -195,7 +195,7 @@ __device__ __forceinline__ bool quick_reject(const Stage& S, const GridP& g,
   bool dead = false;
   auto visit = [&](uint32_t, const Ball& E) {
     ++pairs;
-    if (pairss_sign(P, E) * P.sden < 0) {
+    if (pairss_sign(P, E) < 0) {
       dead = true;
       return true;
     }
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
         push_q(q1, (uint32_t)id, p);
         return false;
       }
-      if (s * P.sden < 0) { /* pi_e < rho: violator */
+      if (s < 0) { /* pi_e < rho: violator */
         stop = 1;
         return true;
       }
@@ -355,13 +355,11 @@ __global__ __launch_bounds__(kBlock) void k_tet_ac2_hot(
       Ball E[4];
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
-      unsigned viol = 0, ties = 0;
-#pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) {
-        const int s = pairss_sign(P, E[k]);
-        viol |= (k < n && s * P.sden < 0) ? 1u << k : 0u;
-        ties |= (k < n && s == 0) ? 1u << k : 0u;
-      }
+      /* a ball repeated past n (E[k] = E[n-1]) repeats ball n-1's bits:
+       * the first violator is unchanged, the ties are cut to n balls */
+      unsigned viol, ties;
+      pairss_masks(P, E, viol, ties);
+      ties &= (1u << n) - 1u;
       /* pairs counted as a scan stopping at its first violator tests them
        * (the group's tests past it are not work the scan needs) */
       pairs += viol ? (unsigned)__builtin_ctz(viol) + 1u : n;
@@ -829,13 +827,9 @@ __global__ __launch_bounds__(kBlock) void k_low_ac2_hot(
       Ball E[4];
 #pragma unroll
       for (uint32_t k = 0; k < 4; ++k) E[k] = S.fball[p + min(k, n - 1)];
-      unsigned viol = 0, ties = 0;
-#pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) {
-        const int s = pairss_sign(P, E[k]);
-        viol |= (k < n && s < 0) ? 1u << k : 0u;
-        ties |= (k < n && s == 0) ? 1u << k : 0u;
-      }
+      unsigned viol, ties; /* bits past n as in k_tet_ac2_hot */
+      pairss_masks(P, E, viol, ties);
+      ties &= (1u << n) - 1u;
       /* pairs counted as a scan stopping at its first violator tests them
        * (the group's tests past it are not work the scan needs) */
       pairs += viol ? (unsigned)__builtin_ctz(viol) + 1u : n;

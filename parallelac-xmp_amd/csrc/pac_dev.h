@@ -699,7 +699,8 @@ __device__ __forceinline__ double fmaR(double a, double b, double c) {
 template <class R>
 struct PairSS {
   R ax, ay, az, wa;
-  R den, nx, ny, nz;
+  R den, nx, ny, nz; /* times sign(den) (exact): the gap below comes out
+                      * as sign(den) g, so that -1 is a violator */
   R K1, K2, Mx, My, Mz;
   int sden;
 };
@@ -714,10 +715,11 @@ __device__ __forceinline__ bool pairss_init(PairSS<R>& P, const V3<FE<R>>& pa,
   P.ay = pa.y.v;
   P.az = pa.z.v;
   P.wa = wa.v;
-  P.den = den.v;
-  P.nx = num.x.v;
-  P.ny = num.y.v;
-  P.nz = num.z.v;
+  const bool neg = P.sden < 0;
+  P.den = neg ? -den.v : den.v;
+  P.nx = neg ? -num.x.v : num.x.v;
+  P.ny = neg ? -num.y.v : num.y.v;
+  P.nz = neg ? -num.z.v : num.z.v;
   R sl = FEc<R>::slack();
   P.K1 = (cu * fabs(den.v) + den.e) * sl;
   P.K2 = (wa.e * fabs(den.v)) * sl + (R)64 * FEc<R>::tiny();
@@ -751,9 +753,13 @@ __device__ __forceinline__ void cross_ss(R ax, R ay, R az, R bx, R by, R bz,
   c[2] = SS<R>{ax * by - ay * bx, aX * bY + aY * bX};
 }
 
+/* The pair test's gap g = sign(den) (L den - y . num) for ball e and the
+ * bound B on its rounding error.  Negating den and num negates every
+ * rounded result exactly, so g is the unnormalised gap times sign(den),
+ * bit for bit, and B is unchanged. */
 template <class R>
-__device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
-                                           const Ball& e) {
+__device__ __forceinline__ void pairss_gap(const PairSS<R>& P, const Ball& e,
+                                           R& g, R& B) {
   /* fused multiply-adds round once where the bound counts two roundings,
    * so the bound still holds */
   R yx = (R)e.x - P.ax, yy = (R)e.y - P.ay, yz = (R)e.z - P.az;
@@ -761,11 +767,35 @@ __device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
   R we = (R)e.w * (R)e.w;
   R L = (q - we) + P.wa;
   R yn = fmaR(yx, P.nx, fmaR(yy, P.ny, yz * P.nz));
-  R g = fmaR(L, P.den, -yn);
+  g = fmaR(L, P.den, -yn);
   R Labs = (q + we) + P.wa;
-  R B = fmaR(Labs, P.K1,
-             fmaR(fabs(yx), P.Mx, fmaR(fabs(yy), P.My, fmaR(fabs(yz), P.Mz, P.K2))));
+  B = fmaR(Labs, P.K1,
+           fmaR(fabs(yx), P.Mx, fmaR(fabs(yy), P.My, fmaR(fabs(yz), P.Mz, P.K2))));
+}
+/* -1: e certainly violates (power below rho), 0: undecided, 1: not */
+template <class R>
+__device__ __forceinline__ int pairss_sign(const PairSS<R>& P,
+                                           const Ball& e) {
+  R g, B;
+  pairss_gap(P, e, g, B);
   return g > B ? 1 : (g < -B ? -1 : 0);
+}
+/* The same for four balls as bit masks: bit k of viol when ball k
+ * certainly violates, of tie when its test is undecided (|g| <= B, or a
+ * NaN: pairss_sign's 0). */
+template <class R>
+__device__ __forceinline__ void pairss_masks(const PairSS<R>& P,
+                                             const Ball E[4], unsigned& viol,
+                                             unsigned& tie) {
+  viol = 0u;
+  tie = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    R g, B;
+    pairss_gap(P, E[k], g, B);
+    viol |= g < -B ? 1u << k : 0u;
+    tie |= !(fabs(g) > B) ? 1u << k : 0u;
+  }
 }
 
 /* Orthocentre numerators N and denominator D of a tet at precision R with
