@@ -392,15 +392,63 @@ __device__ int tet_code(const Ball& a, const Ball& b, const Ball& c,
   if (fe_sign(D) == 0) return UNDEC;
   return code_of(fe_sign(tet_val(D, N, wa, Al<T>::c(alpha))));
 }
+/* The first stages of the edge and triangle filters, semi-static as
+ * tet_dn_ss: each value evaluated once in precision R together with its
+ * permanent (the same expression on absolute values, subtractions made
+ * additions), the error bound gamma_h times the permanent (ss_fe) for
+ * the expression tree's height h, leaves u, v = p - p_a and w = r^2
+ * rounded once, alpha exact, products by 2 and 4 exact.
+ *   edge: V = 4 |u|^2 (w_a + alpha) - A^2, A = |u|^2 + w_a - w_b; h 8;
+ *   triangle: |n|^2 (n = u x v), h 6, and V = 4 |n|^2 (w_a + alpha) -
+ *   ((A^2 |v|^2 - 2 A B u.v) + B^2 |u|^2), A = |u|^2 + w_a - w_b,
+ *   B = |v|^2 + w_a - w_c; h 11.
+ * The same values as edge_val / tri_val; the bound replaces a running
+ * error per operation (~4x the instructions for triangles). */
+template <class R>
+__device__ __forceinline__ int edge_code_ss(const Ball& a, const Ball& b,
+                                            Real alpha) {
+  const R ux = (R)b.x - (R)a.x, uy = (R)b.y - (R)a.y, uz = (R)b.z - (R)a.z;
+  const R wa = (R)a.w * (R)a.w, wb = (R)b.w * (R)b.w, al = (R)alpha;
+  const R uu = (ux * ux + uy * uy) + uz * uz;
+  const R A = (uu + wa) - wb, Ap = (uu + wa) + wb;
+  const R V = ((R)4 * uu) * (wa + al) - A * A;
+  const R Vp = ((R)4 * uu) * (wa + fabs(al)) + Ap * Ap;
+  return code_of(fe_sign(ss_fe(SS<R>{V, Vp}, 8)));
+}
+template <class R>
+__device__ __forceinline__ int tri_code_ss(const Ball& a, const Ball& b,
+                                           const Ball& c, Real alpha) {
+  const R ax = a.x, ay = a.y, az = a.z;
+  const R ux = (R)b.x - ax, uy = (R)b.y - ay, uz = (R)b.z - az;
+  const R vx = (R)c.x - ax, vy = (R)c.y - ay, vz = (R)c.z - az;
+  const R wa = (R)a.w * (R)a.w, wb = (R)b.w * (R)b.w, wc = (R)c.w * (R)c.w;
+  SS<R> n[3];
+  cross_ss(ux, uy, uz, vx, vy, vz, n);
+  const R nn = (n[0].v * n[0].v + n[1].v * n[1].v) + n[2].v * n[2].v;
+  const R nnp = (n[0].p * n[0].p + n[1].p * n[1].p) + n[2].p * n[2].p;
+  if (fe_sign(ss_fe(SS<R>{nn, nnp}, 6)) <= 0) return UNDEC;
+  const R uu = (ux * ux + uy * uy) + uz * uz;
+  const R vv = (vx * vx + vy * vy) + vz * vz;
+  const R uv = (ux * vx + uy * vy) + uz * vz;
+  const R uvp = (fabs(ux) * fabs(vx) + fabs(uy) * fabs(vy)) + fabs(uz) * fabs(vz);
+  const R A = (uu + wa) - wb, Ap = (uu + wa) + wb;
+  const R B = (vv + wa) - wc, Bp = (vv + wa) + wc;
+  const R lhs = ((A * A) * vv - (((R)2 * A) * B) * uv) + (B * B) * uu;
+  const R lhsp = ((Ap * Ap) * vv + (((R)2 * Ap) * Bp) * uvp) + (Bp * Bp) * uu;
+  const R al = (R)alpha;
+  const R V = ((R)4 * nn) * (wa + al) - lhs;
+  const R Vp = ((R)4 * nnp) * (wa + fabs(al)) + lhsp;
+  return code_of(fe_sign(ss_fe(SS<R>{V, Vp}, 11)));
+}
 /* the first stage of the tet filter: D and N with a-priori error bounds
  * (tet_dn_ss: one extra operation per operation, on absolute values),
- * the final 4 D^2 (w_a + alpha) - |N|^2 with running bounds */
+ * the final 4 D^2 (w_a + alpha) - |N|^2 with one bound (tet_val_ss) */
 template <class R>
 __device__ __forceinline__ int tet_code_ss(const Ball* v, Real alpha) {
   TetCert<R> T;
   tet_dn_ss(v, T);
   if (fe_sign(T.D) == 0) return UNDEC;
-  return code_of(fe_sign(tet_val(T.D, T.N, T.wa, Al<FE<R>>::c(alpha))));
+  return code_of(tet_val_ss(T, (R)alpha));
 }
 __device__ int tet_exact(const Ball& a, const Ball& b, const Ball& c,
                          const Ball& d, const ExactScale& X) {
@@ -902,7 +950,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_enum_filter(
         while (s_end[wv][rr] <= f) ++rr; /* f < total = s_end[63] */
         j = s_beg[wv][rr] + f;
         const Ball bj = S.ball[j];
-        code = same_pos(bi, bj) ? REJECT : edge_code<FE<R>>(bi, bj, S.alpha);
+        code = same_pos(bi, bj) ? REJECT : edge_code_ss<R>(bi, bj, S.alpha);
       }
       const uint64_t m = __ballot(code != REJECT);
       if (code != REJECT) {
@@ -1059,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
       const Ball b0 = S.ball[ij.x], b1 = S.ball[ij.y], b2 = S.ball[v[0]];
       /* weights used here: one 16-byte load per ball (as for tets) */
       asm volatile("" ::"v"(b0.w), "v"(b1.w), "v"(b2.w));
-      r = tri_code<FE<R>>(b0, b1, b2, S.alpha);
+      r = tri_code_ss<R>(b0, b1, b2, S.alpha);
     }
     const uint64_t bm = __ballot(r != REJECT);
     if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);

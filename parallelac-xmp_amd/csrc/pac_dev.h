@@ -840,6 +840,38 @@ __device__ __forceinline__ void tet_dn_ss(const Ball v[4], TetCert<R>& T) {
   T.N = V3<FE<R>>{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
   T.wa = FE<R>{wa, FEc<R>::u() * wa * FEc<R>::slack()};
 }
+/* Sign of V = 4 D^2 (w_a + alpha) - |N|^2 (alpha exact) from the
+ * certified D, N, w_a: 1 certainly > 0, -1 certainly < 0, 0 undecided.
+ * One bound instead of a running error per operation:
+ *  - the inputs' errors: |4 D*^2 (w*+a) - 4 D^2 (w+a)| <=
+ *    4 (2|D| eD + eD^2)(|w+a| + ew) + 4 D^2 ew, and
+ *    ||N*|^2 - |N|^2| <= sum_q (2|N_q| + eN_q) eN_q;
+ *  - the evaluation's roundings: 4 D is exact, 4 D^2 (w+a) and |N|^2 each
+ *    carry at most three roundings (gamma_3 / (1 - gamma_3) < 4u of their
+ *    computed magnitudes), the difference one more (u / (1 - u) < 2u of
+ *    the computed V);
+ *  - 64 tiny for products that underflow; the slack covers the rounding
+ *    of the bound's own evaluation. */
+template <class R>
+__device__ __forceinline__ int tet_val_ss(const TetCert<R>& T, R alpha) {
+  const R u = FEc<R>::u();
+  const R D = T.D.v, eD = T.D.e;
+  const R s = T.wa.v + alpha, es = T.wa.e;
+  const R t2 = ((R)4 * D) * D;
+  const R t3 = t2 * s;
+  const R nx = T.N.x.v, ny = T.N.y.v, nz = T.N.z.v;
+  const R n2 = (nx * nx + ny * ny) + nz * nz;
+  const R V = t3 - n2;
+  const R eDD = ((R)2 * fabs(D) + eD) * eD;
+  const R eN = ((R)2 * fabs(nx) + T.N.x.e) * T.N.x.e +
+               ((R)2 * fabs(ny) + T.N.y.e) * T.N.y.e +
+               ((R)2 * fabs(nz) + T.N.z.e) * T.N.z.e;
+  const R ein = (R)4 * eDD * (fabs(s) + es) + t2 * es + eN;
+  const R B = (ein + (R)4 * u * (fabs(t3) + n2) + (R)2 * u * fabs(V) +
+               (R)64 * FEc<R>::tiny()) *
+              FEc<R>::slack();
+  return V > B ? 1 : (V < -B ? -1 : 0);
+}
 
 }  // namespace PAC_V
 }  // namespace pac
