@@ -1056,8 +1056,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
     uint32_t* cseg, uint32_t* cea, uint32_t* ceb, uint8_t* flag,
     uint32_t* bcnt, Queue q1, uint64_t lim) {
   const int64_t ne = c_ne.get();
-  /* 5 blocks per CU fit in LDS with the edge table and block_flush's 16 B */
-  constexpr int CAP = 1912;
+  /* 6 blocks per CU fit in LDS with the edge table and block_flush's 16 B
+   * (a block averages ~1,800 slots on the 100k globule; the rest go to
+   * global memory: measured 6 > 5 > 8 blocks per CU) */
+  constexpr int CAP = 1570;
   __shared__ uint32_t s_out[4 * CAP];
   __shared__ uint2 s_ij[kBlock]; /* the block's edges (i, j) */
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
