@@ -703,25 +703,22 @@ __device__ __forceinline__ bool low_setup(const GridP& g, const Ball* v,
     R w2 = (R)v[2].w * (R)v[2].w;
     R vv = vx * vx + vy * vy + vz * vz;
     SS<R> B{(vv + wa) - w2, (vv + wa) + w2};
-    SS<R> n[3];
-    cross_ss(ux, uy, uz, vx, vy, vz, n);
-    SS<R> nn{n[0].v * n[0].v + n[1].v * n[1].v + n[2].v * n[2].v,
-             n[0].p * n[0].p + n[1].p * n[1].p + n[2].p * n[2].p};
+    /* permanents bounded by max norms: every component of n = u x v by
+     * pn = 2 |u|_inf |v|_inf, of v x n by 2 |v|_inf pn, of u x n by
+     * 2 |u|_inf pn */
+    const R mu = fmax(fabs(ux), fmax(fabs(uy), fabs(uz)));
+    const R mv = fmax(fabs(vx), fmax(fabs(vy), fabs(vz)));
+    const R pn = ((R)2 * mu) * mv;
+    const R nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+    SS<R> nn{nx * nx + ny * ny + nz * nz, ((R)3 * pn) * pn};
     /* X = A (v x n) - B (u x n) */
-    R nx = n[0].v, ny = n[1].v, nz = n[2].v;
-    R NX = n[0].p, NY = n[1].p, NZ = n[2].p;
-    R vX = fabs(vx), vY = fabs(vy), vZ = fabs(vz), uX = fabs(ux), uY = fabs(uy),
-      uZ = fabs(uz);
-    SS<R> vn[3] = {{vy * nz - vz * ny, vY * NZ + vZ * NY},
-                   {vz * nx - vx * nz, vZ * NX + vX * NZ},
-                   {vx * ny - vy * nx, vX * NY + vY * NX}};
-    SS<R> un[3] = {{uy * nz - uz * ny, uY * NZ + uZ * NY},
-                   {uz * nx - ux * nz, uZ * NX + uX * NZ},
-                   {ux * ny - uy * nx, uX * NY + uY * NX}};
+    const R vn[3] = {vy * nz - vz * ny, vz * nx - vx * nz, vx * ny - vy * nx};
+    const R un[3] = {uy * nz - uz * ny, uz * nx - ux * nz, ux * ny - uy * nx};
+    const R xp = ((R)2 * pn) * (A.p * mv + B.p * mu);
     SS<R> X[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q)
-      X[q] = SS<R>{A.v * vn[q].v - B.v * un[q].v, A.p * vn[q].p + B.p * un[q].p};
+      X[q] = SS<R>{A.v * vn[q] - B.v * un[q], xp};
     den = ss_fe(nn, 8);
     num = V3<FE<R>>{ss_fe(X[0], 12), ss_fe(X[1], 12), ss_fe(X[2], 12)};
   } else {

@@ -817,25 +817,26 @@ __device__ __forceinline__ void tet_dn_ss(const Ball v[4], TetCert<R>& T) {
   R wa = (R)v[0].w * (R)v[0].w;
   R w1 = (R)v[1].w * (R)v[1].w, w2 = (R)v[2].w * (R)v[2].w,
     w3 = (R)v[3].w * (R)v[3].w;
-  SS<R> vt[3], tu[3], uv[3];
-  cross_ss(vx, vy, vz, tx, ty, tz, vt);
-  cross_ss(tx, ty, tz, ux, uy, uz, tu);
-  cross_ss(ux, uy, uz, vx, vy, vz, uv);
-  SS<R> D{ux * vt[0].v + uy * vt[1].v + uz * vt[2].v,
-          fabs(ux) * vt[0].p + fabs(uy) * vt[1].p + fabs(uz) * vt[2].p};
+  /* the permanents of the cross products bounded by max norms:
+   * |v_y t_z| + |v_z t_y| <= 2 |v|_inf |t|_inf for every component */
+  const R mu = fmax(fabs(ux), fmax(fabs(uy), fabs(uz)));
+  const R mv = fmax(fabs(vx), fmax(fabs(vy), fabs(vz)));
+  const R mt = fmax(fabs(tx), fmax(fabs(ty), fabs(tz)));
+  const R pvt = ((R)2 * mv) * mt, ptu = ((R)2 * mt) * mu, puv = ((R)2 * mu) * mv;
+  const R vt[3] = {vy * tz - vz * ty, vz * tx - vx * tz, vx * ty - vy * tx};
+  const R tu[3] = {ty * uz - tz * uy, tz * ux - tx * uz, tx * uy - ty * ux};
+  const R uv[3] = {uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx};
+  SS<R> D{ux * vt[0] + uy * vt[1] + uz * vt[2],
+          ((fabs(ux) + fabs(uy)) + fabs(uz)) * pvt};
   R uu = ux * ux + uy * uy + uz * uz, vv = vx * vx + vy * vy + vz * vz,
     tt = tx * tx + ty * ty + tz * tz;
   SS<R> A{(uu + wa) - w1, (uu + wa) + w1}, B{(vv + wa) - w2, (vv + wa) + w2},
       C{(tt + wa) - w3, (tt + wa) + w3};
+  const R np = A.p * pvt + B.p * ptu + C.p * puv;
   SS<R> N[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
-    N[q] = SS<R>{A.v * vt[q].v + B.v * tu[q].v + C.v * uv[q].v,
-                 fabs(A.v) * vt[q].p + fabs(B.v) * tu[q].p + fabs(C.v) * uv[q].p};
-  /* A, B, C permanents exceed |A| etc.; use them for the N bound */
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-    N[q].p = A.p * vt[q].p + B.p * tu[q].p + C.p * uv[q].p;
+    N[q] = SS<R>{A.v * vt[q] + B.v * tu[q] + C.v * uv[q], np};
   T.D = ss_fe(D, 6);
   T.N = V3<FE<R>>{ss_fe(N[0], 10), ss_fe(N[1], 10), ss_fe(N[2], 10)};
   T.wa = FE<R>{wa, FEc<R>::u() * wa * FEc<R>::slack()};
