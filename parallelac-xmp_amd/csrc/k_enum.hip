@@ -560,6 +560,7 @@ void launch_tet_caps(const Stage& S, Cnt m, const uint64_t* trow,
 
 /* number of triangles of alpha edge q = (i,j): the size of row(i) after j
  * intersected with row(j) (computePossibleTriangleCount, Parallel:232-266) */
+template <bool BUF>
 __global__ __launch_bounds__(kBlock) void k_tri_count(Cnt c_ne,
                                                      const uint64_t* erow,
                                                      const uint32_t* ecol,
@@ -569,23 +570,38 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(Cnt c_ne,
   int64_t q = gtid();
   if (q >= ne) return;
   uint32_t i = eleft[q], j = ecol[q];
-  /* alpha-edge ids fit 32 bits (the candidate count is checked below
-   * 2^32), and 32-bit cursors halve the merge's index arithmetic */
-  uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
-  uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
   uint32_t c = 0;
-  while (a < ae && b < be) {
-    uint32_t x = ecol[a], y = ecol[b];
-    c += (x == y);
-    a += (x <= y);
-    b += (y <= x);
+  if constexpr (BUF) {
+    /* byte offsets into ecol (below 4 GiB): no 64-bit address
+     * arithmetic per load */
+    const BufU32 E{ecol, (uint32_t)(c_ne.cap * 4)};
+    uint32_t a = ((uint32_t)q + 1u) << 2, ae = (uint32_t)erow[i + 1] << 2;
+    uint32_t b = (uint32_t)erow[j] << 2, be = (uint32_t)erow[j + 1] << 2;
+    while (a < ae && b < be) {
+      const uint32_t x = E.at(a), y = E.at(b);
+      c += (x == y);
+      a += x <= y ? 4u : 0u;
+      b += y <= x ? 4u : 0u;
+    }
+  } else {
+    /* alpha-edge ids fit 32 bits (the candidate count is checked below
+     * 2^32), and 32-bit cursors halve the merge's index arithmetic */
+    uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
+    uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
+    while (a < ae && b < be) {
+      const uint32_t x = ecol[a], y = ecol[b];
+      c += (x == y);
+      a += (x <= y);
+      b += (y <= x);
+    }
   }
   cnt[q] = c;
 }
 void launch_tri_count(Cnt ne, const uint64_t* erow, const uint32_t* ecol,
                       const uint32_t* eleft, uint32_t* cnt, PipeLaunch L) {
   if (ne.cap <= 0) return;
-  hipLaunchKernelGGL(k_tri_count, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
+  auto kf = ne.cap < ((int64_t)1 << 30) ? k_tri_count<true> : k_tri_count<false>;
+  hipLaunchKernelGGL(kf, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, ne, erow, ecol, eleft, cnt);
 }
 
@@ -683,7 +699,7 @@ constexpr int fam_of(int KIND) {
  * written to global memory during the merges and are filtered from there. */
 /* FP32 first stage: 8 waves per SIMD (at most 64 VGPRs, no spills; the
  * LDS stage admits 8 blocks per CU); FP64: the compiler's choice */
-template <class R>
+template <class R, bool BUF>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_filter(
     Stage S, Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
@@ -708,28 +724,53 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
   if (q < nt) {
     uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
     const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
-    /* alpha-triangle ids fit 32 bits (the slot count is checked below
-     * 2^32): 32-bit cursors halve the merge's index arithmetic */
-    uint32_t a = (uint32_t)q + 1, ae = (uint32_t)trow[eij + 1];
-    uint32_t b = (uint32_t)trow[eik], be = (uint32_t)trow[eik + 1];
-    uint32_t c = (uint32_t)trow[ejk], ce = (uint32_t)trow[ejk + 1];
     uint64_t o = off[q], oe = off[q + 1];
-    /* the slots end with T(i,j) (o < oe adds nothing then) unless the
-     * triangle was given none (chunk sub-inputs) */
-    while (a < ae && b < be && c < ce && o < oe) {
-      uint32_t x = tk[a], y = tk[b], z = tk[c];
-      uint32_t m = max(x, max(y, z));
-      if (x == m && y == m && z == m) {
-        B.put(ctet, o, make_uint4(i, j, k, m));
-        ++o;
-        ++found;
-        ++a;
-        ++b;
-        ++c;
-      } else {
-        a += (x < m);
-        b += (y < m);
-        c += (z < m);
+    if constexpr (BUF) {
+      /* the same merge over byte offsets into tk (below 4 GiB): no 64-bit
+       * address arithmetic per load.  A triangle's slots are its row's
+       * tail after it, so o < oe holds while a < ae, unless it was given
+       * none (chunk sub-inputs): then the merge is empty. */
+      const BufU32 K{tk, (uint32_t)(c_nt.cap * 4)};
+      uint32_t a = ((uint32_t)q + 1u) << 2, ae = (uint32_t)trow[eij + 1] << 2;
+      uint32_t b = (uint32_t)trow[eik] << 2, be = (uint32_t)trow[eik + 1] << 2;
+      uint32_t c = (uint32_t)trow[ejk] << 2, ce = (uint32_t)trow[ejk + 1] << 2;
+      if (o == oe) ae = a;
+      while (a < ae && b < be && c < ce) {
+        const uint32_t x = K.at(a), y = K.at(b), z = K.at(c);
+        const uint32_t m = max(x, max(y, z));
+        const bool all = x == m && y == m && z == m;
+        if (all) {
+          B.put(ctet, o, make_uint4(i, j, k, m));
+          ++o;
+          ++found;
+        }
+        a += (all || x < m) ? 4u : 0u;
+        b += (all || y < m) ? 4u : 0u;
+        c += (all || z < m) ? 4u : 0u;
+      }
+    } else {
+      /* alpha-triangle ids fit 32 bits (the slot count is checked below
+       * 2^32): 32-bit cursors halve the merge's index arithmetic; the
+       * slots end with T(i,j) (o < oe adds nothing then) unless the
+       * triangle was given none (chunk sub-inputs) */
+      uint32_t a = (uint32_t)q + 1, ae = (uint32_t)trow[eij + 1];
+      uint32_t b = (uint32_t)trow[eik], be = (uint32_t)trow[eik + 1];
+      uint32_t c = (uint32_t)trow[ejk], ce = (uint32_t)trow[ejk + 1];
+      while (a < ae && b < be && c < ce && o < oe) {
+        const uint32_t x = tk[a], y = tk[b], z = tk[c];
+        const uint32_t m = max(x, max(y, z));
+        if (x == m && y == m && z == m) {
+          B.put(ctet, o, make_uint4(i, j, k, m));
+          ++o;
+          ++found;
+          ++a;
+          ++b;
+          ++c;
+        } else {
+          a += (x < m);
+          b += (y < m);
+          c += (z < m);
+        }
       }
     }
     for (; o < oe; ++o) B.put(ctet, o, make_uint4(0xffffffffu, 0u, 0u, 0u));
@@ -794,17 +835,21 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             uint64_t lim) {
   if (nt.cap <= 0) return;
   dim3 gr((unsigned)((nt.cap + kBlock - 1) / kBlock));
+  /* byte offsets into tk while it stays below 4 GiB */
+  const bool buf = nt.cap < ((int64_t)1 << 30);
   if constexpr (!kBallF64) {
     if (S.staged) {
-      hipLaunchKernelGGL(k_tet_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
-                         nt, trow, tk, tedge, teik, tejk, eleft, ecol, off,
-                         ctet, flag, bcnt, q1, lim);
+      auto kf = buf ? k_tet_enum_filter<float, true>
+                    : k_tet_enum_filter<float, false>;
+      hipLaunchKernelGGL(kf, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge,
+                         teik, tejk, eleft, ecol, off, ctet, flag, bcnt, q1, lim);
       return;
     }
   }
-  hipLaunchKernelGGL(k_tet_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S, nt,
-                     trow, tk, tedge, teik, tejk, eleft, ecol, off, ctet, flag,
-                     bcnt, q1, lim);
+  auto kd = buf ? k_tet_enum_filter<double, true>
+                : k_tet_enum_filter<double, false>;
+  hipLaunchKernelGGL(kd, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge, teik,
+                     tejk, eleft, ecol, off, ctet, flag, bcnt, q1, lim);
 }
 
 /* After the ladder: the kept tets of each enumeration block (its first
@@ -1049,7 +1094,7 @@ void launch_edge_copy(int64_t n, const uint64_t* off, const uint32_t* bcnt,
  * ones written compacted, in slot order, from the block's first slot on
  * (as in k_tet_enum_filter), four arrays and the flag; the filter never
  * reads an array the enumeration wrote for every slot. */
-template <class R>
+template <class R, bool BUF>
 __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
     Stage S, Cnt c_ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
@@ -1075,19 +1120,36 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
   if (q < ne) {
     const uint32_t i = eleft[q], j = ecol[q];
     s_ij[threadIdx.x] = make_uint2(i, j);
-    uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
-    uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
     uint64_t o = off[q];
-    while (a < ae && b < be) {
-      const uint32_t x = ecol[a], y = ecol[b];
-      if (x == y) {
-        const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
-        B.put(g, o, v);
-        ++o;
-        ++found;
+    if constexpr (BUF) { /* byte offsets into ecol, as in k_tri_count */
+      const BufU32 E{ecol, (uint32_t)(c_ne.cap * 4)};
+      uint32_t a = ((uint32_t)q + 1u) << 2, ae = (uint32_t)erow[i + 1] << 2;
+      uint32_t b = (uint32_t)erow[j] << 2, be = (uint32_t)erow[j + 1] << 2;
+      while (a < ae && b < be) {
+        const uint32_t x = E.at(a), y = E.at(b);
+        if (x == y) {
+          const uint32_t v[4] = {x, (uint32_t)q, a >> 2, b >> 2};
+          B.put(g, o, v);
+          ++o;
+          ++found;
+        }
+        a += x <= y ? 4u : 0u;
+        b += y <= x ? 4u : 0u;
       }
-      a += (x <= y);
-      b += (y <= x);
+    } else {
+      uint32_t a = (uint32_t)q + 1, ae = (uint32_t)erow[i + 1];
+      uint32_t b = (uint32_t)erow[j], be = (uint32_t)erow[j + 1];
+      while (a < ae && b < be) {
+        const uint32_t x = ecol[a], y = ecol[b];
+        if (x == y) {
+          const uint32_t v[4] = {x, (uint32_t)q, (uint32_t)a, (uint32_t)b};
+          B.put(g, o, v);
+          ++o;
+          ++found;
+        }
+        a += (x <= y);
+        b += (y <= x);
+      }
     }
   }
   __syncthreads(); /* LDS stage, edge table and overflow slots written */
@@ -1142,14 +1204,14 @@ void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
                             PipeLaunch L, uint64_t lim) {
   if (ne.cap <= 0) return;
   dim3 gr((unsigned)((ne.cap + kBlock - 1) / kBlock));
-  if (S.staged)
-    hipLaunchKernelGGL(k_tri_enum_filter<float>, gr, dim3(kBlock), 0, L.s, S,
-                       ne, erow, ecol, eleft, off, cval, cseg, cea, ceb, flag,
-                       bcnt, q1, lim);
-  else
-    hipLaunchKernelGGL(k_tri_enum_filter<double>, gr, dim3(kBlock), 0, L.s, S,
-                       ne, erow, ecol, eleft, off, cval, cseg, cea, ceb, flag,
-                       bcnt, q1, lim);
+  /* byte offsets into ecol while it stays below 4 GiB */
+  const bool buf = ne.cap < ((int64_t)1 << 30);
+  auto kf = S.staged ? (buf ? k_tri_enum_filter<float, true>
+                            : k_tri_enum_filter<float, false>)
+                     : (buf ? k_tri_enum_filter<double, true>
+                            : k_tri_enum_filter<double, false>);
+  hipLaunchKernelGGL(kf, gr, dim3(kBlock), 0, L.s, S, ne, erow, ecol, eleft, off,
+                     cval, cseg, cea, ceb, flag, bcnt, q1, lim);
 }
 /* the kept triangles of block b to (tk, tedge, teik, tejk)[bout[b], ...)
  * in order; trow[e] for the block's edges e (and trow[ne], by the block

@@ -107,6 +107,22 @@ __device__ __forceinline__ uint32_t list_at(const uint32_t* data, uint32_t lcap,
   }
   return data[(uint64_t)lo * lcap + (w - s_pre[lo])];
 }
+/* Loads of a u32 array through a buffer resource: one 32-bit byte offset
+ * per load instead of a 64-bit address, for arrays below 4 GiB (offsets
+ * at or past the array's end read 0).  kBufWord3 is the descriptor's
+ * fourth word for raw 32-bit access on gfx9-family parts (data format
+ * 32). */
+constexpr int kBufWord3 = 0x00020000;
+struct BufU32 {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufU32(const uint32_t* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), (short)0,
+                                            (int)bytes, kBufWord3)) {}
+  __device__ __forceinline__ uint32_t at(uint32_t byte_off) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
+  }
+};
+
 /* Block-staged output of a one-thread-per-segment enumeration.  The
  * segments of a block own the contiguous output range [base, base+total)
  * (their exclusive-scan offsets); each thread's entries go to LDS at their
