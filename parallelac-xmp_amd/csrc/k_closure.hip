@@ -16,42 +16,6 @@
 namespace pac {
 using namespace PAC_V; /* type-independent device helpers */
 
-__global__ __launch_bounds__(kBlock) void k_compact_u4(Cnt c_n,
-                                                       const uint4* in,
-                                                       const uint8_t* flag,
-                                                       const uint32_t* pos,
-                                                       uint4* out,
-                                                       uint64_t ocap) {
-  const int64_t n = c_n.get();
-  int64_t i = gtid();
-  if (i >= n || !flag[i] || pos[i] >= ocap) return;
-  out[pos[i]] = in[i];
-}
-void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
-                         const uint32_t* pos, uint4* out, PipeLaunch L,
-                         uint64_t ocap) {
-  if (n.cap <= 0) return;
-  hipLaunchKernelGGL(k_compact_u4, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, in, flag, pos, out, ocap);
-}
-
-__global__ __launch_bounds__(kBlock) void k_compact_ids(Cnt c_n,
-                                                        const uint8_t* flag,
-                                                        const uint32_t* pos,
-                                                        uint32_t* out,
-                                                        uint64_t ocap) {
-  const int64_t n = c_n.get();
-  int64_t i = gtid();
-  if (i >= n || !flag[i] || pos[i] >= ocap) return;
-  out[pos[i]] = (uint32_t)i;
-}
-void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
-                        uint32_t* out, PipeLaunch L, uint64_t ocap) {
-  if (n.cap <= 0) return;
-  hipLaunchKernelGGL(k_compact_ids, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, flag, pos, out, ocap);
-}
-
 /* out[i] = a[i] | b[i] (flags) */
 __global__ __launch_bounds__(kBlock) void k_or_flags(Cnt c_n, const uint8_t* a,
                                                      const uint8_t* b,
@@ -68,25 +32,6 @@ void launch_or_flags(Cnt n, const uint8_t* a, const uint8_t* b, uint8_t* out,
                      dim3(kBlock), 0, L.s, n, a, b, out);
 }
 /* out[pos[i]] = val[i] != 0 for the flagged i */
-__global__ __launch_bounds__(kBlock) void k_compact_u8(Cnt c_n,
-                                                       const uint8_t* flag,
-                                                       const uint32_t* pos,
-                                                       const uint8_t* val,
-                                                       uint8_t* out,
-                                                       uint64_t ocap) {
-  const int64_t n = c_n.get();
-  const int64_t i = gtid();
-  if (i >= n || !flag[i] || pos[i] >= ocap) return;
-  out[pos[i]] = val[i] != 0;
-}
-void launch_compact_u8(Cnt n, const uint8_t* flag, const uint32_t* pos,
-                       const uint8_t* val, uint8_t* out, PipeLaunch L,
-                       uint64_t ocap) {
-  if (n.cap <= 0) return;
-  hipLaunchKernelGGL(k_compact_u8, dim3((unsigned)((n.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, n, flag, pos, val, out, ocap);
-}
-
 /* faces of the final tets -> tri_in */
 __global__ __launch_bounds__(kBlock) void k_mark_faces(
     Cnt c_nk, const uint4* ftets, const uint64_t* erow, const uint32_t* ecol,
@@ -94,12 +39,21 @@ __global__ __launch_bounds__(kBlock) void k_mark_faces(
   const int64_t nk = c_nk.get();
   int64_t id = gtid();
   if (id >= nk) return;
-  uint4 t = ftets[id];
+  /* t = (i, j, k, l), i < j < k < l: faces ijk and ijl in the row of
+   * edge ij, ikl in that of ik, jkl in that of jk (three edge lookups) */
+  const uint4 t = ftets[id];
+  const int64_t eij = find_row(ecol, erow[t.x], erow[t.x + 1], t.y);
+  const int64_t eik = find_row(ecol, erow[t.x], erow[t.x + 1], t.z);
+  const int64_t ejk = find_row(ecol, erow[t.y], erow[t.y + 1], t.z);
   int64_t f;
-  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.y, t.z)) >= 0) tri_in[f] = 1;
-  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.y, t.w)) >= 0) tri_in[f] = 1;
-  if ((f = find_tri(erow, ecol, trow, tk, t.x, t.z, t.w)) >= 0) tri_in[f] = 1;
-  if ((f = find_tri(erow, ecol, trow, tk, t.y, t.z, t.w)) >= 0) tri_in[f] = 1;
+  if (eij >= 0) {
+    if ((f = find_row(tk, trow[eij], trow[eij + 1], t.z)) >= 0) tri_in[f] = 1;
+    if ((f = find_row(tk, trow[eij], trow[eij + 1], t.w)) >= 0) tri_in[f] = 1;
+  }
+  if (eik >= 0 && (f = find_row(tk, trow[eik], trow[eik + 1], t.w)) >= 0)
+    tri_in[f] = 1;
+  if (ejk >= 0 && (f = find_row(tk, trow[ejk], trow[ejk + 1], t.w)) >= 0)
+    tri_in[f] = 1;
 }
 void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
@@ -172,24 +126,6 @@ void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
   hipLaunchKernelGGL(k_emit_tris, dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nt, ftri, ecol, eleft, tk, tedge,
                      perm, out);
-}
-__global__ __launch_bounds__(kBlock) void k_emit_edges(
-    Cnt c_ne, const uint32_t* ecol, const uint32_t* eleft,
-    const uint8_t* flag, const uint32_t* pos, const uint32_t* perm,
-    int2* out, uint64_t ocap) {
-  const int64_t ne = c_ne.get();
-  int64_t id = gtid();
-  if (id >= ne || !flag[id] || pos[id] >= ocap) return;
-  out[pos[id]] = make_int2((int)perm[eleft[id]], (int)perm[ecol[id]]);
-}
-void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
-                       const uint8_t* flag, const uint32_t* pos,
-                       const uint32_t* perm, int32_t* out, PipeLaunch L,
-                       uint64_t ocap) {
-  if (ne.cap <= 0) return;
-  hipLaunchKernelGGL(k_emit_edges, dim3((unsigned)((ne.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, ne, ecol, eleft, flag, pos, perm,
-                     (int2*)out, ocap);
 }
 
 }  // namespace pac

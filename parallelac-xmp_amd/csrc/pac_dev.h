@@ -290,35 +290,24 @@ struct StageOf<FEDD> {
 };
 
 /* ---- CSR lookups ---- */
-__device__ __forceinline__ int64_t find_in(const uint32_t* a, uint64_t lo,
-                                           uint64_t hi, uint32_t v) {
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    uint32_t x = a[mid];
-    if (x < v)
-      lo = mid + 1;
-    else
-      hi = mid;
+/* Position of v in the sorted row a[lo, hi) or -1, reading the row eight
+ * entries at a time: the eight loads issue together, one round trip per
+ * eight entries where a binary search waits on every probe (the rows of
+ * the alpha edge and triangle CSRs hold a few to a few tens of entries) */
+__device__ __forceinline__ int64_t find_row(const uint32_t* a, uint64_t lo,
+                                            uint64_t hi, uint32_t v) {
+  for (uint64_t p = lo; p < hi; p += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = p + k < hi ? a[p + k] : 0xffffffffu;
+    int hit = 8;
+#pragma unroll
+    for (int k = 7; k >= 0; --k)
+      if (x[k] == v && p + k < hi) hit = k;
+    if (hit < 8) return (int64_t)(p + hit);
+    if (x[7] > v) return -1; /* sorted: v would have been in this batch */
   }
-  return (int64_t)lo;
-}
-__device__ __forceinline__ int64_t find_edge(const uint64_t* erow,
-                                             const uint32_t* ecol, uint32_t a,
-                                             uint32_t b) {
-  uint64_t lo = erow[a], hi = erow[a + 1];
-  int64_t p = find_in(ecol, lo, hi, b);
-  return ((uint64_t)p < hi && ecol[p] == b) ? p : -1;
-}
-__device__ __forceinline__ int64_t find_tri(const uint64_t* erow,
-                                            const uint32_t* ecol,
-                                            const uint64_t* trow,
-                                            const uint32_t* tk, uint32_t a,
-                                            uint32_t b, uint32_t c) {
-  int64_t e = find_edge(erow, ecol, a, b);
-  if (e < 0) return -1;
-  uint64_t lo = trow[e], hi = trow[e + 1];
-  int64_t p = find_in(tk, lo, hi, c);
-  return ((uint64_t)p < hi && tk[p] == c) ? p : -1;
+  return -1;
 }
 
 /* ---- emptiness search box ----

@@ -91,6 +91,10 @@ void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, Cnt n, void* tmp,
                         size_t tmp_bytes, hipStream_t s,
                         uint64_t* total64 = nullptr);
 size_t scan_tmp_bytes(int64_t n);
+/* tile offsets of flags (nonzero = 1) for the compactions below, inside
+ * tmp (scan_tmp_bytes(n)); the count of nonzero flags at *total64 */
+const uint64_t* flag_tile_offsets(const uint8_t* in, Cnt n, void* tmp,
+                                  hipStream_t s, uint64_t* total64);
 /* stable LSD radix sort of (key, val) by the low `bits` bits of key */
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                       uint32_t* vals_alt, int64_t n, int bits, void* tmp,
@@ -228,18 +232,26 @@ struct CandCtx {
   const uint4* ctet; /* tet candidates (i, j, k, l); x = ~0: unused slot */
 };
 
-/* ---- closure + emission (k_closure.hip) ----
- * ocap: capacity of a compaction's output (entries past it are dropped) */
+/* ---- compactions by flags over tile offsets (k_prims.hip) ----
+ * offs from flag_tile_offsets of the same flags; ocap: capacity of the
+ * output (entries past it are dropped) */
 void launch_compact_tets(Cnt n, const uint4* in, const uint8_t* flag,
-                         const uint32_t* pos, uint4* out, PipeLaunch L,
+                         const uint64_t* offs, uint4* out, PipeLaunch L,
                          uint64_t ocap = ~0ull);
-void launch_compact_ids(Cnt n, const uint8_t* flag, const uint32_t* pos,
+void launch_compact_ids(Cnt n, const uint8_t* flag, const uint64_t* offs,
                         uint32_t* out, PipeLaunch L, uint64_t ocap = ~0ull);
-void launch_or_flags(Cnt n, const uint8_t* a, const uint8_t* b, uint8_t* out,
-                     PipeLaunch L);
-void launch_compact_u8(Cnt n, const uint8_t* flag, const uint32_t* pos,
+void launch_compact_u8(Cnt n, const uint8_t* flag, const uint64_t* offs,
                        const uint8_t* val, uint8_t* out, PipeLaunch L,
                        uint64_t ocap = ~0ull);
+/* kept alpha edges as (perm[left], perm[col]) pairs */
+void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
+                       const uint8_t* flag, const uint64_t* offs,
+                       const uint32_t* perm, int32_t* out, PipeLaunch L,
+                       uint64_t ocap = ~0ull);
+
+/* ---- closure + emission (k_closure.hip) ---- */
+void launch_or_flags(Cnt n, const uint8_t* a, const uint8_t* b, uint8_t* out,
+                     PipeLaunch L);
 void launch_mark_faces(Cnt nk, const uint4* ftets, const uint64_t* erow,
                        const uint32_t* ecol, const uint64_t* trow,
                        const uint32_t* tk, uint8_t* tri_in, PipeLaunch L);
@@ -252,10 +264,6 @@ void launch_emit_tris(Cnt nt, const uint32_t* ftri, const uint32_t* ecol,
                       const uint32_t* eleft, const uint32_t* tk,
                       const uint32_t* tedge, const uint32_t* perm,
                       int32_t* out, PipeLaunch L);
-void launch_emit_edges(Cnt ne, const uint32_t* ecol, const uint32_t* eleft,
-                       const uint8_t* flag, const uint32_t* pos,
-                       const uint32_t* perm, int32_t* out, PipeLaunch L,
-                       uint64_t ocap = ~0ull);
 
 /* ---- outputs (k_output.hip) ---- */
 /* Filtration keys of all simplices, key[0, n + E + T + K) (balls, edges,

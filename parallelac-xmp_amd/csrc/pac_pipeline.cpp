@@ -731,11 +731,10 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   Cnt Kf(0);
   DBuf<uint4> ftets;
   {
-    DBuf<uint32_t> pos(P, Kalpha.cap + 1);
     auto tmp = scan_tmp(Kalpha.cap);
+    const uint64_t* koffs = nullptr; /* tile offsets of kflag, in tmp */
     auto tail = [&] {
-      scan_u8flag_to_u32(kflag.get(), pos.get(), Kalpha, tmp.get(), 0, s,
-                         dcnt + RC_KF);
+      koffs = flag_tile_offsets(kflag.get(), Kalpha, tmp.get(), s, dcnt + RC_KF);
     };
     if (opt.skip_tet_ac2 || !Kalpha.cap) {
       fill_u8(kflag.get(), Kalpha.cap, 1, s);
@@ -770,8 +769,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           tail, RC_KF);
     }
     ftets = DBuf<uint4>(P, Kf.cap);
-    launch_compact_tets(Kalpha, tets.get(), kflag.get(), pos.get(), ftets.get(),
-                        L, (uint64_t)Kf.cap);
+    launch_compact_tets(Kalpha, tets.get(), kflag.get(), koffs, ftets.get(), L,
+                        (uint64_t)Kf.cap);
     KCHK();
   }
   tets.release();
@@ -800,12 +799,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   Cnt Tf(0);
   DBuf<uint32_t> ftri;
   {
-    DBuf<uint32_t> pos(P, Talpha.cap + 1);
     auto tmp = scan_tmp(Talpha.cap);
+    const uint64_t* toffs = nullptr; /* tile offsets of tfinal, in tmp */
     auto tail = [&] {
       if (filt) launch_or_flags(Talpha, tflag.get(), tri_in.get(), tfin.get(), L);
-      scan_u8flag_to_u32(tfinal, pos.get(), Talpha, tmp.get(), 0, s,
-                         dcnt + RC_TF);
+      toffs = flag_tile_offsets(tfinal, Talpha, tmp.get(), s, dcnt + RC_TF);
     };
     if (!Talpha.cap) {
       tail();
@@ -839,12 +837,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           tail, RC_TF);
     }
     ftri = DBuf<uint32_t>(P, Tf.cap);
-    launch_compact_ids(Talpha, tfinal, pos.get(), ftri.get(), L,
-                       (uint64_t)Tf.cap);
+    launch_compact_ids(Talpha, tfinal, toffs, ftri.get(), L, (uint64_t)Tf.cap);
     if (filt) {
       R->gab[1] = DBuf<uint8_t>(P, Tf.cap);
-      launch_compact_u8(Talpha, tfinal, pos.get(), tflag.get(), R->gab[1].get(),
-                        L, (uint64_t)Tf.cap);
+      launch_compact_u8(Talpha, tfinal, toffs, tflag.get(), R->gab[1].get(), L,
+                        (uint64_t)Tf.cap);
     }
     KCHK();
   }
@@ -867,13 +864,12 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
                   edge_skip, L);
   const uint8_t* efinal = filt ? efin.get() : eflag.get();
   Cnt Ef(0);
-  DBuf<uint32_t> pos_e(P, Ealpha.cap + 1);
+  auto etmp = scan_tmp(Ealpha.cap);
+  const uint64_t* eoffs = nullptr; /* tile offsets of efinal, in etmp */
   {
-    auto tmp = scan_tmp(Ealpha.cap);
     auto tail = [&] {
       if (filt) launch_or_flags(Ealpha, eflag.get(), edge_in.get(), efin.get(), L);
-      scan_u8flag_to_u32(efinal, pos_e.get(), Ealpha, tmp.get(), 0, s,
-                         dcnt + RC_EF);
+      eoffs = flag_tile_offsets(efinal, Ealpha, etmp.get(), s, dcnt + RC_EF);
     };
     if (!Ealpha.cap) {
       tail();
@@ -912,12 +908,12 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   R->lists[0] = DBuf<int32_t>(P, 2 * Ef.cap);
   R->lists[1] = DBuf<int32_t>(P, 3 * Tf.cap);
   R->lists[2] = DBuf<int32_t>(P, 4 * Kf.cap);
-  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), efinal, pos_e.get(),
-                    perm.get(), R->lists[0].get(), L, (uint64_t)Ef.cap);
+  launch_emit_edges(Ealpha, ecol.get(), eleft.get(), efinal, eoffs, perm.get(),
+                    R->lists[0].get(), L, (uint64_t)Ef.cap);
   if (filt) {
     R->gab[0] = DBuf<uint8_t>(P, Ef.cap);
-    launch_compact_u8(Ealpha, efinal, pos_e.get(), eflag.get(), R->gab[0].get(),
-                      L, (uint64_t)Ef.cap);
+    launch_compact_u8(Ealpha, efinal, eoffs, eflag.get(), R->gab[0].get(), L,
+                      (uint64_t)Ef.cap);
     R->rank = DBuf<uint32_t>(P, n);
     launch_rank_of(perm.get(), n, R->rank.get(), s);
   }
