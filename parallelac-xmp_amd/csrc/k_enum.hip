@@ -705,7 +705,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
     Stage S, Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
-    uint4* ctet, uint8_t* flag, uint32_t* bcnt, Queue q1, uint64_t lim) {
+    uint4* ctet, uint8_t* flag, uint32_t* bcnt, uint32_t* bkeep, Queue q1,
+    uint64_t lim) {
   const int64_t nt = c_nt.get();
   /* 8 blocks per CU fit in LDS with block_flush's 16 B (the kernel's 60
    * VGPRs allow 8 waves per SIMD); a block of 256 alpha triangles averages
@@ -714,9 +715,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= nt) { /* whole block past the (device) count */
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = bkeep[blockIdx.x] = 0;
     return;
   }
+  __shared__ unsigned s_und; /* an undecided candidate written */
+  if (threadIdx.x == 0) s_und = 0;
   const int64_t q1e = q0 + kBlock < nt ? q0 + kBlock : nt;
   BlockOut4<CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
   int64_t q = gtid();
@@ -818,12 +821,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
       const uint64_t at = base + pre;
       ctet[at] = v;
       flag[at] = (uint8_t)r;
-      if (r == UNDEC) push_q(q1, (uint32_t)at, v.x);
+      if (r == UNDEC) {
+        push_q(q1, (uint32_t)at, v.x);
+        s_und = 1;
+      }
     }
     run += tot;
     __syncthreads(); /* s_wsum is rewritten next round */
   }
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = run;
+  if (threadIdx.x == 0) {
+    bcnt[blockIdx.x] = run;
+    bkeep[blockIdx.x] = s_und ? kRecount : run;
+  }
   block_flush(S.ctr, CTR_CAND + 2, found);
 }
 void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
@@ -831,8 +840,8 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
                             const uint64_t* off, uint4* ctet, uint8_t* flag,
-                            uint32_t* bcnt, Queue q1, PipeLaunch L,
-                            uint64_t lim) {
+                            uint32_t* bcnt, uint32_t* bkeep, Queue q1,
+                            PipeLaunch L, uint64_t lim) {
   if (nt.cap <= 0) return;
   dim3 gr((unsigned)((nt.cap + kBlock - 1) / kBlock));
   /* byte offsets into tk while it stays below 4 GiB */
@@ -842,14 +851,15 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
       auto kf = buf ? k_tet_enum_filter<float, true>
                     : k_tet_enum_filter<float, false>;
       hipLaunchKernelGGL(kf, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge,
-                         teik, tejk, eleft, ecol, off, ctet, flag, bcnt, q1, lim);
+                         teik, tejk, eleft, ecol, off, ctet, flag, bcnt, bkeep,
+                         q1, lim);
       return;
     }
   }
   auto kd = buf ? k_tet_enum_filter<double, true>
                 : k_tet_enum_filter<double, false>;
   hipLaunchKernelGGL(kd, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge, teik,
-                     tejk, eleft, ecol, off, ctet, flag, bcnt, q1, lim);
+                     tejk, eleft, ecol, off, ctet, flag, bcnt, bkeep, q1, lim);
 }
 
 /* After the ladder: the kept tets of each enumeration block (its first
@@ -863,6 +873,9 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_keep(
   const int64_t b = gtid() >> 6;
   const int lane = threadIdx.x & 63;
   if (b >= nblk) return;
+  /* a block that queued no undecided candidate kept all it wrote: its
+   * enumeration left that count here */
+  if (bkeep[b] != kRecount) return;
   unsigned c = 0;
   if (b * kBlock < nt) {
     const uint64_t base = off[b * kBlock];
@@ -1099,7 +1112,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
     Stage S, Cnt c_ne, const uint64_t* erow, const uint32_t* ecol,
     const uint32_t* eleft, const uint64_t* off, uint32_t* cval,
     uint32_t* cseg, uint32_t* cea, uint32_t* ceb, uint8_t* flag,
-    uint32_t* bcnt, Queue q1, uint64_t lim) {
+    uint32_t* bcnt, uint32_t* bkeep, Queue q1, uint64_t lim) {
   const int64_t ne = c_ne.get();
   /* 6 blocks per CU fit in LDS with the edge table and block_flush's 16 B
    * (a block averages ~1,800 slots on the 100k globule; the rest go to
@@ -1109,9 +1122,11 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
   __shared__ uint2 s_ij[kBlock]; /* the block's edges (i, j) */
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= ne) { /* whole block past the (device) count */
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = bkeep[blockIdx.x] = 0;
     return;
   }
+  __shared__ unsigned s_und; /* an undecided candidate written */
+  if (threadIdx.x == 0) s_und = 0;
   const int64_t q1e = q0 + kBlock < ne ? q0 + kBlock : ne;
   BlockOut<4, CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
   uint32_t* const g[4] = {cval, cseg, cea, ceb};
@@ -1188,20 +1203,26 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
 #pragma unroll
       for (int k = 0; k < 4; ++k) g[k][at] = v[k];
       flag[at] = (uint8_t)r;
-      if (r == UNDEC) push_q(q1, (uint32_t)at, v[1]);
+      if (r == UNDEC) {
+        push_q(q1, (uint32_t)at, v[1]);
+        s_und = 1;
+      }
     }
     run += tot;
     __syncthreads(); /* s_wsum is rewritten next round */
   }
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = run;
+  if (threadIdx.x == 0) {
+    bcnt[blockIdx.x] = run;
+    bkeep[blockIdx.x] = s_und ? kRecount : run;
+  }
   block_flush(S.ctr, CTR_CAND + 1, found);
 }
 void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
                             const uint32_t* ecol, const uint32_t* eleft,
                             const uint64_t* off, uint32_t* cval,
                             uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                            uint8_t* flag, uint32_t* bcnt, Queue q1,
-                            PipeLaunch L, uint64_t lim) {
+                            uint8_t* flag, uint32_t* bcnt, uint32_t* bkeep,
+                            Queue q1, PipeLaunch L, uint64_t lim) {
   if (ne.cap <= 0) return;
   dim3 gr((unsigned)((ne.cap + kBlock - 1) / kBlock));
   /* byte offsets into ecol while it stays below 4 GiB */
@@ -1211,7 +1232,7 @@ void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
                      : (buf ? k_tri_enum_filter<double, true>
                             : k_tri_enum_filter<double, false>);
   hipLaunchKernelGGL(kf, gr, dim3(kBlock), 0, L.s, S, ne, erow, ecol, eleft, off,
-                     cval, cseg, cea, ceb, flag, bcnt, q1, lim);
+                     cval, cseg, cea, ceb, flag, bcnt, bkeep, q1, lim);
 }
 /* the kept triangles of block b to (tk, tedge, teik, tejk)[bout[b], ...)
  * in order; trow[e] for the block's edges e (and trow[ne], by the block

@@ -428,8 +428,8 @@ void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
                             const uint32_t* ecol, const uint32_t* eleft,
                             const uint64_t* off, uint32_t* cval,
                             uint32_t* cseg, uint32_t* cea, uint32_t* ceb,
-                            uint8_t* flag, uint32_t* bcnt, Queue q1,
-                            PipeLaunch L, uint64_t lim = ~0ull);
+                            uint8_t* flag, uint32_t* bcnt, uint32_t* bkeep,
+                            Queue q1, PipeLaunch L, uint64_t lim = ~0ull);
 /* after the ladder (bkeep by launch_tet_block_keep over the ne alpha
  * edges, bout its exclusive scan): the kept triangles of each block to
  * (tk, tedge, teik, tejk) in order, and the alpha-triangle row start of
@@ -451,10 +451,15 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
                             const uint64_t* off, uint4* ctet, uint8_t* flag,
-                            uint32_t* bcnt, Queue q1, PipeLaunch L,
-                            uint64_t lim = ~0ull);
+                            uint32_t* bcnt, uint32_t* bkeep, Queue q1,
+                            PipeLaunch L, uint64_t lim = ~0ull);
+/* The triangle and tet enumerations also write bkeep[b]: bcnt[b] when the
+ * block queued no undecided candidate (the ladder cannot drop any of its
+ * entries), else kRecount. */
+constexpr uint32_t kRecount = 0xffffffffu;
 /* after the ladder: kept tets per enumeration block (ceil(nt.cap / 256)
- * entries), then, with bout their exclusive scan, the kept tets in order */
+ * entries; blocks marked kRecount recounted from their flags), then, with
+ * bout their exclusive scan, the kept tets in order */
 void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
                            const uint8_t* flag, uint32_t* bkeep, PipeLaunch L);
 void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,

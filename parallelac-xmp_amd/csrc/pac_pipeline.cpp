@@ -606,8 +606,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
             zero_u64(ctr + (CTR_CAND + 1) * CTR_SLOTS, CTR_SLOTS, s);
           launch_tri_enum_filter(SG, Ealpha, erow.get(), ecol.get(), eleft.get(),
                                  off.get(), cval.get(), cseg.get(), cea.get(),
-                                 ceb.get(), flag.get(), bcnt.get(), qq, L,
-                                 (uint64_t)Tslots.cap);
+                                 ceb.get(), flag.get(), bcnt.get(), bkeep.get(),
+                                 qq, L, (uint64_t)Tslots.cap);
         },
         [&](QRef qr, Queue q2) {
           launch_filter_esc(SG, 2, C1, qr, cval.get(), flag.get(), q2, L);
@@ -668,7 +668,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_tet_enum_filter(SG, Talpha, trow.get(), tk.get(), tedge.get(),
                                  teik.get(), tejk.get(), eleft.get(), ecol.get(),
                                  off.get(), ctet.get(), flag.get(), bcnt.get(),
-                                 qq, L, (uint64_t)Kslots.cap);
+                                 bkeep.get(), qq, L, (uint64_t)Kslots.cap);
         },
         [&](QRef qr, Queue q2) {
           launch_filter_esc(SG, 3, C2, qr, nullptr, flag.get(), q2, L);
