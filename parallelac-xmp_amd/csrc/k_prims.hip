@@ -534,14 +534,21 @@ __global__ void k_make_view(const QEntry* q, uint32_t cap, unsigned* cnt,
                             QView* v, unsigned* status, unsigned bit,
                             unsigned* fill, unsigned* also, int fresh) {
   if (threadIdx.x != 0) return;
-  uint32_t acc = 0;
+  unsigned c[QSLOTS];
   unsigned mx = 0;
   for (int k = 0; k < QSLOTS; ++k) {
-    const unsigned c = cnt[k];
+    c[k] = cnt[k];
     cnt[k] = 0;
+    mx = c[k] > mx ? c[k] : mx;
+  }
+  /* an eager ladder's first view (fresh) is empty when its queue
+   * overflowed: the ladder runs again with a larger queue, and its
+   * consumers need not work through the partial one */
+  const bool drop = fresh && mx > cap;
+  uint32_t acc = 0;
+  for (int k = 0; k < QSLOTS; ++k) {
     v->off[k] = acc;
-    acc += c < cap ? c : cap;
-    mx = c > mx ? c : mx;
+    acc += drop ? 0u : (c[k] < cap ? c[k] : cap);
   }
   v->off[QSLOTS] = acc;
   v->q = q;
@@ -564,6 +571,23 @@ void launch_make_view(const QEntry* q, uint32_t cap, unsigned* cnt, QView* v,
                       unsigned* also, bool fresh, hipStream_t s) {
   hipLaunchKernelGGL(k_make_view, dim3(1), dim3(64), 0, s, q, cap, cnt, v,
                      status, bit, fill, also, (int)fresh);
+}
+
+/* An eager ladder's state in one buffer for one read-back: the
+ * sub-queue counts of its last queue (then cleared for the next producer)
+ * and its count, beside the status and fill its first view left */
+__global__ void k_ladder_state(unsigned* cnt, const uint64_t* count,
+                               unsigned* lad) {
+  const int t = threadIdx.x;
+  if (t < QSLOTS) {
+    lad[kLadCnt + t] = cnt[t];
+    cnt[t] = 0;
+  }
+  if (t == 0) *reinterpret_cast<uint64_t*>(lad + kLadCount) = *count;
+}
+void launch_ladder_state(unsigned* cnt, const uint64_t* count, unsigned* lad,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_ladder_state, dim3(1), dim3(64), 0, s, cnt, count, lad);
 }
 
 __global__ void k_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,

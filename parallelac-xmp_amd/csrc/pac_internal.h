@@ -217,12 +217,21 @@ enum { REC_CAP = 0,    /* bits 0..8: a count exceeded its capacity */
        REC_STALE = 11  /* the input no longer matches the recorded plan */ };
 /* view of QSLOTS sub-queues (counters cleared after reading, and the
  * LSLOTS list counters at also);
- * fresh: status / fill overwritten, else merged */
+ * fresh (an eager ladder's first view): status / fill overwritten, and
+ * the view left empty when a sub-queue overflowed (the ladder reruns);
+ * else status / fill merged and the view clamped to the capacity */
 void launch_make_view(const QEntry* q, uint32_t cap, unsigned* cnt, QView* v,
                       unsigned* status, unsigned bit, unsigned* fill,
                       unsigned* also, bool fresh, hipStream_t s);
 void launch_cap_check(const uint64_t* v, uint64_t cap, unsigned* status,
                       unsigned bit, hipStream_t s);
+/* eager ladder state (u32 words): [kLadStatus] status, [kLadFill] fill of
+ * the first view, [kLadCount] the count (u64), [kLadCnt, + QSLOTS) the
+ * last queue's sub-queue counts */
+constexpr int kLadStatus = 0, kLadFill = 1, kLadCount = 4, kLadCnt = 6;
+constexpr int kLadWords = kLadCnt + QSLOTS;
+void launch_ladder_state(unsigned* cnt, const uint64_t* count, unsigned* lad,
+                         hipStream_t s);
 void launch_plan_check(const Bounds* now, const Bounds* ref, int64_t m,
                        int scale, unsigned* status, unsigned bit,
                        hipStream_t s);

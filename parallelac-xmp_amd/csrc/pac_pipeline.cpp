@@ -329,6 +329,9 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   unsigned* const qn = rec ? rec->qn : qn_own.get();
   unsigned* const nrest = qn + 2 * QSLOTS;
   unsigned* const nnext = nrest + LSLOTS;
+  /* eager ladders: the device view of q1 and the state read back once */
+  DBuf<QView> ev_view(P, 1);
+  DBuf<unsigned> lad(P, kLadWords);
 
   /* ---- binning: key, stable radix sort, gather, cell starts ---- */
   DBuf<uint32_t> key(P, n), idx(P, n), key2(P, n), idx2(P, n);
@@ -397,10 +400,14 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* Precision ladder: hot kernel (q1) -> [staged: escalation kernel FP64 ->
    * double-double (q1 -> q2)] -> exact wide-integer kernel, then `tail`,
    * the scan that leaves the ladder's count at dcnt[q].
-   * Eager: the queue counters are read back to build each consumer's view
-   * on the host (sizing the escalation queue from the entries actually
-   * queued and skipping empty stages); a queue overflow reruns its
-   * producer with a larger queue (the producers are idempotent).
+   * Eager: the escalation reads q1 through a view built on the device, and
+   * the tail runs before the exact stage on the chance that the last queue
+   * (q2; q1 without escalation) is empty, which it almost always is; one
+   * read-back then brings that queue's counts, the count and the overflow
+   * status.  Only when the last queue holds entries are they read into a
+   * host view (for the exact stage, sorted when long), and the tail run
+   * again after it.  An overflow reruns the ladder with larger queues (the
+   * stages are idempotent).
    * Recorded: views built on the device, queues of fixed size, overflows
    * reported through the status word. */
   const int64_t qmax = (int64_t)(UINT32_MAX / QSLOTS) - 1;
@@ -412,32 +419,6 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   };
   auto grow = [&](uint32_t need) {
     return (uint32_t)std::min<int64_t>((int64_t)need + need / 4 + 256, qmax);
-  };
-  /* counts of the sub-queues -> consumer view; false on overflow, with
-   * *need = the per-slot capacity that would have sufficed; the counters
-   * are left cleared for the next producer */
-  auto view_of = [&](const QEntry* qq, uint32_t cap, unsigned* dcq, QView* v,
-                     uint32_t* need) -> bool {
-    unsigned h[QSLOTS];
-    HIPCHK(hipMemcpyAsync(E->pinned, dcq, sizeof(h), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::memcpy(h, E->pinned, sizeof(h));
-    zero_u32(dcq, QSLOTS, s);
-    v->q = qq;
-    v->cap = cap;
-    v->work = 0;
-    uint64_t acc = 0;
-    unsigned mx = 0;
-    for (int k = 0; k < QSLOTS; ++k) {
-      v->off[k] = (uint32_t)acc;
-      acc += std::min(h[k], cap);
-      mx = std::max(mx, h[k]);
-    }
-    v->off[QSLOTS] = (uint32_t)acc;
-    *need = mx;
-    int64_t& fill = R->sizes[(dcq == qn) ? 9 : 10];
-    fill = std::max<int64_t>(fill, mx);
-    return mx <= cap;
   };
   auto ladder = [&](int64_t cap1_total, int kp_hot, int kp_exact, auto&& hot,
                     auto&& esc, auto&& exact, auto&& tail, int q) -> Cnt {
@@ -460,8 +441,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       return settle(q);
     }
     uint32_t c1 = slot_cap(cap1_total);
+    /* q2 as large as q1: an escalated pair queues at most itself; a
+     * whole-simplex entry queues its undecided pairs, and an overflow
+     * reruns the ladder */
+    uint32_t c2 = staged ? slot_cap((int64_t)c1 * QSLOTS + 4096) : 0;
     DBuf<QEntry> q1, q2;
-    QView v1;
     for (;;) {
       q1 = DBuf<QEntry>(P, (size_t)c1 * QSLOTS);
       zero_u32(nrest, LSLOTS, s);
@@ -469,29 +453,65 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       hot(Queue{q1.get(), qn, c1});
       if (kp_hot >= 0) KP.end();
       KCHK();
-      uint32_t need = 0;
-      if (view_of(q1.get(), c1, qn, &v1, &need)) break;
-      c1 = grow(need);
-    }
-    QView vx = v1;
-    if (staged && v1.size()) {
-      uint32_t c2 = slot_cap(4 * (int64_t)v1.size() + 4096);
-      for (;;) {
+      const QEntry* last = q1.get();
+      uint32_t clast = c1;
+      unsigned* cnt_last = qn;
+      if (staged) {
         q2 = DBuf<QEntry>(P, (size_t)c2 * QSLOTS);
-        esc(QRef{v1, nullptr, 0}, Queue{q2.get(), qn + QSLOTS, c2});
+        launch_make_view(q1.get(), c1, qn, ev_view.get(), lad.get() + kLadStatus,
+                         1u, lad.get() + kLadFill, nullptr, true, s);
+        esc(QRef{QView{}, ev_view.get(), QSLOTS * c1},
+            Queue{q2.get(), qn + QSLOTS, c2});
         KCHK();
-        uint32_t need = 0;
-        if (view_of(q2.get(), c2, qn + QSLOTS, &vx, &need)) break;
-        c2 = grow(need);
+        last = q2.get();
+        clast = c2;
+        cnt_last = qn + QSLOTS;
       }
+      tail();
+      launch_ladder_state(cnt_last, dcnt + q, lad.get(), s);
+      KCHK();
+      unsigned h[kLadWords];
+      HIPCHK(hipMemcpyAsync(E->pinned, lad.get(), sizeof(h),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      std::memcpy(h, E->pinned, sizeof(h));
+      QView vx;
+      vx.q = last;
+      vx.cap = clast;
+      vx.work = 0;
+      uint64_t acc = 0;
+      unsigned mx = 0;
+      for (int k = 0; k < QSLOTS; ++k) {
+        vx.off[k] = (uint32_t)acc;
+        acc += std::min(h[kLadCnt + k], clast);
+        mx = std::max(mx, h[kLadCnt + k]);
+      }
+      vx.off[QSLOTS] = (uint32_t)acc;
+      const unsigned fill1 = staged ? h[kLadFill] : mx;
+      R->sizes[9] = std::max<int64_t>(R->sizes[9], fill1);
+      if (staged) R->sizes[10] = std::max<int64_t>(R->sizes[10], mx);
+      if (fill1 > c1) { /* q1 overflowed: everything again, larger */
+        c1 = grow(fill1);
+        if (staged) c2 = std::max(c2, slot_cap((int64_t)c1 * QSLOTS + 4096));
+        continue;
+      }
+      if (mx > clast) { /* q2 overflowed */
+        c2 = grow(mx);
+        continue;
+      }
+      if (!vx.size()) {
+        uint64_t cnt;
+        std::memcpy(&cnt, h + kLadCount, sizeof(cnt));
+        return Cnt((int64_t)cnt);
+      }
+      if (kp_exact >= 0) KP.begin(kp_exact);
+      exact(QRef{vx, nullptr, 0});
+      if (kp_exact >= 0) KP.end();
+      KCHK();
+      tail();
+      KCHK();
+      return settle(q);
     }
-    if (kp_exact >= 0) KP.begin(kp_exact);
-    exact(QRef{vx, nullptr, 0});
-    if (kp_exact >= 0) KP.end();
-    KCHK();
-    tail();
-    KCHK();
-    return settle(q);
   };
 
   /* hot-stage evaluations booked on the host: one per edge candidate
