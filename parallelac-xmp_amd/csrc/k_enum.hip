@@ -892,30 +892,36 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_copy(
   const int64_t nt = c_nt.get();
   const int64_t b = blockIdx.x;
   if (b * kBlock >= nt) return;
-  __shared__ unsigned s_wsum[kBlock / 64];
   const uint64_t base = off[b * kBlock];
   const uint32_t n = bcnt[b];
+  uint64_t o = bout[b];
+  if (bout[b + 1] - o == n) { /* every entry kept: a straight copy */
+    for (uint32_t t = threadIdx.x; t < n; t += kBlock)
+      if (o + t < ocap) tets[o + t] = ctet[base + t];
+    return;
+  }
+  /* wave sums double-buffered by round parity (one barrier a round) */
+  __shared__ unsigned s_wsum[2][kBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  uint64_t o = bout[b];
-  for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+  int par = 0;
+  for (uint32_t r0 = 0; r0 < n; r0 += kBlock, par ^= 1) {
     const uint32_t t = r0 + threadIdx.x;
     const bool keep = t < n && flag[base + t] == KEEP;
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
     if (keep) v = ctet[base + t];
     const uint64_t bm = __ballot(keep);
-    if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+    if (lane == 0) s_wsum[par][w] = (unsigned)__popcll(bm);
     __syncthreads();
     uint64_t pre = o + (uint64_t)__popcll(bm & lt), tot = 0;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; ++q) {
-      const unsigned c = s_wsum[q];
+      const unsigned c = s_wsum[par][q];
       if (q < w) pre += c;
       tot += c;
     }
     if (keep && pre < ocap) tets[pre] = v;
     o += tot;
-    __syncthreads();
   }
 }
 void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
@@ -1248,7 +1254,9 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
   const int64_t b = blockIdx.x;
   const int64_t q0 = b * kBlock;
   if (q0 > ne) return;
-  __shared__ unsigned s_wsum[kBlock / 64];
+  /* wave sums double-buffered by round parity: a round's sums are
+   * rewritten two rounds later, after the next round's barrier */
+  __shared__ unsigned s_wsum[2][kBlock / 64];
   __shared__ uint32_t s_cnt[kBlock];
   s_cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -1258,22 +1266,44 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
   if (q0 < ne) {
     const uint64_t base = off[q0];
     const uint32_t n = bcnt[b];
+    /* every entry kept (bout's step equals the written count): positions
+     * follow without ranking, and no flag is read */
+    const bool all = bout[b + 1] - o0 == n;
     uint64_t o = o0;
-    for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+    int par = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += kBlock, par ^= 1) {
       const uint32_t t = r0 + threadIdx.x;
-      const bool keep = t < n && flag[base + t] == KEEP;
-      uint32_t seg = 0;
-      if (keep) {
-        seg = cseg[base + t];
-        atomicAdd(&s_cnt[seg - (uint32_t)q0], 1u);
+      const bool keep = t < n && (all || flag[base + t] == KEEP);
+      const uint32_t seg = keep ? cseg[base + t] : 0xffffffffu;
+      /* entries come in edge order: one LDS add per run of lanes with
+       * the same edge (its first lane adds the run's length) instead of
+       * one per lane on the same counter */
+      const uint32_t prev = (uint32_t)__shfl_up((int)seg, 1, 64);
+      const uint64_t heads = __ballot(lane == 0 || prev != seg);
+      if (keep && ((heads >> lane) & 1ull)) {
+        const uint64_t later = lane == 63 ? 0ull : heads >> (lane + 1);
+        const unsigned len = later ? (unsigned)__builtin_ctzll(later) + 1u
+                                   : 64u - (unsigned)lane;
+        atomicAdd(&s_cnt[seg - (uint32_t)q0], len);
+      }
+      if (all) {
+        if (keep && o + t - r0 < ocap) {
+          const uint64_t at = o + t - r0;
+          tk[at] = cval[base + t];
+          tedge[at] = seg;
+          teik[at] = cea[base + t];
+          tejk[at] = ceb[base + t];
+        }
+        o += min((uint32_t)kBlock, n - r0);
+        continue;
       }
       const uint64_t bm = __ballot(keep);
-      if (lane == 0) s_wsum[w] = (unsigned)__popcll(bm);
+      if (lane == 0) s_wsum[par][w] = (unsigned)__popcll(bm);
       __syncthreads();
       uint64_t pre = o + (uint64_t)__popcll(bm & lt), tot = 0;
 #pragma unroll
       for (int k = 0; k < kBlock / 64; ++k) {
-        const unsigned c = s_wsum[k];
+        const unsigned c = s_wsum[par][k];
         if (k < w) pre += c;
         tot += c;
       }
@@ -1284,9 +1314,9 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
         tejk[pre] = ceb[base + t];
       }
       o += tot;
-      __syncthreads();
     }
   }
+  __syncthreads(); /* the per-edge counts complete */
   /* exclusive scan of the per-edge counts (one wave per 64 edges, then
    * the wave totals) */
   const uint32_t c = s_cnt[threadIdx.x];
@@ -1296,12 +1326,12 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
     const uint32_t y = (uint32_t)__shfl_up((int)incl, d, 64);
     if (lane >= d) incl += y;
   }
-  if (lane == 63) s_wsum[w] = incl;
+  if (lane == 63) s_wsum[0][w] = incl;
   __syncthreads();
   uint64_t before = o0;
 #pragma unroll
   for (int k = 0; k < kBlock / 64; ++k)
-    if (k < w) before += s_wsum[k];
+    if (k < w) before += s_wsum[0][k];
   const int64_t e = q0 + threadIdx.x;
   if (e <= ne) {
     const uint64_t v = before + incl - c;
