@@ -553,7 +553,6 @@ __global__ void k_make_view(const QEntry* q, uint32_t cap, unsigned* cnt,
   v->off[QSLOTS] = acc;
   v->q = q;
   v->cap = cap;
-  v->work = 0;
   if (also)
     for (int k = 0; k < LSLOTS; ++k) also[k] = 0;
   const unsigned b = mx > cap ? bit : 0u;

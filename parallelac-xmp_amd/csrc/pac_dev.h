@@ -229,33 +229,19 @@ __device__ __forceinline__ bool live_entry(const QEntry& e,
   return e.aux == 0xffffffffu || flag[e.item] != 0;
 }
 
-/* Entries of a queue for its consumer kernel.  Every thread takes one
- * entry by its index; with a device view, entries past the grid are then
- * handed out 64 at a time per wave from the view's work counter, so that
- * entries of very different cost (exact ties, full scans) spread over all
- * waves of a small grid, and a queue no larger than the grid costs no
- * atomics; with a host view, grid-stride. */
+/* Entries of a queue for its consumer kernel: thread t takes entry t,
+ * then t + grid, ... (a host view's grid covers its entries; a device
+ * view's is at most a few hundred blocks). */
 template <class F>
 __device__ __forceinline__ void for_queue(const QRef& q, F&& body) {
   const uint32_t n = q.size();
   const uint32_t grid = gridDim.x * blockDim.x;
   const uint32_t t0 = (uint32_t)gtid();
   if (t0 < n) body(t0);
-  if (n <= grid) return;
-  if (q.dv) {
-    unsigned* work = const_cast<unsigned*>(&q.dv->work);
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-      unsigned base = 0;
-      if (lane == 0) base = atomicAdd(work, 64u);
-      base = __shfl(base, 0, 64);
-      if (base >= n - grid) break;
-      const uint32_t t = grid + base + lane;
-      if (t < n) body(t);
-    }
-  } else {
-    for (uint32_t t = t0 + grid; t < n; t += grid) body(t);
-  }
+  /* a static stride: a shared work counter took one device-scope atomic
+   * per 64 entries, and with tens of millions of escalated entries
+   * (degenerate inputs) those serialized */
+  for (uint32_t t = t0 + grid; t < n; t += grid) body(t);
 }
 
 template <class T>

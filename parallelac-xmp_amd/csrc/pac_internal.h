@@ -168,7 +168,6 @@ struct QView {
   const QEntry* q;
   uint32_t cap;
   uint32_t off[QSLOTS + 1]; /* exclusive prefix of the sub-queue counts */
-  uint32_t work;            /* device views: entries handed out so far */
   __host__ __device__ uint32_t size() const { return off[QSLOTS]; }
   __device__ QEntry at(uint32_t i) const {
     int lo = 0, hi = QSLOTS - 1; /* last k with off[k] <= i */
@@ -193,8 +192,8 @@ struct QRef {
 };
 /* grid of a queue consumer: host views (size known) one entry per
  * thread; device views (size known only on the device) at most maxb
- * blocks, the consumers taking further entries dynamically, so that a
- * queue sized for many entries but holding few costs few idle blocks */
+ * blocks striding over the entries, so that a queue sized for many
+ * entries but holding few costs few idle blocks */
 constexpr unsigned kEscBlocks = 1024, kExactBlocks = 512;
 inline dim3 qgrid(const QRef& q, unsigned maxb) {
   const unsigned b = (q.launch_items() + kBlock - 1) / kBlock;

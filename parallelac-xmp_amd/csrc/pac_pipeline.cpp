@@ -478,7 +478,6 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
       QView vx;
       vx.q = last;
       vx.cap = clast;
-      vx.work = 0;
       uint64_t acc = 0;
       unsigned mx = 0;
       for (int k = 0; k < QSLOTS; ++k) {

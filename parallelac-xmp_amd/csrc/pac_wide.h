@@ -227,30 +227,15 @@ __device__ __forceinline__ CW<TPI> cw_of(const WI& a) {
 }
 
 /* Queue entries for groups of TPI lanes: group g of the grid takes entry
- * g, then g + groups, ...; with a device view, entries past the grid are
- * handed out per wave (64 / TPI at a time) from the view's work counter.
- * Every lane of a group calls body(t) with the same t. */
+ * g, then g + groups, ...  Every lane of a group calls body(t) with the
+ * same t. */
 template <int TPI, class F>
 __device__ __forceinline__ void for_queue_grp(const QRef& q, F&& body) {
   const uint32_t n = q.size();
   const uint32_t groups = (gridDim.x * blockDim.x) / TPI;
   const uint32_t g0 = (uint32_t)(gtid() / TPI);
   if (g0 < n) body(g0);
-  if (n <= groups) return;
-  if (q.dv) {
-    unsigned* work = const_cast<unsigned*>(&q.dv->work);
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-      unsigned base = 0;
-      if (lane == 0) base = atomicAdd(work, (unsigned)(64 / TPI));
-      base = __shfl(base, 0, 64);
-      if (base >= n - groups) break;
-      const uint32_t t = groups + base + (uint32_t)(lane / TPI);
-      if (t < n) body(t);
-    }
-  } else {
-    for (uint32_t t = g0 + groups; t < n; t += groups) body(t);
-  }
+  for (uint32_t t = g0 + groups; t < n; t += groups) body(t); /* as for_queue */
 }
 
 }  // namespace PAC_V
