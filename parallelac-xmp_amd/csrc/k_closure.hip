@@ -40,15 +40,16 @@ __global__ __launch_bounds__(kBlock) void k_mark_faces(
   int64_t id = gtid();
   if (id >= nk) return;
   /* t = (i, j, k, l), i < j < k < l: faces ijk and ijl in the row of
-   * edge ij, ikl in that of ik, jkl in that of jk (three edge lookups) */
+   * edge ij, ikl in that of ik, jkl in that of jk; edges ij and ik are in
+   * row i, so two row scans find the three edges */
   const uint4 t = ftets[id];
-  const int64_t eij = find_row(ecol, erow[t.x], erow[t.x + 1], t.y);
-  const int64_t eik = find_row(ecol, erow[t.x], erow[t.x + 1], t.z);
+  int64_t eij, eik, f, g;
+  find_row2(ecol, erow[t.x], erow[t.x + 1], t.y, t.z, eij, eik);
   const int64_t ejk = find_row(ecol, erow[t.y], erow[t.y + 1], t.z);
-  int64_t f;
   if (eij >= 0) {
-    if ((f = find_row(tk, trow[eij], trow[eij + 1], t.z)) >= 0) tri_in[f] = 1;
-    if ((f = find_row(tk, trow[eij], trow[eij + 1], t.w)) >= 0) tri_in[f] = 1;
+    find_row2(tk, trow[eij], trow[eij + 1], t.z, t.w, f, g);
+    if (f >= 0) tri_in[f] = 1;
+    if (g >= 0) tri_in[g] = 1;
   }
   if (eik >= 0 && (f = find_row(tk, trow[eik], trow[eik + 1], t.w)) >= 0)
     tri_in[f] = 1;

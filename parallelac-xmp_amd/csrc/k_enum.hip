@@ -697,6 +697,18 @@ constexpr int fam_of(int KIND) {
  * and the four vertex ids only of the slots that are kept or undecided
  * (the later stages read no others).  Slots past the LDS stage were
  * written to global memory during the merges and are filtered from there. */
+/* First slot of alpha triangle q without a per-triangle offset array:
+ * the triangles of edge e occupy rows trow[e] .. trow[e+1]-1 and each has
+ * its row's tail after it as slots, so triangle s + k of a row of L = trow
+ * [e+1] - s triangles starts k (L - 1) - k (k - 1) / 2 slots into the
+ * edge's range, which begins at roff[e] (the exclusive scan over edges of
+ * L (L - 1) / 2). */
+__device__ __forceinline__ uint64_t tet_slot_off(const uint64_t* roff,
+                                                 const uint64_t* trow,
+                                                 uint32_t e, uint32_t q) {
+  const uint64_t s = trow[e], L = trow[e + 1] - s, k = (uint64_t)q - s;
+  return roff[e] + k * (L - 1) - k * (k - 1) / 2;
+}
 /* FP32 first stage: 8 waves per SIMD (at most 64 VGPRs, no spills; the
  * LDS stage admits 8 blocks per CU); FP64: the compiler's choice */
 template <class R, bool BUF>
@@ -705,8 +717,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
     Stage S, Cnt c_nt, const uint64_t* trow, const uint32_t* tk,
     const uint32_t* tedge, const uint32_t* teik, const uint32_t* tejk,
     const uint32_t* eleft, const uint32_t* ecol, const uint64_t* off,
-    uint4* ctet, uint8_t* flag, uint32_t* bcnt, uint32_t* bkeep, Queue q1,
-    uint64_t lim) {
+    const uint64_t* roff, uint64_t* bbase, uint4* ctet, uint8_t* flag,
+    uint32_t* bcnt, uint32_t* bkeep, Queue q1, uint64_t lim) {
   const int64_t nt = c_nt.get();
   /* 8 blocks per CU fit in LDS with block_flush's 16 B (the kernel's 60
    * VGPRs allow 8 waves per SIMD); a block of 256 alpha triangles averages
@@ -715,19 +727,41 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
   __shared__ uint4 s_out[CAP];
   const int64_t q0 = (int64_t)blockIdx.x * kBlock;
   if (q0 >= nt) { /* whole block past the (device) count */
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = bkeep[blockIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      bcnt[blockIdx.x] = bkeep[blockIdx.x] = 0;
+      bbase[blockIdx.x] = 0;
+    }
     return;
   }
   __shared__ unsigned s_und; /* an undecided candidate written */
   if (threadIdx.x == 0) s_und = 0;
   const int64_t q1e = q0 + kBlock < nt ? q0 + kBlock : nt;
-  BlockOut4<CAP> B{s_out, off[q0], off[q1e] - off[q0], lim};
-  int64_t q = gtid();
+  /* the triangle's slots [o, oe): off[] per triangle, or (roff) the
+   * closed form, a triangle's slots being its row's tail; the block's
+   * range runs from its first triangle's o to its last one's oe */
+  const int64_t q = gtid();
+  uint32_t eij = 0;
+  uint64_t o = 0, oe = 0;
+  if (q < nt) {
+    eij = tedge[q];
+    if (off) {
+      o = off[q];
+      oe = off[q + 1];
+    } else {
+      o = tet_slot_off(roff, trow, eij, (uint32_t)q);
+      oe = o + (trow[eij + 1] - (uint64_t)q - 1);
+    }
+  }
+  __shared__ uint64_t s_span[2];
+  if (q == q0) s_span[0] = o;
+  if (q == q1e - 1) s_span[1] = oe;
+  __syncthreads();
+  if (threadIdx.x == 0) bbase[blockIdx.x] = s_span[0];
+  BlockOut4<CAP> B{s_out, s_span[0], s_span[1] - s_span[0], lim};
   unsigned found = 0;
   if (q < nt) {
-    uint32_t eij = tedge[q], eik = teik[q], ejk = tejk[q];
+    const uint32_t eik = teik[q], ejk = tejk[q];
     const uint32_t i = eleft[eij], j = ecol[eij], k = tk[q];
-    uint64_t o = off[q], oe = off[q + 1];
     if constexpr (BUF) {
       /* the same merge over byte offsets into tk (below 4 GiB): no 64-bit
        * address arithmetic per load.  A triangle's slots are its row's
@@ -839,7 +873,8 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* tk, const uint32_t* tedge,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
-                            const uint64_t* off, uint4* ctet, uint8_t* flag,
+                            const uint64_t* off, const uint64_t* roff,
+                            uint64_t* bbase, uint4* ctet, uint8_t* flag,
                             uint32_t* bcnt, uint32_t* bkeep, Queue q1,
                             PipeLaunch L, uint64_t lim) {
   if (nt.cap <= 0) return;
@@ -851,23 +886,24 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
       auto kf = buf ? k_tet_enum_filter<float, true>
                     : k_tet_enum_filter<float, false>;
       hipLaunchKernelGGL(kf, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge,
-                         teik, tejk, eleft, ecol, off, ctet, flag, bcnt, bkeep,
-                         q1, lim);
+                         teik, tejk, eleft, ecol, off, roff, bbase, ctet, flag,
+                         bcnt, bkeep, q1, lim);
       return;
     }
   }
   auto kd = buf ? k_tet_enum_filter<double, true>
                 : k_tet_enum_filter<double, false>;
   hipLaunchKernelGGL(kd, gr, dim3(kBlock), 0, L.s, S, nt, trow, tk, tedge, teik,
-                     tejk, eleft, ecol, off, ctet, flag, bcnt, bkeep, q1, lim);
+                     tejk, eleft, ecol, off, roff, bbase, ctet, flag, bcnt,
+                     bkeep, q1, lim);
 }
 
 /* After the ladder: the kept tets of each enumeration block (its first
  * bcnt[b] positions from off[256 b] hold its kept and formerly undecided
  * tets), one wave per block. */
 __global__ __launch_bounds__(kBlock) void k_tet_block_keep(
-    Cnt c_nt, const uint64_t* off, const uint32_t* bcnt, const uint8_t* flag,
-    uint32_t* bkeep) {
+    Cnt c_nt, const uint64_t* off, int stride, const uint32_t* bcnt,
+    const uint8_t* flag, uint32_t* bkeep) {
   const int64_t nt = c_nt.get();
   const int64_t nblk = (c_nt.cap + kBlock - 1) / kBlock;
   const int64_t b = gtid() >> 6;
@@ -878,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_keep(
   if (bkeep[b] != kRecount) return;
   unsigned c = 0;
   if (b * kBlock < nt) {
-    const uint64_t base = off[b * kBlock];
+    const uint64_t base = off[b * stride];
     const uint32_t n = bcnt[b];
     for (uint32_t t = lane; t < n; t += 64) c += flag[base + t] == KEEP;
   }
@@ -887,12 +923,13 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_keep(
 }
 /* the kept tets of block b to tets[bout[b], ...), in order */
 __global__ __launch_bounds__(kBlock) void k_tet_block_copy(
-    Cnt c_nt, const uint64_t* off, const uint32_t* bcnt, const uint4* ctet,
-    const uint8_t* flag, const uint64_t* bout, uint4* tets, uint64_t ocap) {
+    Cnt c_nt, const uint64_t* off, int stride, const uint32_t* bcnt,
+    const uint4* ctet, const uint8_t* flag, const uint64_t* bout, uint4* tets,
+    uint64_t ocap) {
   const int64_t nt = c_nt.get();
   const int64_t b = blockIdx.x;
   if (b * kBlock >= nt) return;
-  const uint64_t base = off[b * kBlock];
+  const uint64_t base = off[b * stride];
   const uint32_t n = bcnt[b];
   uint64_t o = bout[b];
   if (bout[b + 1] - o == n) { /* every entry kept: a straight copy */
@@ -924,23 +961,24 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_copy(
     o += tot;
   }
 }
-void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
-                           const uint8_t* flag, uint32_t* bkeep, PipeLaunch L) {
+void launch_tet_block_keep(Cnt nt, const uint64_t* off, int stride,
+                           const uint32_t* bcnt, const uint8_t* flag,
+                           uint32_t* bkeep, PipeLaunch L) {
   if (nt.cap <= 0) return;
   const int64_t nblk = (nt.cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_tet_block_keep,
                      dim3((unsigned)((nblk * 64 + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, off, bcnt, flag, bkeep);
+                     dim3(kBlock), 0, L.s, nt, off, stride, bcnt, flag, bkeep);
 }
-void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
-                           const uint4* ctet, const uint8_t* flag,
-                           const uint64_t* bout, uint4* tets, PipeLaunch L,
-                           uint64_t ocap) {
+void launch_tet_block_copy(Cnt nt, const uint64_t* off, int stride,
+                           const uint32_t* bcnt, const uint4* ctet,
+                           const uint8_t* flag, const uint64_t* bout,
+                           uint4* tets, PipeLaunch L, uint64_t ocap) {
   if (nt.cap <= 0) return;
   hipLaunchKernelGGL(k_tet_block_copy,
                      dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, nt, off, bcnt, ctet, flag, bout,
-                     tets, ocap);
+                     dim3(kBlock), 0, L.s, nt, off, stride, bcnt, ctet, flag,
+                     bout, tets, ocap);
 }
 
 /* Edge candidates of ball i (j > i in the stencil rows, Parallel:122-201)
@@ -1249,7 +1287,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
     Cnt c_ne, const uint64_t* off, const uint32_t* bcnt, const uint32_t* cval,
     const uint32_t* cseg, const uint32_t* cea, const uint32_t* ceb,
     const uint8_t* flag, const uint64_t* bout, uint32_t* tk, uint32_t* tedge,
-    uint32_t* teik, uint32_t* tejk, uint64_t* trow, uint64_t ocap) {
+    uint32_t* teik, uint32_t* tejk, uint64_t* trow, uint64_t* tpair,
+    uint64_t ocap) {
   const int64_t ne = c_ne.get();
   const int64_t b = blockIdx.x;
   const int64_t q0 = b * kBlock;
@@ -1337,18 +1376,20 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
     const uint64_t v = before + incl - c;
     trow[e] = v < ocap ? v : ocap;
   }
+  /* the tet slots of the edge's triangles (each one's row tail) */
+  if (tpair && e < ne) tpair[e] = c ? (uint64_t)c * (c - 1) / 2 : 0;
 }
 void launch_tri_block_copy(Cnt ne, const uint64_t* off, const uint32_t* bcnt,
                            const uint32_t* cval, const uint32_t* cseg,
                            const uint32_t* cea, const uint32_t* ceb,
                            const uint8_t* flag, const uint64_t* bout,
                            uint32_t* tk, uint32_t* tedge, uint32_t* teik,
-                           uint32_t* tejk, uint64_t* trow, PipeLaunch L,
-                           uint64_t ocap) {
+                           uint32_t* tejk, uint64_t* trow, uint64_t* tpair,
+                           PipeLaunch L, uint64_t ocap) {
   const int64_t nblk = (std::max<int64_t>(ne.cap, 0) + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_tri_block_copy, dim3((unsigned)(nblk + 1)), dim3(kBlock),
                      0, L.s, ne, off, bcnt, cval, cseg, cea, ceb, flag, bout,
-                     tk, tedge, teik, tejk, trow, ocap);
+                     tk, tedge, teik, tejk, trow, tpair, ocap);
 }
 
 template <int KIND>

@@ -217,6 +217,10 @@ void scan_u32_to_u64(const uint32_t* in, uint64_t* out, Cnt n, void* tmp,
                      size_t, hipStream_t s, uint64_t* total64) {
   scan_impl<uint32_t, false>(in, out, n, (uint64_t*)tmp, s, total64);
 }
+void scan_u64_to_u64(const uint64_t* in, uint64_t* out, Cnt n, void* tmp,
+                     size_t, hipStream_t s, uint64_t* total64) {
+  scan_impl<uint64_t, false>(in, out, n, (uint64_t*)tmp, s, total64);
+}
 /* Flag scans (u8 flags, nonzero = 1 -> u32 positions), the scan every
  * compaction runs: a thread owns kScanItems (8) consecutive flags, read as
  * one 8-byte word, and writes their 8 positions as two 16-byte stores;

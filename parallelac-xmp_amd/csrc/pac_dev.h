@@ -296,6 +296,26 @@ __device__ __forceinline__ int64_t find_row(const uint32_t* a, uint64_t lo,
   return -1;
 }
 
+/* The same for two values at once (one pass over the row): the
+ * positions of v and w (v < w) in a[lo, hi), each -1 when absent */
+__device__ __forceinline__ void find_row2(const uint32_t* a, uint64_t lo,
+                                          uint64_t hi, uint32_t v, uint32_t w,
+                                          int64_t& pv, int64_t& pw) {
+  pv = -1;
+  pw = -1;
+  for (uint64_t p = lo; p < hi; p += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = p + k < hi ? a[p + k] : 0xffffffffu;
+#pragma unroll
+    for (int k = 7; k >= 0; --k) {
+      if (x[k] == v && p + k < hi) pv = (int64_t)(p + k);
+      if (x[k] == w && p + k < hi) pw = (int64_t)(p + k);
+    }
+    if (x[7] > w) return; /* sorted: both would have been seen */
+  }
+}
+
 /* ---- emptiness search box ----
  * Every ball that can violate a simplex lies within `reach` of its centre
  * c.  The centre is known to within err per axis and the FP32 cell keys of

@@ -86,6 +86,9 @@ enum { FAM_EDGE = 0, FAM_TRI = 1, FAM_TET = 2, FAM_TET_AC2 = 3,
 void scan_u32_to_u64(const uint32_t* in, uint64_t* out, Cnt n, void* tmp,
                      size_t tmp_bytes, hipStream_t s,
                      uint64_t* total64 = nullptr);
+void scan_u64_to_u64(const uint64_t* in, uint64_t* out, Cnt n, void* tmp,
+                     size_t tmp_bytes, hipStream_t s,
+                     uint64_t* total64 = nullptr);
 /* flags (nonzero = 1) -> u32 positions; n < 2^32 */
 void scan_u8flag_to_u32(const uint8_t* in, uint32_t* out, Cnt n, void* tmp,
                         size_t tmp_bytes, hipStream_t s,
@@ -440,25 +443,30 @@ void launch_tri_enum_filter(const Stage& S, Cnt ne, const uint64_t* erow,
                             Queue q1, PipeLaunch L, uint64_t lim = ~0ull);
 /* after the ladder (bkeep by launch_tet_block_keep over the ne alpha
  * edges, bout its exclusive scan): the kept triangles of each block to
- * (tk, tedge, teik, tejk) in order, and the alpha-triangle row start of
- * every alpha edge, trow[0 .. ne] */
+ * (tk, tedge, teik, tejk) in order, the alpha-triangle row start of every
+ * alpha edge, trow[0 .. ne], and (tpair, if given) each edge's tet slots,
+ * L (L - 1) / 2 for its L triangles (each one's row tail) */
 void launch_tri_block_copy(Cnt ne, const uint64_t* off, const uint32_t* bcnt,
                            const uint32_t* cval, const uint32_t* cseg,
                            const uint32_t* cea, const uint32_t* ceb,
                            const uint8_t* flag, const uint64_t* bout,
                            uint32_t* tk, uint32_t* tedge, uint32_t* teik,
-                           uint32_t* tejk, uint64_t* trow, PipeLaunch L,
-                           uint64_t ocap = ~0ull);
+                           uint32_t* tejk, uint64_t* trow, uint64_t* tpair,
+                           PipeLaunch L, uint64_t ocap = ~0ull);
 /* tet candidates of the alpha triangles, each filtered by the first
  * precision stage in the same kernel; enumeration block b (256 alpha
  * triangles) writes its kept and undecided tets (i, j, k, l) with their
- * flags, in order, compacted from its first slot off[256 b] on and their
- * number to bcnt[b]; undecided ones are queued on q1 by position */
+ * flags, in order, compacted from its first slot on (bbase[b]) and their
+ * number to bcnt[b]; undecided ones are queued on q1 by position.  Slot
+ * offsets per triangle from off[], or (off null) from roff, the exclusive
+ * scan of launch_tri_block_copy's tpair (each triangle's slots its row's
+ * tail: not for chunk sub-inputs, whose unowned triangles get none) */
 void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
                             const uint32_t* tk, const uint32_t* tedge,
                             const uint32_t* teik, const uint32_t* tejk,
                             const uint32_t* eleft, const uint32_t* ecol,
-                            const uint64_t* off, uint4* ctet, uint8_t* flag,
+                            const uint64_t* off, const uint64_t* roff,
+                            uint64_t* bbase, uint4* ctet, uint8_t* flag,
                             uint32_t* bcnt, uint32_t* bkeep, Queue q1,
                             PipeLaunch L, uint64_t lim = ~0ull);
 /* The triangle and tet enumerations also write bkeep[b]: bcnt[b] when the
@@ -467,13 +475,16 @@ void launch_tet_enum_filter(const Stage& S, Cnt nt, const uint64_t* trow,
 constexpr uint32_t kRecount = 0xffffffffu;
 /* after the ladder: kept tets per enumeration block (ceil(nt.cap / 256)
  * entries; blocks marked kRecount recounted from their flags), then, with
- * bout their exclusive scan, the kept tets in order */
-void launch_tet_block_keep(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
-                           const uint8_t* flag, uint32_t* bkeep, PipeLaunch L);
-void launch_tet_block_copy(Cnt nt, const uint64_t* off, const uint32_t* bcnt,
-                           const uint4* ctet, const uint8_t* flag,
-                           const uint64_t* bout, uint4* tets, PipeLaunch L,
-                           uint64_t ocap = ~0ull);
+ * bout their exclusive scan, the kept tets in order; block b's entries
+ * start at off[b * stride] (per-item offsets: stride 256; per-block
+ * bases: 1) */
+void launch_tet_block_keep(Cnt nt, const uint64_t* off, int stride,
+                           const uint32_t* bcnt, const uint8_t* flag,
+                           uint32_t* bkeep, PipeLaunch L);
+void launch_tet_block_copy(Cnt nt, const uint64_t* off, int stride,
+                           const uint32_t* bcnt, const uint4* ctet,
+                           const uint8_t* flag, const uint64_t* bout,
+                           uint4* tets, PipeLaunch L, uint64_t ocap = ~0ull);
 /* escalation and exact stages of the filters (the first stages are fused
  * with the enumerations): kind 1 edges, 2 triangles, 3 tets */
 void launch_filter_esc(const Stage& S, int kind, CandCtx C, QRef q, const uint32_t* cval, uint8_t* flag,

@@ -592,6 +592,11 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   /* ---- triangles (AlphaComplexParallel.cu:1279-1371) ---- */
   Cnt Tslots(0), Talpha(0);
   DBuf<uint64_t> trow(P, Ealpha.cap + 1);
+  /* the tet slots of each alpha edge's triangles (their row tails), for
+   * the tets' slot offsets in closed form; chunk sub-inputs give unowned
+   * triangles no slots and keep per-triangle offsets */
+  const bool tail_slots = !SG.chunk;
+  DBuf<uint64_t> tpair(P, tail_slots ? std::max<int64_t>(Ealpha.cap, 1) : 1);
   DBuf<uint32_t> tk, tedge, teik, tejk;
   CandCtx C1{eleft.get(), ecol.get(), nullptr, nullptr, nullptr};
   {
@@ -635,7 +640,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
           launch_filter_exact(SG, 2, C1, qr, cval.get(), flag.get(), L);
         },
         [&] {
-          launch_tet_block_keep(Ealpha, off.get(), bcnt.get(), flag.get(),
+          launch_tet_block_keep(Ealpha, off.get(), kBlock, bcnt.get(), flag.get(),
                                 bkeep.get(), L);
           scan_u32_to_u64(bkeep.get(), bout.get(), Cnt(nblk), tmp.get(), 0, s,
                           dcnt + RC_TALPHA);
@@ -647,7 +652,8 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
     tejk = DBuf<uint32_t>(P, Talpha.cap);
     launch_tri_block_copy(Ealpha, off.get(), bcnt.get(), cval.get(), cseg.get(),
                           cea.get(), ceb.get(), flag.get(), bout.get(), tk.get(),
-                          tedge.get(), teik.get(), tejk.get(), trow.get(), L,
+                          tedge.get(), teik.get(), tejk.get(), trow.get(),
+                          tail_slots ? tpair.get() : nullptr, L,
                           (uint64_t)Talpha.cap);
     KCHK();
   }
@@ -658,13 +664,28 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
   DBuf<uint4> tets;
   CandCtx C2{eleft.get(), ecol.get(), tk.get(), tedge.get(), nullptr};
   {
-    DBuf<uint64_t> off(P, Talpha.cap + 1);
-    {
+    /* slot offsets: per alpha edge in closed form (roff), or per alpha
+     * triangle from its cap (off, chunk sub-inputs) */
+    DBuf<uint64_t> off, roff;
+    if (tail_slots) {
+      roff = DBuf<uint64_t>(P, Ealpha.cap + 1);
+      {
+        auto tmp = scan_tmp(Ealpha.cap);
+        scan_u64_to_u64(tpair.get(), roff.get(), Ealpha, tmp.get(), 0, s,
+                        dcnt + RC_KSLOTS);
+      }
+      KCHK();
+      Kslots = settle(RC_KSLOTS);
+      if (Kslots.cap >= ((int64_t)1 << 32))
+        fail(PAC_ERR_OVERFLOW, "tet candidates exceed 2^32");
+    } else {
+      off = DBuf<uint64_t>(P, Talpha.cap + 1);
       DBuf<uint32_t> cap(P, Talpha.cap);
       launch_tet_caps(SG, Talpha, trow.get(), tedge.get(), eleft.get(),
                       ecol.get(), tk.get(), cap.get(), L);
       Kslots = count_scan(cap.get(), Talpha, off.get(), RC_KSLOTS, "tet");
     }
+    tpair.release();
     DBuf<uint4> ctet(P, Kslots.cap);
     DBuf<uint8_t> flag(P, Kslots.cap);
     C2.ctet = ctet.get();
@@ -674,7 +695,7 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
      * are counted, scanned and copied out in order */
     const int64_t nblk = (Talpha.cap + kBlock - 1) / kBlock;
     DBuf<uint32_t> bcnt(P, nblk + 1), bkeep(P, nblk + 1);
-    DBuf<uint64_t> bout(P, nblk + 1);
+    DBuf<uint64_t> bout(P, nblk + 1), bbase(P, nblk + 1);
     auto tmp = scan_tmp(nblk);
     /* a rerun (queue overflow, eager only) recounts the candidates from
      * zero */
@@ -686,23 +707,26 @@ void run_pipeline(Engine* E, const Ball* raw, int64_t n, Real solvent,
             zero_u64(ctr + (CTR_CAND + 2) * CTR_SLOTS, CTR_SLOTS, s);
           launch_tet_enum_filter(SG, Talpha, trow.get(), tk.get(), tedge.get(),
                                  teik.get(), tejk.get(), eleft.get(), ecol.get(),
-                                 off.get(), ctet.get(), flag.get(), bcnt.get(),
-                                 bkeep.get(), qq, L, (uint64_t)Kslots.cap);
+                                 tail_slots ? nullptr : off.get(),
+                                 tail_slots ? roff.get() : nullptr, bbase.get(),
+                                 ctet.get(), flag.get(), bcnt.get(), bkeep.get(),
+                                 qq, L, (uint64_t)Kslots.cap);
         },
         [&](QRef qr, Queue q2) {
           launch_filter_esc(SG, 3, C2, qr, nullptr, flag.get(), q2, L);
         },
         [&](QRef qr) { launch_filter_exact(SG, 3, C2, qr, nullptr, flag.get(), L); },
         [&] {
-          launch_tet_block_keep(Talpha, off.get(), bcnt.get(), flag.get(),
+          launch_tet_block_keep(Talpha, bbase.get(), 1, bcnt.get(), flag.get(),
                                 bkeep.get(), L);
           scan_u32_to_u64(bkeep.get(), bout.get(), Cnt(nblk), tmp.get(), 0, s,
                           dcnt + RC_KALPHA);
         },
         RC_KALPHA);
     tets = DBuf<uint4>(P, Kalpha.cap);
-    launch_tet_block_copy(Talpha, off.get(), bcnt.get(), ctet.get(), flag.get(),
-                          bout.get(), tets.get(), L, (uint64_t)Kalpha.cap);
+    launch_tet_block_copy(Talpha, bbase.get(), 1, bcnt.get(), ctet.get(),
+                          flag.get(), bout.get(), tets.get(), L,
+                          (uint64_t)Kalpha.cap);
     KCHK();
   }
   T.mark(5);
