@@ -577,11 +577,17 @@ __global__ __launch_bounds__(kBlock) void k_tri_count(Cnt c_ne,
     const BufU32 E{ecol, (uint32_t)(c_ne.cap * 4)};
     uint32_t a = ((uint32_t)q + 1u) << 2, ae = (uint32_t)erow[i + 1] << 2;
     uint32_t b = (uint32_t)erow[j] << 2, be = (uint32_t)erow[j + 1] << 2;
+    /* two merge steps a round: each list's next element is loaded with
+     * its current one (~0u past the row's end, above every ball id) */
     while (a < ae && b < be) {
-      const uint32_t x = E.at(a), y = E.at(b);
-      c += (x == y);
-      a += x <= y ? 4u : 0u;
-      b += y <= x ? 4u : 0u;
+      const uint32_t x0 = E.at(a), y0 = E.at(b);
+      const uint32_t x1 = a + 4u < ae ? E.at(a + 4u) : ~0u;
+      const uint32_t y1 = b + 4u < be ? E.at(b + 4u) : ~0u;
+      const bool sa = x0 <= y0, sb = y0 <= x0;
+      const uint32_t x = sa ? x1 : x0, y = sb ? y1 : y0;
+      c += (x0 == y0) + (x == y && x != ~0u);
+      a += (sa ? 4u : 0u) + (x <= y ? 4u : 0u);
+      b += (sb ? 4u : 0u) + (y <= x ? 4u : 0u);
     }
   } else {
     /* alpha-edge ids fit 32 bits (the candidate count is checked below
@@ -772,18 +778,33 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 8 : 1))) void k_tet_enum_fil
       uint32_t b = (uint32_t)trow[eik] << 2, be = (uint32_t)trow[eik + 1] << 2;
       uint32_t c = (uint32_t)trow[ejk] << 2, ce = (uint32_t)trow[ejk + 1] << 2;
       if (o == oe) ae = a;
+      /* two merge steps a round: each list's next element is loaded with
+       * its current one (~0u past the row's end, above every ball id) */
       while (a < ae && b < be && c < ce) {
-        const uint32_t x = K.at(a), y = K.at(b), z = K.at(c);
+        const uint32_t x0 = K.at(a), y0 = K.at(b), z0 = K.at(c);
+        const uint32_t x1 = a + 4u < ae ? K.at(a + 4u) : ~0u;
+        const uint32_t y1 = b + 4u < be ? K.at(b + 4u) : ~0u;
+        const uint32_t z1 = c + 4u < ce ? K.at(c + 4u) : ~0u;
+        const uint32_t m0 = max(x0, max(y0, z0));
+        const bool all0 = x0 == m0 && y0 == m0 && z0 == m0;
+        if (all0) {
+          B.put(ctet, o, make_uint4(i, j, k, m0));
+          ++o;
+          ++found;
+        }
+        const bool sa = all0 || x0 < m0, sb = all0 || y0 < m0,
+                   sc = all0 || z0 < m0;
+        const uint32_t x = sa ? x1 : x0, y = sb ? y1 : y0, z = sc ? z1 : z0;
         const uint32_t m = max(x, max(y, z));
         const bool all = x == m && y == m && z == m;
-        if (all) {
+        if (all && m != ~0u) {
           B.put(ctet, o, make_uint4(i, j, k, m));
           ++o;
           ++found;
         }
-        a += (all || x < m) ? 4u : 0u;
-        b += (all || y < m) ? 4u : 0u;
-        c += (all || z < m) ? 4u : 0u;
+        a += (sa ? 4u : 0u) + ((all || x < m) ? 4u : 0u);
+        b += (sb ? 4u : 0u) + ((all || y < m) ? 4u : 0u);
+        c += (sc ? 4u : 0u) + ((all || z < m) ? 4u : 0u);
       }
     } else {
       /* alpha-triangle ids fit 32 bits (the slot count is checked below
@@ -921,44 +942,39 @@ __global__ __launch_bounds__(kBlock) void k_tet_block_keep(
   c = wave_sum(c);
   if (lane == 0) bkeep[b] = c;
 }
-/* the kept tets of block b to tets[bout[b], ...), in order */
+/* the kept tets of block b to tets[bout[b], ...), in order; one wave per
+ * block (a block holds ~100 entries: with 256-thread workgroups the copy
+ * waits on four times fewer blocks in flight, each a chain of dependent
+ * loads).  Grid as k_tet_block_keep's. */
 __global__ __launch_bounds__(kBlock) void k_tet_block_copy(
     Cnt c_nt, const uint64_t* off, int stride, const uint32_t* bcnt,
     const uint4* ctet, const uint8_t* flag, const uint64_t* bout, uint4* tets,
     uint64_t ocap) {
   const int64_t nt = c_nt.get();
-  const int64_t b = blockIdx.x;
-  if (b * kBlock >= nt) return;
+  const int64_t b = gtid() >> 6;
+  const int lane = threadIdx.x & 63;
+  if (b * kBlock >= nt) return; /* wave-uniform */
   const uint64_t base = off[b * stride];
   const uint32_t n = bcnt[b];
   uint64_t o = bout[b];
   if (bout[b + 1] - o == n) { /* every entry kept: a straight copy */
-    for (uint32_t t = threadIdx.x; t < n; t += kBlock)
+    for (uint32_t t = lane; t < n; t += 64)
       if (o + t < ocap) tets[o + t] = ctet[base + t];
     return;
   }
-  /* wave sums double-buffered by round parity (one barrier a round) */
-  __shared__ unsigned s_wsum[2][kBlock / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  int par = 0;
-  for (uint32_t r0 = 0; r0 < n; r0 += kBlock, par ^= 1) {
-    const uint32_t t = r0 + threadIdx.x;
-    const bool keep = t < n && flag[base + t] == KEEP;
+  for (uint32_t r0 = 0; r0 < n; r0 += 64) {
+    const uint32_t t = r0 + lane;
+    bool keep = false;
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (keep) v = ctet[base + t];
-    const uint64_t bm = __ballot(keep);
-    if (lane == 0) s_wsum[par][w] = (unsigned)__popcll(bm);
-    __syncthreads();
-    uint64_t pre = o + (uint64_t)__popcll(bm & lt), tot = 0;
-#pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) {
-      const unsigned c = s_wsum[par][q];
-      if (q < w) pre += c;
-      tot += c;
+    if (t < n) { /* both loads in flight together */
+      v = ctet[base + t];
+      keep = flag[base + t] == KEEP;
     }
+    const uint64_t bm = __ballot(keep);
+    const uint64_t pre = o + (uint64_t)__popcll(bm & lt);
     if (keep && pre < ocap) tets[pre] = v;
-    o += tot;
+    o += (uint64_t)__popcll(bm);
   }
 }
 void launch_tet_block_keep(Cnt nt, const uint64_t* off, int stride,
@@ -975,8 +991,9 @@ void launch_tet_block_copy(Cnt nt, const uint64_t* off, int stride,
                            const uint8_t* flag, const uint64_t* bout,
                            uint4* tets, PipeLaunch L, uint64_t ocap) {
   if (nt.cap <= 0) return;
+  const int64_t nblk = (nt.cap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_tet_block_copy,
-                     dim3((unsigned)((nt.cap + kBlock - 1) / kBlock)),
+                     dim3((unsigned)((nblk * 64 + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, L.s, nt, off, stride, bcnt, ctet, flag,
                      bout, tets, ocap);
 }
@@ -1184,9 +1201,22 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum_filter(
       const BufU32 E{ecol, (uint32_t)(c_ne.cap * 4)};
       uint32_t a = ((uint32_t)q + 1u) << 2, ae = (uint32_t)erow[i + 1] << 2;
       uint32_t b = (uint32_t)erow[j] << 2, be = (uint32_t)erow[j + 1] << 2;
+      /* two merge steps a round, as in k_tri_count */
       while (a < ae && b < be) {
-        const uint32_t x = E.at(a), y = E.at(b);
-        if (x == y) {
+        const uint32_t x0 = E.at(a), y0 = E.at(b);
+        const uint32_t x1 = a + 4u < ae ? E.at(a + 4u) : ~0u;
+        const uint32_t y1 = b + 4u < be ? E.at(b + 4u) : ~0u;
+        if (x0 == y0) {
+          const uint32_t v[4] = {x0, (uint32_t)q, a >> 2, b >> 2};
+          B.put(g, o, v);
+          ++o;
+          ++found;
+        }
+        const bool sa = x0 <= y0, sb = y0 <= x0;
+        a += sa ? 4u : 0u;
+        b += sb ? 4u : 0u;
+        const uint32_t x = sa ? x1 : x0, y = sb ? y1 : y0;
+        if (x == y && x != ~0u) {
           const uint32_t v[4] = {x, (uint32_t)q, a >> 2, b >> 2};
           B.put(g, o, v);
           ++o;
