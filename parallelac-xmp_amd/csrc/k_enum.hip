@@ -1342,8 +1342,18 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
     int par = 0;
     for (uint32_t r0 = 0; r0 < n; r0 += kBlock, par ^= 1) {
       const uint32_t t = r0 + threadIdx.x;
-      const bool keep = t < n && (all || flag[base + t] == KEEP);
-      const uint32_t seg = keep ? cseg[base + t] : 0xffffffffu;
+      /* every load of the round in flight together: an entry's arrays are
+       * read whether or not it is kept (all were written) */
+      uint32_t fv = KEEP, sv = 0xffffffffu, xv = 0u, av = 0u, bv = 0u;
+      if (t < n) {
+        if (!all) fv = flag[base + t];
+        sv = cseg[base + t];
+        xv = cval[base + t];
+        av = cea[base + t];
+        bv = ceb[base + t];
+      }
+      const bool keep = t < n && fv == KEEP;
+      const uint32_t seg = keep ? sv : 0xffffffffu;
       /* entries come in edge order: one LDS add per run of lanes with
        * the same edge (its first lane adds the run's length) instead of
        * one per lane on the same counter */
@@ -1358,10 +1368,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
       if (all) {
         if (keep && o + t - r0 < ocap) {
           const uint64_t at = o + t - r0;
-          tk[at] = cval[base + t];
+          tk[at] = xv;
           tedge[at] = seg;
-          teik[at] = cea[base + t];
-          tejk[at] = ceb[base + t];
+          teik[at] = av;
+          tejk[at] = bv;
         }
         o += min((uint32_t)kBlock, n - r0);
         continue;
@@ -1377,10 +1387,10 @@ __global__ __launch_bounds__(kBlock) void k_tri_block_copy(
         tot += c;
       }
       if (keep && pre < ocap) {
-        tk[pre] = cval[base + t];
+        tk[pre] = xv;
         tedge[pre] = seg;
-        teik[pre] = cea[base + t];
-        tejk[pre] = ceb[base + t];
+        teik[pre] = av;
+        tejk[pre] = bv;
       }
       o += tot;
     }
